@@ -1,0 +1,208 @@
+"""MI355X-native coupled-cluster-doubles iteration.
+
+Python mirror of the reference's interface for the CCD hot path
+(justinlietz/coupled_cluster_doubles): a basis model (PairingSPBasis,
+ElectronGasSPBasis, src/parse.cpp:231-244) and a device context that
+provides MBPT2Corr, the diagram functions of src/calc.cpp and the
+iteration loop of src/main.cpp:491-682.  All compute runs in
+libccd_amd.so on the GPU; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import CCDError, Problem, check, lib
+
+FP64 = 0
+FP32 = 1
+STOP_REASONS = {0: "converged", 1: "max_iterations", 2: "nan"}
+
+__all__ = ["Model", "Context", "CCDError", "FP64", "FP32", "STOP_REASONS"]
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Model:
+    """Host problem model: s.p. basis, channel blocking and interaction."""
+
+    def __init__(self, handle):
+        self.h = handle
+        n_sp, n_h, n_ch, n_t2 = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int64()
+        check(lib().ccd_model_sizes(self.h, C.byref(n_sp), C.byref(n_h), C.byref(n_ch),
+                                    C.byref(n_t2)), "ccd_model_sizes")
+        self.n_sp, self.n_holes = n_sp.value, n_h.value
+        self.n_chan, self.n_t2 = n_ch.value, n_t2.value
+
+    @classmethod
+    def pairing(cls, n_pair_states=8, n_particles=8, g=0.5, xi=1.0):
+        h = C.c_void_p()
+        check(lib().ccd_model_pairing(n_pair_states, n_particles, g, xi, C.byref(h)),
+              "ccd_model_pairing")
+        return cls(h)
+
+    @classmethod
+    def electron_gas(cls, r_s=1.0, n_shells=4, n_particle_shells=2):
+        h = C.c_void_p()
+        check(lib().ccd_model_electron_gas(r_s, n_shells, n_particle_shells, C.byref(h)),
+              "ccd_model_electron_gas")
+        return cls(h)
+
+    def eref(self) -> float:
+        e = C.c_double()
+        check(lib().ccd_model_eref(self.h, C.byref(e)), "ccd_model_eref")
+        return e.value
+
+    def problem(self, host_v: bool = False):
+        p = C.POINTER(Problem)()
+        f = lib().ccd_model_problem_host_v if host_v else lib().ccd_model_problem
+        check(f(self.h, C.byref(p)), "ccd_model_problem")
+        return p
+
+    def eps(self) -> np.ndarray:
+        p = self.problem().contents
+        return np.ctypeslib.as_array(C.cast(p.eps, C.POINTER(C.c_double)), (self.n_sp,)).copy()
+
+    def dims(self):
+        p = self.problem().contents
+        out = {}
+        for k in ("pp_dim", "hh_dim", "ph_dim", "hp_dim"):
+            n = p.n_chan if k in ("pp_dim", "hh_dim") else p.n_mod
+            out[k] = np.ctypeslib.as_array(C.cast(getattr(p, k), C.POINTER(C.c_int32)), (n,)).copy()
+        return out
+
+    def pairs(self, kind: str) -> np.ndarray:
+        """Concatenated ordered pairs of one class ('pp', 'hh', 'ph', 'hp')."""
+        p = self.problem().contents
+        dims = self.dims()[kind + "_dim"]
+        n = int(dims.sum())
+        if n == 0:
+            return np.zeros((0, 2), dtype=np.int32)
+        arr = np.ctypeslib.as_array(C.cast(getattr(p, kind + "_pairs"), C.POINTER(C.c_int32)), (2 * n,))
+        return arr.reshape(n, 2).copy()
+
+    def t2_labels(self) -> np.ndarray:
+        """(a, b, i, j) of every pphh element."""
+        d = self.dims()
+        pp, hh = self.pairs("pp"), self.pairs("hh")
+        out = np.empty((self.n_t2, 4), dtype=np.int32)
+        e = op = oh = 0
+        for P, H in zip(d["pp_dim"], d["hh_dim"]):
+            if P and H:
+                blk = out[e:e + P * H].reshape(P, H, 4)
+                blk[:, :, 0:2] = pp[op:op + P][:, None, :]
+                blk[:, :, 2:4] = hh[oh:oh + H][None, :, :]
+            e += P * H
+            op += P
+            oh += H
+        return out
+
+    def close(self):
+        if self.h:
+            lib().ccd_model_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    """Device-resident CCD state for one problem on the current HIP device."""
+
+    def __init__(self, model: Model, precision: int = FP64, host_v: bool = False):
+        self.model = model
+        self.h = C.c_void_p()
+        check(lib().ccd_ctx_create(model.problem(host_v), precision, C.byref(self.h)),
+              "ccd_ctx_create")
+        n, b = C.c_int64(), C.c_int64()
+        check(lib().ccd_ctx_info(self.h, C.byref(n), C.byref(b)), "ccd_ctx_info")
+        self.n_t2, self.device_bytes = n.value, b.value
+        self.n_holes = model.n_holes
+        self.n_sp = model.n_sp
+
+    def close(self):
+        if self.h:
+            lib().ccd_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def mbpt2(self) -> float:
+        e = C.c_double()
+        check(lib().ccd_mbpt2(self.h, C.byref(e)), "ccd_mbpt2")
+        return e.value
+
+    def reset(self):
+        check(lib().ccd_reset(self.h), "ccd_reset")
+
+    def iterate(self, mixing: float = 0.5) -> float:
+        e = C.c_double()
+        check(lib().ccd_iterate(self.h, mixing, C.byref(e)), "ccd_iterate")
+        return e.value
+
+    def solve(self, tol: float, max_iters: int = 100, mixing: float = 0.5, cap: int = 256):
+        en = np.zeros(cap)
+        n, stop = C.c_int(), C.c_int()
+        check(lib().ccd_solve(self.h, tol, max_iters, mixing, _ptr(en), cap, C.byref(n),
+                              C.byref(stop)), "ccd_solve")
+        return en[:min(n.value, cap)].copy(), STOP_REASONS[stop.value], n.value
+
+    def get_t2(self) -> np.ndarray:
+        t = np.empty(self.n_t2)
+        check(lib().ccd_get_t2(self.h, _ptr(t)), "ccd_get_t2")
+        return t
+
+    def set_t2(self, t: np.ndarray):
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        assert t.size == self.n_t2
+        check(lib().ccd_set_t2(self.h, _ptr(t)), "ccd_set_t2")
+
+    def _io(self, fn, name, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        assert x.size == self.n_t2
+        y = np.empty(self.n_t2)
+        check(fn(self.h, _ptr(x), _ptr(y)), name)
+        return y
+
+    def ladder(self, t):
+        return self._io(lib().ccd_ladder, "ccd_ladder", t)
+
+    def ring(self, tph):
+        return self._io(lib().ccd_ring, "ccd_ring", tph)
+
+    def remap_ph(self, t):
+        return self._io(lib().ccd_remap_ph, "ccd_remap_ph", t)
+
+    def onebody(self, t):
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        chi = np.empty(self.n_holes)
+        psi = np.empty(self.n_sp - self.n_holes)
+        check(lib().ccd_onebody(self.h, _ptr(t), _ptr(chi), _ptr(psi)), "ccd_onebody")
+        return chi, psi
+
+    def step(self, t_old, mixing: float = 0.5):
+        t_old = np.ascontiguousarray(t_old, dtype=np.float64)
+        t_new = np.empty(self.n_t2)
+        e = C.c_double()
+        check(lib().ccd_step(self.h, _ptr(t_old), _ptr(t_new), mixing, C.byref(e)), "ccd_step")
+        return t_new, e.value
+
+    def time_phases(self, iters: int, mixing: float = 0.5) -> np.ndarray:
+        ms = np.zeros(6)
+        check(lib().ccd_time_phases(self.h, iters, mixing, _ptr(ms)), "ccd_time_phases")
+        return ms
+
+    def work(self):
+        f, fp, b = C.c_double(), C.c_double(), C.c_double()
+        check(lib().ccd_work(self.h, C.byref(f), C.byref(fp), C.byref(b)), "ccd_work")
+        return {"flops": f.value, "flops_pppp": fp.value, "bytes_vpppp": b.value}

@@ -1,0 +1,60 @@
+"""Build libccd_amd.so (and the ccd driver) in-tree for gfx950.
+
+    python -m coupled_cluster_doubles_amd.build
+
+hipcc cross-compiles the device code without a GPU.  The host problem model
+is compiled by g++ with -ffp-contract=off so its arithmetic matches the
+reference's x86-64 evaluation bit for bit.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+SRC = os.path.join(PKG, "csrc")
+OUT = os.path.join(PKG, "_build")
+LIB = os.path.join(PKG, "libccd_amd.so")
+EXE = os.path.join(PKG, "ccd")
+INC = os.path.join(ROOT, "include")
+
+ARCH = os.environ.get("CCD_OFFLOAD_ARCH", "gfx950")
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-munsafe-fp-atomics", f"-I{INC}", f"-I{SRC}",
+]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-ffp-contract=off", f"-I{INC}", f"-I{SRC}"]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build(verbose: bool = True) -> str:
+    os.makedirs(OUT, exist_ok=True)
+    objs = []
+    o = os.path.join(OUT, "model.o")
+    _run(["g++", *CXX_FLAGS, "-c", os.path.join(SRC, "model.cpp"), "-o", o])
+    objs.append(o)
+    o = os.path.join(OUT, "ccd_device.o")
+    _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(SRC, "ccd_device.hip"), "-o", o])
+    objs.append(o)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    main = os.path.join(SRC, "ccd_main.cpp")
+    if os.path.exists(main):
+        _run(["g++", *CXX_FLAGS, main, "-o", EXE, f"-L{PKG}", "-lccd_amd",
+              f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN"])
+    return LIB
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

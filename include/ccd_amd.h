@@ -1,0 +1,187 @@
+/*
+ * ccd_amd.h -- C ABI of the MI355X-native coupled-cluster-doubles iteration.
+ *
+ * Two halves:
+ *
+ *  1. The device context (ccd_ctx_*, ccd_iterate, ccd_solve, per-diagram
+ *     entry points).  This is the drop-in for the reference's hot path:
+ *     the diagram functions of src/calc.cpp, MBPT2Corr of
+ *     src/MBPT2Corr.cpp, the Chain remaps of src/Chain.hpp:710-802 and the
+ *     iteration loop of src/main.cpp:491-682.  It takes a plain descriptor
+ *     of the channel-blocked problem (ccd_problem) that a caller fills from
+ *     the reference's SPBasis arrays and Chain<double> blocks.
+ *
+ *  2. The host problem model (ccd_model_*), a behaviour-identical restatement
+ *     of the reference's basis setup (src/PairingSPBasis.cpp,
+ *     src/ElectronGasSPBasis.cpp, src/SPBasis.cpp, src/balance.cpp at one
+ *     rank, src/load.cpp), so the library runs the reference's input
+ *     format without the reference present.  It fills a ccd_problem.
+ *
+ * Conventions: every function returns 0 on success and a negative
+ * CCD_E_* code on failure (ccd_last_error() gives the message of the
+ * calling thread's last failure); nothing throws across the ABI.  Host
+ * buffers belong to the caller; the context owns its device memory and one
+ * HIP stream.  A context is used from one host thread at a time.
+ *
+ * Layouts are the reference's Chain<double> layouts (src/Chain.hpp:373-426):
+ * channel blocks back to back in the caller's channel order, each block
+ * row-major.  "pphh": block c is pp_dim[c] x hh_dim[c], element
+ * [(a,b),(i,j)] = t_ij^ab.  "phhp": mod block m is ph_dim[m] x hp_dim[m],
+ * element [(a,i),(j,b)] = t_ij^ab.  Pair lists hold ordered pairs (p,q),
+ * two int32 per pair, in the row/column order of the blocks.
+ */
+#ifndef CCD_AMD_H
+#define CCD_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CCD_OK 0
+#define CCD_E_INVALID (-1)   /* bad argument or inconsistent descriptor */
+#define CCD_E_HIP (-2)       /* HIP runtime failure (no device, launch error) */
+#define CCD_E_NOMEM (-3)     /* host or device allocation failed */
+#define CCD_E_STATE (-4)     /* call not valid in the context's state */
+
+/* Arithmetic of the amplitude contractions. */
+#define CCD_PREC_FP64 0      /* everything FP64 (the parity path) */
+#define CCD_PREC_FP32 1      /* FP32 contractions, FP64 update/denominators/energy */
+
+/* Interaction source for a ccd_problem. */
+#define CCD_GEN_NONE 0           /* V blocks supplied by the caller (v_* below) */
+#define CCD_GEN_PAIRING 1        /* generate <pq||rs> of the pairing model on device */
+#define CCD_GEN_ELECTRON_GAS 2   /* generate the antisymmetrised Coulomb element on device */
+
+/* Solve stop reasons (ccd_solve). */
+#define CCD_STOP_CONVERGED 0
+#define CCD_STOP_MAX_ITERS 1     /* main.cpp:512-519: halted after iteration max_iters */
+#define CCD_STOP_NAN 2           /* |dE| > tol was false because dE is NaN (main.cpp:511) */
+
+typedef struct ccd_problem {
+    int32_t n_sp;           /* SPBasis::nSpstates */
+    int32_t n_holes;        /* SPBasis::nParticles; states 0..n_holes-1 are holes */
+    const double *eps;      /* [n_sp] normal-ordered s.p. energies (SPBasis::spEnergy
+                               after rotateSpEnergiesToNormalOrdered) */
+    int32_t n_chan;         /* direct channels (SPBasis::nChannels) */
+    const int32_t *pp_dim;  /* [n_chan] chanDims[c].ppDim */
+    const int32_t *hh_dim;  /* [n_chan] chanDims[c].hhDim */
+    const int32_t *pp_pairs;/* chanMaps[c].ppMap, concatenated over c */
+    const int32_t *hh_pairs;/* chanMaps[c].hhMap, concatenated over c */
+    int32_t n_mod;          /* mod channels (also SPBasis::nChannels in the reference) */
+    const int32_t *ph_dim;  /* [n_mod] chanModDims[m].phDim */
+    const int32_t *hp_dim;  /* [n_mod] chanModDims[m].hpDim */
+    const int32_t *ph_pairs;/* chanModMaps[m].phMap, concatenated */
+    const int32_t *hp_pairs;/* chanModMaps[m].hpMap, concatenated */
+    /* Caller-supplied interaction (gen_kind == CCD_GEN_NONE), FP64, reference
+     * Chain layouts over all channels: vnn_hhhh (hh x hh), vnn_hhpp (hh x pp),
+     * vnn_pppp (pp x pp), vnn_hhpp_hpph_mod (hp x ph), vnn_hpph_hphp_mod
+     * (hp x hp) -- see src/load.cpp:112-318 and src/main.cpp:293-366. */
+    const double *v_hhhh;
+    const double *v_hhpp;
+    const double *v_pppp;
+    const double *v_modA;
+    const double *v_modB;
+    /* On-device generation (gen_kind != CCD_GEN_NONE). */
+    int32_t gen_kind;
+    const int32_t *qn;      /* [n_sp*5] electron gas: nx ny nz sz tz; pairing: level spin - - - */
+    double gen_g;           /* pairing strength g */
+    int32_t gen_nmax;       /* electron gas: max |n_x| of the basis */
+    const double *gen_table;/* electron gas: direct Coulomb term per momentum transfer,
+                               (4*gen_nmax+1)^3 entries, index
+                               ((dx+2m)*(4m+1) + (dy+2m))*(4m+1) + (dz+2m), m = gen_nmax */
+} ccd_problem;
+
+typedef struct ccd_ctx ccd_ctx;
+typedef struct ccd_model ccd_model;
+
+const char *ccd_last_error(void);
+
+/* ---------------------------------------------------------------- device */
+
+/* Validate the descriptor, build the device tables and upload (or generate)
+ * the interaction on the current HIP device.  T2 starts at zero. */
+int ccd_ctx_create(const ccd_problem *prob, int precision, ccd_ctx **out);
+int ccd_ctx_destroy(ccd_ctx *ctx);
+
+/* Sizes: elements of the pphh (= phhp) layout, bytes of device memory held. */
+int ccd_ctx_info(const ccd_ctx *ctx, int64_t *n_t2, int64_t *device_bytes);
+
+/* MBPT2 correlation energy, 1/4 sum v^2/(e_i+e_j-e_a-e_b)
+ * (replaces MBPT2Corr, src/MBPT2Corr.cpp:6-38). */
+int ccd_mbpt2(ccd_ctx *ctx, double *corr);
+
+/* T2 = 0 and the one-body intermediates = 0 (src/main.cpp:497-503). */
+int ccd_reset(ccd_ctx *ctx);
+
+/* One Jacobi step on the device (the loop body of src/main.cpp:589-670):
+ * both ladders and the hhhh quadratic term (calcT2Diff_pphh), the ring
+ * (calcT2Diff_phhp), the one-body terms (calcT2Diff_h2pp_h1,
+ * calcT2Diff_p2_p1hh), all remaps, the update / denominator / mixing
+ * (calcT2_pphh) and the correlation energy of the new amplitudes. */
+int ccd_iterate(ccd_ctx *ctx, double mixing, double *corr);
+
+/* The reference protocol (src/main.cpp:491-519) from T2 = 0: loop while
+ * |dE| > tol, dE starting at 1 and E_prev at 100, halt once more than
+ * max_iters iterations have run (the reference uses max_iters = 100).
+ * energies[k] receives the energy after iteration k for k < cap. */
+int ccd_solve(ccd_ctx *ctx, double tol, int max_iters, double mixing,
+              double *energies, int cap, int *n_iter, int *stop);
+
+/* Copy the current amplitudes out / in (pphh layout, host memory). */
+int ccd_get_t2(ccd_ctx *ctx, double *t2);
+int ccd_set_t2(ccd_ctx *ctx, const double *t2);
+
+/* Per-diagram entry points on caller amplitudes (host arrays), for parity
+ * tests of each piece against calc.cpp:
+ *   ccd_ladder:   d = calcT2Diff_pphh(t)          (calc.cpp:132-231), pphh
+ *   ccd_ring:     r = calcT2Diff_phhp(tph)        (calc.cpp:264-339), phhp
+ *   ccd_remap_ph: tph = mapToDistributed_ai_jb(t) (Chain.hpp:757-802)
+ *   ccd_onebody:  chi[n_holes], psi[n_sp-n_holes], the `value` scalars of
+ *                 calcChannelT2Diff_h2pp_h1 / _p2_p1hh (calc.cpp:81-84, 15-20)
+ *   ccd_step:     t_new = one full iteration from t_old, returns its energy */
+int ccd_ladder(ccd_ctx *ctx, const double *t, double *d);
+int ccd_ring(ccd_ctx *ctx, const double *tph, double *r);
+int ccd_remap_ph(ccd_ctx *ctx, const double *t, double *tph);
+int ccd_onebody(ccd_ctx *ctx, const double *t, double *chi, double *psi);
+int ccd_step(ccd_ctx *ctx, const double *t_old, double *t_new, double mixing,
+             double *corr);
+
+/* Run `iters` iterations from the current amplitudes and report the average
+ * device time of each phase in milliseconds, measured with HIP events on
+ * the context's stream: ms[0] ring, ms[1] ladder X (hhhh + quadratic),
+ * ms[2] ladder D (pppp), ms[3] update + remap, ms[4] reductions,
+ * ms[5] whole iteration.  ms must hold 6 doubles. */
+int ccd_time_phases(ccd_ctx *ctx, int iters, double mixing, double *ms);
+
+/* Algorithmic work of one iteration in the reference's ordered-pair algebra:
+ * flops F = sum_c (2hh^2 pp + 2pp hh^2 + 2pp^2 hh) + sum_m (2hp^2 ph + 2ph hp^2)
+ * and the pppp-ladder part of it; bytes of V^pppp read by one ladder launch. */
+int ccd_work(const ccd_ctx *ctx, double *flops, double *flops_pppp,
+             double *bytes_vpppp);
+
+/* ---------------------------------------------------------------- model */
+
+/* Pairing model, reference PairingSPBasis(0, xi, g, nPairStates, nParticles). */
+int ccd_model_pairing(int n_pair_states, int n_particles, double g, double xi,
+                      ccd_model **out);
+/* Electron gas, reference ElectronGasSPBasis(2, r_s, 1, nShells, nParticleShells). */
+int ccd_model_electron_gas(double r_s, int n_shells, int n_particle_shells,
+                           ccd_model **out);
+int ccd_model_destroy(ccd_model *m);
+/* Reference energy (SPBasis::referenceEnergy, before normal ordering). */
+int ccd_model_eref(const ccd_model *m, double *eref);
+int ccd_model_sizes(const ccd_model *m, int32_t *n_sp, int32_t *n_holes,
+                    int32_t *n_chan, int64_t *n_t2);
+/* Descriptor with on-device generation; valid until the model is destroyed. */
+int ccd_model_problem(ccd_model *m, const ccd_problem **out);
+/* Build the host FP64 V chains (the reference's loadVnn_*) and return a
+ * descriptor that carries them instead of a generator. */
+int ccd_model_problem_host_v(ccd_model *m, const ccd_problem **out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

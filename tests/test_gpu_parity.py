@@ -1,0 +1,134 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Tolerances (FP64 path): energies within 1e-10 relative of the oracle, the
+bar the reference comparison is held to; amplitude arrays within
+1e-12 * max|reference| (the only differences are summation order and FMA
+contraction).  The ph remap is index work and must be exact.
+"""
+import numpy as np
+import pytest
+
+from coupled_cluster_doubles_amd import FP64, Context, Model
+from oracle.pyoracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+E_REL = 1e-10
+A_REL = 1e-12
+
+
+def _rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+def _pair(kind, args):
+    if kind == "pairing":
+        return Model.pairing(*args), Oracle.pairing(*args)
+    return Model.electron_gas(*args), Oracle.electron_gas(*args)
+
+
+def test_pairing_solve(require_gpu):
+    m, o = _pair("pairing", (8, 8, 0.5, 1.0))
+    ctx = Context(m, FP64)
+    assert _rel(ctx.mbpt2(), o.mbpt2()) < 1e-13
+    e_g, stop_g, n_g = ctx.solve(1e-12)
+    e_o, stop_o, _ = o.solve(1e-12)
+    assert stop_g == stop_o == "converged"
+    assert n_g == len(e_o)
+    assert np.max(np.abs(e_g - e_o) / np.abs(e_o)) < E_REL
+    assert e_g[0] == pytest.approx(0.5 * o.mbpt2(), rel=1e-13)
+
+
+def test_electron_gas_n14_solve(require_gpu):
+    m, o = _pair("heg", (1.0, 19, 2))
+    ctx = Context(m, FP64)
+    assert _rel(ctx.mbpt2(), o.mbpt2()) < E_REL
+    e_g, stop_g, n_g = ctx.solve(1e-10)
+    e_o, stop_o, _ = o.solve(1e-10)
+    assert stop_g == stop_o == "converged"
+    assert abs(n_g - len(e_o)) <= 1
+    k = min(n_g, len(e_o))
+    assert np.max(np.abs(e_g[:k] - e_o[:k]) / np.abs(e_o[:k])) < E_REL
+    assert _rel(e_g[-1], e_o[-1]) < E_REL
+    t_g, t_o = ctx.get_t2(), o.t2()
+    assert np.max(np.abs(t_g - t_o)) <= 1e-9 * np.max(np.abs(t_o))
+
+
+@pytest.mark.parametrize("kind,args", [
+    ("pairing", (8, 8, 0.5, 1.0)),
+    ("heg", (1.0, 4, 2)),
+    ("heg", (1.0, 19, 2)),
+])
+def test_diagrams_on_random_amplitudes(require_gpu, kind, args):
+    m, o = _pair(kind, args)
+    ctx = Context(m, FP64)
+    rng = np.random.default_rng(3)
+    t = rng.uniform(-0.05, 0.05, m.n_t2)
+    # ladder: calcT2Diff_pphh
+    d_g, d_o = ctx.ladder(t), o.ladder(t)
+    assert np.max(np.abs(d_g - d_o)) <= A_REL * np.max(np.abs(d_o))
+    # ph remap: exact
+    tph_g, tph_o = ctx.remap_ph(t), o.scatter_ph(t)
+    assert np.array_equal(tph_g, tph_o)
+    # ring: calcT2Diff_phhp
+    r_g, r_o = ctx.ring(tph_o), o.ring(tph_o)
+    assert np.max(np.abs(r_g - r_o)) <= A_REL * np.max(np.abs(r_o))
+    # one-body intermediates
+    chi_g, psi_g = ctx.onebody(t)
+    chi_o, psi_o = o.onebody(t)
+    assert np.max(np.abs(chi_g - chi_o)) <= A_REL * np.max(np.abs(chi_o))
+    assert np.max(np.abs(psi_g - psi_o)) <= A_REL * np.max(np.abs(psi_o))
+    # a full step from antisymmetric amplitudes
+    lab = m.t2_labels()
+    key = {tuple(r): k for k, r in enumerate(lab.tolist())}
+    sw_ab = np.array([key[(b, a, i, j)] for a, b, i, j in lab.tolist()])
+    sw_ij = np.array([key[(a, b, j, i)] for a, b, i, j in lab.tolist()])
+    ta = t - t[sw_ab]
+    ta = ta - ta[sw_ij]
+    tn_g, e_g = ctx.step(ta)
+    tn_o, e_o = o.step(ta)
+    assert np.max(np.abs(tn_g - tn_o)) <= A_REL * np.max(np.abs(tn_o))
+    assert _rel(e_g, e_o) < E_REL
+
+
+def test_host_v_matches_generated_v(require_gpu):
+    m = Model.electron_gas(1.0, 6, 2)
+    c_gen = Context(m, FP64)
+    c_host = Context(m, FP64, host_v=True)
+    assert c_gen.mbpt2() == c_host.mbpt2()
+    t = np.random.default_rng(5).uniform(-0.05, 0.05, m.n_t2)
+    a, ea = c_gen.step(t)
+    b, eb = c_host.step(t)
+    assert np.array_equal(a, b) and ea == eb
+
+
+def test_headline_n54_matches_oracle_two_iterations(require_gpu):
+    m, o = _pair("heg", (1.0, 32, 4))
+    ctx = Context(m, FP64)
+    mb = ctx.mbpt2()
+    assert _rel(mb, o.mbpt2()) < E_REL
+    e0 = ctx.iterate()
+    assert _rel(e0, 0.5 * mb) < 1e-12            # zero start, mixing 1/2
+    e1 = ctx.iterate()
+    t_o = np.zeros(o.nt2)
+    t_o, eo0 = o.step(t_o)
+    t_o, eo1 = o.step(t_o)
+    assert _rel(e0, eo0) < E_REL and _rel(e1, eo1) < E_REL
+    t_g = ctx.get_t2()
+    assert np.max(np.abs(t_g - t_o)) <= A_REL * np.max(np.abs(t_o))
+
+
+def test_headline_n54_amplitudes_stay_antisymmetric(require_gpu):
+    m = Model.electron_gas(1.0, 32, 4)
+    ctx = Context(m, FP64)
+    for _ in range(4):
+        ctx.iterate()
+    t = ctx.get_t2()
+    lab = m.t2_labels()
+    # (b,a) partner of each element via a dictionary on (a,b,i,j) keys
+    n = m.n_sp
+    keys = ((lab[:, 0].astype(np.int64) * n + lab[:, 1]) * n + lab[:, 2]) * n + lab[:, 3]
+    swk = ((lab[:, 1].astype(np.int64) * n + lab[:, 0]) * n + lab[:, 2]) * n + lab[:, 3]
+    order = np.argsort(keys)
+    pos = order[np.searchsorted(keys[order], swk)]
+    assert np.max(np.abs(t + t[pos])) <= 1e-13 * np.max(np.abs(t))
