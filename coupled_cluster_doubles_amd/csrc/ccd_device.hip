@@ -63,10 +63,13 @@ extern "C" const char *ccd_last_error(void) { return ccd::g_err.c_str(); }
 
 struct DChan {          // active direct channel (pp > 0 and hh > 0)
     int64_t off_t;      // block offset in the pphh layout
-    int64_t off_pppp;   // offset of its pp x pp block of V^pppp
-    int64_t off_hhhh;   // offset of its hh x hh block of V^hhhh / X
+    int64_t off_vp;     // packed V^pppp block: npk x ldv, rows/cols = pairs a<b
+    int64_t off_tp;     // packed-row blocks (npk x hh): antisymmetrised T, D_p, V^hhpp rows
+    int64_t off_hhhh;   // hh x hh block of V^hhhh / X
     int32_t pp, hh;
     int32_t pp_off, hh_off;  // first pair in the pp / hh pair lists
+    int32_t npk, ldv;        // number of pairs a<b; leading dimension (multiple of 8)
+    int32_t pr_off, pad;     // first entry in the packed-row lists
 };
 
 struct DMod {           // active mod channel (ph > 0 and hp > 0)
@@ -136,19 +139,59 @@ __device__ double gen_tbme(const Gen &G, int p, int q, int r, int s) {
 
 // ------------------------------------------------------------------ generation kernels
 
-// V^pppp rows of one tile: [(a,b),(c,d)] = <ab||cd>.
-__global__ __launch_bounds__(kThreads) void k_gen_pppp(const DTile *__restrict__ tiles,
-                                                       const DChan *__restrict__ chans,
-                                                       const int2 *__restrict__ ppl, Gen G,
-                                                       double *__restrict__ v) {
+// Packed V^pppp rows of one tile: Vp[u][w] = <a_u b_u||c_w d_w> over pairs a<b,
+// zero in the padding columns w >= npk.
+__global__ __launch_bounds__(kThreads) void k_gen_vp(const DTile *__restrict__ tiles,
+                                                     const DChan *__restrict__ chans,
+                                                     const int2 *__restrict__ ppl,
+                                                     const int32_t *__restrict__ prow, Gen G,
+                                                     double *__restrict__ vp) {
     const DTile tl = tiles[blockIdx.x];
     const DChan ch = chans[tl.chan];
-    const int P = ch.pp;
-    const int64_t n = (int64_t)tl.nrows * P;
+    const int ldv = ch.ldv, npk = ch.npk;
+    const int64_t n = (int64_t)tl.nrows * ldv;
     for (int64_t l = threadIdx.x; l < n; l += kThreads) {
-        const int r = tl.row0 + (int)(l / P), s = (int)(l % P);
-        const int2 ab = ppl[ch.pp_off + r], cd = ppl[ch.pp_off + s];
-        v[ch.off_pppp + (int64_t)r * P + s] = gen_tbme(G, ab.x, ab.y, cd.x, cd.y);
+        const int u = tl.row0 + (int)(l / ldv), w = (int)(l % ldv);
+        double v = 0.0;
+        if (w < npk) {
+            const int2 ab = ppl[ch.pp_off + prow[ch.pr_off + u]];
+            const int2 cd = ppl[ch.pp_off + prow[ch.pr_off + w]];
+            v = gen_tbme(G, ab.x, ab.y, cd.x, cd.y);
+        }
+        vp[ch.off_vp + (int64_t)u * ldv + w] = v;
+    }
+}
+
+// Packed rows of V^hhpp^T: vtp[u][kl] = vt[(a_u,b_u)][kl] (tile = packed rows).
+__global__ __launch_bounds__(kThreads) void k_pack_vt(const DTile *__restrict__ tiles,
+                                                      const DChan *__restrict__ chans,
+                                                      const int32_t *__restrict__ prow,
+                                                      const double *__restrict__ vt,
+                                                      double *__restrict__ vtp) {
+    const DTile tl = tiles[blockIdx.x];
+    const DChan ch = chans[tl.chan];
+    const int H = ch.hh;
+    for (int l = threadIdx.x; l < tl.nrows * H; l += kThreads) {
+        const int u = tl.row0 + l / H, kl = l % H;
+        vtp[ch.off_tp + (int64_t)u * H + kl] = vt[ch.off_t + (int64_t)prow[ch.pr_off + u] * H + kl];
+    }
+}
+
+// Antisymmetrised amplitudes on the packed rows: tas[u][ij] = (t[(a,b)][ij] - t[(b,a)][ij]) / 2.
+__global__ __launch_bounds__(kThreads) void k_antisym(const DTile *__restrict__ tiles,
+                                                      const DChan *__restrict__ chans,
+                                                      const int32_t *__restrict__ prow,
+                                                      const int32_t *__restrict__ prow_sw,
+                                                      const double *__restrict__ t,
+                                                      double *__restrict__ tas) {
+    const DTile tl = tiles[blockIdx.x];
+    const DChan ch = chans[tl.chan];
+    const int H = ch.hh;
+    for (int l = threadIdx.x; l < tl.nrows * H; l += kThreads) {
+        const int u = tl.row0 + l / H, ij = l % H;
+        const double x = t[ch.off_t + (int64_t)prow[ch.pr_off + u] * H + ij];
+        const double y = t[ch.off_t + (int64_t)prow_sw[ch.pr_off + u] * H + ij];
+        tas[ch.off_tp + (int64_t)u * H + ij] = 0.5 * (x - y);
     }
 }
 
@@ -236,120 +279,146 @@ __global__ __launch_bounds__(kThreads) void k_ring(const DMod *__restrict__ mods
 }
 
 // ------------------------------------------------------------------ ladders (calc.cpp:132-203)
+//
+// With t and V antisymmetric in (a,b) and (c,d), the reference's ordered-pair
+// ladder 1/2 sum_{cd} V[ab][cd] t[cd][ij] equals sum_{c<d} V[ab][cd] t[cd][ij],
+// and D[(b,a)] = -D[(a,b)], D[(a,a)] = 0.  Both ladder kernels therefore run on
+// the packed pairs a<b: a quarter of the V^pppp bytes and flops.  The
+// contractions use v_mfma_f64_16x16x4f64: A is 16x4 (lane l: row l%16, k-slot
+// l/16), B is 4x16 (lane l: k-slot l/16, column l%16), C is 16x16 with lane l
+// holding column l%16, rows 4*(l/16)..+3.  Within a 32-wide k chunk, k-slot q
+// of step s is k = 8q + s, so each lane reads 8 consecutive A elements.
 
-constexpr int LBM = 64, LBK = 32, LBN = 64;
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef double dbl2 __attribute__((ext_vector_type(2)));
 
-// acc[4][4] += A(rows x K) * B(K x cols) for the block's 64 x 64 output tile.
-// A is row-major with leading dimension lda (a_trans = false) or stored
-// transposed, element (r,k) at A[k*lda + r] (a_trans = true).
-template <bool ATrans>
-__device__ __forceinline__ void tile_gemm(const double *__restrict__ A, int64_t lda,
-                                          const double *__restrict__ B, int ldb, int K, int rows,
-                                          int col0, int ncols, double (*As)[LBM + 1],
-                                          double (*Bs)[LBN + 1], double acc[4][4]) {
-    const int tid = threadIdx.x, rg = tid >> 4, cg = tid & 15;
-    for (int k0 = 0; k0 < K; k0 += LBK) {
-        if (!ATrans) {
-            const int r = tid >> 2, kb = (tid & 3) * 8;
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                const int k = k0 + kb + q;
-                As[kb + q][r] = (r < rows && k < K) ? A[(int64_t)r * lda + k] : 0.0;
-            }
-        } else {
-            const int kk = tid >> 3, rb = (tid & 7) * 8;
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                const int r = rb + q, k = k0 + kk;
-                As[kk][r] = (r < rows && k < K) ? A[(int64_t)k * lda + r] : 0.0;
-            }
-        }
-        {
-            const int kk = tid >> 3, cb = (tid & 7) * 8;
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                const int c = col0 + cb + q, k = k0 + kk;
-                Bs[kk][cb + q] = (k < K && c < ncols) ? B[(int64_t)k * ldb + c] : 0.0;
-            }
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int kk = 0; kk < LBK; kk++) {
-            double a[4], b[4];
-#pragma unroll
-            for (int i = 0; i < 4; i++) a[i] = As[kk][rg * 4 + i];
-#pragma unroll
-            for (int j = 0; j < 4; j++) b[j] = Bs[kk][cg * 4 + j];
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-#pragma unroll
-                for (int j = 0; j < 4; j++) acc[i][j] += a[i] * b[j];
-        }
-        __syncthreads();
+constexpr int KC = 32;                 // k rows per LDS stage
+constexpr int NTMAX = 5;               // 16-column tiles: hh <= 80
+constexpr int BSTR = NTMAX * 16 + 1;   // LDS row stride (doubles)
+constexpr int LROWS = 64;              // packed rows per ladder block (16 per wave)
+
+// Bs[kk][c] = B[(k0+kk)*ldb + c] for c < nt*16, zero where k >= K or c >= ncols.
+__device__ __forceinline__ void stage_b(double (*Bs)[BSTR], const double *__restrict__ B, int ldb,
+                                        int k0, int K, int ncols, int nt) {
+    const int w = nt * 16;
+    for (int idx = threadIdx.x; idx < KC * w; idx += kThreads) {
+        const int kk = idx / w, c = idx - kk * w, k = k0 + kk;
+        Bs[kk][c] = (k < K && c < ncols) ? B[(int64_t)k * ldb + c] : 0.0;
     }
 }
 
-// X = V^hhhh + 1/2 V^hhpp T   (hh x hh per channel; tile = 64 rows of kl).
-__global__ __launch_bounds__(kThreads) void k_ladder_x(const DTile *__restrict__ tiles,
+__device__ __forceinline__ void mfma_chunk(const double (&a)[8], double (*Bs)[BSTR], int kq, int m, int nt,
+                                           dbl4 (&acc)[NTMAX]) {
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+#pragma unroll
+        for (int n = 0; n < NTMAX; n++) {
+            if (n < nt) acc[n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], Bs[kq * 8 + s][n * 16 + m], acc[n], 0, 0, 0);
+        }
+    }
+}
+
+// X = V^hhhh + sum_{c<d} V^hhpp[kl][cd] tas[cd][ij]; one wave per 16x16 tile of X.
+__global__ __launch_bounds__(kThreads) void k_ladder_x(const DTile *__restrict__ tiles, int ntiles,
                                                        const DChan *__restrict__ chans,
-                                                       const double *__restrict__ t,
-                                                       const double *__restrict__ vt,
+                                                       const double *__restrict__ vtp,
+                                                       const double *__restrict__ tas,
                                                        const double *__restrict__ vh,
                                                        double *__restrict__ x) {
-    __shared__ double As[LBK][LBM + 1];
-    __shared__ double Bs[LBK][LBN + 1];
-    const DTile tl = tiles[blockIdx.x];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
+    const int tix = blockIdx.x * (kThreads / 64) + wave;
+    if (tix >= ntiles) return;
+    const DTile tl = tiles[tix];
     const DChan ch = chans[tl.chan];
-    const int H = ch.hh, P = ch.pp;
-    const int rg = threadIdx.x >> 4, cg = threadIdx.x & 15;
-    for (int col0 = 0; col0 < H; col0 += LBN) {
-        double acc[4][4] = {};
-        // A(kl, cd) = V^hhpp[kl][cd] = vt[cd][kl]  (transposed, lda = H)
-        tile_gemm<true>(vt + ch.off_t + tl.row0, H, t + ch.off_t, H, P, tl.nrows, col0, H, As, Bs, acc);
+    const int H = ch.hh, npk = ch.npk, kl0 = tl.row0, ij0 = tl.pad;
+    const bool aok = kl0 + m < H, bok = ij0 + m < H;
+    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < npk; k0 += KC) {
+        double a[8], b[8];
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int row = rg * 4 + i;
-            if (row >= tl.nrows) continue;
+        for (int s = 0; s < 8; s++) {
+            const int w = k0 + kq * 8 + s;
+            const bool ok = w < npk;
+            a[s] = (ok && aok) ? vtp[ch.off_tp + (int64_t)w * H + kl0 + m] : 0.0;
+            b[s] = (ok && bok) ? tas[ch.off_tp + (int64_t)w * H + ij0 + m] : 0.0;
+        }
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int col = col0 + cg * 4 + j;
-                if (col >= H) continue;
-                const int64_t o = ch.off_hhhh + (int64_t)(tl.row0 + row) * H + col;
-                x[o] = 0.5 * acc[i][j] + vh[o];
-            }
+        for (int s = 0; s < 8; s++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[s], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int row = kl0 + kq * 4 + q, col = ij0 + m;
+        if (row < H && col < H) {
+            const int64_t o = ch.off_hhhh + (int64_t)row * H + col;
+            x[o] = vh[o] + acc[q];
         }
     }
 }
 
-// D = 1/2 T X + 1/2 V^pppp T   (pp x hh per channel; tile = 64 rows of ab).
-__global__ __launch_bounds__(kThreads) void k_ladder_d(const DTile *__restrict__ tiles,
+// D_p[u][ij] = 1/2 sum_kl tas[u][kl] X[kl][ij] + sum_w Vp[u][w] tas[w][ij]
+// for 64 packed rows u of one channel per block (16 per wave), all hh columns.
+__global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__ tiles,
                                                        const DChan *__restrict__ chans,
-                                                       const double *__restrict__ t,
-                                                       const double *__restrict__ x,
                                                        const double *__restrict__ vp,
-                                                       double *__restrict__ d) {
-    __shared__ double As[LBK][LBM + 1];
-    __shared__ double Bs[LBK][LBN + 1];
+                                                       const double *__restrict__ tas,
+                                                       const double *__restrict__ x,
+                                                       double *__restrict__ dp) {
+    __shared__ double Bs[KC][BSTR];
     const DTile tl = tiles[blockIdx.x];
     const DChan ch = chans[tl.chan];
-    const int H = ch.hh, P = ch.pp;
-    const int rg = threadIdx.x >> 4, cg = threadIdx.x & 15;
-    for (int col0 = 0; col0 < H; col0 += LBN) {
-        double acc1[4][4] = {}, acc2[4][4] = {};
-        tile_gemm<false>(t + ch.off_t + (int64_t)tl.row0 * H, H, x + ch.off_hhhh, H, H, tl.nrows,
-                         col0, H, As, Bs, acc1);
-        tile_gemm<false>(vp + ch.off_pppp + (int64_t)tl.row0 * P, P, t + ch.off_t, H, P, tl.nrows,
-                         col0, H, As, Bs, acc2);
+    const int H = ch.hh, npk = ch.npk, ldv = ch.ldv;
+    const int nt = (H + 15) / 16;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
+    const int lrow = wave * 16 + m;
+    const bool rok = lrow < tl.nrows;
+    const int64_t u = tl.row0 + lrow;
+    dbl4 acc[NTMAX];
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int row = rg * 4 + i;
-            if (row >= tl.nrows) continue;
+    for (int n = 0; n < NTMAX; n++) acc[n] = dbl4{0.0, 0.0, 0.0, 0.0};
+
+    // 1/2 T X  (K = hh, ordered kl)
+    const double *at = tas + ch.off_tp + u * H;
+    for (int k0 = 0; k0 < H; k0 += KC) {
+        __syncthreads();
+        stage_b(Bs, x + ch.off_hhhh, H, k0, H, H, nt);
+        __syncthreads();
+        double a[8];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int col = col0 + cg * 4 + j;
-                if (col >= H) continue;
-                d[ch.off_t + (int64_t)(tl.row0 + row) * H + col] = 0.5 * acc1[i][j] + 0.5 * acc2[i][j];
+        for (int s = 0; s < 8; s++) {
+            const int k = k0 + kq * 8 + s;
+            a[s] = (rok && k < H) ? 0.5 * at[k] : 0.0;
+        }
+        mfma_chunk(a, Bs, kq, m, nt, acc);
+    }
+    // V^pppp t  (K = npk packed pairs; rows padded with zeros to ldv)
+    const double *av = vp + ch.off_vp + u * ldv;
+    for (int k0 = 0; k0 < npk; k0 += KC) {
+        __syncthreads();
+        stage_b(Bs, tas + ch.off_tp, H, k0, npk, H, nt);
+        __syncthreads();
+        double a[8];
+        const int kb = k0 + kq * 8;
+        if (rok && kb < ldv) {       // ldv and kb are multiples of 8: whole 64-byte group in range
+            const dbl2 *p2 = reinterpret_cast<const dbl2 *>(av + kb);
+#pragma unroll
+            for (int h = 0; h < 4; h++) {
+                const dbl2 v = p2[h];
+                a[2 * h] = v.x;
+                a[2 * h + 1] = v.y;
             }
+        } else {
+#pragma unroll
+            for (int s = 0; s < 8; s++) a[s] = 0.0;
+        }
+        mfma_chunk(a, Bs, kq, m, nt, acc);
+    }
+#pragma unroll
+    for (int n = 0; n < NTMAX; n++) {
+        if (n >= nt) continue;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int row = wave * 16 + kq * 4 + q, col = n * 16 + m;
+            if (row < tl.nrows && col < H) dp[ch.off_tp + (int64_t)(tl.row0 + row) * H + col] = acc[n][q];
         }
     }
 }
@@ -361,6 +430,7 @@ struct UpdArgs {
     const DChan *chans;
     const int2 *ppl, *hhl;
     const int32_t *pp_swap, *hh_swap;  // index of (b,a) / (j,i) within the channel
+    const int32_t *pkc;                // per pp pair: -1 if a==b, else (packed row << 1) | (a > b)
     const int32_t *rowbase;            // [(a-nh)*nh + i] -> phhp offset of row (a,i)
     const int32_t *colidx;             // [j*np + (b-nh)]  -> column of (j,b)
     const double *eps, *chi, *psi;
@@ -393,7 +463,13 @@ __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
         const double t_ba = a.t_old[ch.off_t + (int64_t)a.pp_swap[ch.pp_off + AB] * H + IJ];
         const double v = a.vt[e];
         double value = v;
-        value += a.d[e];
+        const int code = a.pkc[ch.pp_off + AB];
+        double dv = 0.0;
+        if (code >= 0) {
+            dv = a.d[ch.off_tp + (int64_t)(code >> 1) * H + IJ];
+            if (code & 1) dv = -dv;
+        }
+        value += dv;
         value += a.r[g1];
         value -= a.r[g2];
         value -= a.r[g3];
@@ -480,6 +556,23 @@ __global__ __launch_bounds__(kThreads) void k_remap_ph(UpdArgs a) {
     }
 }
 
+// D in the ordered pphh layout from D_p (for the ladder entry point).
+__global__ __launch_bounds__(kThreads) void k_expand_d(UpdArgs a) {
+    const DTile tl = a.tiles[blockIdx.x];
+    const DChan ch = a.chans[tl.chan];
+    const int H = ch.hh;
+    for (int l = threadIdx.x; l < tl.nrows * H; l += kThreads) {
+        const int AB = tl.row0 + l / H, IJ = l % H;
+        const int code = a.pkc[ch.pp_off + AB];
+        double dv = 0.0;
+        if (code >= 0) {
+            dv = a.d[ch.off_tp + (int64_t)(code >> 1) * H + IJ];
+            if (code & 1) dv = -dv;
+        }
+        a.t_new[ch.off_t + (int64_t)AB * H + IJ] = dv;
+    }
+}
+
 // W = V o T (for the one-body entry point).
 __global__ __launch_bounds__(kThreads) void k_hadamard(const double *__restrict__ vt,
                                                        const double *__restrict__ t,
@@ -508,6 +601,7 @@ struct ccd_ctx {
     DTile *d_ltiles = nullptr, *d_utiles = nullptr, *d_xtiles = nullptr;
     int2 *d_ppl = nullptr, *d_hhl = nullptr, *d_phl = nullptr, *d_hpl = nullptr;
     int32_t *d_pp_swap = nullptr, *d_hh_swap = nullptr, *d_rowbase = nullptr, *d_colidx = nullptr;
+    int32_t *d_prow = nullptr, *d_prow_sw = nullptr, *d_pkc = nullptr;
     int32_t *d_seg_ptr = nullptr;
     DSeg *d_segs = nullptr;
     double *d_eps = nullptr, *d_onebody = nullptr;  // chi [nh] then psi [np]
@@ -515,11 +609,12 @@ struct ccd_ctx {
     int cur = 0;
     double *d_d = nullptr, *d_tph = nullptr, *d_r = nullptr, *d_w = nullptr, *d_x = nullptr;
     double *d_vt = nullptr, *d_vh = nullptr, *d_vp = nullptr, *d_va = nullptr, *d_vb = nullptr;
+    double *d_vtp = nullptr, *d_tas = nullptr;
     double *d_partials = nullptr, *d_scalar = nullptr;
     double *h_scalar = nullptr;  // pinned
     int32_t *d_qn = nullptr;
     double *d_table = nullptr;
-    int64_t n_pppp = 0, n_hhhh = 0, n_b = 0;
+    int64_t n_vp = 0, n_tp = 0, n_hhhh = 0, n_b = 0, n_vp_alg = 0;
     size_t ring_lds_max = 0;
     double flops = 0, flops_pppp = 0;
     std::vector<void *> allocs;
@@ -564,7 +659,7 @@ int check_launch(const char *what) {
 UpdArgs upd_args(ccd_ctx *c, const double *t_old, double *t_new, double mixing) {
     UpdArgs a{};
     a.tiles = c->d_utiles; a.chans = c->d_chans; a.ppl = c->d_ppl; a.hhl = c->d_hhl;
-    a.pp_swap = c->d_pp_swap; a.hh_swap = c->d_hh_swap; a.rowbase = c->d_rowbase;
+    a.pp_swap = c->d_pp_swap; a.hh_swap = c->d_hh_swap; a.pkc = c->d_pkc; a.rowbase = c->d_rowbase;
     a.colidx = c->d_colidx; a.eps = c->d_eps; a.chi = c->d_onebody; a.psi = c->d_onebody + c->nh;
     a.vt = c->d_vt; a.d = c->d_d; a.r = c->d_r; a.t_old = t_old; a.t_new = t_new;
     a.tph = c->d_tph; a.w = c->d_w; a.partials = c->d_partials; a.mixing = mixing;
@@ -581,17 +676,23 @@ int launch_ring(ccd_ctx *c) {
 }
 
 int launch_ladder_x(ccd_ctx *c, const double *t) {
-    if (c->xtiles.empty()) return CCD_OK;
-    hipLaunchKernelGGL(k_ladder_x, dim3((unsigned)c->xtiles.size()), dim3(kThreads), 0, c->stream,
-                       c->d_xtiles, c->d_chans, t, c->d_vt, c->d_vh, c->d_x);
+    if (c->ltiles.empty()) return CCD_OK;
+    hipLaunchKernelGGL(k_antisym, dim3((unsigned)c->ltiles.size()), dim3(kThreads), 0, c->stream, c->d_ltiles,
+                       c->d_chans, c->d_prow, c->d_prow_sw, t, c->d_tas);
+    int rc = check_launch("k_antisym");
+    if (rc || c->xtiles.empty()) return rc;
+    const unsigned nb = (unsigned)((c->xtiles.size() + 3) / 4);
+    hipLaunchKernelGGL(k_ladder_x, dim3(nb), dim3(kThreads), 0, c->stream, c->d_xtiles, (int)c->xtiles.size(),
+                       c->d_chans, c->d_vtp, c->d_tas, c->d_vh, c->d_x);
     return check_launch("k_ladder_x");
 }
 
 int launch_ladder_d(ccd_ctx *c, const double *t) {
+    (void)t;
     if (c->ltiles.empty()) return CCD_OK;
-    hipLaunchKernelGGL(k_ladder_d, dim3((unsigned)c->ltiles.size()), dim3(kThreads), 0, c->stream,
-                       c->d_ltiles, c->d_chans, t, c->d_x, c->d_vp, c->d_d);
-    return check_launch("k_ladder_d");
+    hipLaunchKernelGGL(k_ladder_v, dim3((unsigned)c->ltiles.size()), dim3(kThreads), 0, c->stream,
+                       c->d_ltiles, c->d_chans, c->d_vp, c->d_tas, c->d_x, c->d_d);
+    return check_launch("k_ladder_v");
 }
 
 int launch_update(ccd_ctx *c, const double *t_old, double *t_new, double mixing) {
@@ -691,20 +792,14 @@ int build(ccd_ctx *c, const ccd_problem *p) {
     for (int64_t k = 0; k < nhpp; k++) hpl[k] = make_int2(p->hp_pairs[2 * k], p->hp_pairs[2 * k + 1]);
 
     // direct channels
-    std::vector<int32_t> chan_of_pp(npp, -1), chan_of_hh(nhh, -1);
-    std::vector<int32_t> pp_swap(npp, -1), hh_swap(nhh, -1);
-    int64_t off_t = 0, off_pppp = 0, off_hhhh = 0, ipp = 0, ihh = 0;
-    c->flops = 0; c->flops_pppp = 0;
+    std::vector<int32_t> pp_swap(npp, -1), hh_swap(nhh, -1), pkc(npp, -1);
+    std::vector<int32_t> prow, prow_sw;
+    int64_t off_t = 0, off_vp = 0, off_tp = 0, off_hhhh = 0, ipp = 0, ihh = 0;
+    c->flops = 0; c->flops_pppp = 0; c->n_vp_alg = 0;
     std::unordered_map<int64_t, int32_t> pos;
     for (int k = 0; k < p->n_chan; k++) {
         const int P = p->pp_dim[k], H = p->hh_dim[k];
         if (P > 0 && H > 0) {
-            DChan ch{off_t, off_pppp, off_hhhh, P, H, (int32_t)ipp, (int32_t)ihh};
-            c->chans.push_back(ch);
-            off_pppp += (int64_t)P * P;
-            off_hhhh += (int64_t)H * H;
-            c->flops += 2.0 * H * H * P + 2.0 * P * H * H + 2.0 * P * P * H;
-            c->flops_pppp += 2.0 * P * P * H;
             // swaps within the channel
             pos.clear();
             for (int q = 0; q < P; q++) pos[(int64_t)ppl[ipp + q].x * nsp + ppl[ipp + q].y] = q;
@@ -720,13 +815,39 @@ int build(ccd_ctx *c, const ccd_problem *p) {
                 if (it == pos.end()) return ccd::fail(CCD_E_INVALID, "hh channel %d lacks a swapped pair", k);
                 hh_swap[ihh + q] = it->second;
             }
+            // packed rows: pairs a<b in list order
+            const int pr_off = (int)prow.size();
+            std::vector<int32_t> packed_of(P, -1);
+            for (int q = 0; q < P; q++)
+                if (ppl[ipp + q].x < ppl[ipp + q].y) {
+                    packed_of[q] = (int32_t)(prow.size() - pr_off);
+                    prow.push_back(q);
+                    prow_sw.push_back(pp_swap[ipp + q]);
+                }
+            const int npk = (int)prow.size() - pr_off;
+            for (int q = 0; q < P; q++) {
+                const int a = ppl[ipp + q].x, b = ppl[ipp + q].y;
+                if (a < b) pkc[ipp + q] = packed_of[q] << 1;
+                else if (a > b) pkc[ipp + q] = (packed_of[pp_swap[ipp + q]] << 1) | 1;
+            }
+            const int ldv = (npk + 7) / 8 * 8;
+            DChan ch{off_t, off_vp, off_tp, off_hhhh, P, H, (int32_t)ipp, (int32_t)ihh, npk, ldv, pr_off, 0};
+            c->chans.push_back(ch);
+            off_vp += (int64_t)npk * ldv;
+            off_tp += (int64_t)npk * H;
+            off_hhhh += (int64_t)H * H;
+            c->n_vp_alg += (int64_t)npk * npk;
+            c->flops += 2.0 * H * H * P + 2.0 * P * H * H + 2.0 * P * P * H;
+            c->flops_pppp += 2.0 * P * P * H;
         }
         off_t += (int64_t)P * H;
         ipp += P; ihh += H;
     }
     c->nt2 = off_t;
-    c->n_pppp = off_pppp; c->n_hhhh = off_hhhh;
+    c->n_vp = off_vp; c->n_tp = off_tp; c->n_hhhh = off_hhhh;
     if (c->nt2 >= (int64_t)INT32_MAX) return ccd::fail(CCD_E_INVALID, "T2 has %lld elements (limit 2^31)", (long long)c->nt2);
+    for (const DChan &ch : c->chans)
+        if (ch.hh > NTMAX * 16) return ccd::fail(CCD_E_INVALID, "channel with hh=%d exceeds %d", ch.hh, NTMAX * 16);
 
     // mod channels and position tables
     std::vector<int32_t> rowbase((size_t)np * nh, -1), colidx((size_t)nh * np, -1);
@@ -783,11 +904,11 @@ int build(ccd_ctx *c, const ccd_problem *p) {
         const DChan &ch = c->chans[k];
         const int rows = std::max(1, kThreads / ch.hh);
         for (int r = 0; r < ch.pp; r += rows) c->utiles.push_back(DTile{k, r, std::min(rows, ch.pp - r), 0});
-        for (int r = 0; r < ch.pp; r += LBM) c->ltiles.push_back(DTile{k, r, std::min(LBM, ch.pp - r), 0});
+        for (int r = 0; r < ch.npk; r += LROWS) c->ltiles.push_back(DTile{k, r, std::min(LROWS, ch.npk - r), 0});
     }
     for (int k = 0; k < (int)c->chans.size(); k++)
-        for (int r = 0; r < c->chans[k].hh; r += LBM)
-            c->xtiles.push_back(DTile{k, r, std::min(LBM, c->chans[k].hh - r), 0});
+        for (int r = 0; r < c->chans[k].hh; r += 16)
+            for (int q = 0; q < c->chans[k].hh; q += 16) c->xtiles.push_back(DTile{k, r, 16, q});
     // one-body segments: chi_i over hh columns with first index i, psi_a over pp rows with first index a
     std::vector<std::vector<DSeg>> sg((size_t)nh + np);
     for (const DChan &ch : c->chans) {
@@ -811,6 +932,8 @@ int build(ccd_ctx *c, const ccd_problem *p) {
         (rc = upload(c, &c->d_ppl, ppl)) || (rc = upload(c, &c->d_hhl, hhl)) ||
         (rc = upload(c, &c->d_phl, phl)) || (rc = upload(c, &c->d_hpl, hpl)) ||
         (rc = upload(c, &c->d_pp_swap, pp_swap)) || (rc = upload(c, &c->d_hh_swap, hh_swap)) ||
+        (rc = upload(c, &c->d_pkc, pkc)) || (rc = upload(c, &c->d_prow, prow)) ||
+        (rc = upload(c, &c->d_prow_sw, prow_sw)) ||
         (rc = upload(c, &c->d_rowbase, rowbase)) || (rc = upload(c, &c->d_colidx, colidx)) ||
         (rc = upload(c, &c->d_seg_ptr, seg_ptr)) || (rc = upload(c, &c->d_segs, segs)))
         return rc;
@@ -819,10 +942,11 @@ int build(ccd_ctx *c, const ccd_problem *p) {
     if ((rc = dalloc(c, &c->d_onebody, (size_t)nsp))) return rc;
     for (int k = 0; k < 2; k++)
         if ((rc = dalloc(c, &c->d_t[k], (size_t)c->nt2))) return rc;
-    if ((rc = dalloc(c, &c->d_d, (size_t)c->nt2)) || (rc = dalloc(c, &c->d_tph, (size_t)c->nt2)) ||
+    if ((rc = dalloc(c, &c->d_d, (size_t)c->n_tp)) || (rc = dalloc(c, &c->d_tph, (size_t)c->nt2)) ||
         (rc = dalloc(c, &c->d_r, (size_t)c->nt2)) || (rc = dalloc(c, &c->d_w, (size_t)c->nt2)) ||
         (rc = dalloc(c, &c->d_x, (size_t)c->n_hhhh)) || (rc = dalloc(c, &c->d_vt, (size_t)c->nt2)) ||
-        (rc = dalloc(c, &c->d_vh, (size_t)c->n_hhhh)) || (rc = dalloc(c, &c->d_vp, (size_t)c->n_pppp)) ||
+        (rc = dalloc(c, &c->d_vh, (size_t)c->n_hhhh)) || (rc = dalloc(c, &c->d_vp, (size_t)c->n_vp)) ||
+        (rc = dalloc(c, &c->d_vtp, (size_t)c->n_tp)) || (rc = dalloc(c, &c->d_tas, (size_t)c->n_tp)) ||
         (rc = dalloc(c, &c->d_va, (size_t)c->nt2)) || (rc = dalloc(c, &c->d_vb, (size_t)c->n_b)) ||
         (rc = dalloc(c, &c->d_partials, std::max(c->utiles.size(), (size_t)1))) ||
         (rc = dalloc(c, &c->d_scalar, 4)))
@@ -846,8 +970,8 @@ int build(ccd_ctx *c, const ccd_problem *p) {
             G.table = c->d_table;
         }
         if (!c->ltiles.empty())
-            hipLaunchKernelGGL(k_gen_pppp, dim3((unsigned)c->ltiles.size()), dim3(kThreads), 0, c->stream,
-                               c->d_ltiles, c->d_chans, c->d_ppl, G, c->d_vp);
+            hipLaunchKernelGGL(k_gen_vp, dim3((unsigned)c->ltiles.size()), dim3(kThreads), 0, c->stream,
+                               c->d_ltiles, c->d_chans, c->d_ppl, c->d_prow, G, c->d_vp);
         if (!c->chans.empty())
             hipLaunchKernelGGL(k_gen_direct_small, dim3((unsigned)c->chans.size()), dim3(kThreads), 0,
                                c->stream, c->d_chans, c->d_ppl, c->d_hhl, G, c->d_vt, c->d_vh);
@@ -864,11 +988,27 @@ int build(ccd_ctx *c, const ccd_problem *p) {
         }
         for (int k = 0; k < p->n_mod; k++) o_b[k + 1] = o_b[k] + (int64_t)p->hp_dim[k] * p->hp_dim[k];
         int ci = 0;
+        std::vector<double> blk;
         for (int k = 0; k < p->n_chan; k++) {
             if (!(p->pp_dim[k] > 0 && p->hh_dim[k] > 0)) continue;
             const DChan &ch = c->chans[ci++];
-            HIP_TRY(hipMemcpy(c->d_vp + ch.off_pppp, p->v_pppp + o_pppp[k],
-                              sizeof(double) * ch.pp * (size_t)ch.pp, hipMemcpyHostToDevice));
+            // The packed ladder needs V^pppp antisymmetric in both pairs (exactly,
+            // as calcTBME produces it); check, then keep the a<b, c<d block.
+            const double *V = p->v_pppp + o_pppp[k];
+            const int P = ch.pp;
+            for (int r = 0; r < P; r++) {
+                const int rs = pp_swap[ch.pp_off + r];
+                for (int q = 0; q < P; q++) {
+                    const double v = V[(size_t)r * P + q];
+                    if (V[(size_t)rs * P + q] != -v || V[(size_t)r * P + pp_swap[ch.pp_off + q]] != -v)
+                        return ccd::fail(CCD_E_INVALID, "V^pppp of channel %d is not antisymmetric", k);
+                }
+            }
+            blk.assign((size_t)ch.npk * ch.ldv, 0.0);
+            for (int u = 0; u < ch.npk; u++)
+                for (int w = 0; w < ch.npk; w++)
+                    blk[(size_t)u * ch.ldv + w] = V[(size_t)prow[ch.pr_off + u] * P + prow[ch.pr_off + w]];
+            HIP_TRY(hipMemcpy(c->d_vp + ch.off_vp, blk.data(), sizeof(double) * blk.size(), hipMemcpyHostToDevice));
             HIP_TRY(hipMemcpy(c->d_vh + ch.off_hhhh, p->v_hhhh + o_hhhh[k],
                               sizeof(double) * ch.hh * (size_t)ch.hh, hipMemcpyHostToDevice));
         }
@@ -886,6 +1026,11 @@ int build(ccd_ctx *c, const ccd_problem *p) {
             HIP_TRY(hipMemcpy(c->d_vb + m.off_b, p->v_modB + o_b[k], sizeof(double) * m.hp * (size_t)m.hp,
                               hipMemcpyHostToDevice));
         }
+    }
+    if (!c->ltiles.empty()) {
+        hipLaunchKernelGGL(k_pack_vt, dim3((unsigned)c->ltiles.size()), dim3(kThreads), 0, c->stream, c->d_ltiles,
+                           c->d_chans, c->d_prow, c->d_vt, c->d_vtp);
+        if ((rc = check_launch("k_pack_vt"))) return rc;
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
     return CCD_OK;
@@ -975,7 +1120,7 @@ int ccd_work(const ccd_ctx *ctx, double *flops, double *flops_pppp, double *byte
     if (!ctx) return ccd::fail(CCD_E_INVALID, "null context");
     if (flops) *flops = ctx->flops;
     if (flops_pppp) *flops_pppp = ctx->flops_pppp;
-    if (bytes_vpppp) *bytes_vpppp = 8.0 * (double)ctx->n_pppp;
+    if (bytes_vpppp) *bytes_vpppp = 8.0 * (double)ctx->n_vp_alg;
     return CCD_OK;
 }
 
@@ -1066,7 +1211,11 @@ int ccd_ladder(ccd_ctx *c, const double *t, double *d) {
     HIP_TRY(hipMemcpyAsync(dt, t, sizeof(double) * c->nt2, hipMemcpyHostToDevice, c->stream));
     int rc;
     if ((rc = launch_ladder_x(c, dt)) || (rc = launch_ladder_d(c, dt))) return rc;
-    HIP_TRY(hipMemcpyAsync(d, c->d_d, sizeof(double) * c->nt2, hipMemcpyDeviceToHost, c->stream));
+    // expand D_p to the ordered pphh layout through the sign codes
+    UpdArgs a = upd_args(c, nullptr, c->d_w, 0.0);
+    hipLaunchKernelGGL(k_expand_d, dim3((unsigned)c->utiles.size()), dim3(kThreads), 0, c->stream, a);
+    if ((rc = check_launch("k_expand_d"))) return rc;
+    HIP_TRY(hipMemcpyAsync(d, c->d_w, sizeof(double) * c->nt2, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return CCD_OK;
 }

@@ -135,7 +135,12 @@ int ccd_get_t2(ccd_ctx *ctx, double *t2);
 int ccd_set_t2(ccd_ctx *ctx, const double *t2);
 
 /* Per-diagram entry points on caller amplitudes (host arrays), for parity
- * tests of each piece against calc.cpp:
+ * tests of each piece against calc.cpp.  They reuse the context's device
+ * buffers, so the iteration state is undefined afterwards: call ccd_set_t2
+ * or ccd_reset before iterating again.  ccd_ladder computes the ladder of the
+ * part of t antisymmetric in (a,b) (V^pppp is stored for pairs a<b only);
+ * the reference's amplitudes are always antisymmetric, so on them it is the
+ * reference's calcT2Diff_pphh.
  *   ccd_ladder:   d = calcT2Diff_pphh(t)          (calc.cpp:132-231), pphh
  *   ccd_ring:     r = calcT2Diff_phhp(tph)        (calc.cpp:264-339), phhp
  *   ccd_remap_ph: tph = mapToDistributed_ai_jb(t) (Chain.hpp:757-802)
