@@ -285,8 +285,8 @@ __global__ __launch_bounds__(kThreads) void k_ring(const DMod *__restrict__ mods
 // and D[(b,a)] = -D[(a,b)], D[(a,a)] = 0.  Both ladder kernels therefore run on
 // the packed pairs a<b: a quarter of the V^pppp bytes and flops.  The
 // contractions use v_mfma_f64_16x16x4f64: A is 16x4 (lane l: row l%16, k-slot
-// l/16), B is 4x16 (lane l: k-slot l/16, column l%16), C is 16x16 with lane l
-// holding column l%16, rows 4*(l/16)..+3.  Within a 32-wide k chunk, k-slot q
+// l/16), B is 4x16 (lane l: k-slot l/16, column l%16), C is 16x16 with
+// register q of lane l holding row 4q + l/16, column l%16.  Within a 32-wide k chunk, k-slot q
 // of step s is k = 8q + s, so each lane reads 8 consecutive A elements.
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
@@ -347,7 +347,7 @@ __global__ __launch_bounds__(kThreads) void k_ladder_x(const DTile *__restrict__
     }
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        const int row = kl0 + kq * 4 + q, col = ij0 + m;
+        const int row = kl0 + q * 4 + kq, col = ij0 + m;
         if (row < H && col < H) {
             const int64_t o = ch.off_hhhh + (int64_t)row * H + col;
             x[o] = vh[o] + acc[q];
@@ -417,7 +417,7 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
         if (n >= nt) continue;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const int row = wave * 16 + kq * 4 + q, col = n * 16 + m;
+            const int row = wave * 16 + q * 4 + kq, col = n * 16 + m;
             if (row < tl.nrows && col < H) dp[ch.off_tp + (int64_t)(tl.row0 + row) * H + col] = acc[n][q];
         }
     }
