@@ -205,4 +205,4 @@ class Context:
     def work(self):
         f, fp, b = C.c_double(), C.c_double(), C.c_double()
         check(lib().ccd_work(self.h, C.byref(f), C.byref(fp), C.byref(b)), "ccd_work")
-        return {"flops": f.value, "flops_pppp": fp.value, "bytes_vpppp": b.value}
+        return {"flops": f.value, "flops_pppp": fp.value, "ladder_bytes": b.value}

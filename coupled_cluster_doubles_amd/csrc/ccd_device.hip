@@ -1116,11 +1116,18 @@ int ccd_ctx_info(const ccd_ctx *ctx, int64_t *n_t2, int64_t *device_bytes) {
     return CCD_OK;
 }
 
-int ccd_work(const ccd_ctx *ctx, double *flops, double *flops_pppp, double *bytes_vpppp) {
+int ccd_work(const ccd_ctx *ctx, double *flops, double *flops_pppp, double *ladder_bytes) {
     if (!ctx) return ccd::fail(CCD_E_INVALID, "null context");
     if (flops) *flops = ctx->flops;
     if (flops_pppp) *flops_pppp = ctx->flops_pppp;
-    if (bytes_vpppp) *bytes_vpppp = 8.0 * (double)ctx->n_vp_alg;
+    if (ladder_bytes) {
+        // one k_ladder_v launch reads the packed V^pppp, the antisymmetrised
+        // amplitudes and X once and writes D_p once
+        double b = 0.0;
+        for (const DChan &ch : ctx->chans)
+            b += (double)ch.npk * ch.npk + 2.0 * ch.npk * ch.hh + (double)ch.hh * ch.hh;
+        *ladder_bytes = (ctx->precision == CCD_PREC_FP64 ? 8.0 : 4.0) * b;
+    }
     return CCD_OK;
 }
 
