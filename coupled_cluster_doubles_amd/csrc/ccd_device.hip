@@ -1,23 +1,24 @@
-// MI355X (gfx950) implementation of the CCD amplitude iteration.
+// MI355X (gfx950) implementation of the CCD amplitude iteration: device
+// context, setup tables and the C ABI of include/ccd_amd.h.
 //
 // Replaces, for one GPU, the hot path of justinlietz/coupled_cluster_doubles:
-//   calcT2Diff_pphh   src/calc.cpp:132-231   -> k_ladder_x + k_ladder_d
+//   calcT2Diff_pphh   src/calc.cpp:132-231   -> k_ladder_x + k_ladder_v
 //   calcT2Diff_phhp   src/calc.cpp:264-339   -> k_ring
-//   calcT2Diff_h2pp_h1, calcT2Diff_p2_p1hh   -> k_onebody (chi, psi) + k_update
+//   calcT2Diff_h2pp_h1, calcT2Diff_p2_p1hh   -> chi, psi from k_reduce, applied in k_update
 //   8 x mapFromDistributed, 5 x mapToDistributed (src/Chain.hpp:710-802)
-//                                            -> index arithmetic inside k_update
-//   calcT2_pphh       src/calc.cpp:368-459   -> k_update + k_sum_partials
+//                                            -> k_transpose (2 per iteration) + in-row swaps
+//   calcT2_pphh       src/calc.cpp:368-459   -> k_update + k_reduce + k_energy
 //   MBPT2Corr         src/MBPT2Corr.cpp:6-38 -> k_mbpt2 + k_sum_partials
 //   loop              src/main.cpp:491-682   -> ccd_solve
 //
-// HBM layout: every amplitude-shaped array (T ping-pong, D, W) is the
-// reference pphh Chain layout; T^ph and R the phhp layout; V^hhpp is kept
-// transposed (pphh layout, "vt") so the update reads it with the same index
-// as T.  V^pppp, V^hhhh, V^B are stored for the active channels only.
-// The one-body intermediates are diagonal (chi_i, psi_a), so the reference's
-// 3-body chains and their four remaps reduce to segmented sums of
-// W = V o T and two in-channel swaps; the four ring gathers and the T^ph
-// scatter are computed from two small position tables.
+// HBM layout.  Amplitude-shaped arrays T (ping-pong), G (the ring output in
+// the pphh layout) and V^hhpp (as "vt", pphh layout) use the reference pphh
+// Chain layout; T^ph and R the phhp layout.  V^pppp is stored for channels
+// with holes only, on pairs a<b (a quarter of the ordered block); the ladder
+// output D, the antisymmetrised amplitudes and V^hhpp rows are kept on the
+// same packed rows.  The one-body intermediates are diagonal (chi_i, psi_a),
+// so the reference's 3-body chains and their four remaps reduce to sums of
+// W = V o T and in-channel row swaps.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -26,12 +27,18 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <new>
 #include <string>
 #include <unordered_map>
 #include <vector>
 
 #include "ccd_amd.h"
+#include "device_common.hpp"
 #include "internal.hpp"
+#include "kernels_ladder.hpp"
+#include "kernels_ring.hpp"
+#include "kernels_setup.hpp"
+#include "kernels_update.hpp"
 
 // ------------------------------------------------------------------ errors
 
@@ -59,529 +66,6 @@ extern "C" const char *ccd_last_error(void) { return ccd::g_err.c_str(); }
                              __FILE__, __LINE__);                                           \
     } while (0)
 
-// ------------------------------------------------------------------ types
-
-struct DChan {          // active direct channel (pp > 0 and hh > 0)
-    int64_t off_t;      // block offset in the pphh layout
-    int64_t off_vp;     // packed V^pppp block: npk x ldv, rows/cols = pairs a<b
-    int64_t off_tp;     // packed-row blocks (npk x hh): antisymmetrised T, D_p, V^hhpp rows
-    int64_t off_hhhh;   // hh x hh block of V^hhhh / X
-    int32_t pp, hh;
-    int32_t pp_off, hh_off;  // first pair in the pp / hh pair lists
-    int32_t npk, ldv;        // number of pairs a<b; leading dimension (multiple of 8)
-    int32_t pr_off, pad;     // first entry in the packed-row lists
-};
-
-struct DMod {           // active mod channel (ph > 0 and hp > 0)
-    int64_t off_ph;     // block offset in the phhp layout (also V^A, hp x ph)
-    int64_t off_b;      // offset of its hp x hp block of V^B
-    int32_t ph, hp;
-    int32_t ph_off, hp_off;
-};
-
-struct DTile {          // a run of rows of one active direct channel
-    int32_t chan, row0, nrows, pad;
-};
-
-struct DSeg {           // strided run of pphh elements for a one-body sum
-    int64_t e0;
-    int32_t stride, count;
-};
-
-struct Gen {            // on-device interaction
-    int kind;
-    int nmax;
-    double g;
-    const int32_t *qn;
-    const double *table;
-};
-
-constexpr int kThreads = 256;
-
-// ------------------------------------------------------------------ device helpers
-
-__device__ __forceinline__ double block_sum(double v, double *red) {
-    // fixed-order reduction: wave butterfly, then the four wave sums in order
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    __syncthreads();
-    if (lane == 0) red[wave] = v;
-    __syncthreads();
-    double s = 0.0;
-    if (threadIdx.x == 0)
-        for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += red[w];
-    return s;  // valid on thread 0
-}
-
-// Antisymmetrised <pq||rs> generated from quantum numbers; bitwise equal to
-// the host model (pairing: src/PairingSPBasis.cpp:127-141; electron gas:
-// src/ElectronGasSPBasis.cpp:419-471 with the direct term tabulated).
-__device__ double gen_tbme(const Gen &G, int p, int q, int r, int s) {
-    const int32_t *a = G.qn + 5 * p, *b = G.qn + 5 * q, *c = G.qn + 5 * r, *d = G.qn + 5 * s;
-    if (G.kind == CCD_GEN_PAIRING) {
-        double v = 0.0;
-        if (a[0] == b[0] && c[0] == d[0] && a[1] == -b[1] && -c[1] == d[1]) {
-            v = -0.5 * G.g;
-            if (a[1] == -c[1]) v = -v;
-        }
-        return v;
-    }
-    const bool cons = a[0] + b[0] == c[0] + d[0] && a[1] + b[1] == c[1] + d[1] && a[2] + b[2] == c[2] + d[2];
-    if (!cons) return 0.0;
-    const int m2 = 2 * G.nmax, w = 4 * G.nmax + 1;
-    double d1 = 0.0, d2 = 0.0;
-    if (a[3] == c[3] && b[3] == d[3])
-        d1 = G.table[((a[0] - c[0] + m2) * w + (a[1] - c[1] + m2)) * w + (a[2] - c[2] + m2)];
-    if (a[3] == d[3] && b[3] == c[3])
-        d2 = G.table[((a[0] - d[0] + m2) * w + (a[1] - d[1] + m2)) * w + (a[2] - d[2] + m2)];
-    return d1 - d2;
-}
-
-// ------------------------------------------------------------------ generation kernels
-
-// Packed V^pppp rows of one tile: Vp[u][w] = <a_u b_u||c_w d_w> over pairs a<b,
-// zero in the padding columns w >= npk.
-__global__ __launch_bounds__(kThreads) void k_gen_vp(const DTile *__restrict__ tiles,
-                                                     const DChan *__restrict__ chans,
-                                                     const int2 *__restrict__ ppl,
-                                                     const int32_t *__restrict__ prow, Gen G,
-                                                     double *__restrict__ vp) {
-    const DTile tl = tiles[blockIdx.x];
-    const DChan ch = chans[tl.chan];
-    const int ldv = ch.ldv, npk = ch.npk;
-    const int64_t n = (int64_t)tl.nrows * ldv;
-    for (int64_t l = threadIdx.x; l < n; l += kThreads) {
-        const int u = tl.row0 + (int)(l / ldv), w = (int)(l % ldv);
-        double v = 0.0;
-        if (w < npk) {
-            const int2 ab = ppl[ch.pp_off + prow[ch.pr_off + u]];
-            const int2 cd = ppl[ch.pp_off + prow[ch.pr_off + w]];
-            v = gen_tbme(G, ab.x, ab.y, cd.x, cd.y);
-        }
-        vp[ch.off_vp + (int64_t)u * ldv + w] = v;
-    }
-}
-
-// Packed rows of V^hhpp^T: vtp[u][kl] = vt[(a_u,b_u)][kl] (tile = packed rows).
-__global__ __launch_bounds__(kThreads) void k_pack_vt(const DTile *__restrict__ tiles,
-                                                      const DChan *__restrict__ chans,
-                                                      const int32_t *__restrict__ prow,
-                                                      const double *__restrict__ vt,
-                                                      double *__restrict__ vtp) {
-    const DTile tl = tiles[blockIdx.x];
-    const DChan ch = chans[tl.chan];
-    const int H = ch.hh;
-    for (int l = threadIdx.x; l < tl.nrows * H; l += kThreads) {
-        const int u = tl.row0 + l / H, kl = l % H;
-        vtp[ch.off_tp + (int64_t)u * H + kl] = vt[ch.off_t + (int64_t)prow[ch.pr_off + u] * H + kl];
-    }
-}
-
-// Antisymmetrised amplitudes on the packed rows: tas[u][ij] = (t[(a,b)][ij] - t[(b,a)][ij]) / 2.
-__global__ __launch_bounds__(kThreads) void k_antisym(const DTile *__restrict__ tiles,
-                                                      const DChan *__restrict__ chans,
-                                                      const int32_t *__restrict__ prow,
-                                                      const int32_t *__restrict__ prow_sw,
-                                                      const double *__restrict__ t,
-                                                      double *__restrict__ tas) {
-    const DTile tl = tiles[blockIdx.x];
-    const DChan ch = chans[tl.chan];
-    const int H = ch.hh;
-    for (int l = threadIdx.x; l < tl.nrows * H; l += kThreads) {
-        const int u = tl.row0 + l / H, ij = l % H;
-        const double x = t[ch.off_t + (int64_t)prow[ch.pr_off + u] * H + ij];
-        const double y = t[ch.off_t + (int64_t)prow_sw[ch.pr_off + u] * H + ij];
-        tas[ch.off_tp + (int64_t)u * H + ij] = 0.5 * (x - y);
-    }
-}
-
-// V^hhpp transposed into the pphh layout, [(a,b),(i,j)] = <ij||ab>, and V^hhhh.
-__global__ __launch_bounds__(kThreads) void k_gen_direct_small(const DChan *__restrict__ chans,
-                                                               const int2 *__restrict__ ppl,
-                                                               const int2 *__restrict__ hhl, Gen G,
-                                                               double *__restrict__ vt,
-                                                               double *__restrict__ vh) {
-    const DChan ch = chans[blockIdx.x];
-    const int P = ch.pp, H = ch.hh;
-    for (int l = threadIdx.x; l < P * H; l += kThreads) {
-        const int AB = l / H, IJ = l - AB * H;
-        const int2 ab = ppl[ch.pp_off + AB], ij = hhl[ch.hh_off + IJ];
-        vt[ch.off_t + l] = gen_tbme(G, ij.x, ij.y, ab.x, ab.y);
-    }
-    for (int l = threadIdx.x; l < H * H; l += kThreads) {
-        const int K = l / H, J = l - K * H;
-        const int2 kl = hhl[ch.hh_off + K], ij = hhl[ch.hh_off + J];
-        vh[ch.off_hhhh + l] = gen_tbme(G, kl.x, kl.y, ij.x, ij.y);
-    }
-}
-
-// V^A [(i,a),(b,j)] = <ij||ab> (hp x ph) and V^B [(i,b),(j,a)] = <ia||bj> (hp x hp)
-// of one mod channel (src/load.cpp:275-318).
-__global__ __launch_bounds__(kThreads) void k_gen_mod(const DMod *__restrict__ mods,
-                                                      const int2 *__restrict__ phl,
-                                                      const int2 *__restrict__ hpl, Gen G,
-                                                      double *__restrict__ va,
-                                                      double *__restrict__ vb) {
-    const DMod m = mods[blockIdx.x];
-    const int PH = m.ph, HP = m.hp;
-    for (int l = threadIdx.x; l < HP * PH; l += kThreads) {
-        const int k = l / PH, c = l - k * PH;
-        const int2 ia = hpl[m.hp_off + k], bj = phl[m.ph_off + c];
-        va[m.off_ph + l] = gen_tbme(G, ia.x, bj.y, ia.y, bj.x);
-    }
-    for (int l = threadIdx.x; l < HP * HP; l += kThreads) {
-        const int k = l / HP, c = l - k * HP;
-        const int2 ib = hpl[m.hp_off + k], ja = hpl[m.hp_off + c];
-        vb[m.off_b + l] = gen_tbme(G, ib.x, ja.y, ib.y, ja.x);
-    }
-}
-
-// Caller V^hhpp (hh x pp) -> pphh layout.
-__global__ __launch_bounds__(kThreads) void k_transpose_hhpp(const DChan *__restrict__ chans,
-                                                             const double *__restrict__ vhhpp,
-                                                             double *__restrict__ vt) {
-    const DChan ch = chans[blockIdx.x];
-    const int P = ch.pp, H = ch.hh;
-    for (int l = threadIdx.x; l < P * H; l += kThreads) {
-        const int AB = l / H, IJ = l - AB * H;
-        vt[ch.off_t + l] = vhhpp[ch.off_t + (int64_t)IJ * P + AB];
-    }
-}
-
-// ------------------------------------------------------------------ ring (calc.cpp:264-313)
-
-// One mod channel per block: X = V^B + 1/2 V^A T^ph (hp x hp, in LDS),
-// R = T^ph X (ph x hp).
-__global__ __launch_bounds__(kThreads) void k_ring(const DMod *__restrict__ mods,
-                                                   const double *__restrict__ tph,
-                                                   const double *__restrict__ va,
-                                                   const double *__restrict__ vb,
-                                                   double *__restrict__ r) {
-    extern __shared__ double sx[];
-    const DMod m = mods[blockIdx.x];
-    const int PH = m.ph, HP = m.hp;
-    const double *T = tph + m.off_ph;
-    const double *A = va + m.off_ph;
-    const double *B = vb + m.off_b;
-    for (int o = threadIdx.x; o < HP * HP; o += kThreads) {
-        const int k = o / HP, j = o - k * HP;
-        double acc = 0.0;
-        for (int p = 0; p < PH; p++) acc += A[k * PH + p] * T[p * HP + j];
-        sx[o] = 0.5 * acc + B[o];
-    }
-    __syncthreads();
-    for (int o = threadIdx.x; o < PH * HP; o += kThreads) {
-        const int p = o / HP, j = o - p * HP;
-        double acc = 0.0;
-        for (int k = 0; k < HP; k++) acc += T[p * HP + k] * sx[k * HP + j];
-        r[m.off_ph + o] = acc;
-    }
-}
-
-// ------------------------------------------------------------------ ladders (calc.cpp:132-203)
-//
-// With t and V antisymmetric in (a,b) and (c,d), the reference's ordered-pair
-// ladder 1/2 sum_{cd} V[ab][cd] t[cd][ij] equals sum_{c<d} V[ab][cd] t[cd][ij],
-// and D[(b,a)] = -D[(a,b)], D[(a,a)] = 0.  Both ladder kernels therefore run on
-// the packed pairs a<b: a quarter of the V^pppp bytes and flops.  The
-// contractions use v_mfma_f64_16x16x4f64: A is 16x4 (lane l: row l%16, k-slot
-// l/16), B is 4x16 (lane l: k-slot l/16, column l%16), C is 16x16 with
-// register q of lane l holding row 4q + l/16, column l%16.  Within a 32-wide k chunk, k-slot q
-// of step s is k = 8q + s, so each lane reads 8 consecutive A elements.
-
-typedef double dbl4 __attribute__((ext_vector_type(4)));
-typedef double dbl2 __attribute__((ext_vector_type(2)));
-
-constexpr int KC = 32;                 // k rows per LDS stage
-constexpr int NTMAX = 5;               // 16-column tiles: hh <= 80
-constexpr int BSTR = NTMAX * 16 + 1;   // LDS row stride (doubles)
-constexpr int LROWS = 64;              // packed rows per ladder block (16 per wave)
-
-// Bs[kk][c] = B[(k0+kk)*ldb + c] for c < nt*16, zero where k >= K or c >= ncols.
-__device__ __forceinline__ void stage_b(double (*Bs)[BSTR], const double *__restrict__ B, int ldb,
-                                        int k0, int K, int ncols, int nt) {
-    const int w = nt * 16;
-    for (int idx = threadIdx.x; idx < KC * w; idx += kThreads) {
-        const int kk = idx / w, c = idx - kk * w, k = k0 + kk;
-        Bs[kk][c] = (k < K && c < ncols) ? B[(int64_t)k * ldb + c] : 0.0;
-    }
-}
-
-__device__ __forceinline__ void mfma_chunk(const double (&a)[8], double (*Bs)[BSTR], int kq, int m, int nt,
-                                           dbl4 (&acc)[NTMAX]) {
-#pragma unroll
-    for (int s = 0; s < 8; s++) {
-#pragma unroll
-        for (int n = 0; n < NTMAX; n++) {
-            if (n < nt) acc[n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], Bs[kq * 8 + s][n * 16 + m], acc[n], 0, 0, 0);
-        }
-    }
-}
-
-// X = V^hhhh + sum_{c<d} V^hhpp[kl][cd] tas[cd][ij]; one wave per 16x16 tile of X.
-__global__ __launch_bounds__(kThreads) void k_ladder_x(const DTile *__restrict__ tiles, int ntiles,
-                                                       const DChan *__restrict__ chans,
-                                                       const double *__restrict__ vtp,
-                                                       const double *__restrict__ tas,
-                                                       const double *__restrict__ vh,
-                                                       double *__restrict__ x) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
-    const int tix = blockIdx.x * (kThreads / 64) + wave;
-    if (tix >= ntiles) return;
-    const DTile tl = tiles[tix];
-    const DChan ch = chans[tl.chan];
-    const int H = ch.hh, npk = ch.npk, kl0 = tl.row0, ij0 = tl.pad;
-    const bool aok = kl0 + m < H, bok = ij0 + m < H;
-    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < npk; k0 += KC) {
-        double a[8], b[8];
-#pragma unroll
-        for (int s = 0; s < 8; s++) {
-            const int w = k0 + kq * 8 + s;
-            const bool ok = w < npk;
-            a[s] = (ok && aok) ? vtp[ch.off_tp + (int64_t)w * H + kl0 + m] : 0.0;
-            b[s] = (ok && bok) ? tas[ch.off_tp + (int64_t)w * H + ij0 + m] : 0.0;
-        }
-#pragma unroll
-        for (int s = 0; s < 8; s++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[s], acc, 0, 0, 0);
-    }
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const int row = kl0 + q * 4 + kq, col = ij0 + m;
-        if (row < H && col < H) {
-            const int64_t o = ch.off_hhhh + (int64_t)row * H + col;
-            x[o] = vh[o] + acc[q];
-        }
-    }
-}
-
-// D_p[u][ij] = 1/2 sum_kl tas[u][kl] X[kl][ij] + sum_w Vp[u][w] tas[w][ij]
-// for 64 packed rows u of one channel per block (16 per wave), all hh columns.
-__global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__ tiles,
-                                                       const DChan *__restrict__ chans,
-                                                       const double *__restrict__ vp,
-                                                       const double *__restrict__ tas,
-                                                       const double *__restrict__ x,
-                                                       double *__restrict__ dp) {
-    __shared__ double Bs[KC][BSTR];
-    const DTile tl = tiles[blockIdx.x];
-    const DChan ch = chans[tl.chan];
-    const int H = ch.hh, npk = ch.npk, ldv = ch.ldv;
-    const int nt = (H + 15) / 16;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
-    const int lrow = wave * 16 + m;
-    const bool rok = lrow < tl.nrows;
-    const int64_t u = tl.row0 + lrow;
-    dbl4 acc[NTMAX];
-#pragma unroll
-    for (int n = 0; n < NTMAX; n++) acc[n] = dbl4{0.0, 0.0, 0.0, 0.0};
-
-    // 1/2 T X  (K = hh, ordered kl)
-    const double *at = tas + ch.off_tp + u * H;
-    for (int k0 = 0; k0 < H; k0 += KC) {
-        __syncthreads();
-        stage_b(Bs, x + ch.off_hhhh, H, k0, H, H, nt);
-        __syncthreads();
-        double a[8];
-#pragma unroll
-        for (int s = 0; s < 8; s++) {
-            const int k = k0 + kq * 8 + s;
-            a[s] = (rok && k < H) ? 0.5 * at[k] : 0.0;
-        }
-        mfma_chunk(a, Bs, kq, m, nt, acc);
-    }
-    // V^pppp t  (K = npk packed pairs; rows padded with zeros to ldv)
-    const double *av = vp + ch.off_vp + u * ldv;
-    for (int k0 = 0; k0 < npk; k0 += KC) {
-        __syncthreads();
-        stage_b(Bs, tas + ch.off_tp, H, k0, npk, H, nt);
-        __syncthreads();
-        double a[8];
-        const int kb = k0 + kq * 8;
-        if (rok && kb < ldv) {       // ldv and kb are multiples of 8: whole 64-byte group in range
-            const dbl2 *p2 = reinterpret_cast<const dbl2 *>(av + kb);
-#pragma unroll
-            for (int h = 0; h < 4; h++) {
-                const dbl2 v = p2[h];
-                a[2 * h] = v.x;
-                a[2 * h + 1] = v.y;
-            }
-        } else {
-#pragma unroll
-            for (int s = 0; s < 8; s++) a[s] = 0.0;
-        }
-        mfma_chunk(a, Bs, kq, m, nt, acc);
-    }
-#pragma unroll
-    for (int n = 0; n < NTMAX; n++) {
-        if (n >= nt) continue;
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int row = wave * 16 + q * 4 + kq, col = n * 16 + m;
-            if (row < tl.nrows && col < H) dp[ch.off_tp + (int64_t)(tl.row0 + row) * H + col] = acc[n][q];
-        }
-    }
-}
-
-// ------------------------------------------------------------------ update (calc.cpp:368-421)
-
-struct UpdArgs {
-    const DTile *tiles;
-    const DChan *chans;
-    const int2 *ppl, *hhl;
-    const int32_t *pp_swap, *hh_swap;  // index of (b,a) / (j,i) within the channel
-    const int32_t *pkc;                // per pp pair: -1 if a==b, else (packed row << 1) | (a > b)
-    const int32_t *rowbase;            // [(a-nh)*nh + i] -> phhp offset of row (a,i)
-    const int32_t *colidx;             // [j*np + (b-nh)]  -> column of (j,b)
-    const double *eps, *chi, *psi;
-    const double *vt, *d, *r, *t_old;
-    double *t_new, *tph, *w;
-    double *partials;                  // per block sum of W
-    double mixing;
-    int nh, np;
-};
-
-__global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
-    __shared__ double red[kThreads / 64];
-    const DTile tl = a.tiles[blockIdx.x];
-    const DChan ch = a.chans[tl.chan];
-    const int H = ch.hh;
-    const int nh = a.nh, np = a.np;
-    double wsum = 0.0;
-    for (int l = threadIdx.x; l < tl.nrows * H; l += kThreads) {
-        const int AB = tl.row0 + l / H, IJ = l % H;
-        const int64_t e = ch.off_t + (int64_t)AB * H + IJ;
-        const int2 ab = a.ppl[ch.pp_off + AB], ij = a.hhl[ch.hh_off + IJ];
-        const int ia = ab.x, ib = ab.y, ii = ij.x, jj = ij.y;
-        const int pa = ia - nh, pb = ib - nh;
-        const int g1 = a.rowbase[pa * nh + ii] + a.colidx[jj * np + pb];  // (a,i),(j,b)
-        const int g2 = a.rowbase[pa * nh + jj] + a.colidx[ii * np + pb];  // (a,j),(i,b)
-        const int g3 = a.rowbase[pb * nh + ii] + a.colidx[jj * np + pa];  // (b,i),(j,a)
-        const int g4 = a.rowbase[pb * nh + jj] + a.colidx[ii * np + pa];  // (b,j),(i,a)
-        const double t = a.t_old[e];
-        const double t_ji = a.t_old[ch.off_t + (int64_t)AB * H + a.hh_swap[ch.hh_off + IJ]];
-        const double t_ba = a.t_old[ch.off_t + (int64_t)a.pp_swap[ch.pp_off + AB] * H + IJ];
-        const double v = a.vt[e];
-        double value = v;
-        const int code = a.pkc[ch.pp_off + AB];
-        double dv = 0.0;
-        if (code >= 0) {
-            dv = a.d[ch.off_tp + (int64_t)(code >> 1) * H + IJ];
-            if (code & 1) dv = -dv;
-        }
-        value += dv;
-        value += a.r[g1];
-        value -= a.r[g2];
-        value -= a.r[g3];
-        value += a.r[g4];
-        value += (0.5 * a.chi[ii]) * t_ji;   // h2pp_h1 gathered as jab_i
-        value -= (0.5 * a.chi[jj]) * t;      // h2pp_h1 gathered as iab_j
-        value += (-0.5 * a.psi[pb]) * t;     // p2_p1hh gathered as b_aij
-        value -= (-0.5 * a.psi[pa]) * t_ba;  // p2_p1hh gathered as a_bij
-        value = value / (a.eps[ii] + a.eps[jj] - a.eps[ia] - a.eps[ib]);
-        value = a.mixing * value + (1. - a.mixing) * t;
-        a.t_new[e] = value;
-        a.tph[g1] = value;                   // mapToDistributed(ai_jb)
-        const double wv = v * value;
-        a.w[e] = wv;
-        wsum += wv;
-    }
-    const double s = block_sum(wsum, red);
-    if (threadIdx.x == 0) a.partials[blockIdx.x] = s;
-}
-
-// MBPT2 terms v^2 / denominator, per-block partial sums.
-__global__ __launch_bounds__(kThreads) void k_mbpt2(const DTile *__restrict__ tiles,
-                                                    const DChan *__restrict__ chans,
-                                                    const int2 *__restrict__ ppl,
-                                                    const int2 *__restrict__ hhl,
-                                                    const double *__restrict__ eps,
-                                                    const double *__restrict__ vt,
-                                                    double *__restrict__ partials) {
-    __shared__ double red[kThreads / 64];
-    const DTile tl = tiles[blockIdx.x];
-    const DChan ch = chans[tl.chan];
-    const int H = ch.hh;
-    double s = 0.0;
-    for (int l = threadIdx.x; l < tl.nrows * H; l += kThreads) {
-        const int AB = tl.row0 + l / H, IJ = l % H;
-        const int2 ab = ppl[ch.pp_off + AB], ij = hhl[ch.hh_off + IJ];
-        const double v = vt[ch.off_t + (int64_t)AB * H + IJ];
-        const double den = eps[ij.x] + eps[ij.y] - eps[ab.x] - eps[ab.y];
-        s += v * v / den;
-    }
-    const double b = block_sum(s, red);
-    if (threadIdx.x == 0) partials[blockIdx.x] = b;
-}
-
-// out = scale * sum(partials[0..n)) in a fixed order (one block).
-__global__ __launch_bounds__(kThreads) void k_sum_partials(const double *__restrict__ partials,
-                                                           int n, double scale,
-                                                           double *__restrict__ out) {
-    __shared__ double red[kThreads / 64];
-    double s = 0.0;
-    for (int k = threadIdx.x; k < n; k += kThreads) s += partials[k];
-    const double b = block_sum(s, red);
-    if (threadIdx.x == 0) out[0] = scale * b;
-}
-
-// chi_i and psi_a: block k sums the segments seg_ptr[k]..seg_ptr[k+1] of W.
-__global__ __launch_bounds__(kThreads) void k_onebody(const int32_t *__restrict__ seg_ptr,
-                                                      const DSeg *__restrict__ segs,
-                                                      const double *__restrict__ w,
-                                                      double *__restrict__ out) {
-    __shared__ double red[kThreads / 64];
-    const int s0 = seg_ptr[blockIdx.x], s1 = seg_ptr[blockIdx.x + 1];
-    double s = 0.0;
-    for (int k = s0 + threadIdx.x; k < s1; k += kThreads) {
-        const DSeg sg = segs[k];
-        double t = 0.0;
-        for (int q = 0; q < sg.count; q++) t += w[sg.e0 + (int64_t)q * sg.stride];
-        s += t;
-    }
-    const double b = block_sum(s, red);
-    if (threadIdx.x == 0) out[blockIdx.x] = b;
-}
-
-// Scatter T (pphh) into T^ph (phhp): the standalone remap entry point.
-__global__ __launch_bounds__(kThreads) void k_remap_ph(UpdArgs a) {
-    const DTile tl = a.tiles[blockIdx.x];
-    const DChan ch = a.chans[tl.chan];
-    const int H = ch.hh, nh = a.nh, np = a.np;
-    for (int l = threadIdx.x; l < tl.nrows * H; l += kThreads) {
-        const int AB = tl.row0 + l / H, IJ = l % H;
-        const int2 ab = a.ppl[ch.pp_off + AB], ij = a.hhl[ch.hh_off + IJ];
-        const int g1 = a.rowbase[(ab.x - nh) * nh + ij.x] + a.colidx[ij.y * np + (ab.y - nh)];
-        a.tph[g1] = a.t_old[ch.off_t + (int64_t)AB * H + IJ];
-    }
-}
-
-// D in the ordered pphh layout from D_p (for the ladder entry point).
-__global__ __launch_bounds__(kThreads) void k_expand_d(UpdArgs a) {
-    const DTile tl = a.tiles[blockIdx.x];
-    const DChan ch = a.chans[tl.chan];
-    const int H = ch.hh;
-    for (int l = threadIdx.x; l < tl.nrows * H; l += kThreads) {
-        const int AB = tl.row0 + l / H, IJ = l % H;
-        const int code = a.pkc[ch.pp_off + AB];
-        double dv = 0.0;
-        if (code >= 0) {
-            dv = a.d[ch.off_tp + (int64_t)(code >> 1) * H + IJ];
-            if (code & 1) dv = -dv;
-        }
-        a.t_new[ch.off_t + (int64_t)AB * H + IJ] = dv;
-    }
-}
-
-// W = V o T (for the one-body entry point).
-__global__ __launch_bounds__(kThreads) void k_hadamard(const double *__restrict__ vt,
-                                                       const double *__restrict__ t,
-                                                       double *__restrict__ w, int64_t n) {
-    for (int64_t e = blockIdx.x * (int64_t)kThreads + threadIdx.x; e < n;
-         e += (int64_t)gridDim.x * kThreads)
-        w[e] = vt[e] * t[e];
-}
-
 // ------------------------------------------------------------------ context
 
 struct ccd_ctx {
@@ -589,33 +73,37 @@ struct ccd_ctx {
     int precision = CCD_PREC_FP64;
     hipStream_t stream = nullptr;
     int nsp = 0, nh = 0, np = 0;
-    int64_t nt2 = 0, nph = 0;
+    int64_t nt2 = 0;
     int64_t bytes = 0;
     std::vector<DChan> chans;
     std::vector<DMod> mods;
     std::vector<DTile> ltiles, utiles, xtiles;
-    std::vector<double> hbuf;
-    // device
+    int64_t n_cp = 0;            // column-partial slots
+    int ring_stride = 0;         // LDS row stride of the ring kernel
+    size_t ring_lds = 0, tr_lds = 0;
+    // device tables
     DChan *d_chans = nullptr;
     DMod *d_mods = nullptr;
     DTile *d_ltiles = nullptr, *d_utiles = nullptr, *d_xtiles = nullptr;
     int2 *d_ppl = nullptr, *d_hhl = nullptr, *d_phl = nullptr, *d_hpl = nullptr;
-    int32_t *d_pp_swap = nullptr, *d_hh_swap = nullptr, *d_rowbase = nullptr, *d_colidx = nullptr;
-    int32_t *d_prow = nullptr, *d_prow_sw = nullptr, *d_pkc = nullptr;
-    int32_t *d_seg_ptr = nullptr;
-    DSeg *d_segs = nullptr;
+    int32_t *d_hh_swap = nullptr, *d_prow = nullptr, *d_prow_sw = nullptr;
+    int32_t *d_red_ptr = nullptr, *d_red_idx = nullptr;
+    int32_t *d_tr_ptr_pp = nullptr, *d_tr_ptr_ph = nullptr, *d_tr_base = nullptr;
+    DRow *d_tr_rows_pp = nullptr, *d_tr_rows_ph = nullptr;
+    uint16_t *d_perm_ph = nullptr, *d_perm_pp = nullptr;
+    // device state and interaction
     double *d_eps = nullptr, *d_onebody = nullptr;  // chi [nh] then psi [np]
     double *d_t[2] = {nullptr, nullptr};
     int cur = 0;
-    double *d_d = nullptr, *d_tph = nullptr, *d_r = nullptr, *d_w = nullptr, *d_x = nullptr;
-    double *d_vt = nullptr, *d_vh = nullptr, *d_vp = nullptr, *d_va = nullptr, *d_vb = nullptr;
-    double *d_vtp = nullptr, *d_tas = nullptr;
+    double *d_dp = nullptr, *d_tph = nullptr, *d_r = nullptr, *d_g = nullptr, *d_x = nullptr;
+    double *d_tas = nullptr, *d_rs = nullptr, *d_cp = nullptr;
+    double *d_vt = nullptr, *d_vtp = nullptr, *d_vh = nullptr, *d_vp = nullptr, *d_va = nullptr,
+           *d_vb = nullptr;
     double *d_partials = nullptr, *d_scalar = nullptr;
     double *h_scalar = nullptr;  // pinned
     int32_t *d_qn = nullptr;
     double *d_table = nullptr;
-    int64_t n_vp = 0, n_tp = 0, n_hhhh = 0, n_b = 0, n_vp_alg = 0;
-    size_t ring_lds_max = 0;
+    int64_t n_vp = 0, n_tp = 0, n_hhhh = 0, n_b = 0;
     double flops = 0, flops_pppp = 0;
     std::vector<void *> allocs;
 };
@@ -659,73 +147,89 @@ int check_launch(const char *what) {
 UpdArgs upd_args(ccd_ctx *c, const double *t_old, double *t_new, double mixing) {
     UpdArgs a{};
     a.tiles = c->d_utiles; a.chans = c->d_chans; a.ppl = c->d_ppl; a.hhl = c->d_hhl;
-    a.pp_swap = c->d_pp_swap; a.hh_swap = c->d_hh_swap; a.pkc = c->d_pkc; a.rowbase = c->d_rowbase;
-    a.colidx = c->d_colidx; a.eps = c->d_eps; a.chi = c->d_onebody; a.psi = c->d_onebody + c->nh;
-    a.vt = c->d_vt; a.d = c->d_d; a.r = c->d_r; a.t_old = t_old; a.t_new = t_new;
-    a.tph = c->d_tph; a.w = c->d_w; a.partials = c->d_partials; a.mixing = mixing;
-    a.nh = c->nh; a.np = c->np;
+    a.prow = c->d_prow; a.prow_sw = c->d_prow_sw; a.hh_swap = c->d_hh_swap;
+    a.eps = c->d_eps; a.chi = c->d_onebody; a.psi = c->d_onebody + c->nh;
+    a.vt = c->d_vt; a.dp = c->d_dp; a.g = c->d_g; a.t_old = t_old; a.t_new = t_new;
+    a.tas = c->d_tas; a.rs = c->d_rs; a.cp = c->d_cp; a.mixing = mixing; a.nh = c->nh;
     return a;
 }
 
-// Phases of one iteration; ev (optional) brackets each phase.
+// --- launch helpers, one per phase of the iteration
+
 int launch_ring(ccd_ctx *c) {
     if (c->mods.empty()) return CCD_OK;
-    hipLaunchKernelGGL(k_ring, dim3((unsigned)c->mods.size()), dim3(kThreads), c->ring_lds_max,
-                       c->stream, c->d_mods, c->d_tph, c->d_va, c->d_vb, c->d_r);
+    hipLaunchKernelGGL(k_ring, dim3((unsigned)c->mods.size()), dim3(kThreads), c->ring_lds, c->stream, c->d_mods,
+                       c->ring_stride, c->d_tph, c->d_va, c->d_vb, c->d_r);
     return check_launch("k_ring");
 }
 
-int launch_ladder_x(ccd_ctx *c, const double *t) {
+// pphh -> phhp (T to T^ph) when to_ph, else phhp -> pphh (R to G)
+int launch_transpose(ccd_ctx *c, bool to_ph, const double *src, double *dst) {
+    if (c->np == 0 || c->nt2 == 0) return CCD_OK;
+    if (to_ph)
+        hipLaunchKernelGGL(k_transpose, dim3((unsigned)c->np), dim3(kThreads), c->tr_lds, c->stream, c->d_tr_ptr_pp,
+                           c->d_tr_rows_pp, c->d_tr_ptr_ph, c->d_tr_rows_ph, c->d_tr_base, c->d_perm_ph, src, dst);
+    else
+        hipLaunchKernelGGL(k_transpose, dim3((unsigned)c->np), dim3(kThreads), c->tr_lds, c->stream, c->d_tr_ptr_ph,
+                           c->d_tr_rows_ph, c->d_tr_ptr_pp, c->d_tr_rows_pp, c->d_tr_base, c->d_perm_pp, src, dst);
+    return check_launch("k_transpose");
+}
+
+int launch_antisym(ccd_ctx *c, const double *t) {
     if (c->ltiles.empty()) return CCD_OK;
     hipLaunchKernelGGL(k_antisym, dim3((unsigned)c->ltiles.size()), dim3(kThreads), 0, c->stream, c->d_ltiles,
                        c->d_chans, c->d_prow, c->d_prow_sw, t, c->d_tas);
-    int rc = check_launch("k_antisym");
-    if (rc || c->xtiles.empty()) return rc;
+    return check_launch("k_antisym");
+}
+
+int launch_ladder_x(ccd_ctx *c) {
+    if (c->xtiles.empty()) return CCD_OK;
     const unsigned nb = (unsigned)((c->xtiles.size() + 3) / 4);
     hipLaunchKernelGGL(k_ladder_x, dim3(nb), dim3(kThreads), 0, c->stream, c->d_xtiles, (int)c->xtiles.size(),
                        c->d_chans, c->d_vtp, c->d_tas, c->d_vh, c->d_x);
     return check_launch("k_ladder_x");
 }
 
-int launch_ladder_d(ccd_ctx *c, const double *t) {
-    (void)t;
+int launch_ladder_v(ccd_ctx *c) {
     if (c->ltiles.empty()) return CCD_OK;
     hipLaunchKernelGGL(k_ladder_v, dim3((unsigned)c->ltiles.size()), dim3(kThreads), 0, c->stream,
-                       c->d_ltiles, c->d_chans, c->d_vp, c->d_tas, c->d_x, c->d_d);
+                       c->d_ltiles, c->d_chans, c->d_vp, c->d_tas, c->d_x, c->d_dp);
     return check_launch("k_ladder_v");
 }
 
+template <bool UPDATE>
 int launch_update(ccd_ctx *c, const double *t_old, double *t_new, double mixing) {
     if (c->utiles.empty()) return CCD_OK;
     UpdArgs a = upd_args(c, t_old, t_new, mixing);
-    hipLaunchKernelGGL(k_update, dim3((unsigned)c->utiles.size()), dim3(kThreads), 0, c->stream, a);
+    hipLaunchKernelGGL(k_update<UPDATE>, dim3((unsigned)c->utiles.size()), dim3(kThreads), 0, c->stream, a);
     return check_launch("k_update");
 }
 
-int launch_reductions(ccd_ctx *c) {
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kThreads), 0, c->stream, c->d_partials,
-                       (int)c->utiles.size(), 0.25, c->d_scalar);
-    int rc = check_launch("k_sum_partials");
+int launch_reduce(ccd_ctx *c) {
+    hipLaunchKernelGGL(k_reduce, dim3((unsigned)c->nsp), dim3(kThreads), 0, c->stream, c->d_red_ptr, c->d_red_idx,
+                       c->nh, c->d_cp, c->d_rs, c->d_onebody);
+    int rc = check_launch("k_reduce");
     if (rc) return rc;
-    hipLaunchKernelGGL(k_onebody, dim3((unsigned)(c->nh + c->np)), dim3(kThreads), 0, c->stream,
-                       c->d_seg_ptr, c->d_segs, c->d_w, c->d_onebody);
-    return check_launch("k_onebody");
+    hipLaunchKernelGGL(k_energy, dim3(1), dim3(kThreads), 0, c->stream, c->d_onebody, c->nh, c->d_scalar);
+    return check_launch("k_energy");
 }
 
+// One iteration; ev (optional, 6 events) brackets
+// ring + R remap | X | pppp ladder | update | T remap + reductions.
 int run_iteration(ccd_ctx *c, double mixing, hipEvent_t *ev) {
     const double *t_old = c->d_t[c->cur];
     double *t_new = c->d_t[1 - c->cur];
     int rc;
     if (ev) HIP_TRY(hipEventRecord(ev[0], c->stream));
-    if ((rc = launch_ring(c))) return rc;
+    if ((rc = launch_ring(c)) || (rc = launch_transpose(c, false, c->d_r, c->d_g))) return rc;
     if (ev) HIP_TRY(hipEventRecord(ev[1], c->stream));
-    if ((rc = launch_ladder_x(c, t_old))) return rc;
+    if ((rc = launch_ladder_x(c))) return rc;
     if (ev) HIP_TRY(hipEventRecord(ev[2], c->stream));
-    if ((rc = launch_ladder_d(c, t_old))) return rc;
+    if ((rc = launch_ladder_v(c))) return rc;
     if (ev) HIP_TRY(hipEventRecord(ev[3], c->stream));
-    if ((rc = launch_update(c, t_old, t_new, mixing))) return rc;
+    if ((rc = launch_update<true>(c, t_old, t_new, mixing))) return rc;
     if (ev) HIP_TRY(hipEventRecord(ev[4], c->stream));
-    if ((rc = launch_reductions(c))) return rc;
+    if ((rc = launch_transpose(c, true, t_new, c->d_tph)) || (rc = launch_reduce(c))) return rc;
     if (ev) HIP_TRY(hipEventRecord(ev[5], c->stream));
     c->cur = 1 - c->cur;
     return CCD_OK;
@@ -758,10 +262,12 @@ int validate(const ccd_problem *p) {
     return CCD_OK;
 }
 
+// Host tables: channel descriptors, swaps, packed rows, tiles, transposition
+// plans and reduction lists.  Every index a kernel dereferences is produced
+// and range-checked here.
 int build(ccd_ctx *c, const ccd_problem *p) {
     const int nsp = p->n_sp, nh = p->n_holes, np = nsp - nh;
     c->nsp = nsp; c->nh = nh; c->np = np;
-    // --- pair lists and channel tables (host) ---
     int64_t npp = 0, nhh = 0, nphp = 0, nhpp = 0;
     for (int k = 0; k < p->n_chan; k++) {
         if (p->pp_dim[k] < 0 || p->hh_dim[k] < 0) return ccd::fail(CCD_E_INVALID, "negative dim");
@@ -773,6 +279,7 @@ int build(ccd_ctx *c, const ccd_problem *p) {
     }
     if ((npp && !p->pp_pairs) || (nhh && !p->hh_pairs) || (nphp && !p->ph_pairs) || (nhpp && !p->hp_pairs))
         return ccd::fail(CCD_E_INVALID, "missing pair lists");
+    if (npp >= INT32_MAX || nhh >= INT32_MAX) return ccd::fail(CCD_E_INVALID, "pair lists too long");
     auto pair_ok = [&](const int32_t *l, int64_t n, bool p_hole, bool q_hole) {
         for (int64_t k = 0; k < n; k++) {
             int a = l[2 * k], b = l[2 * k + 1];
@@ -791,18 +298,19 @@ int build(ccd_ctx *c, const ccd_problem *p) {
     for (int64_t k = 0; k < nphp; k++) phl[k] = make_int2(p->ph_pairs[2 * k], p->ph_pairs[2 * k + 1]);
     for (int64_t k = 0; k < nhpp; k++) hpl[k] = make_int2(p->hp_pairs[2 * k], p->hp_pairs[2 * k + 1]);
 
-    // direct channels
-    std::vector<int32_t> pp_swap(npp, -1), hh_swap(nhh, -1), pkc(npp, -1);
+    // ---- direct channels: swaps, packed rows, row pairs
+    std::vector<int32_t> pp_swap(npp, -1), hh_swap(nhh, -1);
     std::vector<int32_t> prow, prow_sw;
     int64_t off_t = 0, off_vp = 0, off_tp = 0, off_hhhh = 0, ipp = 0, ihh = 0;
-    c->flops = 0; c->flops_pppp = 0; c->n_vp_alg = 0;
+    c->flops = 0; c->flops_pppp = 0;
     std::unordered_map<int64_t, int32_t> pos;
     for (int k = 0; k < p->n_chan; k++) {
         const int P = p->pp_dim[k], H = p->hh_dim[k];
         if (P > 0 && H > 0) {
-            // swaps within the channel
+            if (H > NTMAX * 16) return ccd::fail(CCD_E_INVALID, "channel %d has hh=%d > %d", k, H, NTMAX * 16);
             pos.clear();
             for (int q = 0; q < P; q++) pos[(int64_t)ppl[ipp + q].x * nsp + ppl[ipp + q].y] = q;
+            if ((int)pos.size() != P) return ccd::fail(CCD_E_INVALID, "pp channel %d lists a pair twice", k);
             for (int q = 0; q < P; q++) {
                 auto it = pos.find((int64_t)ppl[ipp + q].y * nsp + ppl[ipp + q].x);
                 if (it == pos.end()) return ccd::fail(CCD_E_INVALID, "pp channel %d lacks a swapped pair", k);
@@ -810,33 +318,32 @@ int build(ccd_ctx *c, const ccd_problem *p) {
             }
             pos.clear();
             for (int q = 0; q < H; q++) pos[(int64_t)hhl[ihh + q].x * nsp + hhl[ihh + q].y] = q;
+            if ((int)pos.size() != H) return ccd::fail(CCD_E_INVALID, "hh channel %d lists a pair twice", k);
             for (int q = 0; q < H; q++) {
                 auto it = pos.find((int64_t)hhl[ihh + q].y * nsp + hhl[ihh + q].x);
                 if (it == pos.end()) return ccd::fail(CCD_E_INVALID, "hh channel %d lacks a swapped pair", k);
                 hh_swap[ihh + q] = it->second;
             }
-            // packed rows: pairs a<b in list order
+            // pair list: packed rows (a<b) first, then rows (a,a)
             const int pr_off = (int)prow.size();
-            std::vector<int32_t> packed_of(P, -1);
             for (int q = 0; q < P; q++)
                 if (ppl[ipp + q].x < ppl[ipp + q].y) {
-                    packed_of[q] = (int32_t)(prow.size() - pr_off);
                     prow.push_back(q);
                     prow_sw.push_back(pp_swap[ipp + q]);
                 }
             const int npk = (int)prow.size() - pr_off;
-            for (int q = 0; q < P; q++) {
-                const int a = ppl[ipp + q].x, b = ppl[ipp + q].y;
-                if (a < b) pkc[ipp + q] = packed_of[q] << 1;
-                else if (a > b) pkc[ipp + q] = (packed_of[pp_swap[ipp + q]] << 1) | 1;
-            }
+            for (int q = 0; q < P; q++)
+                if (ppl[ipp + q].x == ppl[ipp + q].y) {
+                    prow.push_back(q);
+                    prow_sw.push_back(q);
+                }
+            const int ndiag = (int)prow.size() - pr_off - npk;
             const int ldv = (npk + 7) / 8 * 8;
-            DChan ch{off_t, off_vp, off_tp, off_hhhh, P, H, (int32_t)ipp, (int32_t)ihh, npk, ldv, pr_off, 0};
+            DChan ch{off_t, off_vp, off_tp, off_hhhh, P, H, (int32_t)ipp, (int32_t)ihh, npk, ldv, pr_off, ndiag};
             c->chans.push_back(ch);
             off_vp += (int64_t)npk * ldv;
             off_tp += (int64_t)npk * H;
             off_hhhh += (int64_t)H * H;
-            c->n_vp_alg += (int64_t)npk * npk;
             c->flops += 2.0 * H * H * P + 2.0 * P * H * H + 2.0 * P * P * H;
             c->flops_pppp += 2.0 * P * P * H;
         }
@@ -845,21 +352,20 @@ int build(ccd_ctx *c, const ccd_problem *p) {
     }
     c->nt2 = off_t;
     c->n_vp = off_vp; c->n_tp = off_tp; c->n_hhhh = off_hhhh;
-    if (c->nt2 >= (int64_t)INT32_MAX) return ccd::fail(CCD_E_INVALID, "T2 has %lld elements (limit 2^31)", (long long)c->nt2);
-    for (const DChan &ch : c->chans)
-        if (ch.hh > NTMAX * 16) return ccd::fail(CCD_E_INVALID, "channel with hh=%d exceeds %d", ch.hh, NTMAX * 16);
+    if (c->nt2 >= (int64_t)INT32_MAX)
+        return ccd::fail(CCD_E_INVALID, "T2 has %lld elements (limit 2^31)", (long long)c->nt2);
 
-    // mod channels and position tables
-    std::vector<int32_t> rowbase((size_t)np * nh, -1), colidx((size_t)nh * np, -1);
+    // ---- mod channels and position tables (host only)
+    std::vector<int32_t> colidx((size_t)nh * np, -1);
     std::vector<int32_t> mod_of_ph((size_t)np * nh, -1), mod_of_hp((size_t)nh * np, -1);
     int64_t off_ph = 0, off_b = 0, iph = 0, ihp = 0;
+    int max_hp = 0;
     for (int k = 0; k < p->n_mod; k++) {
         const int PH = p->ph_dim[k], HP = p->hp_dim[k];
         for (int q = 0; q < PH; q++) {
             int a = phl[iph + q].x, i = phl[iph + q].y;
             size_t s = (size_t)(a - nh) * nh + i;
             if (mod_of_ph[s] >= 0) return ccd::fail(CCD_E_INVALID, "ph pair (%d,%d) listed twice", a, i);
-            rowbase[s] = (int32_t)(off_ph + (int64_t)q * HP);
             mod_of_ph[s] = k;
         }
         for (int q = 0; q < HP; q++) {
@@ -870,96 +376,145 @@ int build(ccd_ctx *c, const ccd_problem *p) {
             mod_of_hp[s] = k;
         }
         if (PH > 0 && HP > 0) {
-            DMod m{off_ph, off_b, PH, HP, (int32_t)iph, (int32_t)ihp};
-            c->mods.push_back(m);
+            if (PH > NTMAX * 16 || HP > NTMAX * 16)
+                return ccd::fail(CCD_E_INVALID, "mod channel %d has ph=%d hp=%d > %d", k, PH, HP, NTMAX * 16);
+            c->mods.push_back(DMod{off_ph, off_b, PH, HP, (int32_t)iph, (int32_t)ihp});
             off_b += (int64_t)HP * HP;
+            max_hp = std::max(max_hp, HP);
             c->flops += 2.0 * HP * HP * PH + 2.0 * PH * HP * HP;
         }
         off_ph += (int64_t)PH * HP;
         iph += PH; ihp += HP;
     }
-    c->nph = off_ph; c->n_b = off_b;
-    if (c->nph != c->nt2)
-        return ccd::fail(CCD_E_INVALID, "phhp layout has %lld elements, pphh has %lld", (long long)c->nph,
+    c->n_b = off_b;
+    if (off_ph != c->nt2)
+        return ccd::fail(CCD_E_INVALID, "phhp layout has %lld elements, pphh has %lld", (long long)off_ph,
                          (long long)c->nt2);
-    // every T2 element must land in one mod block, (a,i) and (j,b) in the same channel
-    for (const DChan &ch : c->chans)
-        for (int AB = 0; AB < ch.pp; AB++)
-            for (int IJ = 0; IJ < ch.hh; IJ++) {
-                int a = ppl[ch.pp_off + AB].x, b = ppl[ch.pp_off + AB].y;
-                int i = hhl[ch.hh_off + IJ].x, j = hhl[ch.hh_off + IJ].y;
-                const int32_t m1 = mod_of_ph[(size_t)(a - nh) * nh + i], m2 = mod_of_hp[(size_t)j * np + (b - nh)];
-                const int32_t m3 = mod_of_ph[(size_t)(b - nh) * nh + j], m4 = mod_of_hp[(size_t)i * np + (a - nh)];
-                if (m1 < 0 || m1 != m2 || m3 < 0 || m3 != m4)
-                    return ccd::fail(CCD_E_INVALID, "T2 element (%d,%d,%d,%d) has no consistent mod channel", a,
-                                     b, i, j);
-            }
-    size_t lds = 0;
-    for (const DMod &m : c->mods) lds = std::max(lds, (size_t)m.hp * m.hp * sizeof(double));
-    if (lds > 64 * 1024) return ccd::fail(CCD_E_INVALID, "mod channel hp=%zu exceeds the ring kernel's LDS", lds);
-    c->ring_lds_max = lds;
+    c->ring_stride = (max_hp + 15) / 16 * 16 + 1;
+    c->ring_lds = (size_t)((max_hp + 15) / 16 * 16) * c->ring_stride * sizeof(double);
+    if (c->ring_lds > 64 * 1024) return ccd::fail(CCD_E_INVALID, "ring LDS %zu bytes > 64 KB", c->ring_lds);
 
-    // tiles: update tiles (~256 elements), ladder tiles (64 rows), X tiles (64 rows of hh)
+    // ---- per-particle transposition plans between the two layouts:
+    // particle a's block holds the elements (a,b,i,j); its pphh rows are the
+    // rows (a,b) in channel order, its phhp rows the rows (a,i) in mod-channel order
+    std::vector<std::vector<DRow>> rpp(np), rph(np);
+    std::vector<int32_t> blk(np, 0), blk2(np, 0), row_start_pp(npp, -1), ph_start((size_t)np * nh, -1);
+    for (const DChan &ch : c->chans)
+        for (int AB = 0; AB < ch.pp; AB++) {
+            const int a = ppl[ch.pp_off + AB].x - nh;
+            row_start_pp[ch.pp_off + AB] = blk[a];
+            rpp[a].push_back(DRow{(int32_t)(ch.off_t + (int64_t)AB * ch.hh), ch.hh, blk[a], 0});
+            blk[a] += ch.hh;
+        }
+    for (const DMod &m : c->mods)
+        for (int q = 0; q < m.ph; q++) {
+            const int a = phl[m.ph_off + q].x - nh, i = phl[m.ph_off + q].y;
+            ph_start[(size_t)a * nh + i] = blk2[a];
+            rph[a].push_back(DRow{(int32_t)(m.off_ph + (int64_t)q * m.hp), m.hp, blk2[a], 0});
+            blk2[a] += m.hp;
+        }
+    std::vector<int32_t> tr_base(np + 1, 0), tr_ptr_pp(np + 1, 0), tr_ptr_ph(np + 1, 0);
+    int32_t max_blk = 0;
+    for (int a = 0; a < np; a++) {
+        if (blk[a] != blk2[a])
+            return ccd::fail(CCD_E_INVALID, "particle %d has %d pphh and %d phhp elements", a + nh, blk[a], blk2[a]);
+        max_blk = std::max(max_blk, blk[a]);
+        tr_base[a + 1] = tr_base[a] + blk[a];
+        tr_ptr_pp[a + 1] = tr_ptr_pp[a] + (int32_t)rpp[a].size();
+        tr_ptr_ph[a + 1] = tr_ptr_ph[a] + (int32_t)rph[a].size();
+    }
+    if (max_blk > 65535 || (size_t)max_blk * sizeof(double) > 64 * 1024)
+        return ccd::fail(CCD_E_INVALID, "a particle block of %d elements exceeds the transpose kernel's LDS", max_blk);
+    c->tr_lds = (size_t)std::max(max_blk, 1) * sizeof(double);
+    std::vector<DRow> rows_pp, rows_ph;
+    for (int a = 0; a < np; a++) {
+        rows_pp.insert(rows_pp.end(), rpp[a].begin(), rpp[a].end());
+        rows_ph.insert(rows_ph.end(), rph[a].begin(), rph[a].end());
+    }
+    std::vector<uint16_t> perm_ph(c->nt2), perm_pp(c->nt2);
+    std::vector<uint8_t> seen(c->nt2, 0);
+    for (const DChan &ch : c->chans)
+        for (int AB = 0; AB < ch.pp; AB++) {
+            const int a = ppl[ch.pp_off + AB].x, b = ppl[ch.pp_off + AB].y, pa = a - nh;
+            for (int IJ = 0; IJ < ch.hh; IJ++) {
+                const int i = hhl[ch.hh_off + IJ].x, j = hhl[ch.hh_off + IJ].y;
+                const int32_t m1 = mod_of_ph[(size_t)pa * nh + i], m2 = mod_of_hp[(size_t)j * np + (b - nh)];
+                if (m1 < 0 || m1 != m2)
+                    return ccd::fail(CCD_E_INVALID, "T2 element (%d,%d,%d,%d) has no consistent mod channel", a, b,
+                                     i, j);
+                const int32_t sph = ph_start[(size_t)pa * nh + i] + colidx[(size_t)j * np + (b - nh)];
+                const int32_t spp = row_start_pp[ch.pp_off + AB] + IJ;
+                if (sph < 0 || sph >= blk[pa] || spp < 0 || spp >= blk[pa] || seen[tr_base[pa] + sph])
+                    return ccd::fail(CCD_E_INVALID, "inconsistent pphh/phhp correspondence at (%d,%d,%d,%d)", a, b,
+                                     i, j);
+                seen[tr_base[pa] + sph] = 1;
+                perm_ph[tr_base[pa] + sph] = (uint16_t)spp;  // phhp slot <- pphh slot
+                perm_pp[tr_base[pa] + spp] = (uint16_t)sph;  // pphh slot <- phhp slot
+            }
+        }
+
+    // ---- tiles
     for (int k = 0; k < (int)c->chans.size(); k++) {
         const DChan &ch = c->chans[k];
-        const int rows = std::max(1, kThreads / ch.hh);
-        for (int r = 0; r < ch.pp; r += rows) c->utiles.push_back(DTile{k, r, std::min(rows, ch.pp - r), 0});
         for (int r = 0; r < ch.npk; r += LROWS) c->ltiles.push_back(DTile{k, r, std::min(LROWS, ch.npk - r), 0});
+        for (int r = 0; r < ch.hh; r += 16)
+            for (int q = 0; q < ch.hh; q += 16) c->xtiles.push_back(DTile{k, r, 16, q});
+        const int npairs = ch.npk + ch.ndiag, per = std::max(1, kThreads / ch.hh);
+        for (int r = 0; r < npairs; r += per) {
+            c->utiles.push_back(DTile{k, r, std::min(per, npairs - r), (int32_t)c->n_cp});
+            c->n_cp += ch.hh;
+        }
     }
-    for (int k = 0; k < (int)c->chans.size(); k++)
-        for (int r = 0; r < c->chans[k].hh; r += 16)
-            for (int q = 0; q < c->chans[k].hh; q += 16) c->xtiles.push_back(DTile{k, r, 16, q});
-    // one-body segments: chi_i over hh columns with first index i, psi_a over pp rows with first index a
-    std::vector<std::vector<DSeg>> sg((size_t)nh + np);
-    for (const DChan &ch : c->chans) {
-        for (int IJ = 0; IJ < ch.hh; IJ++)
-            sg[hhl[ch.hh_off + IJ].x].push_back(DSeg{ch.off_t + IJ, ch.hh, ch.pp});
-        for (int AB = 0; AB < ch.pp; AB++)
-            sg[(size_t)nh + (ppl[ch.pp_off + AB].x - nh)].push_back(DSeg{ch.off_t + (int64_t)AB * ch.hh, 1, ch.hh});
+    if (c->n_cp >= (int64_t)INT32_MAX) return ccd::fail(CCD_E_INVALID, "too many column partials");
+
+    // ---- reduction lists: chi_i over column partials, psi_a over row sums
+    std::vector<std::vector<int32_t>> lists((size_t)nsp);
+    for (const DTile &t : c->utiles) {
+        const DChan &ch = c->chans[t.chan];
+        for (int IJ = 0; IJ < ch.hh; IJ++) lists[hhl[ch.hh_off + IJ].x].push_back(t.aux + IJ);
     }
-    std::vector<int32_t> seg_ptr(1, 0);
-    std::vector<DSeg> segs;
-    for (auto &v : sg) {
-        segs.insert(segs.end(), v.begin(), v.end());
-        seg_ptr.push_back((int32_t)segs.size());
+    for (const DChan &ch : c->chans)
+        for (int AB = 0; AB < ch.pp; AB++) lists[ppl[ch.pp_off + AB].x].push_back(ch.pp_off + AB);
+    std::vector<int32_t> red_ptr(1, 0), red_idx;
+    for (auto &v : lists) {
+        red_idx.insert(red_idx.end(), v.begin(), v.end());
+        red_ptr.push_back((int32_t)red_idx.size());
     }
 
-    // --- device tables ---
+    // ---- upload tables
     int rc;
     if ((rc = upload(c, &c->d_chans, c->chans)) || (rc = upload(c, &c->d_mods, c->mods)) ||
         (rc = upload(c, &c->d_ltiles, c->ltiles)) || (rc = upload(c, &c->d_utiles, c->utiles)) ||
-        (rc = upload(c, &c->d_xtiles, c->xtiles)) ||
-        (rc = upload(c, &c->d_ppl, ppl)) || (rc = upload(c, &c->d_hhl, hhl)) ||
-        (rc = upload(c, &c->d_phl, phl)) || (rc = upload(c, &c->d_hpl, hpl)) ||
-        (rc = upload(c, &c->d_pp_swap, pp_swap)) || (rc = upload(c, &c->d_hh_swap, hh_swap)) ||
-        (rc = upload(c, &c->d_pkc, pkc)) || (rc = upload(c, &c->d_prow, prow)) ||
-        (rc = upload(c, &c->d_prow_sw, prow_sw)) ||
-        (rc = upload(c, &c->d_rowbase, rowbase)) || (rc = upload(c, &c->d_colidx, colidx)) ||
-        (rc = upload(c, &c->d_seg_ptr, seg_ptr)) || (rc = upload(c, &c->d_segs, segs)))
+        (rc = upload(c, &c->d_xtiles, c->xtiles)) || (rc = upload(c, &c->d_ppl, ppl)) ||
+        (rc = upload(c, &c->d_hhl, hhl)) || (rc = upload(c, &c->d_phl, phl)) || (rc = upload(c, &c->d_hpl, hpl)) ||
+        (rc = upload(c, &c->d_hh_swap, hh_swap)) || (rc = upload(c, &c->d_prow, prow)) ||
+        (rc = upload(c, &c->d_prow_sw, prow_sw)) || (rc = upload(c, &c->d_red_ptr, red_ptr)) ||
+        (rc = upload(c, &c->d_red_idx, red_idx)) || (rc = upload(c, &c->d_tr_ptr_pp, tr_ptr_pp)) ||
+        (rc = upload(c, &c->d_tr_ptr_ph, tr_ptr_ph)) || (rc = upload(c, &c->d_tr_base, tr_base)) ||
+        (rc = upload(c, &c->d_tr_rows_pp, rows_pp)) || (rc = upload(c, &c->d_tr_rows_ph, rows_ph)) ||
+        (rc = upload(c, &c->d_perm_ph, perm_ph)) || (rc = upload(c, &c->d_perm_pp, perm_pp)))
         return rc;
     std::vector<double> eps(p->eps, p->eps + nsp);
     if ((rc = upload(c, &c->d_eps, eps))) return rc;
-    if ((rc = dalloc(c, &c->d_onebody, (size_t)nsp))) return rc;
-    for (int k = 0; k < 2; k++)
-        if ((rc = dalloc(c, &c->d_t[k], (size_t)c->nt2))) return rc;
-    if ((rc = dalloc(c, &c->d_d, (size_t)c->n_tp)) || (rc = dalloc(c, &c->d_tph, (size_t)c->nt2)) ||
-        (rc = dalloc(c, &c->d_r, (size_t)c->nt2)) || (rc = dalloc(c, &c->d_w, (size_t)c->nt2)) ||
-        (rc = dalloc(c, &c->d_x, (size_t)c->n_hhhh)) || (rc = dalloc(c, &c->d_vt, (size_t)c->nt2)) ||
+    const size_t nt = (size_t)c->nt2;
+    if ((rc = dalloc(c, &c->d_onebody, (size_t)nsp)) || (rc = dalloc(c, &c->d_t[0], nt)) ||
+        (rc = dalloc(c, &c->d_t[1], nt)) || (rc = dalloc(c, &c->d_tph, nt)) || (rc = dalloc(c, &c->d_r, nt)) ||
+        (rc = dalloc(c, &c->d_g, nt)) || (rc = dalloc(c, &c->d_vt, nt)) || (rc = dalloc(c, &c->d_va, nt)) ||
+        (rc = dalloc(c, &c->d_dp, (size_t)c->n_tp)) || (rc = dalloc(c, &c->d_tas, (size_t)c->n_tp)) ||
+        (rc = dalloc(c, &c->d_vtp, (size_t)c->n_tp)) || (rc = dalloc(c, &c->d_x, (size_t)c->n_hhhh)) ||
         (rc = dalloc(c, &c->d_vh, (size_t)c->n_hhhh)) || (rc = dalloc(c, &c->d_vp, (size_t)c->n_vp)) ||
-        (rc = dalloc(c, &c->d_vtp, (size_t)c->n_tp)) || (rc = dalloc(c, &c->d_tas, (size_t)c->n_tp)) ||
-        (rc = dalloc(c, &c->d_va, (size_t)c->nt2)) || (rc = dalloc(c, &c->d_vb, (size_t)c->n_b)) ||
-        (rc = dalloc(c, &c->d_partials, std::max(c->utiles.size(), (size_t)1))) ||
-        (rc = dalloc(c, &c->d_scalar, 4)))
+        (rc = dalloc(c, &c->d_vb, (size_t)c->n_b)) || (rc = dalloc(c, &c->d_rs, (size_t)std::max<int64_t>(npp, 1))) ||
+        (rc = dalloc(c, &c->d_cp, (size_t)std::max<int64_t>(c->n_cp, 1))) ||
+        (rc = dalloc(c, &c->d_partials, std::max(c->utiles.size(), (size_t)1))) || (rc = dalloc(c, &c->d_scalar, 4)))
         return rc;
     HIP_TRY(hipHostMalloc((void **)&c->h_scalar, 4 * sizeof(double), hipHostMallocDefault));
 
-    // --- interaction ---
+    // ---- interaction
     if (p->gen_kind != CCD_GEN_NONE) {
         std::vector<int32_t> qn(p->qn, p->qn + 5 * (size_t)nsp);
         if ((rc = upload(c, &c->d_qn, qn))) return rc;
         Gen G{p->gen_kind, p->gen_nmax, p->gen_g, c->d_qn, nullptr};
         if (p->gen_kind == CCD_GEN_ELECTRON_GAS) {
-            // every momentum transfer of the basis must index the table
             for (int s = 0; s < nsp; s++)
                 for (int k = 0; k < 3; k++)
                     if (std::abs(p->qn[5 * s + k]) > p->gen_nmax)
@@ -980,7 +535,6 @@ int build(ccd_ctx *c, const ccd_problem *p) {
                                c->d_mods, c->d_phl, c->d_hpl, G, c->d_va, c->d_vb);
         if ((rc = check_launch("generate V"))) return rc;
     } else {
-        // caller V chains: copy the active blocks
         std::vector<int64_t> o_hhhh(p->n_chan + 1, 0), o_pppp(p->n_chan + 1, 0), o_b(p->n_mod + 1, 0);
         for (int k = 0; k < p->n_chan; k++) {
             o_hhhh[k + 1] = o_hhhh[k] + (int64_t)p->hh_dim[k] * p->hh_dim[k];
@@ -988,7 +542,7 @@ int build(ccd_ctx *c, const ccd_problem *p) {
         }
         for (int k = 0; k < p->n_mod; k++) o_b[k + 1] = o_b[k] + (int64_t)p->hp_dim[k] * p->hp_dim[k];
         int ci = 0;
-        std::vector<double> blk;
+        std::vector<double> blkv;
         for (int k = 0; k < p->n_chan; k++) {
             if (!(p->pp_dim[k] > 0 && p->hh_dim[k] > 0)) continue;
             const DChan &ch = c->chans[ci++];
@@ -1004,21 +558,21 @@ int build(ccd_ctx *c, const ccd_problem *p) {
                         return ccd::fail(CCD_E_INVALID, "V^pppp of channel %d is not antisymmetric", k);
                 }
             }
-            blk.assign((size_t)ch.npk * ch.ldv, 0.0);
+            blkv.assign((size_t)ch.npk * ch.ldv, 0.0);
             for (int u = 0; u < ch.npk; u++)
                 for (int w = 0; w < ch.npk; w++)
-                    blk[(size_t)u * ch.ldv + w] = V[(size_t)prow[ch.pr_off + u] * P + prow[ch.pr_off + w]];
-            HIP_TRY(hipMemcpy(c->d_vp + ch.off_vp, blk.data(), sizeof(double) * blk.size(), hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(c->d_vh + ch.off_hhhh, p->v_hhhh + o_hhhh[k],
-                              sizeof(double) * ch.hh * (size_t)ch.hh, hipMemcpyHostToDevice));
+                    blkv[(size_t)u * ch.ldv + w] = V[(size_t)prow[ch.pr_off + u] * P + prow[ch.pr_off + w]];
+            HIP_TRY(hipMemcpy(c->d_vp + ch.off_vp, blkv.data(), sizeof(double) * blkv.size(), hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(c->d_vh + ch.off_hhhh, p->v_hhhh + o_hhhh[k], sizeof(double) * ch.hh * (size_t)ch.hh,
+                              hipMemcpyHostToDevice));
         }
-        // V^hhpp has the pphh element count and offsets; stage through d_w, then transpose
-        HIP_TRY(hipMemcpy(c->d_w, p->v_hhpp, sizeof(double) * (size_t)c->nt2, hipMemcpyHostToDevice));
+        // V^hhpp has the pphh element count and offsets; stage through d_g, then transpose
+        HIP_TRY(hipMemcpy(c->d_g, p->v_hhpp, sizeof(double) * nt, hipMemcpyHostToDevice));
         if (!c->chans.empty())
-            hipLaunchKernelGGL(k_transpose_hhpp, dim3((unsigned)c->chans.size()), dim3(kThreads), 0,
-                               c->stream, c->d_chans, c->d_w, c->d_vt);
+            hipLaunchKernelGGL(k_transpose_hhpp, dim3((unsigned)c->chans.size()), dim3(kThreads), 0, c->stream,
+                               c->d_chans, c->d_g, c->d_vt);
         if ((rc = check_launch("k_transpose_hhpp"))) return rc;
-        HIP_TRY(hipMemcpy(c->d_va, p->v_modA, sizeof(double) * (size_t)c->nt2, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(c->d_va, p->v_modA, sizeof(double) * nt, hipMemcpyHostToDevice));
         int mi = 0;
         for (int k = 0; k < p->n_mod; k++) {
             if (!(p->ph_dim[k] > 0 && p->hp_dim[k] > 0)) continue;
@@ -1037,29 +591,26 @@ int build(ccd_ctx *c, const ccd_problem *p) {
 }
 
 int zero_state(ccd_ctx *c) {
-    for (int k = 0; k < 2; k++) HIP_TRY(hipMemsetAsync(c->d_t[k], 0, sizeof(double) * c->nt2, c->stream));
-    HIP_TRY(hipMemsetAsync(c->d_tph, 0, sizeof(double) * c->nt2, c->stream));
+    const size_t nt = sizeof(double) * (size_t)c->nt2;
+    HIP_TRY(hipMemsetAsync(c->d_t[0], 0, nt, c->stream));
+    HIP_TRY(hipMemsetAsync(c->d_t[1], 0, nt, c->stream));
+    HIP_TRY(hipMemsetAsync(c->d_tph, 0, nt, c->stream));
+    HIP_TRY(hipMemsetAsync(c->d_tas, 0, sizeof(double) * (size_t)std::max<int64_t>(c->n_tp, 1), c->stream));
     HIP_TRY(hipMemsetAsync(c->d_onebody, 0, sizeof(double) * c->nsp, c->stream));
     c->cur = 0;
     return CCD_OK;
 }
 
-// Load host amplitudes as the current T and derive T^ph, chi, psi from them.
+// Load host amplitudes as the current T and derive T^ph, the antisymmetrised
+// packed rows, chi and psi from them.
 int set_state(ccd_ctx *c, const double *t2) {
-    HIP_TRY(hipMemcpyAsync(c->d_t[c->cur], t2, sizeof(double) * c->nt2, hipMemcpyHostToDevice, c->stream));
-    UpdArgs a = upd_args(c, c->d_t[c->cur], nullptr, 0.0);
-    if (!c->utiles.empty()) {
-        hipLaunchKernelGGL(k_remap_ph, dim3((unsigned)c->utiles.size()), dim3(kThreads), 0, c->stream, a);
-        int rc = check_launch("k_remap_ph");
-        if (rc) return rc;
-    }
-    hipLaunchKernelGGL(k_hadamard, dim3(1024), dim3(kThreads), 0, c->stream, c->d_vt, c->d_t[c->cur], c->d_w,
-                       c->nt2);
-    int rc = check_launch("k_hadamard");
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_onebody, dim3((unsigned)c->nsp), dim3(kThreads), 0, c->stream, c->d_seg_ptr,
-                       c->d_segs, c->d_w, c->d_onebody);
-    return check_launch("k_onebody");
+    double *t = c->d_t[c->cur];
+    HIP_TRY(hipMemcpyAsync(t, t2, sizeof(double) * c->nt2, hipMemcpyHostToDevice, c->stream));
+    int rc;
+    if ((rc = launch_transpose(c, true, t, c->d_tph)) || (rc = launch_antisym(c, t)) ||
+        (rc = launch_update<false>(c, t, nullptr, 0.0)) || (rc = launch_reduce(c)))
+        return rc;
+    return CCD_OK;
 }
 
 }  // namespace
@@ -1074,15 +625,14 @@ int ccd_ctx_create(const ccd_problem *prob, int precision, ccd_ctx **out) {
     *out = nullptr;
     int rc = validate(prob);
     if (rc) return rc;
-    if (precision != CCD_PREC_FP64)
-        return ccd::fail(CCD_E_INVALID, "precision %d is not available", precision);
+    if (precision != CCD_PREC_FP64) return ccd::fail(CCD_E_INVALID, "precision %d is not available", precision);
     int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
-        return ccd::fail(CCD_E_HIP, "no HIP device available");
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return ccd::fail(CCD_E_HIP, "no HIP device available");
     ccd_ctx *c = new (std::nothrow) ccd_ctx();
     if (!c) return ccd::fail(CCD_E_NOMEM, "out of host memory");
     c->precision = precision;
-    if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipGetDevice(&c->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return ccd::fail(CCD_E_HIP, "could not create a HIP stream");
     }
@@ -1138,12 +688,12 @@ int ccd_mbpt2(ccd_ctx *c, double *corr) {
         *corr = 0.0;
         return CCD_OK;
     }
-    hipLaunchKernelGGL(k_mbpt2, dim3((unsigned)c->utiles.size()), dim3(kThreads), 0, c->stream, c->d_utiles,
-                       c->d_chans, c->d_ppl, c->d_hhl, c->d_eps, c->d_vt, c->d_partials);
+    UpdArgs a = upd_args(c, nullptr, nullptr, 0.0);
+    hipLaunchKernelGGL(k_mbpt2, dim3((unsigned)c->utiles.size()), dim3(kThreads), 0, c->stream, a, c->d_partials);
     int rc = check_launch("k_mbpt2");
     if (rc) return rc;
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kThreads), 0, c->stream, c->d_partials,
-                       (int)c->utiles.size(), 0.25, c->d_scalar);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kThreads), 0, c->stream, c->d_partials, (int)c->utiles.size(),
+                       0.25, c->d_scalar);
     if ((rc = check_launch("k_sum_partials"))) return rc;
     return read_scalar(c, corr);
 }
@@ -1217,12 +767,13 @@ int ccd_ladder(ccd_ctx *c, const double *t, double *d) {
     double *dt = c->d_t[1 - c->cur];  // scratch: the non-current T buffer
     HIP_TRY(hipMemcpyAsync(dt, t, sizeof(double) * c->nt2, hipMemcpyHostToDevice, c->stream));
     int rc;
-    if ((rc = launch_ladder_x(c, dt)) || (rc = launch_ladder_d(c, dt))) return rc;
-    // expand D_p to the ordered pphh layout through the sign codes
-    UpdArgs a = upd_args(c, nullptr, c->d_w, 0.0);
-    hipLaunchKernelGGL(k_expand_d, dim3((unsigned)c->utiles.size()), dim3(kThreads), 0, c->stream, a);
-    if ((rc = check_launch("k_expand_d"))) return rc;
-    HIP_TRY(hipMemcpyAsync(d, c->d_w, sizeof(double) * c->nt2, hipMemcpyDeviceToHost, c->stream));
+    if ((rc = launch_antisym(c, dt)) || (rc = launch_ladder_x(c)) || (rc = launch_ladder_v(c))) return rc;
+    if (!c->utiles.empty()) {
+        UpdArgs a = upd_args(c, nullptr, nullptr, 0.0);
+        hipLaunchKernelGGL(k_expand_d, dim3((unsigned)c->utiles.size()), dim3(kThreads), 0, c->stream, a, c->d_g);
+        if ((rc = check_launch("k_expand_d"))) return rc;
+    }
+    HIP_TRY(hipMemcpyAsync(d, c->d_g, sizeof(double) * c->nt2, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return CCD_OK;
 }
@@ -1243,12 +794,8 @@ int ccd_remap_ph(ccd_ctx *c, const double *t, double *tph) {
     if (!c || !t || !tph) return ccd::fail(CCD_E_INVALID, "ccd_remap_ph: null argument");
     double *dt = c->d_t[1 - c->cur];
     HIP_TRY(hipMemcpyAsync(dt, t, sizeof(double) * c->nt2, hipMemcpyHostToDevice, c->stream));
-    UpdArgs a = upd_args(c, dt, nullptr, 0.0);
-    if (!c->utiles.empty()) {
-        hipLaunchKernelGGL(k_remap_ph, dim3((unsigned)c->utiles.size()), dim3(kThreads), 0, c->stream, a);
-        int rc = check_launch("k_remap_ph");
-        if (rc) return rc;
-    }
+    int rc = launch_transpose(c, true, dt, c->d_tph);
+    if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(tph, c->d_tph, sizeof(double) * c->nt2, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return CCD_OK;

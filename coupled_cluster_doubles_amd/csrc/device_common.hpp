@@ -1,0 +1,95 @@
+// Device-side types and helpers shared by the CCD kernels (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "ccd_amd.h"
+
+constexpr int kThreads = 256;   // every kernel runs 4 wave64s per block
+
+struct DChan {          // active direct channel (pp > 0 and hh > 0)
+    int64_t off_t;      // block offset in the pphh layout
+    int64_t off_vp;     // packed V^pppp block: npk x ldv, rows/cols = pairs a<b
+    int64_t off_tp;     // packed-row blocks (npk x hh): antisymmetrised T, D_p, V^hhpp rows
+    int64_t off_hhhh;   // hh x hh block of V^hhhh / X
+    int32_t pp, hh;
+    int32_t pp_off, hh_off;  // first pair in the pp / hh pair lists
+    int32_t npk, ldv;        // number of pairs a<b; leading dimension (multiple of 8)
+    int32_t pr_off, ndiag;   // first entry in the row-pair lists; number of pairs (a,a)
+};
+
+struct DMod {           // active mod channel (ph > 0 and hp > 0)
+    int64_t off_ph;     // block offset in the phhp layout (also V^A, hp x ph)
+    int64_t off_b;      // offset of its hp x hp block of V^B
+    int32_t ph, hp;
+    int32_t ph_off, hp_off;
+};
+
+struct DTile {          // a run of rows (or row pairs) of one active direct channel
+    int32_t chan, row0, nrows, aux;
+};
+
+struct DRow {           // one contiguous row of a layout, for the per-particle transposes
+    int32_t goff;       // element offset in the layout
+    int32_t len;
+    int32_t loff;       // offset in the particle's LDS block
+    int32_t pad;
+};
+
+struct Gen {            // on-device interaction
+    int kind;
+    int nmax;
+    double g;
+    const int32_t *qn;
+    const double *table;
+};
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
+// Fixed-order block reduction: wave butterfly, then the four wave sums in order.
+// Every thread of the block must call it; the result is valid on thread 0.
+__device__ __forceinline__ double block_sum(double v, double *red) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += red[w];
+    return s;
+}
+
+// Antisymmetrised <pq||rs> generated from quantum numbers; bitwise equal to
+// the host model (pairing: src/PairingSPBasis.cpp:127-141; electron gas:
+// src/ElectronGasSPBasis.cpp:419-471 with the direct term tabulated).
+__device__ inline double gen_tbme(const Gen &G, int p, int q, int r, int s) {
+    const int32_t *a = G.qn + 5 * p, *b = G.qn + 5 * q, *c = G.qn + 5 * r, *d = G.qn + 5 * s;
+    if (G.kind == CCD_GEN_PAIRING) {
+        double v = 0.0;
+        if (a[0] == b[0] && c[0] == d[0] && a[1] == -b[1] && -c[1] == d[1]) {
+            v = -0.5 * G.g;
+            if (a[1] == -c[1]) v = -v;
+        }
+        return v;
+    }
+    const bool cons = a[0] + b[0] == c[0] + d[0] && a[1] + b[1] == c[1] + d[1] && a[2] + b[2] == c[2] + d[2];
+    if (!cons) return 0.0;
+    const int m2 = 2 * G.nmax, w = 4 * G.nmax + 1;
+    double d1 = 0.0, d2 = 0.0;
+    if (a[3] == c[3] && b[3] == d[3])
+        d1 = G.table[((a[0] - c[0] + m2) * w + (a[1] - c[1] + m2)) * w + (a[2] - c[2] + m2)];
+    if (a[3] == d[3] && b[3] == c[3])
+        d2 = G.table[((a[0] - d[0] + m2) * w + (a[1] - d[1] + m2)) * w + (a[2] - d[2] + m2)];
+    return d1 - d2;
+}
+
+// v_mfma_f64_16x16x4f64: A is 16x4 (lane l: row l%16, k-slot l/16), B is 4x16
+// (lane l: k-slot l/16, column l%16), C is 16x16 with register q of lane l
+// holding row 4q + l/16, column l%16.
+__device__ __forceinline__ dbl4 mfma16(double a, double b, dbl4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}

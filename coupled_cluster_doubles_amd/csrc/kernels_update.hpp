@@ -1,0 +1,242 @@
+// Amplitude update, remaps and reductions:
+//   calcChannelT2_pphh (src/calc.cpp:368-421)              -> k_update
+//   the one-body terms (calc.cpp:9-25, 73-89)              -> chi/psi from k_reduce, applied in k_update
+//   mapToDistributed(ai_jb) / mapFromDistributed x 4       -> k_transpose (per particle, LDS-staged)
+//   (src/Chain.hpp:710-802)
+//   the energy sum (calc.cpp:417-420, main.cpp:667-668)     -> k_reduce + k_energy
+//   MBPT2ChannelCorr (src/MBPT2Corr.cpp:6-28)              -> k_mbpt2
+#pragma once
+
+#include "device_common.hpp"
+
+// Row pairs: the first npk entries of a channel's pair list are the packed rows
+// u (pair a<b, partner row (b,a)); the next ndiag entries are rows (a,a),
+// paired with themselves.  An update tile is a run of pairs of one channel
+// (row0 = first pair, nrows = number of pairs, aux = offset of its column
+// partial sums).
+struct UpdArgs {
+    const DTile *tiles;
+    const DChan *chans;
+    const int2 *ppl, *hhl;
+    const int32_t *prow, *prow_sw;   // row of (a,b) and of (b,a) per pair
+    const int32_t *hh_swap;          // index of (j,i) within the channel
+    const double *eps, *chi, *psi;   // chi [nh], psi [np]
+    const double *vt;                // <ij||ab> in the pphh layout
+    const double *dp;                // ladder D on packed rows
+    const double *g;                 // ring R gathered to the pphh layout: g[(a,b),(i,j)] = R[(a,i),(j,b)]
+    const double *t_old;
+    double *t_new, *tas;             // new amplitudes; their (a,b)-antisymmetric packed rows
+    double *rs;                      // W = V o T row sums, indexed pp_off + row
+    double *cp;                      // W column partial sums per tile, indexed aux + IJ
+    double mixing;
+    int nh;
+};
+
+// UPDATE = true: one Jacobi step for both rows of each pair; UPDATE = false:
+// only the W sums of the current amplitudes (to derive chi, psi, E from them).
+template <bool UPDATE>
+__global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
+    __shared__ double w1s[kThreads], w2s[kThreads];
+    const DTile tl = a.tiles[blockIdx.x];
+    const DChan ch = a.chans[tl.chan];
+    const int H = ch.hh, n = tl.nrows * H;
+    const int l = threadIdx.x;
+    if (l < n) {
+        const int pr = tl.row0 + l / H, IJ = l % H;
+        const int JI = a.hh_swap[ch.hh_off + IJ];
+        const int r1 = a.prow[ch.pr_off + pr], r2 = a.prow_sw[ch.pr_off + pr];
+        const bool diag = r1 == r2;
+        const int64_t b1 = ch.off_t + (int64_t)r1 * H, b2 = ch.off_t + (int64_t)r2 * H;
+        const double v1 = a.vt[b1 + IJ], v2 = a.vt[b2 + IJ];
+        const double t1 = a.t_old[b1 + IJ], t2 = a.t_old[b2 + IJ];
+        double w1, w2;
+        if (UPDATE) {
+            const int2 ab = a.ppl[ch.pp_off + r1], ij = a.hhl[ch.hh_off + IJ];
+            const int ia = ab.x, ib = ab.y, ii = ij.x, jj = ij.y;
+            const double t1s = a.t_old[b1 + JI], t2s = a.t_old[b2 + JI];
+            const double g1 = a.g[b1 + IJ], g1s = a.g[b1 + JI], g2 = a.g[b2 + IJ], g2s = a.g[b2 + JI];
+            const double d = diag ? 0.0 : a.dp[ch.off_tp + (int64_t)pr * H + IJ];
+            const double ci = 0.5 * a.chi[ii], cj = 0.5 * a.chi[jj];
+            const double pa = -0.5 * a.psi[ia - a.nh], pb = -0.5 * a.psi[ib - a.nh];
+            // element (a,b,i,j), in the order of src/calc.cpp:393-408
+            double x1 = v1;
+            x1 += d;
+            x1 += g1;
+            x1 -= g1s;
+            x1 -= g2;
+            x1 += g2s;
+            x1 += ci * t1s;
+            x1 -= cj * t1;
+            x1 += pb * t1;
+            x1 -= pa * t2;
+            x1 = x1 / (a.eps[ii] + a.eps[jj] - a.eps[ia] - a.eps[ib]);
+            x1 = a.mixing * x1 + (1. - a.mixing) * t1;
+            a.t_new[b1 + IJ] = x1;
+            w1 = v1 * x1;
+            w2 = 0.0;
+            if (!diag) {
+                // element (b,a,i,j)
+                double x2 = v2;
+                x2 -= d;
+                x2 += g2;
+                x2 -= g2s;
+                x2 -= g1;
+                x2 += g1s;
+                x2 += ci * t2s;
+                x2 -= cj * t2;
+                x2 += pa * t2;
+                x2 -= pb * t1;
+                x2 = x2 / (a.eps[ii] + a.eps[jj] - a.eps[ib] - a.eps[ia]);
+                x2 = a.mixing * x2 + (1. - a.mixing) * t2;
+                a.t_new[b2 + IJ] = x2;
+                a.tas[ch.off_tp + (int64_t)pr * H + IJ] = 0.5 * (x1 - x2);
+                w2 = v2 * x2;
+            }
+        } else {
+            w1 = v1 * t1;
+            w2 = diag ? 0.0 : v2 * t2;
+        }
+        w1s[l] = w1;
+        w2s[l] = w2;
+    }
+    __syncthreads();
+    // deterministic row sums (one thread per pair) and column partials (one per column)
+    if (l < tl.nrows) {
+        const int pr = tl.row0 + l;
+        double s1 = 0.0, s2 = 0.0;
+        for (int q = 0; q < H; q++) {
+            s1 += w1s[l * H + q];
+            s2 += w2s[l * H + q];
+        }
+        const int r1 = a.prow[ch.pr_off + pr], r2 = a.prow_sw[ch.pr_off + pr];
+        a.rs[ch.pp_off + r1] = s1;
+        if (r1 != r2) a.rs[ch.pp_off + r2] = s2;
+    }
+    if (l < H) {
+        double s = 0.0;
+        for (int q = 0; q < tl.nrows; q++) s += w1s[q * H + l] + w2s[q * H + l];
+        a.cp[tl.aux + l] = s;
+    }
+}
+
+// Per-particle transposition between the pphh and phhp layouts (both hold
+// every amplitude once; the block of particle a holds the elements whose
+// first particle index is a).  Rows of `src` are loaded contiguously into LDS,
+// rows of `dst` are written contiguously, dst position p taking LDS slot
+// perm[base + p].  With src = pphh it is the reference's
+// mapToDistributed(ai_jb); with src = phhp it gathers R to the pphh layout.
+__global__ __launch_bounds__(kThreads) void k_transpose(const int32_t *__restrict__ ptr_src,
+                                                        const DRow *__restrict__ rows_src,
+                                                        const int32_t *__restrict__ ptr_dst,
+                                                        const DRow *__restrict__ rows_dst,
+                                                        const int32_t *__restrict__ base,
+                                                        const uint16_t *__restrict__ perm,
+                                                        const double *__restrict__ src,
+                                                        double *__restrict__ dst) {
+    extern __shared__ double buf[];
+    const int a = blockIdx.x;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int r = ptr_src[a] + wave; r < ptr_src[a + 1]; r += kThreads / 64) {
+        const DRow rr = rows_src[r];
+        for (int c = lane; c < rr.len; c += 64) buf[rr.loff + c] = src[rr.goff + c];
+    }
+    __syncthreads();
+    const int64_t b = base[a];
+    for (int r = ptr_dst[a] + wave; r < ptr_dst[a + 1]; r += kThreads / 64) {
+        const DRow rr = rows_dst[r];
+        for (int c = lane; c < rr.len; c += 64) dst[rr.goff + c] = buf[perm[b + rr.loff + c]];
+    }
+}
+
+// chi_i (blocks 0..nh-1) and psi_a (blocks nh..nsp-1) as fixed-order sums of
+// listed column partials / row sums.
+__global__ __launch_bounds__(kThreads) void k_reduce(const int32_t *__restrict__ ptr,
+                                                     const int32_t *__restrict__ idx, int nh,
+                                                     const double *__restrict__ cp,
+                                                     const double *__restrict__ rs,
+                                                     double *__restrict__ out) {
+    __shared__ double red[kThreads / 64];
+    const int k = blockIdx.x;
+    const double *srcv = k < nh ? cp : rs;
+    double s = 0.0;
+    for (int q = ptr[k] + threadIdx.x; q < ptr[k + 1]; q += kThreads) s += srcv[idx[q]];
+    const double b = block_sum(s, red);
+    if (threadIdx.x == 0) out[k] = b;
+}
+
+// E = 1/4 sum_i chi_i = 1/4 sum V o T (every element has one first hole index).
+__global__ __launch_bounds__(kThreads) void k_energy(const double *__restrict__ chi, int nh,
+                                                     double *__restrict__ out) {
+    __shared__ double red[kThreads / 64];
+    double s = 0.0;
+    for (int k = threadIdx.x; k < nh; k += kThreads) s += chi[k];
+    const double b = block_sum(s, red);
+    if (threadIdx.x == 0) out[0] = 0.25 * b;
+}
+
+// MBPT2 terms v^2 / denominator over the pair tiles, per-block partial sums.
+__global__ __launch_bounds__(kThreads) void k_mbpt2(UpdArgs a, double *__restrict__ partials) {
+    __shared__ double red[kThreads / 64];
+    const DTile tl = a.tiles[blockIdx.x];
+    const DChan ch = a.chans[tl.chan];
+    const int H = ch.hh, l = threadIdx.x;
+    double s = 0.0;
+    if (l < tl.nrows * H) {
+        const int pr = tl.row0 + l / H, IJ = l % H;
+        const int r1 = a.prow[ch.pr_off + pr], r2 = a.prow_sw[ch.pr_off + pr];
+        const int2 ab = a.ppl[ch.pp_off + r1], ij = a.hhl[ch.hh_off + IJ];
+        const double v1 = a.vt[ch.off_t + (int64_t)r1 * H + IJ];
+        s = v1 * v1 / (a.eps[ij.x] + a.eps[ij.y] - a.eps[ab.x] - a.eps[ab.y]);
+        if (r1 != r2) {
+            const double v2 = a.vt[ch.off_t + (int64_t)r2 * H + IJ];
+            s += v2 * v2 / (a.eps[ij.x] + a.eps[ij.y] - a.eps[ab.y] - a.eps[ab.x]);
+        }
+    }
+    const double b = block_sum(s, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = b;
+}
+
+// out = scale * sum(partials[0..n)) in a fixed order (one block).
+__global__ __launch_bounds__(kThreads) void k_sum_partials(const double *__restrict__ partials, int n,
+                                                           double scale, double *__restrict__ out) {
+    __shared__ double red[kThreads / 64];
+    double s = 0.0;
+    for (int k = threadIdx.x; k < n; k += kThreads) s += partials[k];
+    const double b = block_sum(s, red);
+    if (threadIdx.x == 0) out[0] = scale * b;
+}
+
+// Antisymmetrised packed rows of caller amplitudes: tas[u] = (t[(a,b)] - t[(b,a)]) / 2.
+__global__ __launch_bounds__(kThreads) void k_antisym(const DTile *__restrict__ tiles,
+                                                      const DChan *__restrict__ chans,
+                                                      const int32_t *__restrict__ prow,
+                                                      const int32_t *__restrict__ prow_sw,
+                                                      const double *__restrict__ t,
+                                                      double *__restrict__ tas) {
+    const DTile tl = tiles[blockIdx.x];
+    const DChan ch = chans[tl.chan];
+    const int H = ch.hh;
+    for (int l = threadIdx.x; l < tl.nrows * H; l += kThreads) {
+        const int u = tl.row0 + l / H, ij = l % H;
+        const double x = t[ch.off_t + (int64_t)prow[ch.pr_off + u] * H + ij];
+        const double y = t[ch.off_t + (int64_t)prow_sw[ch.pr_off + u] * H + ij];
+        tas[ch.off_tp + (int64_t)u * H + ij] = 0.5 * (x - y);
+    }
+}
+
+// D in the ordered pphh layout from the packed ladder output (ladder entry point).
+__global__ __launch_bounds__(kThreads) void k_expand_d(UpdArgs a, double *__restrict__ d) {
+    const DTile tl = a.tiles[blockIdx.x];
+    const DChan ch = a.chans[tl.chan];
+    const int H = ch.hh, l = threadIdx.x;
+    if (l >= tl.nrows * H) return;
+    const int pr = tl.row0 + l / H, IJ = l % H;
+    const int r1 = a.prow[ch.pr_off + pr], r2 = a.prow_sw[ch.pr_off + pr];
+    if (r1 == r2) {
+        d[ch.off_t + (int64_t)r1 * H + IJ] = 0.0;
+    } else {
+        const double v = a.dp[ch.off_tp + (int64_t)pr * H + IJ];
+        d[ch.off_t + (int64_t)r1 * H + IJ] = v;
+        d[ch.off_t + (int64_t)r2 * H + IJ] = -v;
+    }
+}
