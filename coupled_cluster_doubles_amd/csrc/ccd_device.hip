@@ -77,19 +77,19 @@ struct ccd_ctx {
     int64_t bytes = 0;
     std::vector<DChan> chans;
     std::vector<DMod> mods;
-    std::vector<DTile> ltiles, utiles, xtiles;
+    std::vector<DTile> rtiles, ltiles, utiles, xtiles;  // packed-row, ladder, update, X tiles
+    int lt_off[NTMAX + 1] = {0};  // ladder tiles grouped by 16-column tile count: [lt_off[n-1], lt_off[n])
     int64_t n_cp = 0;            // column-partial slots
     int ring_stride = 0;         // LDS row stride of the ring kernel
     size_t ring_lds = 0, tr_lds = 0;
     // device tables
     DChan *d_chans = nullptr;
     DMod *d_mods = nullptr;
-    DTile *d_ltiles = nullptr, *d_utiles = nullptr, *d_xtiles = nullptr;
+    DTile *d_rtiles = nullptr, *d_ltiles = nullptr, *d_utiles = nullptr, *d_xtiles = nullptr;
     int2 *d_ppl = nullptr, *d_hhl = nullptr, *d_phl = nullptr, *d_hpl = nullptr;
     int32_t *d_hh_swap = nullptr, *d_prow = nullptr, *d_prow_sw = nullptr;
     int32_t *d_red_ptr = nullptr, *d_red_idx = nullptr;
-    int32_t *d_tr_ptr_pp = nullptr, *d_tr_ptr_ph = nullptr, *d_tr_base = nullptr;
-    DRow *d_tr_rows_pp = nullptr, *d_tr_rows_ph = nullptr;
+    int32_t *d_tr_base = nullptr, *d_addr_pp = nullptr, *d_addr_ph = nullptr;
     uint16_t *d_perm_ph = nullptr, *d_perm_pp = nullptr;
     // device state and interaction
     double *d_eps = nullptr, *d_onebody = nullptr;  // chi [nh] then psi [np]
@@ -103,7 +103,7 @@ struct ccd_ctx {
     double *h_scalar = nullptr;  // pinned
     int32_t *d_qn = nullptr;
     double *d_table = nullptr;
-    int64_t n_vp = 0, n_tp = 0, n_hhhh = 0, n_b = 0;
+    int64_t n_vp = 0, n_tp = 0, n_dp = 0, n_hhhh = 0, n_b = 0;
     double flops = 0, flops_pppp = 0;
     std::vector<void *> allocs;
 };
@@ -167,33 +167,43 @@ int launch_ring(ccd_ctx *c) {
 int launch_transpose(ccd_ctx *c, bool to_ph, const double *src, double *dst) {
     if (c->np == 0 || c->nt2 == 0) return CCD_OK;
     if (to_ph)
-        hipLaunchKernelGGL(k_transpose, dim3((unsigned)c->np), dim3(kThreads), c->tr_lds, c->stream, c->d_tr_ptr_pp,
-                           c->d_tr_rows_pp, c->d_tr_ptr_ph, c->d_tr_rows_ph, c->d_tr_base, c->d_perm_ph, src, dst);
+        hipLaunchKernelGGL(k_transpose, dim3((unsigned)c->np), dim3(kThreads), c->tr_lds, c->stream, c->d_tr_base,
+                           c->d_addr_pp, c->d_addr_ph, c->d_perm_ph, src, dst);
     else
-        hipLaunchKernelGGL(k_transpose, dim3((unsigned)c->np), dim3(kThreads), c->tr_lds, c->stream, c->d_tr_ptr_ph,
-                           c->d_tr_rows_ph, c->d_tr_ptr_pp, c->d_tr_rows_pp, c->d_tr_base, c->d_perm_pp, src, dst);
+        hipLaunchKernelGGL(k_transpose, dim3((unsigned)c->np), dim3(kThreads), c->tr_lds, c->stream, c->d_tr_base,
+                           c->d_addr_ph, c->d_addr_pp, c->d_perm_pp, src, dst);
     return check_launch("k_transpose");
 }
 
 int launch_antisym(ccd_ctx *c, const double *t) {
-    if (c->ltiles.empty()) return CCD_OK;
-    hipLaunchKernelGGL(k_antisym, dim3((unsigned)c->ltiles.size()), dim3(kThreads), 0, c->stream, c->d_ltiles,
+    if (c->rtiles.empty()) return CCD_OK;
+    hipLaunchKernelGGL(k_antisym, dim3((unsigned)c->rtiles.size()), dim3(kThreads), 0, c->stream, c->d_rtiles,
                        c->d_chans, c->d_prow, c->d_prow_sw, t, c->d_tas);
     return check_launch("k_antisym");
 }
 
 int launch_ladder_x(ccd_ctx *c) {
     if (c->xtiles.empty()) return CCD_OK;
-    const unsigned nb = (unsigned)((c->xtiles.size() + 3) / 4);
-    hipLaunchKernelGGL(k_ladder_x, dim3(nb), dim3(kThreads), 0, c->stream, c->d_xtiles, (int)c->xtiles.size(),
+    hipLaunchKernelGGL(k_ladder_x, dim3((unsigned)c->xtiles.size()), dim3(kThreads), 0, c->stream, c->d_xtiles,
                        c->d_chans, c->d_vtp, c->d_tas, c->d_vh, c->d_x);
     return check_launch("k_ladder_x");
 }
 
+template <int NT>
+void launch_ladder_v_nt(ccd_ctx *c) {
+    const int n = c->lt_off[NT] - c->lt_off[NT - 1];
+    if (n > 0)
+        hipLaunchKernelGGL(k_ladder_v<NT>, dim3((unsigned)n), dim3(kThreads), 0, c->stream,
+                           c->d_ltiles + c->lt_off[NT - 1], c->d_chans, c->d_vp, c->d_tas, c->d_x, c->d_dp);
+}
+
 int launch_ladder_v(ccd_ctx *c) {
     if (c->ltiles.empty()) return CCD_OK;
-    hipLaunchKernelGGL(k_ladder_v, dim3((unsigned)c->ltiles.size()), dim3(kThreads), 0, c->stream,
-                       c->d_ltiles, c->d_chans, c->d_vp, c->d_tas, c->d_x, c->d_dp);
+    launch_ladder_v_nt<1>(c);
+    launch_ladder_v_nt<2>(c);
+    launch_ladder_v_nt<3>(c);
+    launch_ladder_v_nt<4>(c);
+    launch_ladder_v_nt<5>(c);
     return check_launch("k_ladder_v");
 }
 
@@ -301,7 +311,7 @@ int build(ccd_ctx *c, const ccd_problem *p) {
     // ---- direct channels: swaps, packed rows, row pairs
     std::vector<int32_t> pp_swap(npp, -1), hh_swap(nhh, -1);
     std::vector<int32_t> prow, prow_sw;
-    int64_t off_t = 0, off_vp = 0, off_tp = 0, off_hhhh = 0, ipp = 0, ihh = 0;
+    int64_t off_t = 0, off_vp = 0, off_tp = 0, off_dp = 0, off_hhhh = 0, ipp = 0, ihh = 0;
     c->flops = 0; c->flops_pppp = 0;
     std::unordered_map<int64_t, int32_t> pos;
     for (int k = 0; k < p->n_chan; k++) {
@@ -339,10 +349,13 @@ int build(ccd_ctx *c, const ccd_problem *p) {
                 }
             const int ndiag = (int)prow.size() - pr_off - npk;
             const int ldv = (npk + 7) / 8 * 8;
-            DChan ch{off_t, off_vp, off_tp, off_hhhh, P, H, (int32_t)ipp, (int32_t)ihh, npk, ldv, pr_off, ndiag};
+            const int nseg = std::max(1, (npk + KSEG - 1) / KSEG);
+            DChan ch{off_t, off_vp, off_tp, off_hhhh, P, H, (int32_t)ipp, (int32_t)ihh, npk, ldv, pr_off, ndiag,
+                     off_dp, nseg, 0};
             c->chans.push_back(ch);
             off_vp += (int64_t)npk * ldv;
             off_tp += (int64_t)npk * H;
+            off_dp += (int64_t)nseg * npk * H;
             off_hhhh += (int64_t)H * H;
             c->flops += 2.0 * H * H * P + 2.0 * P * H * H + 2.0 * P * P * H;
             c->flops_pppp += 2.0 * P * P * H;
@@ -351,7 +364,7 @@ int build(ccd_ctx *c, const ccd_problem *p) {
         ipp += P; ihh += H;
     }
     c->nt2 = off_t;
-    c->n_vp = off_vp; c->n_tp = off_tp; c->n_hhhh = off_hhhh;
+    c->n_vp = off_vp; c->n_tp = off_tp; c->n_dp = off_dp; c->n_hhhh = off_hhhh;
     if (c->nt2 >= (int64_t)INT32_MAX)
         return ccd::fail(CCD_E_INVALID, "T2 has %lld elements (limit 2^31)", (long long)c->nt2);
 
@@ -395,42 +408,47 @@ int build(ccd_ctx *c, const ccd_problem *p) {
     if (c->ring_lds > 64 * 1024) return ccd::fail(CCD_E_INVALID, "ring LDS %zu bytes > 64 KB", c->ring_lds);
 
     // ---- per-particle transposition plans between the two layouts:
-    // particle a's block holds the elements (a,b,i,j); its pphh rows are the
-    // rows (a,b) in channel order, its phhp rows the rows (a,i) in mod-channel order
-    std::vector<std::vector<DRow>> rpp(np), rph(np);
+    // particle a's block holds the elements (a,b,i,j); its pphh slots run over
+    // the rows (a,b) in channel order, its phhp slots over the rows (a,i) in
+    // mod-channel order
     std::vector<int32_t> blk(np, 0), blk2(np, 0), row_start_pp(npp, -1), ph_start((size_t)np * nh, -1);
     for (const DChan &ch : c->chans)
         for (int AB = 0; AB < ch.pp; AB++) {
             const int a = ppl[ch.pp_off + AB].x - nh;
             row_start_pp[ch.pp_off + AB] = blk[a];
-            rpp[a].push_back(DRow{(int32_t)(ch.off_t + (int64_t)AB * ch.hh), ch.hh, blk[a], 0});
             blk[a] += ch.hh;
         }
     for (const DMod &m : c->mods)
         for (int q = 0; q < m.ph; q++) {
             const int a = phl[m.ph_off + q].x - nh, i = phl[m.ph_off + q].y;
             ph_start[(size_t)a * nh + i] = blk2[a];
-            rph[a].push_back(DRow{(int32_t)(m.off_ph + (int64_t)q * m.hp), m.hp, blk2[a], 0});
             blk2[a] += m.hp;
         }
-    std::vector<int32_t> tr_base(np + 1, 0), tr_ptr_pp(np + 1, 0), tr_ptr_ph(np + 1, 0);
+    std::vector<int32_t> tr_base(np + 1, 0);
     int32_t max_blk = 0;
     for (int a = 0; a < np; a++) {
         if (blk[a] != blk2[a])
             return ccd::fail(CCD_E_INVALID, "particle %d has %d pphh and %d phhp elements", a + nh, blk[a], blk2[a]);
         max_blk = std::max(max_blk, blk[a]);
         tr_base[a + 1] = tr_base[a] + blk[a];
-        tr_ptr_pp[a + 1] = tr_ptr_pp[a] + (int32_t)rpp[a].size();
-        tr_ptr_ph[a + 1] = tr_ptr_ph[a] + (int32_t)rph[a].size();
     }
+    if (tr_base[np] != c->nt2) return ccd::fail(CCD_E_INVALID, "particle blocks do not cover T2");
     if (max_blk > 65535 || (size_t)max_blk * sizeof(double) > 64 * 1024)
         return ccd::fail(CCD_E_INVALID, "a particle block of %d elements exceeds the transpose kernel's LDS", max_blk);
     c->tr_lds = (size_t)std::max(max_blk, 1) * sizeof(double);
-    std::vector<DRow> rows_pp, rows_ph;
-    for (int a = 0; a < np; a++) {
-        rows_pp.insert(rows_pp.end(), rpp[a].begin(), rpp[a].end());
-        rows_ph.insert(rows_ph.end(), rph[a].begin(), rph[a].end());
-    }
+    std::vector<int32_t> addr_pp(c->nt2, -1), addr_ph(c->nt2, -1);
+    for (const DChan &ch : c->chans)
+        for (int AB = 0; AB < ch.pp; AB++) {
+            const int a = ppl[ch.pp_off + AB].x - nh;
+            for (int IJ = 0; IJ < ch.hh; IJ++)
+                addr_pp[tr_base[a] + row_start_pp[ch.pp_off + AB] + IJ] = (int32_t)(ch.off_t + (int64_t)AB * ch.hh + IJ);
+        }
+    for (const DMod &m : c->mods)
+        for (int q = 0; q < m.ph; q++) {
+            const int a = phl[m.ph_off + q].x - nh, i = phl[m.ph_off + q].y;
+            for (int JB = 0; JB < m.hp; JB++)
+                addr_ph[tr_base[a] + ph_start[(size_t)a * nh + i] + JB] = (int32_t)(m.off_ph + (int64_t)q * m.hp + JB);
+        }
     std::vector<uint16_t> perm_ph(c->nt2), perm_pp(c->nt2);
     std::vector<uint8_t> seen(c->nt2, 0);
     for (const DChan &ch : c->chans)
@@ -452,11 +470,26 @@ int build(ccd_ctx *c, const ccd_problem *p) {
                 perm_pp[tr_base[pa] + spp] = (uint16_t)sph;  // pphh slot <- phhp slot
             }
         }
+    for (int64_t k = 0; k < c->nt2; k++)
+        if (addr_pp[k] < 0 || addr_ph[k] < 0) return ccd::fail(CCD_E_INVALID, "transposition plan has a hole");
 
-    // ---- tiles
+    // ---- tiles: packed-row tiles; ladder tiles = row tiles x k segments, grouped
+    // by their 16-column tile count (channel order kept inside a group)
+    for (int k = 0; k < (int)c->chans.size(); k++)
+        for (int r = 0; r < c->chans[k].npk; r += LROWS)
+            c->rtiles.push_back(DTile{k, r, std::min(LROWS, c->chans[k].npk - r), 0});
+    for (int nt = 1; nt <= NTMAX; nt++) {
+        for (int k = 0; k < (int)c->chans.size(); k++) {
+            const DChan &ch = c->chans[k];
+            if ((ch.hh + 15) / 16 != nt) continue;
+            for (int sgm = 0; sgm < ch.nseg; sgm++)
+                for (int r = 0; r < ch.npk; r += LROWS)
+                    c->ltiles.push_back(DTile{k, r, std::min(LROWS, ch.npk - r), sgm});
+        }
+        c->lt_off[nt] = (int)c->ltiles.size();
+    }
     for (int k = 0; k < (int)c->chans.size(); k++) {
         const DChan &ch = c->chans[k];
-        for (int r = 0; r < ch.npk; r += LROWS) c->ltiles.push_back(DTile{k, r, std::min(LROWS, ch.npk - r), 0});
         for (int r = 0; r < ch.hh; r += 16)
             for (int q = 0; q < ch.hh; q += 16) c->xtiles.push_back(DTile{k, r, 16, q});
         const int npairs = ch.npk + ch.ndiag, per = std::max(1, kThreads / ch.hh);
@@ -484,14 +517,14 @@ int build(ccd_ctx *c, const ccd_problem *p) {
     // ---- upload tables
     int rc;
     if ((rc = upload(c, &c->d_chans, c->chans)) || (rc = upload(c, &c->d_mods, c->mods)) ||
-        (rc = upload(c, &c->d_ltiles, c->ltiles)) || (rc = upload(c, &c->d_utiles, c->utiles)) ||
+        (rc = upload(c, &c->d_rtiles, c->rtiles)) || (rc = upload(c, &c->d_ltiles, c->ltiles)) ||
+        (rc = upload(c, &c->d_utiles, c->utiles)) ||
         (rc = upload(c, &c->d_xtiles, c->xtiles)) || (rc = upload(c, &c->d_ppl, ppl)) ||
         (rc = upload(c, &c->d_hhl, hhl)) || (rc = upload(c, &c->d_phl, phl)) || (rc = upload(c, &c->d_hpl, hpl)) ||
         (rc = upload(c, &c->d_hh_swap, hh_swap)) || (rc = upload(c, &c->d_prow, prow)) ||
         (rc = upload(c, &c->d_prow_sw, prow_sw)) || (rc = upload(c, &c->d_red_ptr, red_ptr)) ||
-        (rc = upload(c, &c->d_red_idx, red_idx)) || (rc = upload(c, &c->d_tr_ptr_pp, tr_ptr_pp)) ||
-        (rc = upload(c, &c->d_tr_ptr_ph, tr_ptr_ph)) || (rc = upload(c, &c->d_tr_base, tr_base)) ||
-        (rc = upload(c, &c->d_tr_rows_pp, rows_pp)) || (rc = upload(c, &c->d_tr_rows_ph, rows_ph)) ||
+        (rc = upload(c, &c->d_red_idx, red_idx)) || (rc = upload(c, &c->d_tr_base, tr_base)) ||
+        (rc = upload(c, &c->d_addr_pp, addr_pp)) || (rc = upload(c, &c->d_addr_ph, addr_ph)) ||
         (rc = upload(c, &c->d_perm_ph, perm_ph)) || (rc = upload(c, &c->d_perm_pp, perm_pp)))
         return rc;
     std::vector<double> eps(p->eps, p->eps + nsp);
@@ -500,7 +533,7 @@ int build(ccd_ctx *c, const ccd_problem *p) {
     if ((rc = dalloc(c, &c->d_onebody, (size_t)nsp)) || (rc = dalloc(c, &c->d_t[0], nt)) ||
         (rc = dalloc(c, &c->d_t[1], nt)) || (rc = dalloc(c, &c->d_tph, nt)) || (rc = dalloc(c, &c->d_r, nt)) ||
         (rc = dalloc(c, &c->d_g, nt)) || (rc = dalloc(c, &c->d_vt, nt)) || (rc = dalloc(c, &c->d_va, nt)) ||
-        (rc = dalloc(c, &c->d_dp, (size_t)c->n_tp)) || (rc = dalloc(c, &c->d_tas, (size_t)c->n_tp)) ||
+        (rc = dalloc(c, &c->d_dp, (size_t)c->n_dp)) || (rc = dalloc(c, &c->d_tas, (size_t)c->n_tp)) ||
         (rc = dalloc(c, &c->d_vtp, (size_t)c->n_tp)) || (rc = dalloc(c, &c->d_x, (size_t)c->n_hhhh)) ||
         (rc = dalloc(c, &c->d_vh, (size_t)c->n_hhhh)) || (rc = dalloc(c, &c->d_vp, (size_t)c->n_vp)) ||
         (rc = dalloc(c, &c->d_vb, (size_t)c->n_b)) || (rc = dalloc(c, &c->d_rs, (size_t)std::max<int64_t>(npp, 1))) ||
@@ -524,9 +557,9 @@ int build(ccd_ctx *c, const ccd_problem *p) {
             if ((rc = upload(c, &c->d_table, tab))) return rc;
             G.table = c->d_table;
         }
-        if (!c->ltiles.empty())
-            hipLaunchKernelGGL(k_gen_vp, dim3((unsigned)c->ltiles.size()), dim3(kThreads), 0, c->stream,
-                               c->d_ltiles, c->d_chans, c->d_ppl, c->d_prow, G, c->d_vp);
+        if (!c->rtiles.empty())
+            hipLaunchKernelGGL(k_gen_vp, dim3((unsigned)c->rtiles.size()), dim3(kThreads), 0, c->stream,
+                               c->d_rtiles, c->d_chans, c->d_ppl, c->d_prow, G, c->d_vp);
         if (!c->chans.empty())
             hipLaunchKernelGGL(k_gen_direct_small, dim3((unsigned)c->chans.size()), dim3(kThreads), 0,
                                c->stream, c->d_chans, c->d_ppl, c->d_hhl, G, c->d_vt, c->d_vh);
@@ -581,8 +614,8 @@ int build(ccd_ctx *c, const ccd_problem *p) {
                               hipMemcpyHostToDevice));
         }
     }
-    if (!c->ltiles.empty()) {
-        hipLaunchKernelGGL(k_pack_vt, dim3((unsigned)c->ltiles.size()), dim3(kThreads), 0, c->stream, c->d_ltiles,
+    if (!c->rtiles.empty()) {
+        hipLaunchKernelGGL(k_pack_vt, dim3((unsigned)c->rtiles.size()), dim3(kThreads), 0, c->stream, c->d_rtiles,
                            c->d_chans, c->d_prow, c->d_vt, c->d_vtp);
         if ((rc = check_launch("k_pack_vt"))) return rc;
     }

@@ -22,7 +22,7 @@ struct UpdArgs {
     const int32_t *hh_swap;          // index of (j,i) within the channel
     const double *eps, *chi, *psi;   // chi [nh], psi [np]
     const double *vt;                // <ij||ab> in the pphh layout
-    const double *dp;                // ladder D on packed rows
+    const double *dp;                // ladder D on packed rows, one slab per k segment
     const double *g;                 // ring R gathered to the pphh layout: g[(a,b),(i,j)] = R[(a,i),(j,b)]
     const double *t_old;
     double *t_new, *tas;             // new amplitudes; their (a,b)-antisymmetric packed rows
@@ -55,7 +55,12 @@ __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
             const int ia = ab.x, ib = ab.y, ii = ij.x, jj = ij.y;
             const double t1s = a.t_old[b1 + JI], t2s = a.t_old[b2 + JI];
             const double g1 = a.g[b1 + IJ], g1s = a.g[b1 + JI], g2 = a.g[b2 + IJ], g2s = a.g[b2 + JI];
-            const double d = diag ? 0.0 : a.dp[ch.off_tp + (int64_t)pr * H + IJ];
+            double d = 0.0;
+            if (!diag) {
+                const int64_t slab = (int64_t)ch.npk * H;
+                const double *dq = a.dp + ch.off_dp + (int64_t)pr * H + IJ;
+                for (int q = 0; q < ch.nseg; q++) d += dq[q * slab];
+            }
             const double ci = 0.5 * a.chi[ii], cj = 0.5 * a.chi[jj];
             const double pa = -0.5 * a.psi[ia - a.nh], pb = -0.5 * a.psi[ib - a.nh];
             // element (a,b,i,j), in the order of src/calc.cpp:393-408
@@ -121,31 +126,24 @@ __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
 
 // Per-particle transposition between the pphh and phhp layouts (both hold
 // every amplitude once; the block of particle a holds the elements whose
-// first particle index is a).  Rows of `src` are loaded contiguously into LDS,
-// rows of `dst` are written contiguously, dst position p taking LDS slot
-// perm[base + p].  With src = pphh it is the reference's
-// mapToDistributed(ai_jb); with src = phhp it gathers R to the pphh layout.
-__global__ __launch_bounds__(kThreads) void k_transpose(const int32_t *__restrict__ ptr_src,
-                                                        const DRow *__restrict__ rows_src,
-                                                        const int32_t *__restrict__ ptr_dst,
-                                                        const DRow *__restrict__ rows_dst,
-                                                        const int32_t *__restrict__ base,
+// first particle index is a, base[a] .. base[a+1]).  Slot l of the source
+// order is loaded from src[src_addr[base+l]] into LDS (row runs, so the loads
+// coalesce), then slot l of the destination order is written to
+// dst[dst_addr[base+l]] from LDS slot perm[base+l].  With src = pphh it is the
+// reference's mapToDistributed(ai_jb); with src = phhp it gathers R into the
+// pphh layout.
+__global__ __launch_bounds__(kThreads) void k_transpose(const int32_t *__restrict__ base,
+                                                        const int32_t *__restrict__ src_addr,
+                                                        const int32_t *__restrict__ dst_addr,
                                                         const uint16_t *__restrict__ perm,
                                                         const double *__restrict__ src,
                                                         double *__restrict__ dst) {
     extern __shared__ double buf[];
     const int a = blockIdx.x;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int r = ptr_src[a] + wave; r < ptr_src[a + 1]; r += kThreads / 64) {
-        const DRow rr = rows_src[r];
-        for (int c = lane; c < rr.len; c += 64) buf[rr.loff + c] = src[rr.goff + c];
-    }
+    const int b0 = base[a], n = base[a + 1] - b0;
+    for (int l = threadIdx.x; l < n; l += kThreads) buf[l] = src[src_addr[b0 + l]];
     __syncthreads();
-    const int64_t b = base[a];
-    for (int r = ptr_dst[a] + wave; r < ptr_dst[a + 1]; r += kThreads / 64) {
-        const DRow rr = rows_dst[r];
-        for (int c = lane; c < rr.len; c += 64) dst[rr.goff + c] = buf[perm[b + rr.loff + c]];
-    }
+    for (int l = threadIdx.x; l < n; l += kThreads) dst[dst_addr[b0 + l]] = buf[perm[b0 + l]];
 }
 
 // chi_i (blocks 0..nh-1) and psi_a (blocks nh..nsp-1) as fixed-order sums of
@@ -235,7 +233,10 @@ __global__ __launch_bounds__(kThreads) void k_expand_d(UpdArgs a, double *__rest
     if (r1 == r2) {
         d[ch.off_t + (int64_t)r1 * H + IJ] = 0.0;
     } else {
-        const double v = a.dp[ch.off_tp + (int64_t)pr * H + IJ];
+        const int64_t slab = (int64_t)ch.npk * H;
+        const double *dq = a.dp + ch.off_dp + (int64_t)pr * H + IJ;
+        double v = 0.0;
+        for (int q = 0; q < ch.nseg; q++) v += dq[q * slab];
         d[ch.off_t + (int64_t)r1 * H + IJ] = v;
         d[ch.off_t + (int64_t)r2 * H + IJ] = -v;
     }
