@@ -70,6 +70,9 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first copy this many MiB device-to-device (a kernel of known traffic, "
+                         "to calibrate hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the CCD iteration on a single "
@@ -77,6 +80,15 @@ def main():
         sys.exit(2)
 
     from coupled_cluster_doubles_amd import FP32, FP64, Context
+
+    if args.copy_probe_mib > 0:
+        import torch
+        n = args.copy_probe_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     cfg = CONFIGS[args.config]
     prec = FP64 if args.precision == "fp64" else FP32

@@ -348,8 +348,8 @@ int build(ccd_ctx *c, const ccd_problem *p) {
                     prow_sw.push_back(q);
                 }
             const int ndiag = (int)prow.size() - pr_off - npk;
-            const int ldv = (npk + 7) / 8 * 8;
-            const int nseg = std::max(1, (npk + KSEG - 1) / KSEG);
+            const int ldv = (npk + 15) / 16 * 16;
+            const int nseg = ladder_segments(npk, (H + 15) / 16);
             DChan ch{off_t, off_vp, off_tp, off_hhhh, P, H, (int32_t)ipp, (int32_t)ihh, npk, ldv, pr_off, ndiag,
                      off_dp, nseg, 0};
             c->chans.push_back(ch);

@@ -6,16 +6,13 @@
 // and D[(b,a)] = -D[(a,b)], D[(a,a)] = 0.  Both ladder kernels therefore run on
 // the packed pairs a<b: a quarter of the V^pppp bytes and flops.  The
 // contractions use v_mfma_f64_16x16x4f64 (layout in device_common.hpp).
-// Within a 32-wide k chunk, k-slot q of MFMA step s is k = 8q + s, so each
-// lane reads 8 consecutive elements of its A row.
 #pragma once
 
 #include "device_common.hpp"
 
-constexpr int KC = 32;                 // k per pipeline stage
+constexpr int KC = 32;                 // k per chunk of k_ladder_x (k-slot q of step s: k = 8q + s)
 constexpr int NTMAX = 5;               // 16-column tiles: hh <= 80
 constexpr int LROWS = 64;              // packed rows per ladder block (16 per wave)
-constexpr int KSEG = 8 * KC;           // packed pairs per k segment of the pppp ladder
 
 // X = V^hhhh + sum_{c<d} V^hhpp[kl][cd] tas[cd][ij], one 16x16 tile of X per
 // block; the four waves take every fourth k chunk and their partial tiles are
@@ -63,14 +60,25 @@ __global__ __launch_bounds__(kThreads) void k_ladder_x(const DTile *__restrict__
 
 // D_p[u][ij] = 1/2 sum_kl tas[u][kl] X[kl][ij] + sum_w Vp[u][w] tas[w][ij]
 // for 64 packed rows u of one channel per block (16 per wave) and NT*16 >= hh
-// columns, split over k segments of KSEG packed pairs (tile.aux = segment):
-// segment s writes its partial sum to slab s of the channel's D_p, and the
-// update adds the slabs in segment order.  The k range of a block is one
-// sequence of 32-wide chunks: for segment 0 first the hh columns of T X, then
-// its packed pairs of V t.  Software pipeline: while the
-// MFMAs of chunk c run, the A values of chunk c+1 (HBM) and its B tile
-// (L2-resident amplitudes / X) are loaded into registers; the B tile then
-// goes to the other LDS buffer, one barrier per chunk.
+// columns, split over k segments of kseg_pairs(NT) packed pairs (tile.aux =
+// segment): segment s writes its partial sum to slab s of the channel's D_p,
+// and the update adds the slabs in segment order.  The k range of a block is
+// one sequence of KCH-wide chunks: for segment 0 first the hh columns of T X,
+// then its packed pairs of V t.  In a chunk, k-slot q of MFMA step s is
+// k = 8 (s/2) + 2q + s%2: the four lanes of a row read 64 contiguous bytes per
+// 16-byte load, so a load instruction touches half cache lines whose other
+// halves the next instruction reads (rows are 128-byte aligned).  Software
+// pipeline: while the MFMAs of chunk c run, the A values of chunk c+1 (HBM)
+// and its B tile (L2-resident amplitudes / X) are loaded into registers; the
+// B tile then goes to the other LDS buffer, one barrier per chunk.
+__device__ __forceinline__ int kslot(int q, int s) { return 8 * (s >> 1) + 2 * q + (s & 1); }
+
+template <int NT>
+struct LadderShape {
+    static constexpr int KCH = NT <= 2 ? 64 : 32;   // k per pipeline stage
+    static constexpr int KSEG = NT <= 2 ? 256 : 64; // packed pairs per k segment
+};
+
 template <int NT>
 __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__ tiles,
                                                        const DChan *__restrict__ chans,
@@ -78,13 +86,14 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
                                                        const double *__restrict__ tas,
                                                        const double *__restrict__ x,
                                                        double *__restrict__ dp) {
-    constexpr int W = NT * 16, STR = W + 1, BPT = (KC * W + kThreads - 1) / kThreads;
-    __shared__ double Bs[2][KC][STR];
+    constexpr int KCH = LadderShape<NT>::KCH, KSEG = LadderShape<NT>::KSEG, G = KCH / 4;
+    constexpr int W = NT * 16, STR = W + 1, BPT = (KCH * W + kThreads - 1) / kThreads;
+    __shared__ double Bs[2][KCH][STR];
     const DTile tl = tiles[blockIdx.x];
     const DChan ch = chans[tl.chan];
     const int H = ch.hh, npk = ch.npk, ldv = ch.ldv, seg = tl.aux;
     const int kbeg = seg * KSEG, kend = min(npk, kbeg + KSEG);
-    const int nx = seg == 0 ? (H + KC - 1) / KC : 0, nchunk = nx + (kend - kbeg + KC - 1) / KC;
+    const int nx = seg == 0 ? (H + KCH - 1) / KCH : 0, nchunk = nx + (kend - kbeg + KCH - 1) / KCH;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
     const int lrow = wave * 16 + m;
     const bool rok = lrow < tl.nrows;
@@ -92,50 +101,50 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
     const double *at = tas + ch.off_tp + u * H;
     const double *av = vp + ch.off_vp + u * ldv;
 
-    auto load_a = [&](int c, double (&a)[8]) {
+    auto load_a = [&](int c, double (&a)[G]) {
         if (c < nx) {
 #pragma unroll
-            for (int s = 0; s < 8; s++) {
-                const int k = c * KC + kq * 8 + s;
+            for (int s = 0; s < G; s++) {
+                const int k = c * KCH + kslot(kq, s);
                 a[s] = (rok && k < H) ? 0.5 * at[k] : 0.0;
             }
         } else {
-            const int kb = kbeg + (c - nx) * KC + kq * 8;
-            if (rok && kb < ldv && kb < kend) {  // ldv, kb, kend - kbeg multiples of 8: whole 64-byte group
-                const dbl2 *p2 = reinterpret_cast<const dbl2 *>(av + kb);
+            const int kc0 = kbeg + (c - nx) * KCH;
 #pragma unroll
-                for (int h = 0; h < 4; h++) {
-                    const dbl2 v = p2[h];
+            for (int h = 0; h < G / 2; h++) {
+                const int k = kc0 + 8 * h + 2 * kq;  // even; ldv is even, so k+1 < ldv too
+                if (rok && k < kend && k < ldv) {
+                    const dbl2 v = *reinterpret_cast<const dbl2 *>(av + k);
                     a[2 * h] = v.x;
                     a[2 * h + 1] = v.y;
+                } else {
+                    a[2 * h] = 0.0;
+                    a[2 * h + 1] = 0.0;
                 }
-            } else {
-#pragma unroll
-                for (int s = 0; s < 8; s++) a[s] = 0.0;
             }
         }
     };
     auto load_b = [&](int c, double (&b)[BPT]) {
         const double *B = c < nx ? x + ch.off_hhhh : tas + ch.off_tp;
-        const int K = c < nx ? H : kend, k0 = c < nx ? c * KC : kbeg + (c - nx) * KC;
+        const int K = c < nx ? H : kend, k0 = c < nx ? c * KCH : kbeg + (c - nx) * KCH;
 #pragma unroll
         for (int q = 0; q < BPT; q++) {
             const int idx = threadIdx.x + kThreads * q, kk = idx / W, col = idx - kk * W, k = k0 + kk;
-            b[q] = (idx < KC * W && k < K && col < H) ? B[(int64_t)k * H + col] : 0.0;
+            b[q] = (idx < KCH * W && k < K && col < H) ? B[(int64_t)k * H + col] : 0.0;
         }
     };
     auto store_b = [&](int buf, const double (&b)[BPT]) {
 #pragma unroll
         for (int q = 0; q < BPT; q++) {
             const int idx = threadIdx.x + kThreads * q, kk = idx / W, col = idx - kk * W;
-            if (idx < KC * W) Bs[buf][kk][col] = b[q];
+            if (idx < KCH * W) Bs[buf][kk][col] = b[q];
         }
     };
 
     dbl4 acc[NT];
 #pragma unroll
     for (int n = 0; n < NT; n++) acc[n] = dbl4{0.0, 0.0, 0.0, 0.0};
-    double a[8], an[8] = {}, b[BPT];
+    double a[G], an[G] = {}, b[BPT];
     load_a(0, a);
     load_b(0, b);
     store_b(0, b);
@@ -148,14 +157,14 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
         }
         const int cb = c & 1;
 #pragma unroll
-        for (int s = 0; s < 8; s++) {
+        for (int s = 0; s < G; s++) {
 #pragma unroll
-            for (int n = 0; n < NT; n++) acc[n] = mfma16(a[s], Bs[cb][kq * 8 + s][n * 16 + m], acc[n]);
+            for (int n = 0; n < NT; n++) acc[n] = mfma16(a[s], Bs[cb][kslot(kq, s)][n * 16 + m], acc[n]);
         }
         if (more) store_b(cb ^ 1, b);
         __syncthreads();
 #pragma unroll
-        for (int s = 0; s < 8; s++) a[s] = an[s];
+        for (int s = 0; s < G; s++) a[s] = an[s];
     }
 #pragma unroll
     for (int n = 0; n < NT; n++) {
@@ -166,4 +175,10 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
                 dp[ch.off_dp + (int64_t)seg * npk * H + (int64_t)(tl.row0 + row) * H + col] = acc[n][q];
         }
     }
+}
+
+// Number of k segments of a channel's pppp ladder.
+inline int ladder_segments(int npk, int nt) {
+    const int kseg = nt <= 2 ? LadderShape<1>::KSEG : LadderShape<3>::KSEG;
+    return npk > 0 ? (npk + kseg - 1) / kseg : 1;
 }
