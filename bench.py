@@ -1,11 +1,15 @@
 """Benchmark: CCD iterations/s on the electron gas, N=54, r_s=1.0, n^2<=36.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64]
-                    [--no-cpu-baseline] [--config heg54|heg14|pairing]
+                    [--mode stored|free] [--no-cpu-baseline]
+                    [--config heg54|heg14|pairing]
 
 A step is one full Jacobi iteration of the CCD amplitude equations over the
 whole problem (both ladders, ring, one-body terms, remaps, update, energy),
-with the amplitudes and interaction resident in HBM.  Prints one JSON line.
+with the amplitudes and interaction resident in HBM.  --mode free is the
+reference's matrix-free mode (-m 1): V^pppp is rebuilt inside the pppp
+ladder every iteration instead of being read from HBM; the iterates are
+bit-identical to the stored mode.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -43,6 +47,22 @@ def make_model(cfg):
     return Model.pairing(*cfg["args"])
 
 
+def packed_ladder_flops(model):
+    """MFMA-side flops of one pass of the packed pppp ladder: per direct
+    channel 2 npk^2 hh (V t over pairs c<d) + 2 npk hh^2 (T X), npk = pairs a<b."""
+    d = model.dims()
+    pairs = model.pairs("pp")
+    f, o = 0.0, 0
+    for pp, hh in zip(d["pp_dim"], d["hh_dim"]):
+        blk = pairs[o:o + pp]
+        o += pp
+        if pp == 0 or hh == 0:
+            continue
+        npk = int(np.count_nonzero(blk[:, 0] < blk[:, 1]))
+        f += 2.0 * npk * npk * hh + 2.0 * npk * hh * hh
+    return f
+
+
 def cpu_baseline(cfg, iters, threads):
     """The FP64 CPU oracle (plain C + OpenMP) timed on this host."""
     from oracle.pyoracle import Oracle, set_threads
@@ -68,6 +88,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="heg54", choices=sorted(CONFIGS))
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--mode", default="stored", choices=["stored", "free"],
+                    help="stored: V^pppp read from HBM; free: rebuilt in the ladder (reference -m 1)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -94,7 +116,7 @@ def main():
     prec = FP64 if args.precision == "fp64" else FP32
     model = make_model(cfg)
     t_setup = time.perf_counter()
-    ctx = Context(model, prec)
+    ctx = Context(model, prec, matrix_free=args.mode == "free")
     setup_s = time.perf_counter() - t_setup
     mbpt2 = ctx.mbpt2()
     work = ctx.work()
@@ -112,11 +134,22 @@ def main():
     phases = ctx.time_phases(max(3, min(args.steps, 10)))
     ladder_ms = float(phases[2])
     lb = work["ladder_bytes"]
-    achieved = lb / (ladder_ms * 1e-3) / 1e9
     gemm_tflops = work["flops"] / (ms_step * 1e-3) / 1e12
+    if args.mode == "stored":
+        achieved = lb / (ladder_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": "k_ladder_v", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes": lb}
+    else:
+        lf = packed_ladder_flops(model)
+        peak = FP64_PEAK_TFLOPS if prec == FP64 else FP32_PEAK_TFLOPS
+        achieved = lf / (ladder_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "kernel": "k_ladder_v", "achieved": achieved, "peak": peak,
+                "unit": "TFLOP/s", "frac": achieved / peak, "algorithmic_flops": lf,
+                "algorithmic_bytes": lb}
 
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"ladder_d_traffic_{args.config}_{args.precision}.json")
+    tfile = os.path.join(ROOT, "profiles",
+                         f"ladder_d_traffic_{args.config}_{args.precision}_{args.mode}.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("bytes_per_launch")
@@ -142,7 +175,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if prec == FP64 else "f32+f64",
         "data": "synthetic: the reference's closed-form basis and Coulomb interaction, T2 from zero",
-        "config": {"workload": cfg["name"], "precision": args.precision,
+        "config": {"workload": cfg["name"], "precision": args.precision, "mode": args.mode,
                    "n_t2": ctx.n_t2, "device_bytes": ctx.device_bytes},
         "gemm_tflops": gemm_tflops,
         "gemm_flops_per_iter": work["flops"],
@@ -151,9 +184,7 @@ def main():
         "phase_ms": {"ring": phases[0], "ladder_x": phases[1], "ladder_d": phases[2],
                      "update": phases[3], "reductions": phases[4], "iteration": phases[5]},
         "setup_s": setup_s,
-        "roofline": {"bound": "hbm", "kernel": "k_ladder_v", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic, "algorithmic_bytes": lb},
+        "roofline": dict(roof, traffic=traffic),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -17,9 +17,10 @@ from ._lib import CCDError, Problem, check, lib
 
 FP64 = 0
 FP32 = 1
+MATRIX_FREE = 0x100   # OR into the precision: V^pppp rebuilt inside the ladder (reference -m 1)
 STOP_REASONS = {0: "converged", 1: "max_iterations", 2: "nan"}
 
-__all__ = ["Model", "Context", "CCDError", "FP64", "FP32", "STOP_REASONS"]
+__all__ = ["Model", "Context", "CCDError", "FP64", "FP32", "MATRIX_FREE", "STOP_REASONS"]
 
 
 def _ptr(a: np.ndarray):
@@ -115,10 +116,12 @@ class Model:
 class Context:
     """Device-resident CCD state for one problem on the current HIP device."""
 
-    def __init__(self, model: Model, precision: int = FP64, host_v: bool = False):
+    def __init__(self, model: Model, precision: int = FP64, host_v: bool = False,
+                 matrix_free: bool = False):
         self.model = model
         self.h = C.c_void_p()
-        check(lib().ccd_ctx_create(model.problem(host_v), precision, C.byref(self.h)),
+        mode = precision | (MATRIX_FREE if matrix_free else 0)
+        check(lib().ccd_ctx_create(model.problem(host_v), mode, C.byref(self.h)),
               "ccd_ctx_create")
         n, b = C.c_int64(), C.c_int64()
         check(lib().ccd_ctx_info(self.h, C.byref(n), C.byref(b)), "ccd_ctx_info")

@@ -71,6 +71,10 @@ extern "C" const char *ccd_last_error(void) { return ccd::g_err.c_str(); }
 struct ccd_ctx {
     int device = 0;
     int precision = CCD_PREC_FP64;
+    bool matrix_free = false;    // V^pppp generated inside the ladder, never stored
+    int lsrc = LV_STORED;        // LadderSource of the pppp ladder
+    int gen_blocks = 1024;       // grid of the generated-V ladder launches
+    LadderGen lgen{};
     hipStream_t stream = nullptr;
     int nsp = 0, nh = 0, np = 0;
     int64_t nt2 = 0;
@@ -100,9 +104,12 @@ struct ccd_ctx {
     double *d_vt = nullptr, *d_vtp = nullptr, *d_vh = nullptr, *d_vp = nullptr, *d_va = nullptr,
            *d_vb = nullptr;
     double *d_partials = nullptr, *d_scalar = nullptr;
+    double *d_zero = nullptr;    // kZeroDoubles zeros, never written: the target of masked-off loads
     double *h_scalar = nullptr;  // pinned
     int32_t *d_qn = nullptr;
     double *d_table = nullptr;
+    int2 *d_rec1 = nullptr, *d_rec2 = nullptr;  // generated-V pair records
+    double *d_gtab = nullptr;                   // generated-V electron gas table
     int64_t n_vp = 0, n_tp = 0, n_dp = 0, n_hhhh = 0, n_b = 0;
     double flops = 0, flops_pppp = 0;
     std::vector<void *> allocs;
@@ -189,21 +196,33 @@ int launch_ladder_x(ccd_ctx *c) {
     return check_launch("k_ladder_x");
 }
 
-template <int NT>
+// Stored V: one block per tile.  Generated V: at most gen_blocks blocks
+// looping over the tiles, the gas table in dynamic LDS.
+template <int NT, int SRC>
 void launch_ladder_v_nt(ccd_ctx *c) {
     const int n = c->lt_off[NT] - c->lt_off[NT - 1];
-    if (n > 0)
-        hipLaunchKernelGGL(k_ladder_v<NT>, dim3((unsigned)n), dim3(kThreads), 0, c->stream,
-                           c->d_ltiles + c->lt_off[NT - 1], c->d_chans, c->d_vp, c->d_tas, c->d_x, c->d_dp);
+    if (n <= 0) return;
+    const int grid = SRC == LV_STORED ? n : std::min(n, c->gen_blocks);
+    const size_t lds = SRC == LV_GAS ? sizeof(double) * (size_t)c->lgen.ntab : 0;
+    hipLaunchKernelGGL((k_ladder_v<NT, SRC>), dim3((unsigned)grid), dim3(kThreads), lds, c->stream,
+                       c->d_ltiles + c->lt_off[NT - 1], n, c->d_chans, c->d_vp, c->lgen, c->d_tas, c->d_x,
+                       c->d_zero, c->d_dp);
+}
+
+template <int SRC>
+void launch_ladder_v_src(ccd_ctx *c) {
+    launch_ladder_v_nt<1, SRC>(c);
+    launch_ladder_v_nt<2, SRC>(c);
+    launch_ladder_v_nt<3, SRC>(c);
+    launch_ladder_v_nt<4, SRC>(c);
+    launch_ladder_v_nt<5, SRC>(c);
 }
 
 int launch_ladder_v(ccd_ctx *c) {
     if (c->ltiles.empty()) return CCD_OK;
-    launch_ladder_v_nt<1>(c);
-    launch_ladder_v_nt<2>(c);
-    launch_ladder_v_nt<3>(c);
-    launch_ladder_v_nt<4>(c);
-    launch_ladder_v_nt<5>(c);
+    if (c->lsrc == LV_GAS) launch_ladder_v_src<LV_GAS>(c);
+    else if (c->lsrc == LV_GENERIC) launch_ladder_v_src<LV_GENERIC>(c);
+    else launch_ladder_v_src<LV_STORED>(c);
     return check_launch("k_ladder_v");
 }
 
@@ -269,6 +288,93 @@ int validate(const ccd_problem *p) {
     } else {
         return ccd::fail(CCD_E_INVALID, "unknown gen_kind %d", p->gen_kind);
     }
+    return CCD_OK;
+}
+
+// Pair records of the generated-V ladder (LadderGen).  The electron-gas form
+// needs, and checks here: two spin values, conserved total momentum and spin
+// in every direct channel, and a direct-term table even in each component;
+// otherwise the ladder calls the generic generator per element.
+int build_generated_ladder(ccd_ctx *c, const ccd_problem *p, const std::vector<int2> &ppl,
+                           const std::vector<int32_t> &prow) {
+    const size_t nrec = std::max<size_t>(prow.size(), 1);
+    std::vector<int2> rec1(nrec, int2{0, 0}), rec2(nrec, int2{0, 0});
+    bool gas = p->gen_kind == CCD_GEN_ELECTRON_GAS;
+    const int m = p->gen_nmax, D = 2 * m + 1, w = 4 * m + 1;
+    if (gas && (int64_t)D * D * D + 1 > kGasTableMax) gas = false;
+    const int32_t *qn = p->qn;
+    int s_lo = 0, s_hi = 0;
+    if (gas) {
+        s_lo = s_hi = qn[3];
+        for (int s = 0; s < c->nsp; s++) {
+            s_lo = std::min(s_lo, qn[5 * s + 3]);
+            s_hi = std::max(s_hi, qn[5 * s + 3]);
+        }
+        for (int s = 0; s < c->nsp && gas; s++)
+            if (qn[5 * s + 3] != s_lo && qn[5 * s + 3] != s_hi) gas = false;
+    }
+    if (gas)
+        for (const DChan &ch : c->chans) {
+            const int2 f = ppl[ch.pp_off];
+            int tot[4];
+            for (int k = 0; k < 4; k++) tot[k] = qn[5 * f.x + k] + qn[5 * f.y + k];
+            for (int q = 1; q < ch.pp && gas; q++) {
+                const int2 e = ppl[ch.pp_off + q];
+                for (int k = 0; k < 4; k++)
+                    if (qn[5 * e.x + k] + qn[5 * e.y + k] != tot[k]) gas = false;
+            }
+            if (!gas) break;
+        }
+    std::vector<double> tab;
+    if (gas) {
+        tab.assign((size_t)D * D * D + 1, 0.0);
+        const double *T = p->gen_table;
+        auto at = [&](int dx, int dy, int dz) {
+            return T[((size_t)(dx + 2 * m) * w + (dy + 2 * m)) * w + (dz + 2 * m)];
+        };
+        for (int x = 0; x < D && gas; x++)
+            for (int y = 0; y < D && gas; y++)
+                for (int z = 0; z < D && gas; z++) {
+                    const double v = at(x, y, z);
+                    for (int sg = 1; sg < 8; sg++)
+                        if (at(sg & 1 ? -x : x, sg & 2 ? -y : y, sg & 4 ? -z : z) != v) gas = false;
+                    tab[((size_t)x * D + y) * D + z] = v;
+                }
+    }
+    // 16-bit fields: 8 D^2 (2m) + 8 D (2m) + 8 (2m) < zoff = 8 D^3 <= 32768
+    const uint32_t zoff = (uint32_t)(8 * D * D * D);
+    auto scaled = [&](int s) {
+        const int32_t *a = qn + 5 * (size_t)s;
+        const uint32_t x = 8u * D * D * (a[0] + m), y = 8u * D * (a[1] + m), z = 8u * (a[2] + m);
+        const uint32_t sp = a[3] == s_hi && s_hi != s_lo ? zoff : 0u;
+        return int2{(int)(x | y << 16), (int)(z | sp << 16)};
+    };
+    for (const DChan &ch : c->chans)
+        for (int u = 0; u < ch.npk + ch.ndiag; u++) {
+            const int2 ab = ppl[ch.pp_off + prow[ch.pr_off + u]];
+            if (gas) {
+                rec1[ch.pr_off + u] = scaled(ab.x);
+                rec2[ch.pr_off + u] = scaled(ab.y);
+            } else {
+                rec1[ch.pr_off + u] = int2{ab.x, ab.y};
+            }
+        }
+    int rc;
+    if ((rc = upload(c, &c->d_rec1, rec1)) || (rc = upload(c, &c->d_rec2, rec2))) return rc;
+    c->lgen.rec1 = c->d_rec1;
+    c->lgen.rec2 = c->d_rec2;
+    if (gas) {
+        if ((rc = upload(c, &c->d_gtab, tab))) return rc;
+        c->lgen.tab = c->d_gtab;
+        c->lgen.ntab = (int)tab.size();
+        c->lgen.zoff = zoff;
+        c->lsrc = LV_GAS;
+    } else {
+        c->lsrc = LV_GENERIC;
+    }
+    int ncu = 0;
+    HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+    c->gen_blocks = std::max(1, ncu) * 4;
     return CCD_OK;
 }
 
@@ -535,11 +641,14 @@ int build(ccd_ctx *c, const ccd_problem *p) {
         (rc = dalloc(c, &c->d_g, nt)) || (rc = dalloc(c, &c->d_vt, nt)) || (rc = dalloc(c, &c->d_va, nt)) ||
         (rc = dalloc(c, &c->d_dp, (size_t)c->n_dp)) || (rc = dalloc(c, &c->d_tas, (size_t)c->n_tp)) ||
         (rc = dalloc(c, &c->d_vtp, (size_t)c->n_tp)) || (rc = dalloc(c, &c->d_x, (size_t)c->n_hhhh)) ||
-        (rc = dalloc(c, &c->d_vh, (size_t)c->n_hhhh)) || (rc = dalloc(c, &c->d_vp, (size_t)c->n_vp)) ||
+        (rc = dalloc(c, &c->d_vh, (size_t)c->n_hhhh)) ||
+        (rc = dalloc(c, &c->d_vp, c->matrix_free ? (size_t)1 : (size_t)c->n_vp)) ||
         (rc = dalloc(c, &c->d_vb, (size_t)c->n_b)) || (rc = dalloc(c, &c->d_rs, (size_t)std::max<int64_t>(npp, 1))) ||
         (rc = dalloc(c, &c->d_cp, (size_t)std::max<int64_t>(c->n_cp, 1))) ||
-        (rc = dalloc(c, &c->d_partials, std::max(c->utiles.size(), (size_t)1))) || (rc = dalloc(c, &c->d_scalar, 4)))
+        (rc = dalloc(c, &c->d_partials, std::max(c->utiles.size(), (size_t)1))) || (rc = dalloc(c, &c->d_scalar, 4)) ||
+        (rc = dalloc(c, &c->d_zero, (size_t)kZeroDoubles)))
         return rc;
+    HIP_TRY(hipMemset(c->d_zero, 0, sizeof(double) * kZeroDoubles));
     HIP_TRY(hipHostMalloc((void **)&c->h_scalar, 4 * sizeof(double), hipHostMallocDefault));
 
     // ---- interaction
@@ -557,9 +666,13 @@ int build(ccd_ctx *c, const ccd_problem *p) {
             if ((rc = upload(c, &c->d_table, tab))) return rc;
             G.table = c->d_table;
         }
-        if (!c->rtiles.empty())
+        if (c->matrix_free) {
+            if ((rc = build_generated_ladder(c, p, ppl, prow))) return rc;
+            c->lgen.G = G;
+        } else if (!c->rtiles.empty()) {
             hipLaunchKernelGGL(k_gen_vp, dim3((unsigned)c->rtiles.size()), dim3(kThreads), 0, c->stream,
                                c->d_rtiles, c->d_chans, c->d_ppl, c->d_prow, G, c->d_vp);
+        }
         if (!c->chans.empty())
             hipLaunchKernelGGL(k_gen_direct_small, dim3((unsigned)c->chans.size()), dim3(kThreads), 0,
                                c->stream, c->d_chans, c->d_ppl, c->d_hhl, G, c->d_vt, c->d_vh);
@@ -658,12 +771,17 @@ int ccd_ctx_create(const ccd_problem *prob, int precision, ccd_ctx **out) {
     *out = nullptr;
     int rc = validate(prob);
     if (rc) return rc;
+    const bool matrix_free = (precision & CCD_MATRIX_FREE) != 0;
+    precision &= ~CCD_MATRIX_FREE;
     if (precision != CCD_PREC_FP64) return ccd::fail(CCD_E_INVALID, "precision %d is not available", precision);
+    if (matrix_free && prob->gen_kind == CCD_GEN_NONE)
+        return ccd::fail(CCD_E_INVALID, "the matrix-free mode needs an on-device generator (gen_kind)");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return ccd::fail(CCD_E_HIP, "no HIP device available");
     ccd_ctx *c = new (std::nothrow) ccd_ctx();
     if (!c) return ccd::fail(CCD_E_NOMEM, "out of host memory");
     c->precision = precision;
+    c->matrix_free = matrix_free;
     if (hipGetDevice(&c->device) != hipSuccess ||
         hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
@@ -704,11 +822,11 @@ int ccd_work(const ccd_ctx *ctx, double *flops, double *flops_pppp, double *ladd
     if (flops) *flops = ctx->flops;
     if (flops_pppp) *flops_pppp = ctx->flops_pppp;
     if (ladder_bytes) {
-        // one k_ladder_v launch reads the packed V^pppp, the antisymmetrised
-        // amplitudes and X once and writes D_p once
+        // one k_ladder_v launch reads the packed V^pppp (unless generated), the
+        // antisymmetrised amplitudes and X once and writes D_p once
         double b = 0.0;
         for (const DChan &ch : ctx->chans)
-            b += (double)ch.npk * ch.npk + 2.0 * ch.npk * ch.hh + (double)ch.hh * ch.hh;
+            b += (ctx->matrix_free ? 0.0 : (double)ch.npk * ch.npk) + 2.0 * ch.npk * ch.hh + (double)ch.hh * ch.hh;
         *ladder_bytes = (ctx->precision == CCD_PREC_FP64 ? 8.0 : 4.0) * b;
     }
     return CCD_OK;

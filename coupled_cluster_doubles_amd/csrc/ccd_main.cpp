@@ -157,7 +157,8 @@ int main(int argc, char **argv) {
     const ccd_problem *prob = nullptr;
     ccd_model_problem(model, &prob);
     ccd_ctx *ctx = nullptr;
-    if ((rc = ccd_ctx_create(prob, o.precision, &ctx))) return die("ccd_ctx_create", rc);
+    const int mode = o.precision | (o.saveMemory ? CCD_MATRIX_FREE : 0);
+    if ((rc = ccd_ctx_create(prob, mode, &ctx))) return die("ccd_ctx_create", rc);
     int64_t dbytes = 0;
     ccd_ctx_info(ctx, nullptr, &dbytes);
     printf("rank %d device arrays require %g GB of memory\n", 0, dbytes / 1.e9);

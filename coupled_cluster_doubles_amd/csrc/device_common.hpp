@@ -8,6 +8,7 @@
 #include "ccd_amd.h"
 
 constexpr int kThreads = 256;   // every kernel runs 4 wave64s per block
+constexpr int kZeroDoubles = 32;  // size of the zero buffer masked-off loads read (256 B)
 
 struct DChan {          // active direct channel (pp > 0 and hh > 0)
     int64_t off_t;      // block offset in the pphh layout

@@ -8,11 +8,14 @@
 // contractions use v_mfma_f64_16x16x4f64 (layout in device_common.hpp).
 #pragma once
 
+#include <type_traits>
+
 #include "device_common.hpp"
 
 constexpr int KC = 32;                 // k per chunk of k_ladder_x (k-slot q of step s: k = 8q + s)
 constexpr int NTMAX = 5;               // 16-column tiles: hh <= 80
 constexpr int LROWS = 64;              // packed rows per ladder block (16 per wave)
+constexpr int kGasTableMax = 4096;     // entries of the generated-V gas table (LDS, 32 KB)
 
 // X = V^hhhh + sum_{c<d} V^hhpp[kl][cd] tas[cd][ij], one 16x16 tile of X per
 // block; the four waves take every fourth k chunk and their partial tiles are
@@ -58,19 +61,82 @@ __global__ __launch_bounds__(kThreads) void k_ladder_x(const DTile *__restrict__
     }
 }
 
+// Where the pppp ladder takes V^pppp from.
+enum LadderSource {
+    LV_STORED = 0,    // packed blocks in HBM (the stored-interaction mode)
+    LV_GAS = 1,       // electron gas, generated per element from a table in LDS
+    LV_GENERIC = 2,   // any generator, per element through gen_tbme
+};
+
+// Per-element generation of packed V^pppp (the matrix-free mode, the
+// calcFree* twins of src/calc.cpp, which rebuild V per channel with calcTBME).
+// Entries are indexed like the row-pair lists (DChan::pr_off + packed index).
+// Electron gas: inside a channel ka+kb = kc+kd and sa+sb = sc+sd, so
+//   <ab||cd> = [sa==sc] T(ka-kc) - [sb==sc] T(kb-kc),
+// with T the direct term, even in each component of the momentum transfer.
+// rec1 holds for the first state of a pair, rec2 for the second, the
+// momentum components and the spin pre-scaled into four 16-bit fields,
+// (x | y << 16, z | s << 16), so that two v_sad_u16 give
+//   sum of |rec(a) - rec(c)| over the fields = byte offset of T(|ka-kc|),
+// and a spin mismatch adds at least the offset of the table's final zero.
+// Generic: rec1 = (first state, second state).
+struct LadderGen {
+    const int2 *rec1;
+    const int2 *rec2;
+    const double *tab;   // gas: T over (|dx|,|dy|,|dz|), row-major, then one 0
+    uint32_t zoff;       // gas: byte offset of the final 0
+    int ntab;            // gas: entries including the final 0
+    Gen G;               // generic
+};
+
+typedef __attribute__((address_space(3))) const double lds_double;
+
+// LDS address of T(|ka-kc|) (or of the final 0): base + the field distances,
+// clamped to the final 0.
+__device__ __forceinline__ uint32_t gas_addr(int2 r, int2 c, uint32_t base, uint32_t zaddr) {
+    return min(__builtin_amdgcn_sad_u16((uint32_t)r.x, (uint32_t)c.x,
+                                        __builtin_amdgcn_sad_u16((uint32_t)r.y, (uint32_t)c.y, base)),
+               zaddr);
+}
+
+__device__ __forceinline__ double lds_load(uint32_t addr) {
+    return *reinterpret_cast<lds_double *>(static_cast<uintptr_t>(addr));
+}
+
+// Value of lane S of this lane's 16-lane row (DPP row_newbcast).
+template <int S>
+__device__ __forceinline__ int row_bcast(int v) {
+    return __builtin_amdgcn_mov_dpp(v, 0x150 + S, 0xF, 0xF, false);
+}
+
+template <int S, int N, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (S < N) {
+        f(std::integral_constant<int, S>{});
+        static_for<S + 1, N>(f);
+    }
+}
+
 // D_p[u][ij] = 1/2 sum_kl tas[u][kl] X[kl][ij] + sum_w Vp[u][w] tas[w][ij]
-// for 64 packed rows u of one channel per block (16 per wave) and NT*16 >= hh
-// columns, split over k segments of kseg_pairs(NT) packed pairs (tile.aux =
-// segment): segment s writes its partial sum to slab s of the channel's D_p,
-// and the update adds the slabs in segment order.  The k range of a block is
-// one sequence of KCH-wide chunks: for segment 0 first the hh columns of T X,
-// then its packed pairs of V t.  In a chunk, k-slot q of MFMA step s is
+// for 64 packed rows u of one channel per tile (16 per wave) and NT*16 >= hh
+// columns, split over k segments of KSEG packed pairs (tile.aux = segment):
+// segment s writes its partial sum to slab s of the channel's D_p, and the
+// update adds the slabs in segment order.  The k range of a tile is one
+// sequence of KCH-wide chunks: for segment 0 first the hh columns of T X, then
+// its packed pairs of V t.  In a chunk, k-slot q of MFMA step s is
 // k = 8 (s/2) + 2q + s%2: the four lanes of a row read 64 contiguous bytes per
 // 16-byte load, so a load instruction touches half cache lines whose other
 // halves the next instruction reads (rows are 128-byte aligned).  Software
 // pipeline: while the MFMAs of chunk c run, the A values of chunk c+1 (HBM)
 // and its B tile (L2-resident amplitudes / X) are loaded into registers; the
-// B tile then goes to the other LDS buffer, one barrier per chunk.
+// B tile then goes to the other LDS buffer, one barrier per chunk.  With a
+// generated V, lane j of 16-lane row q holds the record of k-slot (q, j) of
+// the chunk (G <= 16), prefetched one chunk ahead; MFMA step s broadcasts
+// lane s of each row, and each lane builds its A value from that record and
+// its row's records.  Every source gives the same values in the same
+// summation order, so the modes agree bit for bit.  Blocks loop over tiles
+// (the generated modes launch fewer blocks than tiles, so the gas table is
+// staged into LDS once per block).
 __device__ __forceinline__ int kslot(int q, int s) { return 8 * (s >> 1) + 2 * q + (s & 1); }
 
 template <int NT>
@@ -79,100 +145,158 @@ struct LadderShape {
     static constexpr int KSEG = NT <= 2 ? 256 : 64; // packed pairs per k segment
 };
 
-template <int NT>
-__global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__ tiles,
+template <int NT, int SRC>
+__global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__ tiles, int ntiles,
                                                        const DChan *__restrict__ chans,
-                                                       const double *__restrict__ vp,
+                                                       const double *__restrict__ vp, LadderGen lg,
                                                        const double *__restrict__ tas,
                                                        const double *__restrict__ x,
+                                                       const double *__restrict__ zero,
                                                        double *__restrict__ dp) {
     constexpr int KCH = LadderShape<NT>::KCH, KSEG = LadderShape<NT>::KSEG, G = KCH / 4;
     constexpr int W = NT * 16, STR = W + 1, BPT = (KCH * W + kThreads - 1) / kThreads;
+    constexpr bool GEN = SRC != LV_STORED;
+    static_assert(G <= 16, "one k-slot record per lane of a 16-lane row");
     __shared__ double Bs[2][KCH][STR];
-    const DTile tl = tiles[blockIdx.x];
-    const DChan ch = chans[tl.chan];
-    const int H = ch.hh, npk = ch.npk, ldv = ch.ldv, seg = tl.aux;
-    const int kbeg = seg * KSEG, kend = min(npk, kbeg + KSEG);
-    const int nx = seg == 0 ? (H + KCH - 1) / KCH : 0, nchunk = nx + (kend - kbeg + KCH - 1) / KCH;
+    extern __shared__ double gtab[];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
-    const int lrow = wave * 16 + m;
-    const bool rok = lrow < tl.nrows;
-    const int64_t u = tl.row0 + lrow;
-    const double *at = tas + ch.off_tp + u * H;
-    const double *av = vp + ch.off_vp + u * ldv;
+    if (SRC == LV_GAS) {
+        for (int i = threadIdx.x; i < lg.ntab; i += kThreads) gtab[i] = lg.tab[i];
+        __syncthreads();
+    }
+    const uint32_t tbase = (uint32_t)(uintptr_t)(lds_double *)gtab;  // LDS byte address
+    const uint32_t zaddr = tbase + lg.zoff;
 
-    auto load_a = [&](int c, double (&a)[G]) {
-        if (c < nx) {
+    for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        const DTile tl = tiles[ti];
+        const DChan ch = chans[tl.chan];
+        const int H = ch.hh, npk = ch.npk, ldv = ch.ldv, seg = tl.aux;
+        const int kbeg = seg * KSEG, kend = min(npk, kbeg + KSEG);
+        const int nx = seg == 0 ? (H + KCH - 1) / KCH : 0, nv = (kend - kbeg + KCH - 1) / KCH;
+        const int lrow = wave * 16 + m;
+        const bool rok = lrow < tl.nrows;
+        const int64_t u = tl.row0 + lrow;
+        const double *at = tas + ch.off_tp + u * H;
+        const double *av = GEN ? nullptr : vp + ch.off_vp + u * ldv;
+        int2 ra = {0, 0}, rb = {0, 0};
+        if (GEN) {
+            const int2 *z2 = reinterpret_cast<const int2 *>(zero);
+            ra = *(rok ? lg.rec1 + ch.pr_off + u : z2);
+            if (SRC == LV_GAS) rb = *(rok ? lg.rec2 + ch.pr_off + u : z2);
+        }
+
+        // Every load is unconditional: an element outside the block reads the
+        // zero buffer instead, so no load sits behind a branch and the
+        // prefetches stay in flight across the MFMAs.
+        auto load_t = [&](int c, double (&a)[G]) {   // T X chunk c
 #pragma unroll
             for (int s = 0; s < G; s++) {
                 const int k = c * KCH + kslot(kq, s);
-                a[s] = (rok && k < H) ? 0.5 * at[k] : 0.0;
+                a[s] = 0.5 * *((rok && k < H) ? at + k : zero);
             }
-        } else {
-            const int kc0 = kbeg + (c - nx) * KCH;
+        };
+        auto load_a = [&](int v, double (&a)[G]) {   // stored V: V chunk v
+            const int kc0 = kbeg + v * KCH;
 #pragma unroll
             for (int h = 0; h < G / 2; h++) {
                 const int k = kc0 + 8 * h + 2 * kq;  // even; ldv is even, so k+1 < ldv too
-                if (rok && k < kend && k < ldv) {
-                    const dbl2 v = *reinterpret_cast<const dbl2 *>(av + k);
-                    a[2 * h] = v.x;
-                    a[2 * h + 1] = v.y;
-                } else {
-                    a[2 * h] = 0.0;
-                    a[2 * h + 1] = 0.0;
-                }
+                const dbl2 w = *reinterpret_cast<const dbl2 *>((rok && k < kend && k < ldv) ? av + k : zero);
+                a[2 * h] = w.x;
+                a[2 * h + 1] = w.y;
             }
-        }
-    };
-    auto load_b = [&](int c, double (&b)[BPT]) {
-        const double *B = c < nx ? x + ch.off_hhhh : tas + ch.off_tp;
-        const int K = c < nx ? H : kend, k0 = c < nx ? c * KCH : kbeg + (c - nx) * KCH;
+        };
+        auto load_r = [&](int v, int2 &rr) {  // generated V: record of k-slot (kq, m) of V chunk v
+            const int k = kbeg + v * KCH + kslot(kq, m);
+            rr = *((m < G && k < kend) ? lg.rec1 + ch.pr_off + k : reinterpret_cast<const int2 *>(zero));
+        };
+        auto gen_a = [&](int2 rr, double (&a)[G]) {  // generated V from the chunk's records
+            static_for<0, G>([&](auto S) {
+                constexpr int s = decltype(S)::value;
+                const int2 r = {row_bcast<s>(rr.x), row_bcast<s>(rr.y)};
+                if (SRC == LV_GAS) {
+                    a[s] = lds_load(gas_addr(ra, r, tbase, zaddr)) - lds_load(gas_addr(rb, r, tbase, zaddr));
+                } else {
+                    a[s] = gen_tbme(lg.G, ra.x, ra.y, r.x, r.y);
+                }
+            });
+        };
+        // B tile of k rows k0.. of a K x H row-major block
+        auto load_b = [&](const double *B, int K, int k0, double (&b)[BPT]) {
 #pragma unroll
-        for (int q = 0; q < BPT; q++) {
-            const int idx = threadIdx.x + kThreads * q, kk = idx / W, col = idx - kk * W, k = k0 + kk;
-            b[q] = (idx < KCH * W && k < K && col < H) ? B[(int64_t)k * H + col] : 0.0;
-        }
-    };
-    auto store_b = [&](int buf, const double (&b)[BPT]) {
+            for (int q = 0; q < BPT; q++) {
+                const int idx = threadIdx.x + kThreads * q, kk = idx / W, col = idx - kk * W, k = k0 + kk;
+                b[q] = *((idx < KCH * W && k < K && col < H) ? B + (int64_t)k * H + col : zero);
+            }
+        };
+        auto store_b = [&](int buf, const double (&b)[BPT]) {
 #pragma unroll
-        for (int q = 0; q < BPT; q++) {
-            const int idx = threadIdx.x + kThreads * q, kk = idx / W, col = idx - kk * W;
-            if (idx < KCH * W) Bs[buf][kk][col] = b[q];
-        }
-    };
+            for (int q = 0; q < BPT; q++) {
+                const int idx = threadIdx.x + kThreads * q, kk = idx / W, col = idx - kk * W;
+                if (idx < KCH * W) Bs[buf][kk][col] = b[q];
+            }
+        };
 
-    dbl4 acc[NT];
+        dbl4 acc[NT];
 #pragma unroll
-    for (int n = 0; n < NT; n++) acc[n] = dbl4{0.0, 0.0, 0.0, 0.0};
-    double a[G], an[G] = {}, b[BPT];
-    load_a(0, a);
-    load_b(0, b);
-    store_b(0, b);
-    __syncthreads();
-    for (int c = 0; c < nchunk; c++) {
-        const bool more = c + 1 < nchunk;
-        if (more) {
-            load_a(c + 1, an);
-            load_b(c + 1, b);
+        for (int n = 0; n < NT; n++) acc[n] = dbl4{0.0, 0.0, 0.0, 0.0};
+        auto mfma_chunk = [&](int buf, const double (&a)[G]) {
+#pragma unroll
+            for (int s = 0; s < G; s++) {
+#pragma unroll
+                for (int n = 0; n < NT; n++) acc[n] = mfma16(a[s], Bs[buf][kslot(kq, s)][n * 16 + m], acc[n]);
+            }
+        };
+        double b[BPT];
+
+        // T X chunks (segment 0 only), unpipelined: at most a few per tile
+        for (int c = 0; c < nx; c++) {
+            double a[G];
+            load_t(c, a);
+            load_b(x + ch.off_hhhh, H, c * KCH, b);
+            __syncthreads();            // earlier readers of buffer 0 are done
+            store_b(0, b);
+            __syncthreads();
+            mfma_chunk(0, a);
         }
-        const int cb = c & 1;
-#pragma unroll
-        for (int s = 0; s < G; s++) {
-#pragma unroll
-            for (int n = 0; n < NT; n++) acc[n] = mfma16(a[s], Bs[cb][kslot(kq, s)][n * 16 + m], acc[n]);
-        }
-        if (more) store_b(cb ^ 1, b);
         __syncthreads();
+
+        // V chunks, software-pipelined over two register sets and two LDS
+        // buffers, unrolled by two so that no copy joins the sets: while the
+        // MFMAs of chunk v run, chunk v+1 is in flight, and the wait for it
+        // comes only before its own MFMAs, with chunk v+2 already issued.
+        // Prefetches past the last chunk read the zero buffer.  B is loaded
+        // before A: loads complete in issue order, so the LDS store of the B
+        // tile does not wait for the A prefetch.
+        const double *tb0 = tas + ch.off_tp;
+        double a0[G], a1[G];
+        int2 r0 = {0, 0}, r1 = {0, 0};
+        auto stage = [&](int v, int buf, double (&acur)[G], double (&anext)[G], int2 rcur, int2 &rnext) {
+            load_b(tb0, kend, kbeg + (v + 1) * KCH, b);
+            if (!GEN) load_a(v + 1, anext);
+            else load_r(v + 1, rnext);
+            if (GEN) gen_a(rcur, acur);
+            mfma_chunk(buf, acur);
+            store_b(buf ^ 1, b);
+            __syncthreads();
+        };
+        load_b(tb0, kend, kbeg, b);
+        if (!GEN) load_a(0, a0);
+        else load_r(0, r0);
+        store_b(0, b);
+        __syncthreads();
+        for (int v = 0; v < nv; v += 2) {
+            stage(v, 0, a0, a1, r0, r1);
+            if (v + 1 >= nv) break;
+            stage(v + 1, 1, a1, a0, r1, r0);
+        }
 #pragma unroll
-        for (int s = 0; s < G; s++) a[s] = an[s];
-    }
+        for (int n = 0; n < NT; n++) {
 #pragma unroll
-    for (int n = 0; n < NT; n++) {
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int row = wave * 16 + q * 4 + kq, col = n * 16 + m;
-            if (row < tl.nrows && col < H)
-                dp[ch.off_dp + (int64_t)seg * npk * H + (int64_t)(tl.row0 + row) * H + col] = acc[n][q];
+            for (int q = 0; q < 4; q++) {
+                const int row = wave * 16 + q * 4 + kq, col = n * 16 + m;
+                if (row < tl.nrows && col < H)
+                    dp[ch.off_dp + (int64_t)seg * npk * H + (int64_t)(tl.row0 + row) * H + col] = acc[n][q];
+            }
         }
     }
 }

@@ -49,6 +49,12 @@ extern "C" {
 /* Arithmetic of the amplitude contractions. */
 #define CCD_PREC_FP64 0      /* everything FP64 (the parity path) */
 #define CCD_PREC_FP32 1      /* FP32 contractions, FP64 update/denominators/energy */
+/* OR-ed into ccd_ctx_create's precision: the matrix-free mode of the reference
+ * (-m 1, the calcFree* twins of src/calc.cpp:45-504 driven by
+ * src/main.cpp:521-588).  V^pppp is never stored: the pppp ladder rebuilds
+ * each element from the generator as it goes, bit-identical to the stored
+ * mode.  Needs gen_kind != CCD_GEN_NONE. */
+#define CCD_MATRIX_FREE 0x100
 
 /* Interaction source for a ccd_problem. */
 #define CCD_GEN_NONE 0           /* V blocks supplied by the caller (v_* below) */
@@ -102,7 +108,8 @@ const char *ccd_last_error(void);
 /* ---------------------------------------------------------------- device */
 
 /* Validate the descriptor, build the device tables and upload (or generate)
- * the interaction on the current HIP device.  T2 starts at zero. */
+ * the interaction on the current HIP device.  T2 starts at zero.
+ * precision: CCD_PREC_*, optionally | CCD_MATRIX_FREE. */
 int ccd_ctx_create(const ccd_problem *prob, int precision, ccd_ctx **out);
 int ccd_ctx_destroy(ccd_ctx *ctx);
 
@@ -163,7 +170,9 @@ int ccd_time_phases(ccd_ctx *ctx, int iters, double mixing, double *ms);
 
 /* Algorithmic work of one iteration in the reference's ordered-pair algebra:
  * flops F = sum_c (2hh^2 pp + 2pp hh^2 + 2pp^2 hh) + sum_m (2hp^2 ph + 2ph hp^2)
- * and the pppp-ladder part of it; bytes of V^pppp read by one ladder launch. */
+ * and the pppp-ladder part of it; bytes one pass of the packed pppp ladder
+ * moves: V^pppp for pairs a<b, c<d (none in the matrix-free mode), the
+ * antisymmetrised amplitudes and X read once, D_p written once. */
 int ccd_work(const ccd_ctx *ctx, double *flops, double *flops_pppp,
              double *bytes_vpppp);
 

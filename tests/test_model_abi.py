@@ -78,3 +78,15 @@ def test_model_rejects_bad_arguments():
 def test_headline_sizes():
     m = Model.electron_gas(1.0, 32, 4)
     assert (m.n_sp, m.n_holes, m.n_t2) == (1850, 54, 2925948)
+
+
+def test_context_rejects_bad_modes_before_touching_a_device():
+    """Argument errors come back as CCD_E_INVALID from host validation."""
+    from coupled_cluster_doubles_amd import FP64, MATRIX_FREE, Context, _lib
+    m = Model.electron_gas(1.0, 4, 2)
+    with pytest.raises(CCDError, match="generator"):
+        Context(m, FP64, host_v=True, matrix_free=True)   # nothing to regenerate V from
+    with pytest.raises(CCDError, match="precision"):
+        Context(m, 7)
+    h = _lib.C.c_void_p()
+    assert _lib.lib().ccd_ctx_create(m.problem(True), FP64 | MATRIX_FREE, _lib.C.byref(h)) == -1
