@@ -160,6 +160,20 @@ class Context:
                               C.byref(stop)), "ccd_solve")
         return en[:min(n.value, cap)].copy(), STOP_REASONS[stop.value], n.value
 
+    def solve_mixed(self, tol: float, switch_tol: float, max_iters: int = 100, mixing: float = 0.5,
+                    cap: int = 256):
+        """FP32 contractions while |dE| > switch_tol, then FP64 sweeps to |dE| <= tol
+        (contexts created with FP32).  Returns (energies, stop, n_iter, n_fp32)."""
+        en = np.zeros(cap)
+        n, n32, stop = C.c_int(), C.c_int(), C.c_int()
+        check(lib().ccd_solve_mixed(self.h, tol, switch_tol, max_iters, mixing, _ptr(en), cap,
+                                    C.byref(n), C.byref(n32), C.byref(stop)), "ccd_solve_mixed")
+        return en[:min(n.value, cap)].copy(), STOP_REASONS[stop.value], n.value, n32.value
+
+    def set_precision(self, precision: int):
+        """Arithmetic of the contractions of the following iterations (FP64 or FP32)."""
+        check(lib().ccd_set_precision(self.h, precision), "ccd_set_precision")
+
     def get_t2(self) -> np.ndarray:
         t = np.empty(self.n_t2)
         check(lib().ccd_get_t2(self.h, _ptr(t)), "ccd_get_t2")

@@ -39,11 +39,14 @@ SIGNATURES = [
     ("ccd_ctx_create", C.c_int, [vp, C.c_int, C.POINTER(vp)]),
     ("ccd_ctx_destroy", C.c_int, [vp]),
     ("ccd_ctx_info", C.c_int, [vp, i64p, i64p]),
+    ("ccd_set_precision", C.c_int, [vp, C.c_int]),
     ("ccd_mbpt2", C.c_int, [vp, dp]),
     ("ccd_reset", C.c_int, [vp]),
     ("ccd_iterate", C.c_int, [vp, C.c_double, dp]),
     ("ccd_solve", C.c_int, [vp, C.c_double, C.c_int, C.c_double, ip, C.c_int,
                             C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    ("ccd_solve_mixed", C.c_int, [vp, C.c_double, C.c_double, C.c_int, C.c_double, ip, C.c_int,
+                                  C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("ccd_get_t2", C.c_int, [vp, ip]),
     ("ccd_set_t2", C.c_int, [vp, ip]),
     ("ccd_ladder", C.c_int, [vp, ip, ip]),

@@ -84,8 +84,9 @@ struct ccd_ctx {
     std::vector<DTile> rtiles, ltiles, utiles, xtiles;  // packed-row, ladder, update, X tiles
     int lt_off[NTMAX + 1] = {0};  // ladder tiles grouped by 16-column tile count: [lt_off[n-1], lt_off[n])
     int64_t n_cp = 0;            // column-partial slots
-    int ring_stride = 0;         // LDS row stride of the ring kernel
-    size_t ring_lds = 0, tr_lds = 0;
+    std::vector<RingItem> ritems;  // ring work items, largest channels first
+    size_t tr_lds = 0;
+    int iter_prec = CCD_PREC_FP64;  // arithmetic of the contractions of the next iterations
     // device tables
     DChan *d_chans = nullptr;
     DMod *d_mods = nullptr;
@@ -109,6 +110,8 @@ struct ccd_ctx {
     int32_t *d_qn = nullptr;
     double *d_table = nullptr;
     int2 *d_rec1 = nullptr, *d_rec2 = nullptr;  // generated-V pair records
+    RingItem *d_ritems = nullptr;
+    float *d_vp32 = nullptr, *d_va32 = nullptr, *d_vb32 = nullptr;  // FP32 copies (mixed precision)
     double *d_gtab = nullptr;                   // generated-V electron gas table
     int64_t n_vp = 0, n_tp = 0, n_dp = 0, n_hhhh = 0, n_b = 0;
     double flops = 0, flops_pppp = 0;
@@ -164,9 +167,15 @@ UpdArgs upd_args(ccd_ctx *c, const double *t_old, double *t_new, double mixing) 
 // --- launch helpers, one per phase of the iteration
 
 int launch_ring(ccd_ctx *c) {
-    if (c->mods.empty()) return CCD_OK;
-    hipLaunchKernelGGL(k_ring, dim3((unsigned)c->mods.size()), dim3(kThreads), c->ring_lds, c->stream, c->d_mods,
-                       c->ring_stride, c->d_tph, c->d_va, c->d_vb, c->d_r);
+    if (c->ritems.empty()) return CCD_OK;
+    const int n = (int)c->ritems.size();
+    const dim3 grid((unsigned)((n + kThreads / 64 - 1) / (kThreads / 64)));
+    if (c->iter_prec == CCD_PREC_FP32)
+        hipLaunchKernelGGL(k_ring<float>, grid, dim3(kThreads), 0, c->stream, c->d_ritems, n, c->d_mods, c->d_tph,
+                           c->d_va32, c->d_vb32, c->d_zero, c->d_r);
+    else
+        hipLaunchKernelGGL(k_ring<double>, grid, dim3(kThreads), 0, c->stream, c->d_ritems, n, c->d_mods, c->d_tph,
+                           c->d_va, c->d_vb, c->d_zero, c->d_r);
     return check_launch("k_ring");
 }
 
@@ -198,31 +207,37 @@ int launch_ladder_x(ccd_ctx *c) {
 
 // Stored V: one block per tile.  Generated V: at most gen_blocks blocks
 // looping over the tiles, the gas table in dynamic LDS.
-template <int NT, int SRC>
-void launch_ladder_v_nt(ccd_ctx *c) {
+template <typename CT, int NT, int SRC>
+void launch_ladder_v_nt(ccd_ctx *c, const CT *vp) {
     const int n = c->lt_off[NT] - c->lt_off[NT - 1];
     if (n <= 0) return;
     const int grid = SRC == LV_STORED ? n : std::min(n, c->gen_blocks);
     const size_t lds = SRC == LV_GAS ? sizeof(double) * (size_t)c->lgen.ntab : 0;
-    hipLaunchKernelGGL((k_ladder_v<NT, SRC>), dim3((unsigned)grid), dim3(kThreads), lds, c->stream,
-                       c->d_ltiles + c->lt_off[NT - 1], n, c->d_chans, c->d_vp, c->lgen, c->d_tas, c->d_x,
+    hipLaunchKernelGGL((k_ladder_v<CT, NT, SRC>), dim3((unsigned)grid), dim3(kThreads), lds, c->stream,
+                       c->d_ltiles + c->lt_off[NT - 1], n, c->d_chans, vp, c->lgen, c->d_tas, c->d_x,
                        c->d_zero, c->d_dp);
 }
 
-template <int SRC>
-void launch_ladder_v_src(ccd_ctx *c) {
-    launch_ladder_v_nt<1, SRC>(c);
-    launch_ladder_v_nt<2, SRC>(c);
-    launch_ladder_v_nt<3, SRC>(c);
-    launch_ladder_v_nt<4, SRC>(c);
-    launch_ladder_v_nt<5, SRC>(c);
+template <typename CT, int SRC>
+void launch_ladder_v_src(ccd_ctx *c, const CT *vp) {
+    launch_ladder_v_nt<CT, 1, SRC>(c, vp);
+    launch_ladder_v_nt<CT, 2, SRC>(c, vp);
+    launch_ladder_v_nt<CT, 3, SRC>(c, vp);
+    launch_ladder_v_nt<CT, 4, SRC>(c, vp);
+    launch_ladder_v_nt<CT, 5, SRC>(c, vp);
+}
+
+template <typename CT>
+void launch_ladder_v_ct(ccd_ctx *c, const CT *vp) {
+    if (c->lsrc == LV_GAS) launch_ladder_v_src<CT, LV_GAS>(c, vp);
+    else if (c->lsrc == LV_GENERIC) launch_ladder_v_src<CT, LV_GENERIC>(c, vp);
+    else launch_ladder_v_src<CT, LV_STORED>(c, vp);
 }
 
 int launch_ladder_v(ccd_ctx *c) {
     if (c->ltiles.empty()) return CCD_OK;
-    if (c->lsrc == LV_GAS) launch_ladder_v_src<LV_GAS>(c);
-    else if (c->lsrc == LV_GENERIC) launch_ladder_v_src<LV_GENERIC>(c);
-    else launch_ladder_v_src<LV_STORED>(c);
+    if (c->iter_prec == CCD_PREC_FP32) launch_ladder_v_ct<float>(c, c->d_vp32);
+    else launch_ladder_v_ct<double>(c, c->d_vp);
     return check_launch("k_ladder_v");
 }
 
@@ -478,7 +493,6 @@ int build(ccd_ctx *c, const ccd_problem *p) {
     std::vector<int32_t> colidx((size_t)nh * np, -1);
     std::vector<int32_t> mod_of_ph((size_t)np * nh, -1), mod_of_hp((size_t)nh * np, -1);
     int64_t off_ph = 0, off_b = 0, iph = 0, ihp = 0;
-    int max_hp = 0;
     for (int k = 0; k < p->n_mod; k++) {
         const int PH = p->ph_dim[k], HP = p->hp_dim[k];
         for (int q = 0; q < PH; q++) {
@@ -497,9 +511,9 @@ int build(ccd_ctx *c, const ccd_problem *p) {
         if (PH > 0 && HP > 0) {
             if (PH > NTMAX * 16 || HP > NTMAX * 16)
                 return ccd::fail(CCD_E_INVALID, "mod channel %d has ph=%d hp=%d > %d", k, PH, HP, NTMAX * 16);
+            for (int jt = 0; jt < (HP + 15) / 16; jt++) c->ritems.push_back(RingItem{(int32_t)c->mods.size(), jt});
             c->mods.push_back(DMod{off_ph, off_b, PH, HP, (int32_t)iph, (int32_t)ihp});
             off_b += (int64_t)HP * HP;
-            max_hp = std::max(max_hp, HP);
             c->flops += 2.0 * HP * HP * PH + 2.0 * PH * HP * HP;
         }
         off_ph += (int64_t)PH * HP;
@@ -509,9 +523,13 @@ int build(ccd_ctx *c, const ccd_problem *p) {
     if (off_ph != c->nt2)
         return ccd::fail(CCD_E_INVALID, "phhp layout has %lld elements, pphh has %lld", (long long)off_ph,
                          (long long)c->nt2);
-    c->ring_stride = (max_hp + 15) / 16 * 16 + 1;
-    c->ring_lds = (size_t)((max_hp + 15) / 16 * 16) * c->ring_stride * sizeof(double);
-    if (c->ring_lds > 64 * 1024) return ccd::fail(CCD_E_INVALID, "ring LDS %zu bytes > 64 KB", c->ring_lds);
+    {   // ring items, channels with more 16-wide tiles first (stable: channel order inside a size)
+        auto tiles_of = [&](const RingItem &w) {
+            return (std::max(c->mods[w.mod].ph, c->mods[w.mod].hp) + 15) / 16;
+        };
+        std::stable_sort(c->ritems.begin(), c->ritems.end(),
+                         [&](const RingItem &a, const RingItem &b) { return tiles_of(a) > tiles_of(b); });
+    }
 
     // ---- per-particle transposition plans between the two layouts:
     // particle a's block holds the elements (a,b,i,j); its pphh slots run over
@@ -625,7 +643,8 @@ int build(ccd_ctx *c, const ccd_problem *p) {
     if ((rc = upload(c, &c->d_chans, c->chans)) || (rc = upload(c, &c->d_mods, c->mods)) ||
         (rc = upload(c, &c->d_rtiles, c->rtiles)) || (rc = upload(c, &c->d_ltiles, c->ltiles)) ||
         (rc = upload(c, &c->d_utiles, c->utiles)) ||
-        (rc = upload(c, &c->d_xtiles, c->xtiles)) || (rc = upload(c, &c->d_ppl, ppl)) ||
+        (rc = upload(c, &c->d_xtiles, c->xtiles)) || (rc = upload(c, &c->d_ritems, c->ritems)) ||
+        (rc = upload(c, &c->d_ppl, ppl)) ||
         (rc = upload(c, &c->d_hhl, hhl)) || (rc = upload(c, &c->d_phl, phl)) || (rc = upload(c, &c->d_hpl, hpl)) ||
         (rc = upload(c, &c->d_hh_swap, hh_swap)) || (rc = upload(c, &c->d_prow, prow)) ||
         (rc = upload(c, &c->d_prow_sw, prow_sw)) || (rc = upload(c, &c->d_red_ptr, red_ptr)) ||
@@ -732,6 +751,18 @@ int build(ccd_ctx *c, const ccd_problem *p) {
                            c->d_chans, c->d_prow, c->d_vt, c->d_vtp);
         if ((rc = check_launch("k_pack_vt"))) return rc;
     }
+    if (c->precision == CCD_PREC_FP32) {   // FP32 copies for the mixed-precision iterations
+        auto to_f32 = [&](const double *src, float **dst, int64_t n) {
+            int r = dalloc(c, dst, (size_t)std::max<int64_t>(n, 1));
+            if (r || n == 0) return r;
+            const unsigned g = (unsigned)std::min<int64_t>((n + kThreads - 1) / kThreads, 4096);
+            hipLaunchKernelGGL(k_to_f32, dim3(g), dim3(kThreads), 0, c->stream, src, *dst, n);
+            return check_launch("k_to_f32");
+        };
+        if ((rc = to_f32(c->d_va, &c->d_va32, c->nt2)) || (rc = to_f32(c->d_vb, &c->d_vb32, c->n_b)) ||
+            (rc = to_f32(c->d_vp, &c->d_vp32, c->matrix_free ? 0 : c->n_vp)))
+            return rc;
+    }
     HIP_TRY(hipStreamSynchronize(c->stream));
     return CCD_OK;
 }
@@ -773,7 +804,8 @@ int ccd_ctx_create(const ccd_problem *prob, int precision, ccd_ctx **out) {
     if (rc) return rc;
     const bool matrix_free = (precision & CCD_MATRIX_FREE) != 0;
     precision &= ~CCD_MATRIX_FREE;
-    if (precision != CCD_PREC_FP64) return ccd::fail(CCD_E_INVALID, "precision %d is not available", precision);
+    if (precision != CCD_PREC_FP64 && precision != CCD_PREC_FP32)
+        return ccd::fail(CCD_E_INVALID, "precision %d is not available", precision);
     if (matrix_free && prob->gen_kind == CCD_GEN_NONE)
         return ccd::fail(CCD_E_INVALID, "the matrix-free mode needs an on-device generator (gen_kind)");
     int ndev = 0;
@@ -781,6 +813,7 @@ int ccd_ctx_create(const ccd_problem *prob, int precision, ccd_ctx **out) {
     ccd_ctx *c = new (std::nothrow) ccd_ctx();
     if (!c) return ccd::fail(CCD_E_NOMEM, "out of host memory");
     c->precision = precision;
+    c->iter_prec = precision;
     c->matrix_free = matrix_free;
     if (hipGetDevice(&c->device) != hipSuccess ||
         hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -824,10 +857,13 @@ int ccd_work(const ccd_ctx *ctx, double *flops, double *flops_pppp, double *ladd
     if (ladder_bytes) {
         // one k_ladder_v launch reads the packed V^pppp (unless generated), the
         // antisymmetrised amplitudes and X once and writes D_p once
+        // (V^pppp in the arithmetic type of the iteration, the rest in FP64)
+        const double vb = ctx->iter_prec == CCD_PREC_FP64 ? 8.0 : 4.0;
         double b = 0.0;
         for (const DChan &ch : ctx->chans)
-            b += (ctx->matrix_free ? 0.0 : (double)ch.npk * ch.npk) + 2.0 * ch.npk * ch.hh + (double)ch.hh * ch.hh;
-        *ladder_bytes = (ctx->precision == CCD_PREC_FP64 ? 8.0 : 4.0) * b;
+            b += (ctx->matrix_free ? 0.0 : vb * ch.npk * ch.npk) +
+                 8.0 * (2.0 * ch.npk * ch.hh + (double)ch.hh * ch.hh);
+        *ladder_bytes = b;
     }
     return CCD_OK;
 }
@@ -891,6 +927,59 @@ int ccd_solve(ccd_ctx *c, double tol, int max_iters, double mixing, double *ener
     }
     if (why == CCD_STOP_CONVERGED && std::isnan(diff)) why = CCD_STOP_NAN;
     if (n_iter) *n_iter = it;
+    if (stop) *stop = why;
+    return CCD_OK;
+}
+
+int ccd_set_precision(ccd_ctx *c, int precision) {
+    ccd::clear_error();
+    if (!c) return ccd::fail(CCD_E_INVALID, "null context");
+    if (precision != CCD_PREC_FP64 && precision != CCD_PREC_FP32)
+        return ccd::fail(CCD_E_INVALID, "unknown precision %d", precision);
+    if (precision == CCD_PREC_FP32 && c->precision != CCD_PREC_FP32)
+        return ccd::fail(CCD_E_STATE, "FP32 iterations need a context created with CCD_PREC_FP32");
+    c->iter_prec = precision;
+    return CCD_OK;
+}
+
+int ccd_solve_mixed(ccd_ctx *c, double tol, double switch_tol, int max_iters, double mixing, double *energies,
+                    int cap, int *n_iter, int *n_fp32, int *stop) {
+    ccd::clear_error();
+    if (!c) return ccd::fail(CCD_E_INVALID, "null context");
+    if (c->precision != CCD_PREC_FP32)
+        return ccd::fail(CCD_E_STATE, "ccd_solve_mixed needs a context created with CCD_PREC_FP32");
+    int rc = zero_state(c);
+    if (rc) return rc;
+    const int saved = c->iter_prec;
+    c->iter_prec = CCD_PREC_FP32;
+    double diff = 1.0, prev = 100.0, corr = 100.0;
+    int it = 0, n32 = 0, why = CCD_STOP_CONVERGED;
+    while (std::fabs(diff) > tol) {
+        if (it > max_iters) {
+            why = CCD_STOP_MAX_ITERS;
+            break;
+        }
+        // FP32 contractions while the iteration moves by more than switch_tol,
+        // then FP64 sweeps to the FP64 fixed point
+        if (c->iter_prec == CCD_PREC_FP32 && std::fabs(diff) <= switch_tol) c->iter_prec = CCD_PREC_FP64;
+        if (c->iter_prec == CCD_PREC_FP32) n32++;
+        if ((rc = run_iteration(c, mixing, nullptr))) {
+            c->iter_prec = saved;
+            return rc;
+        }
+        if ((rc = read_scalar(c, &corr))) {
+            c->iter_prec = saved;
+            return rc;
+        }
+        diff = corr - prev;
+        prev = corr;
+        if (energies && it < cap) energies[it] = corr;
+        it++;
+    }
+    c->iter_prec = saved;
+    if (why == CCD_STOP_CONVERGED && std::isnan(diff)) why = CCD_STOP_NAN;
+    if (n_iter) *n_iter = it;
+    if (n_fp32) *n_fp32 = n32;
     if (stop) *stop = why;
     return CCD_OK;
 }

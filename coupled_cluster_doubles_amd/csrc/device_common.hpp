@@ -51,6 +51,8 @@ struct Gen {            // on-device interaction
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef double dbl2 __attribute__((ext_vector_type(2)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+typedef float flt2 __attribute__((ext_vector_type(2)));
 
 // Fixed-order block reduction: wave butterfly, then the four wave sums in order.
 // Every thread of the block must call it; the result is valid on thread 0.
@@ -96,3 +98,21 @@ __device__ inline double gen_tbme(const Gen &G, int p, int q, int r, int s) {
 __device__ __forceinline__ dbl4 mfma16(double a, double b, dbl4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
+
+// v_mfma_f32_16x16x4f32: A and B as above; C register q of lane l holds row
+// 4 (l/16) + q, column l%16.
+__device__ __forceinline__ flt4 mfma16(float a, float b, flt4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Per-type view of the 16x16 MFMA tile: accumulator type and the output row
+// that register q of lane l holds.
+template <typename CT> struct Mma;
+template <> struct Mma<double> {
+    typedef dbl4 acc;
+    static __device__ __forceinline__ int row(int q, int lane) { return 4 * q + (lane >> 4); }
+};
+template <> struct Mma<float> {
+    typedef flt4 acc;
+    static __device__ __forceinline__ int row(int q, int lane) { return 4 * (lane >> 4) + q; }
+};

@@ -137,7 +137,16 @@ __device__ __forceinline__ void static_for(F &&f) {
 // summation order, so the modes agree bit for bit.  Blocks loop over tiles
 // (the generated modes launch fewer blocks than tiles, so the gas table is
 // staged into LDS once per block).
-__device__ __forceinline__ int kslot(int q, int s) { return 8 * (s >> 1) + 2 * q + (s & 1); }
+//
+// Arithmetic type CT: double, or float for the mixed-precision iterations
+// (V^pppp stored in FP32, or generated in FP64 and rounded; amplitudes and X
+// rounded on load; FP32 accumulation; D_p written in FP64).  For float the
+// k-slot order is k = 16 (s/4) + 4q + s%4, again 64 contiguous bytes per row
+// and load instruction.
+template <typename CT>
+__device__ __forceinline__ int kslot(int q, int s) {
+    return sizeof(CT) == 8 ? 8 * (s >> 1) + 2 * q + (s & 1) : 16 * (s >> 2) + 4 * q + (s & 3);
+}
 
 template <int NT>
 struct LadderShape {
@@ -145,10 +154,10 @@ struct LadderShape {
     static constexpr int KSEG = NT <= 2 ? 256 : 64; // packed pairs per k segment
 };
 
-template <int NT, int SRC>
+template <typename CT, int NT, int SRC>
 __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__ tiles, int ntiles,
                                                        const DChan *__restrict__ chans,
-                                                       const double *__restrict__ vp, LadderGen lg,
+                                                       const CT *__restrict__ vp, LadderGen lg,
                                                        const double *__restrict__ tas,
                                                        const double *__restrict__ x,
                                                        const double *__restrict__ zero,
@@ -156,8 +165,11 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
     constexpr int KCH = LadderShape<NT>::KCH, KSEG = LadderShape<NT>::KSEG, G = KCH / 4;
     constexpr int W = NT * 16, STR = W + 1, BPT = (KCH * W + kThreads - 1) / kThreads;
     constexpr bool GEN = SRC != LV_STORED;
+    constexpr int VW = 16 / sizeof(CT);   // elements per 16-byte load of stored V
+    typedef typename Mma<CT>::acc Acc;
+    typedef CT vec __attribute__((ext_vector_type(VW)));
     static_assert(G <= 16, "one k-slot record per lane of a 16-lane row");
-    __shared__ double Bs[2][KCH][STR];
+    __shared__ CT Bs[2][KCH][STR];
     extern __shared__ double gtab[];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
     if (SRC == LV_GAS) {
@@ -177,7 +189,7 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
         const bool rok = lrow < tl.nrows;
         const int64_t u = tl.row0 + lrow;
         const double *at = tas + ch.off_tp + u * H;
-        const double *av = GEN ? nullptr : vp + ch.off_vp + u * ldv;
+        const CT *av = GEN ? nullptr : vp + ch.off_vp + u * ldv;
         int2 ra = {0, 0}, rb = {0, 0};
         if (GEN) {
             const int2 *z2 = reinterpret_cast<const int2 *>(zero);
@@ -188,47 +200,51 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
         // Every load is unconditional: an element outside the block reads the
         // zero buffer instead, so no load sits behind a branch and the
         // prefetches stay in flight across the MFMAs.
-        auto load_t = [&](int c, double (&a)[G]) {   // T X chunk c
+        auto load_t = [&](int c, CT (&a)[G]) {   // T X chunk c
 #pragma unroll
             for (int s = 0; s < G; s++) {
-                const int k = c * KCH + kslot(kq, s);
-                a[s] = 0.5 * *((rok && k < H) ? at + k : zero);
+                const int k = c * KCH + kslot<CT>(kq, s);
+                a[s] = (CT)(0.5 * *((rok && k < H) ? at + k : zero));
             }
         };
-        auto load_a = [&](int v, double (&a)[G]) {   // stored V: V chunk v
+        auto load_a = [&](int v, CT (&a)[G]) {   // stored V: V chunk v
             const int kc0 = kbeg + v * KCH;
 #pragma unroll
-            for (int h = 0; h < G / 2; h++) {
-                const int k = kc0 + 8 * h + 2 * kq;  // even; ldv is even, so k+1 < ldv too
-                const dbl2 w = *reinterpret_cast<const dbl2 *>((rok && k < kend && k < ldv) ? av + k : zero);
-                a[2 * h] = w.x;
-                a[2 * h + 1] = w.y;
+            for (int h = 0; h < G / VW; h++) {
+                // k-slots (kq, VW h .. VW h + VW-1) are VW consecutive k; k and
+                // ldv are multiples of VW, so the group lies inside the row
+                const int k = kc0 + kslot<CT>(kq, VW * h);
+                const vec w = *reinterpret_cast<const vec *>((rok && k < kend && k < ldv)
+                                                                 ? av + k
+                                                                 : reinterpret_cast<const CT *>(zero));
+#pragma unroll
+                for (int e = 0; e < VW; e++) a[VW * h + e] = w[e];
             }
         };
         auto load_r = [&](int v, int2 &rr) {  // generated V: record of k-slot (kq, m) of V chunk v
-            const int k = kbeg + v * KCH + kslot(kq, m);
+            const int k = kbeg + v * KCH + kslot<CT>(kq, m);
             rr = *((m < G && k < kend) ? lg.rec1 + ch.pr_off + k : reinterpret_cast<const int2 *>(zero));
         };
-        auto gen_a = [&](int2 rr, double (&a)[G]) {  // generated V from the chunk's records
+        auto gen_a = [&](int2 rr, CT (&a)[G]) {  // generated V from the chunk's records
             static_for<0, G>([&](auto S) {
                 constexpr int s = decltype(S)::value;
                 const int2 r = {row_bcast<s>(rr.x), row_bcast<s>(rr.y)};
                 if (SRC == LV_GAS) {
-                    a[s] = lds_load(gas_addr(ra, r, tbase, zaddr)) - lds_load(gas_addr(rb, r, tbase, zaddr));
+                    a[s] = (CT)(lds_load(gas_addr(ra, r, tbase, zaddr)) - lds_load(gas_addr(rb, r, tbase, zaddr)));
                 } else {
-                    a[s] = gen_tbme(lg.G, ra.x, ra.y, r.x, r.y);
+                    a[s] = (CT)gen_tbme(lg.G, ra.x, ra.y, r.x, r.y);
                 }
             });
         };
         // B tile of k rows k0.. of a K x H row-major block
-        auto load_b = [&](const double *B, int K, int k0, double (&b)[BPT]) {
+        auto load_b = [&](const double *B, int K, int k0, CT (&b)[BPT]) {
 #pragma unroll
             for (int q = 0; q < BPT; q++) {
                 const int idx = threadIdx.x + kThreads * q, kk = idx / W, col = idx - kk * W, k = k0 + kk;
-                b[q] = *((idx < KCH * W && k < K && col < H) ? B + (int64_t)k * H + col : zero);
+                b[q] = (CT) * ((idx < KCH * W && k < K && col < H) ? B + (int64_t)k * H + col : zero);
             }
         };
-        auto store_b = [&](int buf, const double (&b)[BPT]) {
+        auto store_b = [&](int buf, const CT (&b)[BPT]) {
 #pragma unroll
             for (int q = 0; q < BPT; q++) {
                 const int idx = threadIdx.x + kThreads * q, kk = idx / W, col = idx - kk * W;
@@ -236,21 +252,22 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
             }
         };
 
-        dbl4 acc[NT];
+        Acc acc[NT];
 #pragma unroll
-        for (int n = 0; n < NT; n++) acc[n] = dbl4{0.0, 0.0, 0.0, 0.0};
-        auto mfma_chunk = [&](int buf, const double (&a)[G]) {
+        for (int n = 0; n < NT; n++) acc[n] = Acc{};
+        auto mfma_chunk = [&](int buf, const CT (&a)[G]) {
 #pragma unroll
             for (int s = 0; s < G; s++) {
 #pragma unroll
-                for (int n = 0; n < NT; n++) acc[n] = mfma16(a[s], Bs[buf][kslot(kq, s)][n * 16 + m], acc[n]);
+                for (int n = 0; n < NT; n++)
+                    acc[n] = mfma16(a[s], Bs[buf][kslot<CT>(kq, s)][n * 16 + m], acc[n]);
             }
         };
-        double b[BPT];
+        CT b[BPT];
 
         // T X chunks (segment 0 only), unpipelined: at most a few per tile
         for (int c = 0; c < nx; c++) {
-            double a[G];
+            CT a[G];
             load_t(c, a);
             load_b(x + ch.off_hhhh, H, c * KCH, b);
             __syncthreads();            // earlier readers of buffer 0 are done
@@ -268,9 +285,9 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
         // before A: loads complete in issue order, so the LDS store of the B
         // tile does not wait for the A prefetch.
         const double *tb0 = tas + ch.off_tp;
-        double a0[G], a1[G];
+        CT a0[G], a1[G];
         int2 r0 = {0, 0}, r1 = {0, 0};
-        auto stage = [&](int v, int buf, double (&acur)[G], double (&anext)[G], int2 rcur, int2 &rnext) {
+        auto stage = [&](int v, int buf, CT (&acur)[G], CT (&anext)[G], int2 rcur, int2 &rnext) {
             load_b(tb0, kend, kbeg + (v + 1) * KCH, b);
             if (!GEN) load_a(v + 1, anext);
             else load_r(v + 1, rnext);
@@ -293,9 +310,9 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
         for (int n = 0; n < NT; n++) {
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const int row = wave * 16 + q * 4 + kq, col = n * 16 + m;
+                const int row = wave * 16 + Mma<CT>::row(q, lane), col = n * 16 + m;
                 if (row < tl.nrows && col < H)
-                    dp[ch.off_dp + (int64_t)seg * npk * H + (int64_t)(tl.row0 + row) * H + col] = acc[n][q];
+                    dp[ch.off_dp + (int64_t)seg * npk * H + (int64_t)(tl.row0 + row) * H + col] = (double)acc[n][q];
             }
         }
     }

@@ -1,74 +1,111 @@
-// Particle-hole ring (calcChannelT2Diff_phhp, src/calc.cpp:264-313):
-// per mod channel, X = V^B + 1/2 V^A T^ph (hp x hp, K = ph) and
-// R = T^ph X (ph x hp, K = hp), both as v_mfma_f64_16x16x4f64 tiles.
+// Particle-hole ring (calcChannelT2Diff_phhp, src/calc.cpp:264-313): per mod
+// channel, X = V^B + 1/2 V^A T^ph (hp x hp, K = ph) and R = T^ph X
+// (ph x hp, K = hp).  Mod blocks are small (at most a few 16-wide tiles per
+// side), so the kernel works per wavefront: one wave owns one
+// 16-column tile jt of one channel, computes the column tiles X(:, jt) into
+// registers and then R(:, jt) = T^ph X(:, jt) from them.  No LDS and no
+// barriers: the MFMA accumulator layout of X is reused directly as the B
+// operand of the second product by ordering the k-slots to match it.
+//
+// Arithmetic type CT: double (v_mfma_f64_16x16x4f64) or float
+// (v_mfma_f32_16x16x4f32, V^A and V^B stored in FP32, T^ph rounded on load);
+// R is written in FP64 either way.  Every load is unconditional: elements
+// outside the block read the zero buffer.
 #pragma once
 
 #include "device_common.hpp"
 
-// One mod channel per block.  X lives in LDS (row stride xs_stride doubles,
-// at least hp rounded up to 16); T^ph, V^A and V^B are read from global
-// memory (a channel's blocks are at most a few tens of KB and stay in L1/L2).
-// Within a 32-wide k chunk, k-slot q of MFMA step s is k = 8q + s, so each
-// lane reads 8 consecutive elements of its A row.
-__global__ __launch_bounds__(kThreads) void k_ring(const DMod *__restrict__ mods, int xs_stride,
-                                                   const double *__restrict__ tph,
-                                                   const double *__restrict__ va,
-                                                   const double *__restrict__ vb,
-                                                   double *__restrict__ r) {
-    extern __shared__ double xs[];
-    const DMod m = mods[blockIdx.x];
-    const int PH = m.ph, HP = m.hp;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, mm = lane & 15, kq = lane >> 4;
-    const double *T = tph + m.off_ph;  // PH x HP
-    const double *A = va + m.off_ph;   // HP x PH
-    const double *B = vb + m.off_b;    // HP x HP
-    const int nth = (HP + 15) / 16, ntp = (PH + 15) / 16;
+struct RingItem {       // one wave's work: mod channel and 16-column tile
+    int32_t mod, jt;
+};
 
-    // X tiles
-    for (int t = wave; t < nth * nth; t += kThreads / 64) {
-        const int kt = t / nth, jt = t - kt * nth;
-        const int row = kt * 16 + mm, col = jt * 16 + mm;
-        dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-        for (int k0 = 0; k0 < PH; k0 += 32) {
-            double a[8], b[8];
+template <typename CT, int NT>
+__device__ __forceinline__ void ring_wave(const DMod &md, int jt, const double *__restrict__ tph,
+                                          const CT *__restrict__ va, const CT *__restrict__ vb,
+                                          const double *__restrict__ zero, double *__restrict__ r) {
+    typedef typename Mma<CT>::acc Acc;
+    constexpr bool F64 = sizeof(CT) == 8;
+    const int lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
+    const int PH = md.ph, HP = md.hp;
+    const double *T = tph + md.off_ph;         // PH x HP, [(a,i),(j,b)]
+    const CT *A = va + md.off_ph;              // HP x PH, V^A [(i,a),(b,j)]
+    const CT *B = vb + md.off_b;               // HP x HP, V^B
+    const CT *zc = reinterpret_cast<const CT *>(zero);
+    const int col = jt * 16 + m;
+    const bool cok = col < HP;
+
+    // k-slot of MFMA step j for lane group kq inside a 16-wide k tile: chosen so
+    // that the accumulator register j of a lane holds exactly the X row that the
+    // lane supplies as B operand at step j of the second product.
+    auto ks = [&](int j) { return F64 ? 4 * j + kq : 4 * kq + j; };
+
+    // X(:, jt) = V^B(:, jt) + 1/2 V^A T(:, jt)
+    Acc x[NT];
 #pragma unroll
-            for (int s = 0; s < 8; s++) {
-                const int p = k0 + kq * 8 + s;
-                const bool ok = p < PH;
-                a[s] = (ok && row < HP) ? A[row * PH + p] : 0.0;
-                b[s] = (ok && col < HP) ? T[p * HP + col] : 0.0;
+    for (int kt = 0; kt < NT; kt++) {
+        const int row = kt * 16 + m;
+        const bool rok = row < HP;
+        Acc acc = {};
+#pragma unroll
+        for (int pt = 0; pt < NT; pt++) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int k = pt * 16 + ks(j);
+                const bool kok = k < PH;
+                const CT a = *((rok && kok) ? A + row * PH + k : zc);
+                const CT b = (CT) * ((kok && cok) ? T + k * HP + col : zero);
+                acc = mfma16(a, b, acc);
             }
-#pragma unroll
-            for (int s = 0; s < 8; s++) acc = mfma16(a[s], b[s], acc);
         }
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const int i = kt * 16 + 4 * q + kq;
-            if (i < HP && col < HP) xs[i * xs_stride + col] = 0.5 * acc[q] + B[i * HP + col];
+            const int i = kt * 16 + Mma<CT>::row(q, lane);
+            const CT bv = *((i < HP && cok) ? B + i * HP + col : zc);
+            x[kt][q] = (CT)0.5 * acc[q] + bv;
         }
     }
-    __syncthreads();
-    // R tiles
-    for (int t = wave; t < ntp * nth; t += kThreads / 64) {
-        const int pt = t / nth, jt = t - pt * nth;
-        const int row = pt * 16 + mm, col = jt * 16 + mm;
-        dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-        for (int k0 = 0; k0 < HP; k0 += 32) {
-            double a[8], b[8];
+    // R(:, jt) = T X(:, jt); step (kt, j) uses k = kt*16 + ks(j), whose X value
+    // is x[kt][j] in this lane.
 #pragma unroll
-            for (int s = 0; s < 8; s++) {
-                const int k = k0 + kq * 8 + s;
-                const bool ok = k < HP;
-                a[s] = (ok && row < PH) ? T[row * HP + k] : 0.0;
-                b[s] = (ok && col < HP) ? xs[k * xs_stride + col] : 0.0;
+    for (int pt = 0; pt < NT; pt++) {
+        const int row = pt * 16 + m;
+        const bool rok = row < PH;
+        Acc acc = {};
+#pragma unroll
+        for (int kt = 0; kt < NT; kt++) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int k = kt * 16 + ks(j);
+                const CT a = (CT) * ((rok && k < HP) ? T + row * HP + k : zero);
+                acc = mfma16(a, x[kt][j], acc);
             }
-#pragma unroll
-            for (int s = 0; s < 8; s++) acc = mfma16(a[s], b[s], acc);
         }
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const int i = pt * 16 + 4 * q + kq;
-            if (i < PH && col < HP) r[m.off_ph + i * HP + col] = acc[q];
+            const int i = pt * 16 + Mma<CT>::row(q, lane);
+            if (i < PH && cok) r[md.off_ph + i * HP + col] = (double)acc[q];
         }
+    }
+}
+
+// Items are sorted by tile count, largest first; wave w of block b takes item
+// 4b + w.  A channel with NT = ceil(max(ph, hp) / 16) has ceil(hp / 16) items.
+template <typename CT>
+__global__ __launch_bounds__(kThreads) void k_ring(const RingItem *__restrict__ items, int nitems,
+                                                   const DMod *__restrict__ mods,
+                                                   const double *__restrict__ tph,
+                                                   const CT *__restrict__ va, const CT *__restrict__ vb,
+                                                   const double *__restrict__ zero,
+                                                   double *__restrict__ r) {
+    const int it = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+    if (it >= nitems) return;
+    const RingItem w = items[it];
+    const DMod md = mods[w.mod];
+    switch ((max(md.ph, md.hp) + 15) / 16) {
+        case 1: ring_wave<CT, 1>(md, w.jt, tph, va, vb, zero, r); break;
+        case 2: ring_wave<CT, 2>(md, w.jt, tph, va, vb, zero, r); break;
+        case 3: ring_wave<CT, 3>(md, w.jt, tph, va, vb, zero, r); break;
+        case 4: ring_wave<CT, 4>(md, w.jt, tph, va, vb, zero, r); break;
+        default: ring_wave<CT, 5>(md, w.jt, tph, va, vb, zero, r); break;
     }
 }

@@ -95,3 +95,11 @@ __global__ __launch_bounds__(kThreads) void k_pack_vt(const DTile *__restrict__ 
         vtp[ch.off_tp + (int64_t)u * H + kl] = vt[ch.off_t + (int64_t)prow[ch.pr_off + u] * H + kl];
     }
 }
+
+// FP32 copy of an FP64 array (round to nearest), for the mixed-precision
+// contractions.
+__global__ __launch_bounds__(kThreads) void k_to_f32(const double *__restrict__ src, float *__restrict__ dst,
+                                                     int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+        dst[i] = (float)src[i];
+}

@@ -48,7 +48,10 @@ extern "C" {
 
 /* Arithmetic of the amplitude contractions. */
 #define CCD_PREC_FP64 0      /* everything FP64 (the parity path) */
-#define CCD_PREC_FP32 1      /* FP32 contractions, FP64 update/denominators/energy */
+#define CCD_PREC_FP32 1      /* mixed: FP32 MFMA contractions (pppp ladder and ring,
+                                FP32 accumulation), FP64 hhhh term, update,
+                                denominators, one-body sums and energy; the
+                                context can also run FP64 iterations */
 /* OR-ed into ccd_ctx_create's precision: the matrix-free mode of the reference
  * (-m 1, the calcFree* twins of src/calc.cpp:45-504 driven by
  * src/main.cpp:521-588).  V^pppp is never stored: the pppp ladder rebuilds
@@ -116,6 +119,12 @@ int ccd_ctx_destroy(ccd_ctx *ctx);
 /* Sizes: elements of the pphh (= phhp) layout, bytes of device memory held. */
 int ccd_ctx_info(const ccd_ctx *ctx, int64_t *n_t2, int64_t *device_bytes);
 
+/* Arithmetic of the contractions of the following iterations: CCD_PREC_FP64
+ * (always available) or CCD_PREC_FP32 (contexts created with CCD_PREC_FP32;
+ * their initial setting).  No counterpart in the reference, which is FP64
+ * only. */
+int ccd_set_precision(ccd_ctx *ctx, int precision);
+
 /* MBPT2 correlation energy, 1/4 sum v^2/(e_i+e_j-e_a-e_b)
  * (replaces MBPT2Corr, src/MBPT2Corr.cpp:6-38). */
 int ccd_mbpt2(ccd_ctx *ctx, double *corr);
@@ -136,6 +145,16 @@ int ccd_iterate(ccd_ctx *ctx, double mixing, double *corr);
  * energies[k] receives the energy after iteration k for k < cap. */
 int ccd_solve(ccd_ctx *ctx, double tol, int max_iters, double mixing,
               double *energies, int cap, int *n_iter, int *stop);
+
+/* The same protocol with mixed precision (contexts created with
+ * CCD_PREC_FP32): FP32 contractions while |dE| > switch_tol, then FP64
+ * iterations (defect-correction sweeps back to the FP64 fixed point) until
+ * |dE| <= tol.  n_fp32 receives the number of FP32 iterations.  The
+ * context's precision setting is left as it was.  No counterpart in the
+ * reference. */
+int ccd_solve_mixed(ccd_ctx *ctx, double tol, double switch_tol, int max_iters,
+                    double mixing, double *energies, int cap, int *n_iter,
+                    int *n_fp32, int *stop);
 
 /* Copy the current amplitudes out / in (pphh layout, host memory). */
 int ccd_get_t2(ccd_ctx *ctx, double *t2);

@@ -163,3 +163,54 @@ def test_matrix_free_solve_matches_oracle(require_gpu):
     k = min(n_g, len(e_o))
     assert abs(n_g - len(e_o)) <= 1
     assert np.max(np.abs(e_g[:k] - e_o[:k]) / np.abs(e_o[:k])) < E_REL
+
+
+# ---- mixed precision (FP32 contractions, FP64 update and energy)
+
+F32_REL = 2e-5          # FP32 contraction vs FP64 oracle, relative to the largest element
+
+
+def test_fp32_diagrams_close_to_oracle(require_gpu):
+    from coupled_cluster_doubles_amd import FP32
+    m, o = _pair("heg", (1.0, 19, 2))
+    ctx = Context(m, FP32)
+    rng = np.random.default_rng(7)
+    t = rng.uniform(-0.05, 0.05, m.n_t2)
+    lab = m.t2_labels()
+    key = {tuple(r): k for k, r in enumerate(lab.tolist())}
+    sw_ab = np.array([key[(b, a, i, j)] for a, b, i, j in lab.tolist()])
+    sw_ij = np.array([key[(a, b, j, i)] for a, b, i, j in lab.tolist()])
+    ta = t - t[sw_ab]
+    ta = ta - ta[sw_ij]
+    d_g, d_o = ctx.ladder(ta), o.ladder(ta)
+    assert np.max(np.abs(d_g - d_o)) <= F32_REL * np.max(np.abs(d_o))
+    tph = o.scatter_ph(t)
+    r_g, r_o = ctx.ring(tph), o.ring(tph)
+    assert np.max(np.abs(r_g - r_o)) <= F32_REL * np.max(np.abs(r_o))
+    # the same context in FP64 is the FP64 path
+    ctx.set_precision(FP64)
+    assert np.array_equal(ctx.ladder(ta), Context(m, FP64).ladder(ta))
+
+
+@pytest.mark.parametrize("free", [False, True])
+def test_fp32_stored_and_generated_v_agree_bitwise(require_gpu, free):
+    from coupled_cluster_doubles_amd import FP32
+    m = Model.electron_gas(1.0, 19, 2)
+    a = Context(m, FP32)
+    b = Context(m, FP32, matrix_free=free)
+    for _ in range(3):
+        assert a.iterate() == b.iterate()
+    assert np.array_equal(a.get_t2(), b.get_t2())
+
+
+@pytest.mark.parametrize("args,n_part", [((1.0, 19, 2), 14), ((1.0, 32, 4), 54)])
+def test_mixed_solve_within_1e7_per_particle(require_gpu, args, n_part):
+    """FP32 contractions then FP64 sweeps: converged energy within 1e-7 Ha per
+    particle of the FP64 solve (in practice within the tolerance itself)."""
+    from coupled_cluster_doubles_amd import FP32
+    m = Model.electron_gas(*args)
+    e64, stop64, n64 = Context(m, FP64, matrix_free=True).solve(1e-10)
+    e, stop, n, n32 = Context(m, FP32, matrix_free=True).solve_mixed(1e-10, 1e-6)
+    assert stop == stop64 == "converged"
+    assert n32 > 0 and n > n32
+    assert abs(e[-1] - e64[-1]) / n_part <= 1e-7
