@@ -150,7 +150,7 @@ __device__ __forceinline__ int kslot(int q, int s) {
 
 template <int NT>
 struct LadderShape {
-    static constexpr int KCH = NT <= 2 ? 64 : 32;   // k per pipeline stage
+    static constexpr int KCH = 32;                  // k per pipeline stage
     static constexpr int KSEG = NT <= 2 ? 256 : 64; // packed pairs per k segment
 };
 
@@ -283,28 +283,38 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
         // comes only before its own MFMAs, with chunk v+2 already issued.
         // Prefetches past the last chunk read the zero buffer.  B is loaded
         // before A: loads complete in issue order, so the LDS store of the B
-        // tile does not wait for the A prefetch.
+        // tile does not wait for the A prefetch.  With a generated V the A
+        // values of chunk v+1 are built between the (dependent) MFMAs of chunk
+        // v, from records loaded two chunks ahead.
         const double *tb0 = tas + ch.off_tp;
         CT a0[G], a1[G];
         int2 r0 = {0, 0}, r1 = {0, 0};
-        auto stage = [&](int v, int buf, CT (&acur)[G], CT (&anext)[G], int2 rcur, int2 &rnext) {
+        auto stage = [&](int v, int buf, CT (&acur)[G], CT (&anext)[G], int2 &rnext, int2 &rnext2) {
             load_b(tb0, kend, kbeg + (v + 1) * KCH, b);
-            if (!GEN) load_a(v + 1, anext);
-            else load_r(v + 1, rnext);
-            if (GEN) gen_a(rcur, acur);
+            if (!GEN) {
+                load_a(v + 1, anext);
+            } else {
+                load_r(v + 2, rnext2);
+                gen_a(rnext, anext);
+            }
             mfma_chunk(buf, acur);
             store_b(buf ^ 1, b);
             __syncthreads();
         };
         load_b(tb0, kend, kbeg, b);
-        if (!GEN) load_a(0, a0);
-        else load_r(0, r0);
+        if (!GEN) {
+            load_a(0, a0);
+        } else {
+            load_r(0, r0);
+            load_r(1, r1);
+            gen_a(r0, a0);
+        }
         store_b(0, b);
         __syncthreads();
         for (int v = 0; v < nv; v += 2) {
-            stage(v, 0, a0, a1, r0, r1);
+            stage(v, 0, a0, a1, r1, r0);
             if (v + 1 >= nv) break;
-            stage(v + 1, 1, a1, a0, r1, r0);
+            stage(v + 1, 1, a1, a0, r0, r1);
         }
 #pragma unroll
         for (int n = 0; n < NT; n++) {
