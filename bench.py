@@ -1,15 +1,22 @@
 """Benchmark: CCD iterations/s on the electron gas, N=54, r_s=1.0, n^2<=36.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64]
-                    [--mode stored|free] [--no-cpu-baseline]
-                    [--config heg54|heg14|pairing]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--precision mixed|fp64|fp32] [--mode free|stored]
+                    [--no-cpu-baseline] [--config heg54|heg14|pairing]
 
 A step is one full Jacobi iteration of the CCD amplitude equations over the
 whole problem (both ladders, ring, one-body terms, remaps, update, energy),
-with the amplitudes and interaction resident in HBM.  --mode free is the
-reference's matrix-free mode (-m 1): V^pppp is rebuilt inside the pppp
-ladder every iteration instead of being read from HBM; the iterates are
-bit-identical to the stored mode.  Prints one JSON line.
+with the amplitudes and interaction resident in HBM.
+
+--precision mixed (default): the steps are FP32-contraction iterations (the
+bulk of a mixed solve); afterwards a full mixed solve (FP32 iterations, then
+FP64 sweeps to |dE| <= 1e-10) and a full FP64 solve are timed from T2 = 0
+and their energies compared per particle.  fp64 / fp32: steps in that
+arithmetic only.
+
+--mode free (default) is the reference's matrix-free mode (-m 1): V^pppp is
+rebuilt inside the pppp ladder every iteration instead of being read from
+HBM; the iterates are bit-identical to --mode stored.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -87,8 +94,10 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="heg54", choices=sorted(CONFIGS))
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
-    ap.add_argument("--mode", default="stored", choices=["stored", "free"],
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64", "fp32"])
+    ap.add_argument("--switch-tol", type=float, default=1e-6,
+                    help="mixed solve: |dE| below which FP64 sweeps replace FP32 iterations")
+    ap.add_argument("--mode", default="free", choices=["stored", "free"],
                     help="stored: V^pppp read from HBM; free: rebuilt in the ladder (reference -m 1)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=3)
@@ -114,18 +123,19 @@ def main():
 
     cfg = CONFIGS[args.config]
     prec = FP64 if args.precision == "fp64" else FP32
+    free = args.mode == "free"
     model = make_model(cfg)
     t_setup = time.perf_counter()
-    ctx = Context(model, prec, matrix_free=args.mode == "free")
+    ctx = Context(model, prec, matrix_free=free)
     setup_s = time.perf_counter() - t_setup
     mbpt2 = ctx.mbpt2()
     work = ctx.work()
 
-    for _ in range(args.warmup):
-        ctx.iterate()
+    # the steps are queued back to back (no host round trip between them);
+    # the call returns after the last one has finished on the device
+    ctx.iterate_n(args.warmup)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        e = ctx.iterate()
+    e = float(ctx.iterate_n(args.steps)[-1])
     dt = time.perf_counter() - t0
     ms_step = 1e3 * dt / args.steps
     iters_per_s = args.steps / dt
@@ -154,6 +164,27 @@ def main():
         with open(tfile) as f:
             traffic = json.load(f).get("bytes_per_launch")
 
+    mixed = None
+    if args.precision == "mixed":
+        # full solves from T2 = 0, timed on the host clock around the call
+        n_part = model.n_holes
+        t0 = time.perf_counter()
+        e_m, stop_m, n_m, n32 = ctx.solve_mixed(1e-10, args.switch_tol)
+        t_m = time.perf_counter() - t0
+        ctx64 = Context(model, FP64, matrix_free=free)
+        ctx64.solve(1e-10)                       # warm
+        t0 = time.perf_counter()
+        e_d, stop_d, n_d = ctx64.solve(1e-10)
+        t_d = time.perf_counter() - t0
+        mixed = {"tol": 1e-10, "switch_tol": args.switch_tol,
+                 "mixed_iters": n_m, "mixed_fp32_iters": n32, "mixed_stop": stop_m,
+                 "mixed_solve_s": t_m, "mixed_energy": float(e_m[-1]),
+                 "fp64_iters": n_d, "fp64_stop": stop_d, "fp64_solve_s": t_d,
+                 "fp64_energy": float(e_d[-1]),
+                 "abs_dE_per_particle": abs(float(e_m[-1]) - float(e_d[-1])) / n_part,
+                 "speedup_vs_fp64_solve": t_d / t_m}
+        ctx64.close()
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, 16)
@@ -174,6 +205,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if prec == FP64 else "f32+f64",
+        "mixed_vs_fp64": mixed,
         "data": "synthetic: the reference's closed-form basis and Coulomb interaction, T2 from zero",
         "config": {"workload": cfg["name"], "precision": args.precision, "mode": args.mode,
                    "n_t2": ctx.n_t2, "device_bytes": ctx.device_bytes},

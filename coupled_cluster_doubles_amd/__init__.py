@@ -153,6 +153,12 @@ class Context:
         check(lib().ccd_iterate(self.h, mixing, C.byref(e)), "ccd_iterate")
         return e.value
 
+    def iterate_n(self, n: int, mixing: float = 0.5) -> np.ndarray:
+        """n iterations with no host round trip in between; their energies."""
+        en = np.zeros(max(n, 1))
+        check(lib().ccd_iterate_n(self.h, n, mixing, _ptr(en)), "ccd_iterate_n")
+        return en[:n].copy()
+
     def solve(self, tol: float, max_iters: int = 100, mixing: float = 0.5, cap: int = 256):
         en = np.zeros(cap)
         n, stop = C.c_int(), C.c_int()

@@ -43,6 +43,7 @@ SIGNATURES = [
     ("ccd_mbpt2", C.c_int, [vp, dp]),
     ("ccd_reset", C.c_int, [vp]),
     ("ccd_iterate", C.c_int, [vp, C.c_double, dp]),
+    ("ccd_iterate_n", C.c_int, [vp, C.c_int, C.c_double, ip]),
     ("ccd_solve", C.c_int, [vp, C.c_double, C.c_int, C.c_double, ip, C.c_int,
                             C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("ccd_solve_mixed", C.c_int, [vp, C.c_double, C.c_double, C.c_int, C.c_double, ip, C.c_int,

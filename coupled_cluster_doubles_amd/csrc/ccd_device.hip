@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
@@ -75,7 +76,9 @@ struct ccd_ctx {
     int lsrc = LV_STORED;        // LadderSource of the pppp ladder
     int gen_blocks = 1024;       // grid of the generated-V ladder launches
     LadderGen lgen{};
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;         // main stream: every entry point is ordered on it
+    hipStream_t aux[2] = {nullptr, nullptr};  // concurrent phases of an iteration
+    hipEvent_t sync_ev[6] = {};
     int nsp = 0, nh = 0, np = 0;
     int64_t nt2 = 0;
     int64_t bytes = 0;
@@ -106,6 +109,9 @@ struct ccd_ctx {
            *d_vb = nullptr;
     double *d_partials = nullptr, *d_scalar = nullptr;
     double *d_zero = nullptr;    // kZeroDoubles zeros, never written: the target of masked-off loads
+    LoopCtl *d_ctl = nullptr;    // device loop state
+    LoopCtl *h_ctl = nullptr;    // pinned staging for it
+    double *d_elog = nullptr;    // energy of every iteration of a device-run loop (kElogCap)
     double *h_scalar = nullptr;  // pinned
     int32_t *d_qn = nullptr;
     double *d_table = nullptr;
@@ -142,8 +148,15 @@ int upload(ccd_ctx *c, T **p, const std::vector<T> &h) {
 void free_ctx(ccd_ctx *c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (hipStream_t s : c->aux)
+        if (s) (void)hipStreamSynchronize(s);
+    for (hipEvent_t e : c->sync_ev)
+        if (e) (void)hipEventDestroy(e);
+    for (hipStream_t s : c->aux)
+        if (s) (void)hipStreamDestroy(s);
     for (void *p : c->allocs) (void)hipFree(p);
     if (c->h_scalar) (void)hipHostFree(c->h_scalar);
+    if (c->h_ctl) (void)hipHostFree(c->h_ctl);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -160,34 +173,35 @@ UpdArgs upd_args(ccd_ctx *c, const double *t_old, double *t_new, double mixing) 
     a.prow = c->d_prow; a.prow_sw = c->d_prow_sw; a.hh_swap = c->d_hh_swap;
     a.eps = c->d_eps; a.chi = c->d_onebody; a.psi = c->d_onebody + c->nh;
     a.vt = c->d_vt; a.dp = c->d_dp; a.g = c->d_g; a.t_old = t_old; a.t_new = t_new;
-    a.tas = c->d_tas; a.rs = c->d_rs; a.cp = c->d_cp; a.mixing = mixing; a.nh = c->nh;
+    a.tas = c->d_tas; a.rs = c->d_rs; a.cp = c->d_cp; a.mixing = mixing; a.nh = c->nh; a.ctl = c->d_ctl;
     return a;
 }
 
-// --- launch helpers, one per phase of the iteration
+// --- launch helpers, one per phase of the iteration; each takes the stream
+// it launches on
 
-int launch_ring(ccd_ctx *c) {
+int launch_ring(ccd_ctx *c, hipStream_t st) {
     if (c->ritems.empty()) return CCD_OK;
     const int n = (int)c->ritems.size();
     const dim3 grid((unsigned)((n + kThreads / 64 - 1) / (kThreads / 64)));
     if (c->iter_prec == CCD_PREC_FP32)
-        hipLaunchKernelGGL(k_ring<float>, grid, dim3(kThreads), 0, c->stream, c->d_ritems, n, c->d_mods, c->d_tph,
-                           c->d_va32, c->d_vb32, c->d_zero, c->d_r);
+        hipLaunchKernelGGL(k_ring<float>, grid, dim3(kThreads), 0, st, c->d_ritems, n, c->d_mods, c->d_tph,
+                           c->d_va32, c->d_vb32, c->d_zero, c->d_r, c->d_ctl);
     else
-        hipLaunchKernelGGL(k_ring<double>, grid, dim3(kThreads), 0, c->stream, c->d_ritems, n, c->d_mods, c->d_tph,
-                           c->d_va, c->d_vb, c->d_zero, c->d_r);
+        hipLaunchKernelGGL(k_ring<double>, grid, dim3(kThreads), 0, st, c->d_ritems, n, c->d_mods, c->d_tph,
+                           c->d_va, c->d_vb, c->d_zero, c->d_r, c->d_ctl);
     return check_launch("k_ring");
 }
 
 // pphh -> phhp (T to T^ph) when to_ph, else phhp -> pphh (R to G)
-int launch_transpose(ccd_ctx *c, bool to_ph, const double *src, double *dst) {
+int launch_transpose(ccd_ctx *c, bool to_ph, const double *src, double *dst, hipStream_t st) {
     if (c->np == 0 || c->nt2 == 0) return CCD_OK;
     if (to_ph)
-        hipLaunchKernelGGL(k_transpose, dim3((unsigned)c->np), dim3(kThreads), c->tr_lds, c->stream, c->d_tr_base,
-                           c->d_addr_pp, c->d_addr_ph, c->d_perm_ph, src, dst);
+        hipLaunchKernelGGL(k_transpose, dim3((unsigned)c->np), dim3(kThreads), c->tr_lds, st, c->d_tr_base,
+                           c->d_addr_pp, c->d_addr_ph, c->d_perm_ph, src, dst, c->d_ctl);
     else
-        hipLaunchKernelGGL(k_transpose, dim3((unsigned)c->np), dim3(kThreads), c->tr_lds, c->stream, c->d_tr_base,
-                           c->d_addr_ph, c->d_addr_pp, c->d_perm_pp, src, dst);
+        hipLaunchKernelGGL(k_transpose, dim3((unsigned)c->np), dim3(kThreads), c->tr_lds, st, c->d_tr_base,
+                           c->d_addr_ph, c->d_addr_pp, c->d_perm_pp, src, dst, c->d_ctl);
     return check_launch("k_transpose");
 }
 
@@ -198,83 +212,116 @@ int launch_antisym(ccd_ctx *c, const double *t) {
     return check_launch("k_antisym");
 }
 
-int launch_ladder_x(ccd_ctx *c) {
+int launch_ladder_x(ccd_ctx *c, hipStream_t st) {
     if (c->xtiles.empty()) return CCD_OK;
-    hipLaunchKernelGGL(k_ladder_x, dim3((unsigned)c->xtiles.size()), dim3(kThreads), 0, c->stream, c->d_xtiles,
-                       c->d_chans, c->d_vtp, c->d_tas, c->d_vh, c->d_x);
+    hipLaunchKernelGGL(k_ladder_x, dim3((unsigned)c->xtiles.size()), dim3(kThreads), 0, st, c->d_xtiles,
+                       c->d_chans, c->d_vtp, c->d_tas, c->d_vh, c->d_x, c->d_ctl);
     return check_launch("k_ladder_x");
 }
 
 // Stored V: one block per tile.  Generated V: at most gen_blocks blocks
 // looping over the tiles, the gas table in dynamic LDS.
 template <typename CT, int NT, int SRC>
-void launch_ladder_v_nt(ccd_ctx *c, const CT *vp) {
+void launch_ladder_v_nt(ccd_ctx *c, const CT *vp, hipStream_t st) {
     const int n = c->lt_off[NT] - c->lt_off[NT - 1];
     if (n <= 0) return;
     const int grid = SRC == LV_STORED ? n : std::min(n, c->gen_blocks);
     const size_t lds = SRC == LV_GAS ? sizeof(double) * (size_t)c->lgen.ntab : 0;
-    hipLaunchKernelGGL((k_ladder_v<CT, NT, SRC>), dim3((unsigned)grid), dim3(kThreads), lds, c->stream,
+    hipLaunchKernelGGL((k_ladder_v<CT, NT, SRC>), dim3((unsigned)grid), dim3(kThreads), lds, st,
                        c->d_ltiles + c->lt_off[NT - 1], n, c->d_chans, vp, c->lgen, c->d_tas, c->d_x,
-                       c->d_zero, c->d_dp);
+                       c->d_zero, c->d_dp, c->d_ctl);
 }
 
+// NT classes [lo, hi] of the pppp ladder
 template <typename CT, int SRC>
-void launch_ladder_v_src(ccd_ctx *c, const CT *vp) {
-    launch_ladder_v_nt<CT, 1, SRC>(c, vp);
-    launch_ladder_v_nt<CT, 2, SRC>(c, vp);
-    launch_ladder_v_nt<CT, 3, SRC>(c, vp);
-    launch_ladder_v_nt<CT, 4, SRC>(c, vp);
-    launch_ladder_v_nt<CT, 5, SRC>(c, vp);
+void launch_ladder_v_src(ccd_ctx *c, const CT *vp, int lo, int hi, hipStream_t st) {
+    if (lo <= 1 && 1 <= hi) launch_ladder_v_nt<CT, 1, SRC>(c, vp, st);
+    if (lo <= 2 && 2 <= hi) launch_ladder_v_nt<CT, 2, SRC>(c, vp, st);
+    if (lo <= 3 && 3 <= hi) launch_ladder_v_nt<CT, 3, SRC>(c, vp, st);
+    if (lo <= 4 && 4 <= hi) launch_ladder_v_nt<CT, 4, SRC>(c, vp, st);
+    if (lo <= 5 && 5 <= hi) launch_ladder_v_nt<CT, 5, SRC>(c, vp, st);
 }
 
 template <typename CT>
-void launch_ladder_v_ct(ccd_ctx *c, const CT *vp) {
-    if (c->lsrc == LV_GAS) launch_ladder_v_src<CT, LV_GAS>(c, vp);
-    else if (c->lsrc == LV_GENERIC) launch_ladder_v_src<CT, LV_GENERIC>(c, vp);
-    else launch_ladder_v_src<CT, LV_STORED>(c, vp);
+void launch_ladder_v_ct(ccd_ctx *c, const CT *vp, int lo, int hi, hipStream_t st) {
+    if (c->lsrc == LV_GAS) launch_ladder_v_src<CT, LV_GAS>(c, vp, lo, hi, st);
+    else if (c->lsrc == LV_GENERIC) launch_ladder_v_src<CT, LV_GENERIC>(c, vp, lo, hi, st);
+    else launch_ladder_v_src<CT, LV_STORED>(c, vp, lo, hi, st);
 }
 
-int launch_ladder_v(ccd_ctx *c) {
+int launch_ladder_v(ccd_ctx *c, int lo, int hi, hipStream_t st) {
     if (c->ltiles.empty()) return CCD_OK;
-    if (c->iter_prec == CCD_PREC_FP32) launch_ladder_v_ct<float>(c, c->d_vp32);
-    else launch_ladder_v_ct<double>(c, c->d_vp);
+    if (c->iter_prec == CCD_PREC_FP32) launch_ladder_v_ct<float>(c, c->d_vp32, lo, hi, st);
+    else launch_ladder_v_ct<double>(c, c->d_vp, lo, hi, st);
     return check_launch("k_ladder_v");
 }
 
 template <bool UPDATE>
-int launch_update(ccd_ctx *c, const double *t_old, double *t_new, double mixing) {
+int launch_update(ccd_ctx *c, const double *t_old, double *t_new, double mixing, hipStream_t st) {
     if (c->utiles.empty()) return CCD_OK;
     UpdArgs a = upd_args(c, t_old, t_new, mixing);
-    hipLaunchKernelGGL(k_update<UPDATE>, dim3((unsigned)c->utiles.size()), dim3(kThreads), 0, c->stream, a);
+    hipLaunchKernelGGL(k_update<UPDATE>, dim3((unsigned)c->utiles.size()), dim3(kThreads), 0, st, a);
     return check_launch("k_update");
 }
 
-int launch_reduce(ccd_ctx *c) {
-    hipLaunchKernelGGL(k_reduce, dim3((unsigned)c->nsp), dim3(kThreads), 0, c->stream, c->d_red_ptr, c->d_red_idx,
-                       c->nh, c->d_cp, c->d_rs, c->d_onebody);
+int launch_reduce(ccd_ctx *c, hipStream_t st) {
+    hipLaunchKernelGGL(k_reduce, dim3((unsigned)c->nsp), dim3(kThreads), 0, st, c->d_red_ptr, c->d_red_idx,
+                       c->nh, c->d_cp, c->d_rs, c->d_onebody, c->d_ctl);
     int rc = check_launch("k_reduce");
     if (rc) return rc;
-    hipLaunchKernelGGL(k_energy, dim3(1), dim3(kThreads), 0, c->stream, c->d_onebody, c->nh, c->d_scalar);
+    hipLaunchKernelGGL(k_energy, dim3(1), dim3(kThreads), 0, st, c->d_onebody, c->nh, c->d_scalar, c->d_ctl,
+                       c->d_elog);
     return check_launch("k_energy");
 }
 
-// One iteration; ev (optional, 6 events) brackets
-// ring + R remap | X | pppp ladder | update | T remap + reductions.
+// One iteration.  Timed (ev = 6 events): every phase in order on the main
+// stream, the events bracketing ring + R remap | X | pppp ladder | update |
+// T remap + reductions.  Untimed: the independent phases run concurrently
+// on three streams, joined by events --
+//   aux1: ring, R remap                 | after the update: T remap
+//   main: X, pppp ladder (1 tile/side)  | update | reductions
+//   aux2:    pppp ladder (2+ tiles)     |
+// and the iteration ends with everything joined on the main stream.
 int run_iteration(ccd_ctx *c, double mixing, hipEvent_t *ev) {
     const double *t_old = c->d_t[c->cur];
     double *t_new = c->d_t[1 - c->cur];
     int rc;
-    if (ev) HIP_TRY(hipEventRecord(ev[0], c->stream));
-    if ((rc = launch_ring(c)) || (rc = launch_transpose(c, false, c->d_r, c->d_g))) return rc;
-    if (ev) HIP_TRY(hipEventRecord(ev[1], c->stream));
-    if ((rc = launch_ladder_x(c))) return rc;
-    if (ev) HIP_TRY(hipEventRecord(ev[2], c->stream));
-    if ((rc = launch_ladder_v(c))) return rc;
-    if (ev) HIP_TRY(hipEventRecord(ev[3], c->stream));
-    if ((rc = launch_update<true>(c, t_old, t_new, mixing))) return rc;
-    if (ev) HIP_TRY(hipEventRecord(ev[4], c->stream));
-    if ((rc = launch_transpose(c, true, t_new, c->d_tph)) || (rc = launch_reduce(c))) return rc;
-    if (ev) HIP_TRY(hipEventRecord(ev[5], c->stream));
+    if (ev) {
+        const hipStream_t st = c->stream;
+        HIP_TRY(hipEventRecord(ev[0], st));
+        if ((rc = launch_ring(c, st)) || (rc = launch_transpose(c, false, c->d_r, c->d_g, st))) return rc;
+        HIP_TRY(hipEventRecord(ev[1], st));
+        if ((rc = launch_ladder_x(c, st))) return rc;
+        HIP_TRY(hipEventRecord(ev[2], st));
+        if ((rc = launch_ladder_v(c, 1, NTMAX, st))) return rc;
+        HIP_TRY(hipEventRecord(ev[3], st));
+        if ((rc = launch_update<true>(c, t_old, t_new, mixing, st))) return rc;
+        HIP_TRY(hipEventRecord(ev[4], st));
+        if ((rc = launch_transpose(c, true, t_new, c->d_tph, st)) || (rc = launch_reduce(c, st))) return rc;
+        HIP_TRY(hipEventRecord(ev[5], st));
+    } else {
+        const hipStream_t s0 = c->stream, s1 = c->aux[0], s2 = c->aux[1];
+        hipEvent_t *e = c->sync_ev;
+        HIP_TRY(hipEventRecord(e[0], s0));
+        HIP_TRY(hipStreamWaitEvent(s1, e[0], 0));
+        if ((rc = launch_ring(c, s1)) || (rc = launch_transpose(c, false, c->d_r, c->d_g, s1))) return rc;
+        HIP_TRY(hipEventRecord(e[1], s1));
+        if ((rc = launch_ladder_x(c, s0))) return rc;
+        HIP_TRY(hipEventRecord(e[2], s0));
+        HIP_TRY(hipStreamWaitEvent(s2, e[2], 0));
+        if ((rc = launch_ladder_v(c, 2, NTMAX, s2))) return rc;
+        HIP_TRY(hipEventRecord(e[3], s2));
+        if ((rc = launch_ladder_v(c, 1, 1, s0))) return rc;
+        HIP_TRY(hipStreamWaitEvent(s0, e[1], 0));
+        HIP_TRY(hipStreamWaitEvent(s0, e[3], 0));
+        if ((rc = launch_update<true>(c, t_old, t_new, mixing, s0))) return rc;
+        HIP_TRY(hipEventRecord(e[4], s0));
+        HIP_TRY(hipStreamWaitEvent(s1, e[4], 0));
+        if ((rc = launch_transpose(c, true, t_new, c->d_tph, s1))) return rc;
+        HIP_TRY(hipEventRecord(e[5], s1));
+        if ((rc = launch_reduce(c, s0))) return rc;
+        HIP_TRY(hipStreamWaitEvent(s0, e[5], 0));
+    }
     c->cur = 1 - c->cur;
     return CCD_OK;
 }
@@ -665,8 +712,10 @@ int build(ccd_ctx *c, const ccd_problem *p) {
         (rc = dalloc(c, &c->d_vb, (size_t)c->n_b)) || (rc = dalloc(c, &c->d_rs, (size_t)std::max<int64_t>(npp, 1))) ||
         (rc = dalloc(c, &c->d_cp, (size_t)std::max<int64_t>(c->n_cp, 1))) ||
         (rc = dalloc(c, &c->d_partials, std::max(c->utiles.size(), (size_t)1))) || (rc = dalloc(c, &c->d_scalar, 4)) ||
-        (rc = dalloc(c, &c->d_zero, (size_t)kZeroDoubles)))
+        (rc = dalloc(c, &c->d_zero, (size_t)kZeroDoubles)) || (rc = dalloc(c, &c->d_ctl, 1)) ||
+        (rc = dalloc(c, &c->d_elog, (size_t)kElogCap)))
         return rc;
+    HIP_TRY(hipHostMalloc((void **)&c->h_ctl, sizeof(LoopCtl), hipHostMallocDefault));
     HIP_TRY(hipMemset(c->d_zero, 0, sizeof(double) * kZeroDoubles));
     HIP_TRY(hipHostMalloc((void **)&c->h_scalar, 4 * sizeof(double), hipHostMallocDefault));
 
@@ -767,6 +816,54 @@ int build(ccd_ctx *c, const ccd_problem *p) {
     return CCD_OK;
 }
 
+// Arm the device loop state.  fresh: iteration count 0 and E_prev = 100.0
+// (src/main.cpp:494); otherwise the count and E_prev carry on.  tol < 0 and
+// max_iters = INT_MAX give a free run that never stops by itself.
+int arm_loop(ccd_ctx *c, double tol, int max_iters, bool fresh) {
+    HIP_TRY(hipStreamSynchronize(c->stream));   // the staging buffer may still be in use
+    if (fresh) {
+        *c->h_ctl = LoopCtl{0, 0, CCD_STOP_CONVERGED, max_iters, 100.0, tol, kElogCap, 0};
+    } else {
+        HIP_TRY(hipMemcpy(c->h_ctl, c->d_ctl, sizeof(LoopCtl), hipMemcpyDeviceToHost));
+        c->h_ctl->done = 0;
+        c->h_ctl->reason = CCD_STOP_CONVERGED;
+        c->h_ctl->tol = tol;
+        c->h_ctl->max_iters = max_iters;
+    }
+    HIP_TRY(hipMemcpyAsync(c->d_ctl, c->h_ctl, sizeof(LoopCtl), hipMemcpyHostToDevice, c->stream));
+    return CCD_OK;
+}
+
+int arm_free_run(ccd_ctx *c) { return arm_loop(c, -1.0, INT_MAX, true); }
+
+// Queue iterations in batches without waiting for each energy; after each
+// batch read the loop state.  Iterations queued past the stopping point do
+// nothing.  Returns with the loop state in h_ctl and the current amplitudes
+// set to those of the last real iteration.
+int run_loop(ccd_ctx *c, double mixing, int batch) {
+    const int cur0 = c->cur, count0 = c->h_ctl->count;
+    for (;;) {
+        for (int k = 0; k < batch; k++) {
+            int rc = run_iteration(c, mixing, nullptr);
+            if (rc) return rc;
+        }
+        HIP_TRY(hipMemcpyAsync(c->h_ctl, c->d_ctl, sizeof(LoopCtl), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if (c->h_ctl->done) break;
+    }
+    c->cur = (cur0 + (c->h_ctl->count - count0)) & 1;
+    return CCD_OK;
+}
+
+int copy_log(ccd_ctx *c, double *energies, int cap, int count) {
+    const int n = std::min(std::min(cap, count), kElogCap);
+    if (energies && n > 0) {
+        HIP_TRY(hipMemcpyAsync(energies, c->d_elog, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    return CCD_OK;
+}
+
 int zero_state(ccd_ctx *c) {
     const size_t nt = sizeof(double) * (size_t)c->nt2;
     HIP_TRY(hipMemsetAsync(c->d_t[0], 0, nt, c->stream));
@@ -775,17 +872,18 @@ int zero_state(ccd_ctx *c) {
     HIP_TRY(hipMemsetAsync(c->d_tas, 0, sizeof(double) * (size_t)std::max<int64_t>(c->n_tp, 1), c->stream));
     HIP_TRY(hipMemsetAsync(c->d_onebody, 0, sizeof(double) * c->nsp, c->stream));
     c->cur = 0;
-    return CCD_OK;
+    return arm_free_run(c);
 }
 
 // Load host amplitudes as the current T and derive T^ph, the antisymmetrised
 // packed rows, chi and psi from them.
 int set_state(ccd_ctx *c, const double *t2) {
+    if (int rc0 = arm_free_run(c)) return rc0;
     double *t = c->d_t[c->cur];
     HIP_TRY(hipMemcpyAsync(t, t2, sizeof(double) * c->nt2, hipMemcpyHostToDevice, c->stream));
     int rc;
-    if ((rc = launch_transpose(c, true, t, c->d_tph)) || (rc = launch_antisym(c, t)) ||
-        (rc = launch_update<false>(c, t, nullptr, 0.0)) || (rc = launch_reduce(c)))
+    if ((rc = launch_transpose(c, true, t, c->d_tph, c->stream)) || (rc = launch_antisym(c, t)) ||
+        (rc = launch_update<false>(c, t, nullptr, 0.0, c->stream)) || (rc = launch_reduce(c, c->stream)))
         return rc;
     return CCD_OK;
 }
@@ -816,10 +914,17 @@ int ccd_ctx_create(const ccd_problem *prob, int precision, ccd_ctx **out) {
     c->iter_prec = precision;
     c->matrix_free = matrix_free;
     if (hipGetDevice(&c->device) != hipSuccess ||
-        hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete c;
-        return ccd::fail(CCD_E_HIP, "could not create a HIP stream");
+        hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->aux[0], hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->aux[1], hipStreamNonBlocking) != hipSuccess) {
+        free_ctx(c);
+        return ccd::fail(CCD_E_HIP, "could not create the HIP streams");
     }
+    for (hipEvent_t &e : c->sync_ev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            free_ctx(c);
+            return ccd::fail(CCD_E_HIP, "could not create HIP events");
+        }
     try {
         rc = build(c, prob);
     } catch (const std::bad_alloc &) {
@@ -897,7 +1002,9 @@ int ccd_reset(ccd_ctx *c) {
 int ccd_iterate(ccd_ctx *c, double mixing, double *corr) {
     ccd::clear_error();
     if (!c) return ccd::fail(CCD_E_INVALID, "null context");
-    int rc = run_iteration(c, mixing, nullptr);
+    int rc = arm_free_run(c);
+    if (rc) return rc;
+    rc = run_iteration(c, mixing, nullptr);
     if (rc) return rc;
     double e = 0.0;
     if ((rc = read_scalar(c, &e))) return rc;
@@ -909,26 +1016,14 @@ int ccd_solve(ccd_ctx *c, double tol, int max_iters, double mixing, double *ener
               int *stop) {
     ccd::clear_error();
     if (!c) return ccd::fail(CCD_E_INVALID, "null context");
+    if (max_iters < 0) return ccd::fail(CCD_E_INVALID, "max_iters < 0");
     int rc = zero_state(c);
-    if (rc) return rc;
-    double diff = 1.0, prev = 100.0, corr = 100.0;
-    int it = 0, why = CCD_STOP_CONVERGED;
-    while (std::fabs(diff) > tol) {
-        if (it > max_iters) {
-            why = CCD_STOP_MAX_ITERS;
-            break;
-        }
-        if ((rc = run_iteration(c, mixing, nullptr))) return rc;
-        if ((rc = read_scalar(c, &corr))) return rc;
-        diff = corr - prev;
-        prev = corr;
-        if (energies && it < cap) energies[it] = corr;
-        it++;
-    }
-    if (why == CCD_STOP_CONVERGED && std::isnan(diff)) why = CCD_STOP_NAN;
-    if (n_iter) *n_iter = it;
-    if (stop) *stop = why;
-    return CCD_OK;
+    if (rc || (rc = arm_loop(c, tol, max_iters, true)) || (rc = run_loop(c, mixing, kSolveBatch))) return rc;
+    const LoopCtl st = *c->h_ctl;
+    if ((rc = copy_log(c, energies, cap, st.count))) return rc;
+    if (n_iter) *n_iter = st.count;
+    if (stop) *stop = st.reason;
+    return arm_free_run(c);
 }
 
 int ccd_set_precision(ccd_ctx *c, int precision) {
@@ -948,40 +1043,43 @@ int ccd_solve_mixed(ccd_ctx *c, double tol, double switch_tol, int max_iters, do
     if (!c) return ccd::fail(CCD_E_INVALID, "null context");
     if (c->precision != CCD_PREC_FP32)
         return ccd::fail(CCD_E_STATE, "ccd_solve_mixed needs a context created with CCD_PREC_FP32");
-    int rc = zero_state(c);
-    if (rc) return rc;
+    if (max_iters < 0) return ccd::fail(CCD_E_INVALID, "max_iters < 0");
     const int saved = c->iter_prec;
+    int rc = zero_state(c);
+    // FP32 contractions while |dE| > max(tol, switch_tol) ...
     c->iter_prec = CCD_PREC_FP32;
-    double diff = 1.0, prev = 100.0, corr = 100.0;
-    int it = 0, n32 = 0, why = CCD_STOP_CONVERGED;
-    while (std::fabs(diff) > tol) {
-        if (it > max_iters) {
-            why = CCD_STOP_MAX_ITERS;
-            break;
-        }
-        // FP32 contractions while the iteration moves by more than switch_tol,
-        // then FP64 sweeps to the FP64 fixed point
-        if (c->iter_prec == CCD_PREC_FP32 && std::fabs(diff) <= switch_tol) c->iter_prec = CCD_PREC_FP64;
-        if (c->iter_prec == CCD_PREC_FP32) n32++;
-        if ((rc = run_iteration(c, mixing, nullptr))) {
-            c->iter_prec = saved;
-            return rc;
-        }
-        if ((rc = read_scalar(c, &corr))) {
-            c->iter_prec = saved;
-            return rc;
-        }
-        diff = corr - prev;
-        prev = corr;
-        if (energies && it < cap) energies[it] = corr;
-        it++;
+    if (!rc) rc = arm_loop(c, std::max(tol, switch_tol), max_iters, true);
+    if (!rc) rc = run_loop(c, mixing, kSolveBatch);
+    int n32 = rc ? 0 : c->h_ctl->count;
+    // ... then FP64 sweeps, continuing the same loop, while |dE| > tol
+    if (!rc && c->h_ctl->reason == CCD_STOP_CONVERGED && switch_tol > tol) {
+        c->iter_prec = CCD_PREC_FP64;
+        rc = arm_loop(c, tol, max_iters, false);
+        if (!rc) rc = run_loop(c, mixing, kSolveBatch);
     }
     c->iter_prec = saved;
-    if (why == CCD_STOP_CONVERGED && std::isnan(diff)) why = CCD_STOP_NAN;
-    if (n_iter) *n_iter = it;
+    if (rc) return rc;
+    const LoopCtl st = *c->h_ctl;
+    if ((rc = copy_log(c, energies, cap, st.count))) return rc;
+    if (n_iter) *n_iter = st.count;
     if (n_fp32) *n_fp32 = n32;
-    if (stop) *stop = why;
-    return CCD_OK;
+    if (stop) *stop = st.reason;
+    return arm_free_run(c);
+}
+
+int ccd_iterate_n(ccd_ctx *c, int n, double mixing, double *energies) {
+    ccd::clear_error();
+    if (!c || n < 0) return ccd::fail(CCD_E_INVALID, "ccd_iterate_n: bad argument");
+    int rc = arm_free_run(c);
+    for (int k = 0; k < n && !rc; k++) rc = run_iteration(c, mixing, nullptr);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(c->h_ctl, c->d_ctl, sizeof(LoopCtl), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->h_ctl->count != n) {   // a NaN energy stops even a free run
+        c->cur = (c->cur + c->h_ctl->count - n) & 1;
+        return ccd::fail(CCD_E_STATE, "iteration %d produced a NaN energy", c->h_ctl->count);
+    }
+    return copy_log(c, energies, n, n);
 }
 
 int ccd_get_t2(ccd_ctx *c, double *t2) {
@@ -1003,11 +1101,14 @@ int ccd_set_t2(ccd_ctx *c, const double *t2) {
 
 int ccd_ladder(ccd_ctx *c, const double *t, double *d) {
     ccd::clear_error();
-    if (!c || !t || !d) return ccd::fail(CCD_E_INVALID, "ccd_ladder: null argument");
+    if (!c || !t || !d) return ccd::fail(CCD_E_INVALID, "ccd_ladder: null argument");    if (int rc0 = arm_free_run(c)) return rc0;
+
     double *dt = c->d_t[1 - c->cur];  // scratch: the non-current T buffer
     HIP_TRY(hipMemcpyAsync(dt, t, sizeof(double) * c->nt2, hipMemcpyHostToDevice, c->stream));
     int rc;
-    if ((rc = launch_antisym(c, dt)) || (rc = launch_ladder_x(c)) || (rc = launch_ladder_v(c))) return rc;
+    if ((rc = launch_antisym(c, dt)) || (rc = launch_ladder_x(c, c->stream)) ||
+        (rc = launch_ladder_v(c, 1, NTMAX, c->stream)))
+        return rc;
     if (!c->utiles.empty()) {
         UpdArgs a = upd_args(c, nullptr, nullptr, 0.0);
         hipLaunchKernelGGL(k_expand_d, dim3((unsigned)c->utiles.size()), dim3(kThreads), 0, c->stream, a, c->d_g);
@@ -1020,9 +1121,10 @@ int ccd_ladder(ccd_ctx *c, const double *t, double *d) {
 
 int ccd_ring(ccd_ctx *c, const double *tph, double *r) {
     ccd::clear_error();
-    if (!c || !tph || !r) return ccd::fail(CCD_E_INVALID, "ccd_ring: null argument");
+    if (!c || !tph || !r) return ccd::fail(CCD_E_INVALID, "ccd_ring: null argument");    if (int rc0 = arm_free_run(c)) return rc0;
+
     HIP_TRY(hipMemcpyAsync(c->d_tph, tph, sizeof(double) * c->nt2, hipMemcpyHostToDevice, c->stream));
-    int rc = launch_ring(c);
+    int rc = launch_ring(c, c->stream);
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(r, c->d_r, sizeof(double) * c->nt2, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1031,10 +1133,11 @@ int ccd_ring(ccd_ctx *c, const double *tph, double *r) {
 
 int ccd_remap_ph(ccd_ctx *c, const double *t, double *tph) {
     ccd::clear_error();
-    if (!c || !t || !tph) return ccd::fail(CCD_E_INVALID, "ccd_remap_ph: null argument");
+    if (!c || !t || !tph) return ccd::fail(CCD_E_INVALID, "ccd_remap_ph: null argument");    if (int rc0 = arm_free_run(c)) return rc0;
+
     double *dt = c->d_t[1 - c->cur];
     HIP_TRY(hipMemcpyAsync(dt, t, sizeof(double) * c->nt2, hipMemcpyHostToDevice, c->stream));
-    int rc = launch_transpose(c, true, dt, c->d_tph);
+    int rc = launch_transpose(c, true, dt, c->d_tph, c->stream);
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(tph, c->d_tph, sizeof(double) * c->nt2, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1069,6 +1172,8 @@ int ccd_step(ccd_ctx *c, const double *t_old, double *t_new, double mixing, doub
 int ccd_time_phases(ccd_ctx *c, int iters, double mixing, double *ms) {
     ccd::clear_error();
     if (!c || !ms || iters <= 0) return ccd::fail(CCD_E_INVALID, "ccd_time_phases: bad argument");
+    int rc0 = arm_free_run(c);
+    if (rc0) return rc0;
     hipEvent_t ev[6];
     for (int k = 0; k < 6; k++) HIP_TRY(hipEventCreate(&ev[k]));
     double acc[6] = {0, 0, 0, 0, 0, 0};

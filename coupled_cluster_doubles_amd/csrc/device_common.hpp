@@ -9,6 +9,8 @@
 
 constexpr int kThreads = 256;   // every kernel runs 4 wave64s per block
 constexpr int kZeroDoubles = 32;  // size of the zero buffer masked-off loads read (256 B)
+constexpr int kElogCap = 1024;    // energies logged by a device-run iteration loop
+constexpr int kSolveBatch = 4;    // iterations queued between two reads of the loop state
 
 struct DChan {          // active direct channel (pp > 0 and hh > 0)
     int64_t off_t;      // block offset in the pphh layout
@@ -39,6 +41,21 @@ struct DRow {           // one contiguous row of a layout, for the per-particle 
     int32_t len;
     int32_t loff;       // offset in the particle's LDS block
     int32_t pad;
+};
+
+// Device-side state of the iteration loop (src/main.cpp:491-519 run on the
+// device).  k_energy applies the reference's stop rule after each iteration;
+// once `done` is set every kernel of later iterations returns at entry, so
+// the host can queue iterations ahead and read the outcome in batches.
+struct LoopCtl {
+    int done;        // set once the loop would have stopped
+    int count;       // iterations completed
+    int reason;      // CCD_STOP_* once done
+    int max_iters;   // halt once more than this many iterations have run
+    double prev;     // energy of the previous iteration (100.0 at the start)
+    double tol;      // continue while |dE| > tol
+    int cap;         // entries of the energy log
+    int pad;
 };
 
 struct Gen {            // on-device interaction

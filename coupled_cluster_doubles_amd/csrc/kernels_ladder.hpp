@@ -25,7 +25,9 @@ __global__ __launch_bounds__(kThreads) void k_ladder_x(const DTile *__restrict__
                                                        const double *__restrict__ vtp,
                                                        const double *__restrict__ tas,
                                                        const double *__restrict__ vh,
-                                                       double *__restrict__ x) {
+                                                       double *__restrict__ x,
+                                                       const LoopCtl *__restrict__ ctl) {
+    if (ctl->done) return;
     __shared__ double part[kThreads / 64][64][4];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
     const DTile tl = tiles[blockIdx.x];
@@ -161,7 +163,9 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
                                                        const double *__restrict__ tas,
                                                        const double *__restrict__ x,
                                                        const double *__restrict__ zero,
-                                                       double *__restrict__ dp) {
+                                                       double *__restrict__ dp,
+                                                       const LoopCtl *__restrict__ ctl) {
+    if (ctl->done) return;
     constexpr int KCH = LadderShape<NT>::KCH, KSEG = LadderShape<NT>::KSEG, G = KCH / 4;
     constexpr int W = NT * 16, STR = W + 1, BPT = (KCH * W + kThreads - 1) / kThreads;
     constexpr bool GEN = SRC != LV_STORED;

@@ -96,7 +96,9 @@ __global__ __launch_bounds__(kThreads) void k_ring(const RingItem *__restrict__ 
                                                    const double *__restrict__ tph,
                                                    const CT *__restrict__ va, const CT *__restrict__ vb,
                                                    const double *__restrict__ zero,
-                                                   double *__restrict__ r) {
+                                                   double *__restrict__ r,
+                                                   const LoopCtl *__restrict__ ctl) {
+    if (ctl->done) return;
     const int it = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
     if (it >= nitems) return;
     const RingItem w = items[it];

@@ -30,12 +30,14 @@ struct UpdArgs {
     double *cp;                      // W column partial sums per tile, indexed aux + IJ
     double mixing;
     int nh;
+    const LoopCtl *ctl;
 };
 
 // UPDATE = true: one Jacobi step for both rows of each pair; UPDATE = false:
 // only the W sums of the current amplitudes (to derive chi, psi, E from them).
 template <bool UPDATE>
 __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
+    if (a.ctl->done) return;
     __shared__ double w1s[kThreads], w2s[kThreads];
     const DTile tl = a.tiles[blockIdx.x];
     const DChan ch = a.chans[tl.chan];
@@ -137,7 +139,9 @@ __global__ __launch_bounds__(kThreads) void k_transpose(const int32_t *__restric
                                                         const int32_t *__restrict__ dst_addr,
                                                         const uint16_t *__restrict__ perm,
                                                         const double *__restrict__ src,
-                                                        double *__restrict__ dst) {
+                                                        double *__restrict__ dst,
+                                                        const LoopCtl *__restrict__ ctl) {
+    if (ctl->done) return;
     extern __shared__ double buf[];
     const int a = blockIdx.x;
     const int b0 = base[a], n = base[a + 1] - b0;
@@ -152,7 +156,9 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const int32_t *__restrict__
                                                      const int32_t *__restrict__ idx, int nh,
                                                      const double *__restrict__ cp,
                                                      const double *__restrict__ rs,
-                                                     double *__restrict__ out) {
+                                                     double *__restrict__ out,
+                                                     const LoopCtl *__restrict__ ctl) {
+    if (ctl->done) return;
     __shared__ double red[kThreads / 64];
     const int k = blockIdx.x;
     const double *srcv = k < nh ? cp : rs;
@@ -162,14 +168,34 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const int32_t *__restrict__
     if (threadIdx.x == 0) out[k] = b;
 }
 
-// E = 1/4 sum_i chi_i = 1/4 sum V o T (every element has one first hole index).
+// E = 1/4 sum_i chi_i = 1/4 sum V o T (every element has one first hole
+// index), then the loop bookkeeping of src/main.cpp:511-519, 667-670:
+// dE = E - E_prev, E_prev = E, iterations++, and the loop goes on while
+// |dE| > tol unless more than max_iters iterations have run.
 __global__ __launch_bounds__(kThreads) void k_energy(const double *__restrict__ chi, int nh,
-                                                     double *__restrict__ out) {
+                                                     double *__restrict__ out, LoopCtl *__restrict__ ctl,
+                                                     double *__restrict__ log) {
+    if (ctl->done) return;
     __shared__ double red[kThreads / 64];
     double s = 0.0;
     for (int k = threadIdx.x; k < nh; k += kThreads) s += chi[k];
     const double b = block_sum(s, red);
-    if (threadIdx.x == 0) out[0] = 0.25 * b;
+    if (threadIdx.x == 0) {
+        const double e = 0.25 * b;
+        out[0] = e;
+        const double diff = e - ctl->prev;
+        ctl->prev = e;
+        const int it = ctl->count;
+        if (it < ctl->cap) log[it] = e;
+        ctl->count = it + 1;
+        if (!(fabs(diff) > ctl->tol)) {
+            ctl->reason = isnan(diff) ? CCD_STOP_NAN : CCD_STOP_CONVERGED;
+            ctl->done = 1;
+        } else if (it + 1 > ctl->max_iters) {
+            ctl->reason = CCD_STOP_MAX_ITERS;
+            ctl->done = 1;
+        }
+    }
 }
 
 // MBPT2 terms v^2 / denominator over the pair tiles, per-block partial sums.

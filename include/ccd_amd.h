@@ -139,10 +139,17 @@ int ccd_reset(ccd_ctx *ctx);
  * (calcT2_pphh) and the correlation energy of the new amplitudes. */
 int ccd_iterate(ccd_ctx *ctx, double mixing, double *corr);
 
+/* n iterations queued back to back with no host round trip in between;
+ * energies[k] (if not NULL) receives the energy after the k-th.  Fails with
+ * CCD_E_STATE if an energy is NaN (the iterations after it do nothing). */
+int ccd_iterate_n(ccd_ctx *ctx, int n, double mixing, double *energies);
+
 /* The reference protocol (src/main.cpp:491-519) from T2 = 0: loop while
  * |dE| > tol, dE starting at 1 and E_prev at 100, halt once more than
  * max_iters iterations have run (the reference uses max_iters = 100).
- * energies[k] receives the energy after iteration k for k < cap. */
+ * energies[k] receives the energy after iteration k for k < min(cap, 1024).
+ * The stop rule runs on the device after every iteration; the host queues
+ * iterations in small batches and reads the outcome between them. */
 int ccd_solve(ccd_ctx *ctx, double tol, int max_iters, double mixing,
               double *energies, int cap, int *n_iter, int *stop);
 

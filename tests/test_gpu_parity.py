@@ -214,3 +214,43 @@ def test_mixed_solve_within_1e7_per_particle(require_gpu, args, n_part):
     assert stop == stop64 == "converged"
     assert n32 > 0 and n > n32
     assert abs(e[-1] - e64[-1]) / n_part <= 1e-7
+
+
+# ---- device-side loop control
+
+def _host_loop(ctx, tol, max_iters=100):
+    """The reference protocol run from the host, one synchronous step at a time."""
+    ctx.reset()
+    diff, prev, out = 1.0, 100.0, []
+    while abs(diff) > tol:
+        if len(out) > max_iters:
+            return np.array(out), "max_iterations"
+        e = ctx.iterate()
+        diff, prev = e - prev, e
+        out.append(e)
+    return np.array(out), ("nan" if np.isnan(diff) else "converged")
+
+
+@pytest.mark.parametrize("tol", [1e-12, 0.0, -1.0])
+def test_device_loop_matches_host_protocol(require_gpu, tol):
+    m = Model.pairing(8, 8, 0.5, 1.0)
+    ctx = Context(m, FP64)
+    e_d, stop_d, n_d = ctx.solve(tol)
+    e_h, stop_h = _host_loop(ctx, tol)
+    assert stop_d == stop_h and n_d == len(e_h)
+    assert np.array_equal(e_d, e_h)
+    if tol < 0:
+        assert n_d == 101
+    # the amplitudes left behind are those of the last real iteration
+    t_d = ctx.get_t2()
+    ctx.solve(tol)
+    assert np.array_equal(ctx.get_t2(), t_d)
+
+
+def test_iterate_n_matches_single_steps(require_gpu):
+    m = Model.electron_gas(1.0, 19, 2)
+    a, b = Context(m, FP64, matrix_free=True), Context(m, FP64, matrix_free=True)
+    e_n = a.iterate_n(7)
+    e_1 = np.array([b.iterate() for _ in range(7)])
+    assert np.array_equal(e_n, e_1)
+    assert np.array_equal(a.get_t2(), b.get_t2())
