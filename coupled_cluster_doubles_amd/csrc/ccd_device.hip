@@ -7,7 +7,7 @@
 //   calcT2Diff_h2pp_h1, calcT2Diff_p2_p1hh   -> chi, psi from k_reduce, applied in k_update
 //   8 x mapFromDistributed, 5 x mapToDistributed (src/Chain.hpp:710-802)
 //                                            -> k_transpose (2 per iteration) + in-row swaps
-//   calcT2_pphh       src/calc.cpp:368-459   -> k_update + k_reduce + k_energy
+//   calcT2_pphh       src/calc.cpp:368-459   -> k_update + k_reduce (which ends with the energy)
 //   MBPT2Corr         src/MBPT2Corr.cpp:6-38 -> k_mbpt2 + k_sum_partials
 //   loop              src/main.cpp:491-682   -> ccd_solve
 //
@@ -77,8 +77,8 @@ struct ccd_ctx {
     int gen_blocks = 1024;       // grid of the generated-V ladder launches
     LadderGen lgen{};
     hipStream_t stream = nullptr;         // main stream: every entry point is ordered on it
-    hipStream_t aux[2] = {nullptr, nullptr};  // concurrent phases of an iteration
-    hipEvent_t sync_ev[6] = {};
+    hipStream_t aux[3] = {nullptr, nullptr, nullptr};  // concurrent phases of an iteration
+    hipEvent_t sync_ev[8] = {};
     int nsp = 0, nh = 0, np = 0;
     int64_t nt2 = 0;
     int64_t bytes = 0;
@@ -96,7 +96,8 @@ struct ccd_ctx {
     DTile *d_rtiles = nullptr, *d_ltiles = nullptr, *d_utiles = nullptr, *d_xtiles = nullptr;
     int2 *d_ppl = nullptr, *d_hhl = nullptr, *d_phl = nullptr, *d_hpl = nullptr;
     int32_t *d_hh_swap = nullptr, *d_prow = nullptr, *d_prow_sw = nullptr;
-    int32_t *d_red_ptr = nullptr, *d_red_idx = nullptr;
+    int32_t *d_red_ptr = nullptr, *d_cpos = nullptr, *d_rpos = nullptr;
+    int64_t n_red = 0;
     int32_t *d_tr_base = nullptr, *d_addr_pp = nullptr, *d_addr_ph = nullptr;
     uint16_t *d_perm_ph = nullptr, *d_perm_pp = nullptr;
     // device state and interaction
@@ -104,12 +105,13 @@ struct ccd_ctx {
     double *d_t[2] = {nullptr, nullptr};
     int cur = 0;
     double *d_dp = nullptr, *d_tph = nullptr, *d_r = nullptr, *d_g = nullptr, *d_x = nullptr;
-    double *d_tas = nullptr, *d_rs = nullptr, *d_cp = nullptr;
+    double *d_tas = nullptr, *d_red = nullptr;  // d_red: W partial sums in reduction-list order
     double *d_vt = nullptr, *d_vtp = nullptr, *d_vh = nullptr, *d_vp = nullptr, *d_va = nullptr,
            *d_vb = nullptr;
     double *d_partials = nullptr, *d_scalar = nullptr;
     double *d_zero = nullptr;    // kZeroDoubles zeros, never written: the target of masked-off loads
     LoopCtl *d_ctl = nullptr;    // device loop state
+    int *d_ticket = nullptr;     // last-block counter of k_reduce (zero between launches)
     LoopCtl *h_ctl = nullptr;    // pinned staging for it
     double *d_elog = nullptr;    // energy of every iteration of a device-run loop (kElogCap)
     double *h_scalar = nullptr;  // pinned
@@ -173,7 +175,7 @@ UpdArgs upd_args(ccd_ctx *c, const double *t_old, double *t_new, double mixing) 
     a.prow = c->d_prow; a.prow_sw = c->d_prow_sw; a.hh_swap = c->d_hh_swap;
     a.eps = c->d_eps; a.chi = c->d_onebody; a.psi = c->d_onebody + c->nh;
     a.vt = c->d_vt; a.dp = c->d_dp; a.g = c->d_g; a.t_old = t_old; a.t_new = t_new;
-    a.tas = c->d_tas; a.rs = c->d_rs; a.cp = c->d_cp; a.mixing = mixing; a.nh = c->nh; a.ctl = c->d_ctl;
+    a.tas = c->d_tas; a.red = c->d_red; a.cpos = c->d_cpos; a.rpos = c->d_rpos; a.mixing = mixing; a.nh = c->nh; a.ctl = c->d_ctl;
     return a;
 }
 
@@ -265,23 +267,38 @@ int launch_update(ccd_ctx *c, const double *t_old, double *t_new, double mixing,
 }
 
 int launch_reduce(ccd_ctx *c, hipStream_t st) {
-    hipLaunchKernelGGL(k_reduce, dim3((unsigned)c->nsp), dim3(kThreads), 0, st, c->d_red_ptr, c->d_red_idx,
-                       c->nh, c->d_cp, c->d_rs, c->d_onebody, c->d_ctl);
-    int rc = check_launch("k_reduce");
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_energy, dim3(1), dim3(kThreads), 0, st, c->d_onebody, c->nh, c->d_scalar, c->d_ctl,
-                       c->d_elog);
-    return check_launch("k_energy");
+    hipLaunchKernelGGL(k_reduce, dim3((unsigned)c->nsp), dim3(kThreads), 0, st, c->d_red_ptr, c->nh, c->d_red,
+                       c->d_onebody, c->d_ticket, c->d_scalar, c->d_ctl, c->d_elog);
+    return check_launch("k_reduce");
+}
+
+// The auxiliary streams start after everything queued so far on the main
+// stream (fork) / the main stream continues after everything queued on them
+// (join).  Untimed iterations run between a fork and a join.
+int fork_aux(ccd_ctx *c) {
+    HIP_TRY(hipEventRecord(c->sync_ev[6], c->stream));
+    for (hipStream_t s : c->aux) HIP_TRY(hipStreamWaitEvent(s, c->sync_ev[6], 0));
+    return CCD_OK;
+}
+
+int join_aux(ccd_ctx *c) {
+    for (hipStream_t s : c->aux) {
+        HIP_TRY(hipEventRecord(c->sync_ev[7], s));
+        HIP_TRY(hipStreamWaitEvent(c->stream, c->sync_ev[7], 0));
+    }
+    return CCD_OK;
 }
 
 // One iteration.  Timed (ev = 6 events): every phase in order on the main
 // stream, the events bracketing ring + R remap | X | pppp ladder | update |
 // T remap + reductions.  Untimed: the independent phases run concurrently
-// on three streams, joined by events --
-//   aux1: ring, R remap                 | after the update: T remap
-//   main: X, pppp ladder (1 tile/side)  | update | reductions
-//   aux2:    pppp ladder (2+ tiles)     |
-// and the iteration ends with everything joined on the main stream.
+// on four streams, ordered by events --
+//   aux1: ring, R remap                 | after the update: T remap (which the
+//                                       |   next iteration's ring follows)
+//   main: X, pppp ladder (1 tile/side)  | update | reductions + energy
+//   aux2:    pppp ladder (2 tiles/side) |
+//   aux3:    pppp ladder (3+)           |
+// Untimed iterations must run between fork_aux and join_aux.
 int run_iteration(ccd_ctx *c, double mixing, hipEvent_t *ev) {
     const double *t_old = c->d_t[c->cur];
     double *t_new = c->d_t[1 - c->cur];
@@ -300,27 +317,27 @@ int run_iteration(ccd_ctx *c, double mixing, hipEvent_t *ev) {
         if ((rc = launch_transpose(c, true, t_new, c->d_tph, st)) || (rc = launch_reduce(c, st))) return rc;
         HIP_TRY(hipEventRecord(ev[5], st));
     } else {
-        const hipStream_t s0 = c->stream, s1 = c->aux[0], s2 = c->aux[1];
+        const hipStream_t s0 = c->stream, s1 = c->aux[0], s2 = c->aux[1], s3 = c->aux[2];
         hipEvent_t *e = c->sync_ev;
-        HIP_TRY(hipEventRecord(e[0], s0));
-        HIP_TRY(hipStreamWaitEvent(s1, e[0], 0));
         if ((rc = launch_ring(c, s1)) || (rc = launch_transpose(c, false, c->d_r, c->d_g, s1))) return rc;
-        HIP_TRY(hipEventRecord(e[1], s1));
+        HIP_TRY(hipEventRecord(e[0], s1));
         if ((rc = launch_ladder_x(c, s0))) return rc;
-        HIP_TRY(hipEventRecord(e[2], s0));
-        HIP_TRY(hipStreamWaitEvent(s2, e[2], 0));
-        if ((rc = launch_ladder_v(c, 2, NTMAX, s2))) return rc;
-        HIP_TRY(hipEventRecord(e[3], s2));
+        HIP_TRY(hipEventRecord(e[1], s0));
+        HIP_TRY(hipStreamWaitEvent(s2, e[1], 0));
+        HIP_TRY(hipStreamWaitEvent(s3, e[1], 0));
+        if ((rc = launch_ladder_v(c, 2, 2, s2))) return rc;
+        HIP_TRY(hipEventRecord(e[2], s2));
+        if ((rc = launch_ladder_v(c, 3, NTMAX, s3))) return rc;
+        HIP_TRY(hipEventRecord(e[3], s3));
         if ((rc = launch_ladder_v(c, 1, 1, s0))) return rc;
-        HIP_TRY(hipStreamWaitEvent(s0, e[1], 0));
+        HIP_TRY(hipStreamWaitEvent(s0, e[0], 0));
+        HIP_TRY(hipStreamWaitEvent(s0, e[2], 0));
         HIP_TRY(hipStreamWaitEvent(s0, e[3], 0));
         if ((rc = launch_update<true>(c, t_old, t_new, mixing, s0))) return rc;
         HIP_TRY(hipEventRecord(e[4], s0));
         HIP_TRY(hipStreamWaitEvent(s1, e[4], 0));
         if ((rc = launch_transpose(c, true, t_new, c->d_tph, s1))) return rc;
-        HIP_TRY(hipEventRecord(e[5], s1));
         if ((rc = launch_reduce(c, s0))) return rc;
-        HIP_TRY(hipStreamWaitEvent(s0, e[5], 0));
     }
     c->cur = 1 - c->cur;
     return CCD_OK;
@@ -684,6 +701,14 @@ int build(ccd_ctx *c, const ccd_problem *p) {
         red_idx.insert(red_idx.end(), v.begin(), v.end());
         red_ptr.push_back((int32_t)red_idx.size());
     }
+    // the update writes each partial straight to its place in the list order,
+    // so every chi_i / psi_a is a sum over a contiguous range
+    std::vector<int32_t> cpos((size_t)std::max<int64_t>(c->n_cp, 1), -1), rpos((size_t)std::max<int64_t>(npp, 1), -1);
+    for (int32_t q = 0; q < (int32_t)red_idx.size(); q++) {
+        if (q < red_ptr[nh]) cpos[red_idx[q]] = q;
+        else rpos[red_idx[q]] = q;
+    }
+    c->n_red = (int64_t)red_idx.size();
 
     // ---- upload tables
     int rc;
@@ -695,7 +720,7 @@ int build(ccd_ctx *c, const ccd_problem *p) {
         (rc = upload(c, &c->d_hhl, hhl)) || (rc = upload(c, &c->d_phl, phl)) || (rc = upload(c, &c->d_hpl, hpl)) ||
         (rc = upload(c, &c->d_hh_swap, hh_swap)) || (rc = upload(c, &c->d_prow, prow)) ||
         (rc = upload(c, &c->d_prow_sw, prow_sw)) || (rc = upload(c, &c->d_red_ptr, red_ptr)) ||
-        (rc = upload(c, &c->d_red_idx, red_idx)) || (rc = upload(c, &c->d_tr_base, tr_base)) ||
+        (rc = upload(c, &c->d_cpos, cpos)) || (rc = upload(c, &c->d_rpos, rpos)) || (rc = upload(c, &c->d_tr_base, tr_base)) ||
         (rc = upload(c, &c->d_addr_pp, addr_pp)) || (rc = upload(c, &c->d_addr_ph, addr_ph)) ||
         (rc = upload(c, &c->d_perm_ph, perm_ph)) || (rc = upload(c, &c->d_perm_pp, perm_pp)))
         return rc;
@@ -709,13 +734,13 @@ int build(ccd_ctx *c, const ccd_problem *p) {
         (rc = dalloc(c, &c->d_vtp, (size_t)c->n_tp)) || (rc = dalloc(c, &c->d_x, (size_t)c->n_hhhh)) ||
         (rc = dalloc(c, &c->d_vh, (size_t)c->n_hhhh)) ||
         (rc = dalloc(c, &c->d_vp, c->matrix_free ? (size_t)1 : (size_t)c->n_vp)) ||
-        (rc = dalloc(c, &c->d_vb, (size_t)c->n_b)) || (rc = dalloc(c, &c->d_rs, (size_t)std::max<int64_t>(npp, 1))) ||
-        (rc = dalloc(c, &c->d_cp, (size_t)std::max<int64_t>(c->n_cp, 1))) ||
+        (rc = dalloc(c, &c->d_vb, (size_t)c->n_b)) || (rc = dalloc(c, &c->d_red, (size_t)std::max<int64_t>(c->n_red, 1))) ||
         (rc = dalloc(c, &c->d_partials, std::max(c->utiles.size(), (size_t)1))) || (rc = dalloc(c, &c->d_scalar, 4)) ||
-        (rc = dalloc(c, &c->d_zero, (size_t)kZeroDoubles)) || (rc = dalloc(c, &c->d_ctl, 1)) ||
+        (rc = dalloc(c, &c->d_zero, (size_t)kZeroDoubles)) || (rc = dalloc(c, &c->d_ctl, 1)) || (rc = dalloc(c, &c->d_ticket, 1)) ||
         (rc = dalloc(c, &c->d_elog, (size_t)kElogCap)))
         return rc;
     HIP_TRY(hipHostMalloc((void **)&c->h_ctl, sizeof(LoopCtl), hipHostMallocDefault));
+    HIP_TRY(hipMemset(c->d_ticket, 0, sizeof(int)));
     HIP_TRY(hipMemset(c->d_zero, 0, sizeof(double) * kZeroDoubles));
     HIP_TRY(hipHostMalloc((void **)&c->h_scalar, 4 * sizeof(double), hipHostMallocDefault));
 
@@ -842,16 +867,18 @@ int arm_free_run(ccd_ctx *c) { return arm_loop(c, -1.0, INT_MAX, true); }
 // set to those of the last real iteration.
 int run_loop(ccd_ctx *c, double mixing, int batch) {
     const int cur0 = c->cur, count0 = c->h_ctl->count;
+    int rc = fork_aux(c);
+    if (rc) return rc;
     for (;;) {
-        for (int k = 0; k < batch; k++) {
-            int rc = run_iteration(c, mixing, nullptr);
-            if (rc) return rc;
-        }
+        for (int k = 0; k < batch; k++)
+            if ((rc = run_iteration(c, mixing, nullptr))) return rc;
         HIP_TRY(hipMemcpyAsync(c->h_ctl, c->d_ctl, sizeof(LoopCtl), hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
         if (c->h_ctl->done) break;
     }
     c->cur = (cur0 + (c->h_ctl->count - count0)) & 1;
+    if ((rc = join_aux(c))) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return CCD_OK;
 }
 
@@ -916,7 +943,8 @@ int ccd_ctx_create(const ccd_problem *prob, int precision, ccd_ctx **out) {
     if (hipGetDevice(&c->device) != hipSuccess ||
         hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->aux[0], hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&c->aux[1], hipStreamNonBlocking) != hipSuccess) {
+        hipStreamCreateWithFlags(&c->aux[1], hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->aux[2], hipStreamNonBlocking) != hipSuccess) {
         free_ctx(c);
         return ccd::fail(CCD_E_HIP, "could not create the HIP streams");
     }
@@ -1003,9 +1031,7 @@ int ccd_iterate(ccd_ctx *c, double mixing, double *corr) {
     ccd::clear_error();
     if (!c) return ccd::fail(CCD_E_INVALID, "null context");
     int rc = arm_free_run(c);
-    if (rc) return rc;
-    rc = run_iteration(c, mixing, nullptr);
-    if (rc) return rc;
+    if (rc || (rc = fork_aux(c)) || (rc = run_iteration(c, mixing, nullptr)) || (rc = join_aux(c))) return rc;
     double e = 0.0;
     if ((rc = read_scalar(c, &e))) return rc;
     if (corr) *corr = e;
@@ -1071,8 +1097,9 @@ int ccd_iterate_n(ccd_ctx *c, int n, double mixing, double *energies) {
     ccd::clear_error();
     if (!c || n < 0) return ccd::fail(CCD_E_INVALID, "ccd_iterate_n: bad argument");
     int rc = arm_free_run(c);
+    if (!rc) rc = fork_aux(c);
     for (int k = 0; k < n && !rc; k++) rc = run_iteration(c, mixing, nullptr);
-    if (rc) return rc;
+    if (rc || (rc = join_aux(c))) return rc;
     HIP_TRY(hipMemcpyAsync(c->h_ctl, c->d_ctl, sizeof(LoopCtl), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (c->h_ctl->count != n) {   // a NaN energy stops even a free run
@@ -1160,7 +1187,7 @@ int ccd_step(ccd_ctx *c, const double *t_old, double *t_new, double mixing, doub
     if (!c || !t_old || !t_new) return ccd::fail(CCD_E_INVALID, "ccd_step: null argument");
     int rc = set_state(c, t_old);
     if (rc) return rc;
-    if ((rc = run_iteration(c, mixing, nullptr))) return rc;
+    if ((rc = fork_aux(c)) || (rc = run_iteration(c, mixing, nullptr)) || (rc = join_aux(c))) return rc;
     double e = 0.0;
     if ((rc = read_scalar(c, &e))) return rc;
     HIP_TRY(hipMemcpyAsync(t_new, c->d_t[c->cur], sizeof(double) * c->nt2, hipMemcpyDeviceToHost, c->stream));

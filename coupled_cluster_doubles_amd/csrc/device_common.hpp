@@ -44,7 +44,7 @@ struct DRow {           // one contiguous row of a layout, for the per-particle 
 };
 
 // Device-side state of the iteration loop (src/main.cpp:491-519 run on the
-// device).  k_energy applies the reference's stop rule after each iteration;
+// device).  k_reduce applies the reference's stop rule after each iteration;
 // once `done` is set every kernel of later iterations returns at entry, so
 // the host can queue iterations ahead and read the outcome in batches.
 struct LoopCtl {

@@ -153,7 +153,7 @@ __device__ __forceinline__ int kslot(int q, int s) {
 template <int NT>
 struct LadderShape {
     static constexpr int KCH = 32;                  // k per pipeline stage
-    static constexpr int KSEG = NT <= 2 ? 256 : 64; // packed pairs per k segment
+    static constexpr int KSEG = 256;                // packed pairs per k segment
 };
 
 template <typename CT, int NT, int SRC>

@@ -3,7 +3,7 @@
 //   the one-body terms (calc.cpp:9-25, 73-89)              -> chi/psi from k_reduce, applied in k_update
 //   mapToDistributed(ai_jb) / mapFromDistributed x 4       -> k_transpose (per particle, LDS-staged)
 //   (src/Chain.hpp:710-802)
-//   the energy sum (calc.cpp:417-420, main.cpp:667-668)     -> k_reduce + k_energy
+//   the energy sum (calc.cpp:417-420, main.cpp:667-668)     -> k_reduce (last block)
 //   MBPT2ChannelCorr (src/MBPT2Corr.cpp:6-28)              -> k_mbpt2
 #pragma once
 
@@ -26,8 +26,9 @@ struct UpdArgs {
     const double *g;                 // ring R gathered to the pphh layout: g[(a,b),(i,j)] = R[(a,i),(j,b)]
     const double *t_old;
     double *t_new, *tas;             // new amplitudes; their (a,b)-antisymmetric packed rows
-    double *rs;                      // W = V o T row sums, indexed pp_off + row
-    double *cp;                      // W column partial sums per tile, indexed aux + IJ
+    double *red;                     // W = V o T partial sums in reduction-list order:
+    const int32_t *cpos;             //   column partial (tile aux + IJ) -> place (chi_i lists)
+    const int32_t *rpos;             //   row sum (pp_off + row) -> place (psi_a lists)
     double mixing;
     int nh;
     const LoopCtl *ctl;
@@ -116,13 +117,13 @@ __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
             s2 += w2s[l * H + q];
         }
         const int r1 = a.prow[ch.pr_off + pr], r2 = a.prow_sw[ch.pr_off + pr];
-        a.rs[ch.pp_off + r1] = s1;
-        if (r1 != r2) a.rs[ch.pp_off + r2] = s2;
+        a.red[a.rpos[ch.pp_off + r1]] = s1;
+        if (r1 != r2) a.red[a.rpos[ch.pp_off + r2]] = s2;
     }
     if (l < H) {
         double s = 0.0;
         for (int q = 0; q < tl.nrows; q++) s += w1s[q * H + l] + w2s[q * H + l];
-        a.cp[tl.aux + l] = s;
+        a.red[a.cpos[tl.aux + l]] = s;
     }
 }
 
@@ -151,38 +152,45 @@ __global__ __launch_bounds__(kThreads) void k_transpose(const int32_t *__restric
 }
 
 // chi_i (blocks 0..nh-1) and psi_a (blocks nh..nsp-1) as fixed-order sums of
-// listed column partials / row sums.
-__global__ __launch_bounds__(kThreads) void k_reduce(const int32_t *__restrict__ ptr,
-                                                     const int32_t *__restrict__ idx, int nh,
-                                                     const double *__restrict__ cp,
-                                                     const double *__restrict__ rs,
-                                                     double *__restrict__ out,
-                                                     const LoopCtl *__restrict__ ctl) {
-    if (ctl->done) return;
-    __shared__ double red[kThreads / 64];
-    const int k = blockIdx.x;
-    const double *srcv = k < nh ? cp : rs;
-    double s = 0.0;
-    for (int q = ptr[k] + threadIdx.x; q < ptr[k + 1]; q += kThreads) s += srcv[idx[q]];
-    const double b = block_sum(s, red);
-    if (threadIdx.x == 0) out[k] = b;
-}
-
-// E = 1/4 sum_i chi_i = 1/4 sum V o T (every element has one first hole
-// index), then the loop bookkeeping of src/main.cpp:511-519, 667-670:
-// dE = E - E_prev, E_prev = E, iterations++, and the loop goes on while
-// |dE| > tol unless more than max_iters iterations have run.
-__global__ __launch_bounds__(kThreads) void k_energy(const double *__restrict__ chi, int nh,
-                                                     double *__restrict__ out, LoopCtl *__restrict__ ctl,
+// their column partials / row sums, which the update stored contiguously
+// (parts[ptr[k] .. ptr[k+1])).  The chi block that finishes last
+// (ticket counter) then forms E = 1/4 sum_i chi_i = 1/4 sum V o T (every element has
+// one first hole index) in a fixed order and runs the loop bookkeeping of
+// src/main.cpp:511-519, 667-670: dE = E - E_prev, E_prev = E, iterations++,
+// and the loop goes on while |dE| > tol unless more than max_iters iterations
+// have run.  The last block resets the ticket counter for the next launch.
+__global__ __launch_bounds__(kThreads) void k_reduce(const int32_t *__restrict__ ptr, int nh,
+                                                     const double *__restrict__ parts,
+                                                     double *__restrict__ out, int *__restrict__ ticket,
+                                                     double *__restrict__ energy, LoopCtl *__restrict__ ctl,
                                                      double *__restrict__ log) {
     if (ctl->done) return;
     __shared__ double red[kThreads / 64];
+    __shared__ int last;
+    const int k = blockIdx.x;
     double s = 0.0;
-    for (int k = threadIdx.x; k < nh; k += kThreads) s += chi[k];
+    for (int q = ptr[k] + threadIdx.x; q < ptr[k + 1]; q += kThreads) s += parts[q];
     const double b = block_sum(s, red);
     if (threadIdx.x == 0) {
-        const double e = 0.25 * b;
-        out[0] = e;
+        out[k] = b;
+        if (k < nh) {           // only the chi blocks take a ticket
+            __threadfence();
+            last = atomicAdd(ticket, 1) == nh - 1;
+        } else {
+            last = 0;
+        }
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double c = 0.0;
+    for (int i = threadIdx.x; i < nh; i += kThreads)
+        c += __hip_atomic_load(out + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const double tot = block_sum(c, red);
+    if (threadIdx.x == 0) {
+        *ticket = 0;
+        const double e = 0.25 * tot;
+        energy[0] = e;
         const double diff = e - ctl->prev;
         ctl->prev = e;
         const int it = ctl->count;
