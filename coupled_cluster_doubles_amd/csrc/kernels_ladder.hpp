@@ -153,7 +153,9 @@ __device__ __forceinline__ int kslot(int q, int s) {
 template <int NT>
 struct LadderShape {
     static constexpr int KCH = 32;                  // k per pipeline stage
-    static constexpr int KSEG = 256;                // packed pairs per k segment
+    // packed pairs per k segment: whole channels for the narrow classes (many
+    // tiles, one prologue each), 256 for the few wide channels
+    static constexpr int KSEG = NT <= 2 ? 1024 : 256;
 };
 
 template <typename CT, int NT, int SRC>
@@ -334,6 +336,10 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
 
 // Number of k segments of a channel's pppp ladder.
 inline int ladder_segments(int npk, int nt) {
-    const int kseg = nt <= 2 ? LadderShape<1>::KSEG : LadderShape<3>::KSEG;
+    const int kseg = nt == 1   ? LadderShape<1>::KSEG
+                     : nt == 2 ? LadderShape<2>::KSEG
+                     : nt == 3 ? LadderShape<3>::KSEG
+                     : nt == 4 ? LadderShape<4>::KSEG
+                               : LadderShape<5>::KSEG;
     return npk > 0 ? (npk + kseg - 1) / kseg : 1;
 }
