@@ -84,7 +84,7 @@ struct ccd_ctx {
     int64_t bytes = 0;
     std::vector<DChan> chans;
     std::vector<DMod> mods;
-    std::vector<DTile> rtiles, ltiles, utiles, xtiles;  // packed-row, ladder, update, X tiles
+    std::vector<DTile> rtiles, ltiles, utiles, xtiles, ttiles;  // packed-row, ladder, update, X, T X tiles
     int lt_off[NTMAX + 1] = {0};  // ladder tiles grouped by 16-column tile count: [lt_off[n-1], lt_off[n])
     int64_t n_cp = 0;            // column-partial slots
     std::vector<RingItem> ritems;  // ring work items, largest channels first
@@ -93,7 +93,7 @@ struct ccd_ctx {
     // device tables
     DChan *d_chans = nullptr;
     DMod *d_mods = nullptr;
-    DTile *d_rtiles = nullptr, *d_ltiles = nullptr, *d_utiles = nullptr, *d_xtiles = nullptr;
+    DTile *d_rtiles = nullptr, *d_ltiles = nullptr, *d_utiles = nullptr, *d_xtiles = nullptr, *d_ttiles = nullptr;
     int2 *d_ppl = nullptr, *d_hhl = nullptr, *d_phl = nullptr, *d_hpl = nullptr;
     int32_t *d_hh_swap = nullptr, *d_prow = nullptr, *d_prow_sw = nullptr;
     int32_t *d_red_ptr = nullptr, *d_cpos = nullptr, *d_rpos = nullptr;
@@ -251,6 +251,20 @@ void launch_ladder_v_ct(ccd_ctx *c, const CT *vp, int lo, int hi, hipStream_t st
     else launch_ladder_v_src<CT, LV_STORED>(c, vp, lo, hi, st);
 }
 
+// 1/2 T X into the last D_p slab of every channel
+int launch_ladder_tx(ccd_ctx *c, hipStream_t st) {
+    if (c->ttiles.empty()) return CCD_OK;
+    const int n = (int)c->ttiles.size();
+    const dim3 grid((unsigned)((n + kThreads / 64 - 1) / (kThreads / 64)));
+    if (c->iter_prec == CCD_PREC_FP32)
+        hipLaunchKernelGGL(k_ladder_tx<float>, grid, dim3(kThreads), 0, st, c->d_ttiles, n, c->d_chans, c->d_tas,
+                           c->d_x, c->d_zero, c->d_dp, c->d_ctl);
+    else
+        hipLaunchKernelGGL(k_ladder_tx<double>, grid, dim3(kThreads), 0, st, c->d_ttiles, n, c->d_chans, c->d_tas,
+                           c->d_x, c->d_zero, c->d_dp, c->d_ctl);
+    return check_launch("k_ladder_tx");
+}
+
 int launch_ladder_v(ccd_ctx *c, int lo, int hi, hipStream_t st) {
     if (c->ltiles.empty()) return CCD_OK;
     if (c->iter_prec == CCD_PREC_FP32) launch_ladder_v_ct<float>(c, c->d_vp32, lo, hi, st);
@@ -293,12 +307,15 @@ int join_aux(ccd_ctx *c) {
 // stream, the events bracketing ring + R remap | X | pppp ladder | update |
 // T remap + reductions.  Untimed: the independent phases run concurrently
 // on four streams, ordered by events --
-//   aux1: ring, R remap                 | after the update: T remap (which the
-//                                       |   next iteration's ring follows)
-//   main: X, pppp ladder (1 tile/side)  | update | reductions + energy
-//   aux2:    pppp ladder (2 tiles/side) |
-//   aux3:    pppp ladder (3+)           |
-// Untimed iterations must run between fork_aux and join_aux.
+//   main: pppp ladder (1 tile/side)          | update
+//   aux1: ring, R remap                      | after the update: T remap (which
+//                                            |   the next iteration's ring follows)
+//   aux2: X, 1/2 T X, pppp ladder (2 tiles)  |
+//   aux3: pppp ladder (3+ tiles)             | after the update: reductions +
+//                                            |   energy (which the next update waits for)
+// (the pppp ladder does not need X; aux2 starts after the previous update).
+// Untimed iterations must run between fork_aux and join_aux; the loop state
+// is final only after a join.
 int run_iteration(ccd_ctx *c, double mixing, hipEvent_t *ev) {
     const double *t_old = c->d_t[c->cur];
     double *t_new = c->d_t[1 - c->cur];
@@ -310,7 +327,7 @@ int run_iteration(ccd_ctx *c, double mixing, hipEvent_t *ev) {
         HIP_TRY(hipEventRecord(ev[1], st));
         if ((rc = launch_ladder_x(c, st))) return rc;
         HIP_TRY(hipEventRecord(ev[2], st));
-        if ((rc = launch_ladder_v(c, 1, NTMAX, st))) return rc;
+        if ((rc = launch_ladder_tx(c, st)) || (rc = launch_ladder_v(c, 1, NTMAX, st))) return rc;
         HIP_TRY(hipEventRecord(ev[3], st));
         if ((rc = launch_update<true>(c, t_old, t_new, mixing, st))) return rc;
         HIP_TRY(hipEventRecord(ev[4], st));
@@ -319,17 +336,17 @@ int run_iteration(ccd_ctx *c, double mixing, hipEvent_t *ev) {
     } else {
         const hipStream_t s0 = c->stream, s1 = c->aux[0], s2 = c->aux[1], s3 = c->aux[2];
         hipEvent_t *e = c->sync_ev;
+        // e[4] still holds the previous iteration's update (or nothing in the
+        // first iteration of a batch, which fork_aux orders instead)
+        HIP_TRY(hipStreamWaitEvent(s2, e[4], 0));
+        if ((rc = launch_ladder_v(c, 1, 1, s0))) return rc;
         if ((rc = launch_ring(c, s1)) || (rc = launch_transpose(c, false, c->d_r, c->d_g, s1))) return rc;
         HIP_TRY(hipEventRecord(e[0], s1));
-        if ((rc = launch_ladder_x(c, s0))) return rc;
-        HIP_TRY(hipEventRecord(e[1], s0));
-        HIP_TRY(hipStreamWaitEvent(s2, e[1], 0));
-        HIP_TRY(hipStreamWaitEvent(s3, e[1], 0));
-        if ((rc = launch_ladder_v(c, 2, 2, s2))) return rc;
+        if ((rc = launch_ladder_x(c, s2)) || (rc = launch_ladder_tx(c, s2)) || (rc = launch_ladder_v(c, 2, 2, s2)))
+            return rc;
         HIP_TRY(hipEventRecord(e[2], s2));
         if ((rc = launch_ladder_v(c, 3, NTMAX, s3))) return rc;
         HIP_TRY(hipEventRecord(e[3], s3));
-        if ((rc = launch_ladder_v(c, 1, 1, s0))) return rc;
         HIP_TRY(hipStreamWaitEvent(s0, e[0], 0));
         HIP_TRY(hipStreamWaitEvent(s0, e[2], 0));
         HIP_TRY(hipStreamWaitEvent(s0, e[3], 0));
@@ -337,7 +354,8 @@ int run_iteration(ccd_ctx *c, double mixing, hipEvent_t *ev) {
         HIP_TRY(hipEventRecord(e[4], s0));
         HIP_TRY(hipStreamWaitEvent(s1, e[4], 0));
         if ((rc = launch_transpose(c, true, t_new, c->d_tph, s1))) return rc;
-        if ((rc = launch_reduce(c, s0))) return rc;
+        HIP_TRY(hipStreamWaitEvent(s3, e[4], 0));
+        if ((rc = launch_reduce(c, s3))) return rc;
     }
     c->cur = 1 - c->cur;
     return CCD_OK;
@@ -540,7 +558,7 @@ int build(ccd_ctx *c, const ccd_problem *p) {
             c->chans.push_back(ch);
             off_vp += (int64_t)npk * ldv;
             off_tp += (int64_t)npk * H;
-            off_dp += (int64_t)nseg * npk * H;
+            off_dp += (int64_t)(nseg + 1) * npk * H;
             off_hhhh += (int64_t)H * H;
             c->flops += 2.0 * H * H * P + 2.0 * P * H * H + 2.0 * P * P * H;
             c->flops_pppp += 2.0 * P * P * H;
@@ -676,6 +694,9 @@ int build(ccd_ctx *c, const ccd_problem *p) {
         }
         c->lt_off[nt] = (int)c->ltiles.size();
     }
+    for (int k = 0; k < (int)c->chans.size(); k++)
+        for (int r = 0; r < c->chans[k].npk; r += WROWS)
+            c->ttiles.push_back(DTile{k, r, std::min(WROWS, c->chans[k].npk - r), 0});
     for (int k = 0; k < (int)c->chans.size(); k++) {
         const DChan &ch = c->chans[k];
         for (int r = 0; r < ch.hh; r += 16)
@@ -715,7 +736,7 @@ int build(ccd_ctx *c, const ccd_problem *p) {
     if ((rc = upload(c, &c->d_chans, c->chans)) || (rc = upload(c, &c->d_mods, c->mods)) ||
         (rc = upload(c, &c->d_rtiles, c->rtiles)) || (rc = upload(c, &c->d_ltiles, c->ltiles)) ||
         (rc = upload(c, &c->d_utiles, c->utiles)) ||
-        (rc = upload(c, &c->d_xtiles, c->xtiles)) || (rc = upload(c, &c->d_ritems, c->ritems)) ||
+        (rc = upload(c, &c->d_xtiles, c->xtiles)) || (rc = upload(c, &c->d_ttiles, c->ttiles)) || (rc = upload(c, &c->d_ritems, c->ritems)) ||
         (rc = upload(c, &c->d_ppl, ppl)) ||
         (rc = upload(c, &c->d_hhl, hhl)) || (rc = upload(c, &c->d_phl, phl)) || (rc = upload(c, &c->d_hpl, hpl)) ||
         (rc = upload(c, &c->d_hh_swap, hh_swap)) || (rc = upload(c, &c->d_prow, prow)) ||
@@ -872,6 +893,7 @@ int run_loop(ccd_ctx *c, double mixing, int batch) {
     for (;;) {
         for (int k = 0; k < batch; k++)
             if ((rc = run_iteration(c, mixing, nullptr))) return rc;
+        if ((rc = join_aux(c))) return rc;
         HIP_TRY(hipMemcpyAsync(c->h_ctl, c->d_ctl, sizeof(LoopCtl), hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
         if (c->h_ctl->done) break;
@@ -1134,7 +1156,7 @@ int ccd_ladder(ccd_ctx *c, const double *t, double *d) {
     HIP_TRY(hipMemcpyAsync(dt, t, sizeof(double) * c->nt2, hipMemcpyHostToDevice, c->stream));
     int rc;
     if ((rc = launch_antisym(c, dt)) || (rc = launch_ladder_x(c, c->stream)) ||
-        (rc = launch_ladder_v(c, 1, NTMAX, c->stream)))
+        (rc = launch_ladder_tx(c, c->stream)) || (rc = launch_ladder_v(c, 1, NTMAX, c->stream)))
         return rc;
     if (!c->utiles.empty()) {
         UpdArgs a = upd_args(c, nullptr, nullptr, 0.0);

@@ -21,8 +21,8 @@ struct DChan {          // active direct channel (pp > 0 and hh > 0)
     int32_t pp_off, hh_off;  // first pair in the pp / hh pair lists
     int32_t npk, ldv;        // number of pairs a<b; leading dimension (multiple of 8)
     int32_t pr_off, ndiag;   // first entry in the row-pair lists; number of pairs (a,a)
-    int64_t off_dp;     // ladder output: nseg slabs of npk x hh partial sums
-    int32_t nseg, pad;  // number of k segments of the pppp ladder
+    int64_t off_dp;     // ladder output: nseg + 1 slabs of npk x hh partial sums
+    int32_t nseg, pad;  // number of k segments of the pppp ladder (slab nseg: 1/2 T X)
 };
 
 struct DMod {           // active mod channel (ph > 0 and hp > 0)

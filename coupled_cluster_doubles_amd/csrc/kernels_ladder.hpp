@@ -15,6 +15,7 @@
 constexpr int KC = 32;                 // k per chunk of k_ladder_x (k-slot q of step s: k = 8q + s)
 constexpr int NTMAX = 5;               // 16-column tiles: hh <= 80
 constexpr int LROWS = 64;              // packed rows per ladder block (16 per wave)
+constexpr int WROWS = 16;              // packed rows per wave tile (T X term)
 constexpr int kGasTableMax = 4096;     // entries of the generated-V gas table (LDS, 32 KB)
 
 // X = V^hhhh + sum_{c<d} V^hhpp[kl][cd] tas[cd][ij], one 16x16 tile of X per
@@ -119,13 +120,12 @@ __device__ __forceinline__ void static_for(F &&f) {
     }
 }
 
-// D_p[u][ij] = 1/2 sum_kl tas[u][kl] X[kl][ij] + sum_w Vp[u][w] tas[w][ij]
-// for 64 packed rows u of one channel per tile (16 per wave) and NT*16 >= hh
-// columns, split over k segments of KSEG packed pairs (tile.aux = segment):
-// segment s writes its partial sum to slab s of the channel's D_p, and the
-// update adds the slabs in segment order.  The k range of a tile is one
-// sequence of KCH-wide chunks: for segment 0 first the hh columns of T X, then
-// its packed pairs of V t.  In a chunk, k-slot q of MFMA step s is
+// D_p[u][ij] = sum_w Vp[u][w] tas[w][ij] for 64 packed rows u of one channel
+// per tile (16 per wave) and NT*16 >= hh columns, split over k segments of
+// KSEG packed pairs (tile.aux = segment): segment s writes its partial sum to
+// slab s of the channel's D_p; k_ladder_tx writes 1/2 tas X to slab nseg, and
+// the update adds the slabs in order.  The k range of a tile is one
+// sequence of KCH-wide chunks.  In a chunk, k-slot q of MFMA step s is
 // k = 8 (s/2) + 2q + s%2: the four lanes of a row read 64 contiguous bytes per
 // 16-byte load, so a load instruction touches half cache lines whose other
 // halves the next instruction reads (rows are 128-byte aligned).  Software
@@ -190,11 +190,10 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
         const DChan ch = chans[tl.chan];
         const int H = ch.hh, npk = ch.npk, ldv = ch.ldv, seg = tl.aux;
         const int kbeg = seg * KSEG, kend = min(npk, kbeg + KSEG);
-        const int nx = seg == 0 ? (H + KCH - 1) / KCH : 0, nv = (kend - kbeg + KCH - 1) / KCH;
+        const int nv = (kend - kbeg + KCH - 1) / KCH;
         const int lrow = wave * 16 + m;
         const bool rok = lrow < tl.nrows;
         const int64_t u = tl.row0 + lrow;
-        const double *at = tas + ch.off_tp + u * H;
         const CT *av = GEN ? nullptr : vp + ch.off_vp + u * ldv;
         int2 ra = {0, 0}, rb = {0, 0};
         if (GEN) {
@@ -206,13 +205,6 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
         // Every load is unconditional: an element outside the block reads the
         // zero buffer instead, so no load sits behind a branch and the
         // prefetches stay in flight across the MFMAs.
-        auto load_t = [&](int c, CT (&a)[G]) {   // T X chunk c
-#pragma unroll
-            for (int s = 0; s < G; s++) {
-                const int k = c * KCH + kslot<CT>(kq, s);
-                a[s] = (CT)(0.5 * *((rok && k < H) ? at + k : zero));
-            }
-        };
         auto load_a = [&](int v, CT (&a)[G]) {   // stored V: V chunk v
             const int kc0 = kbeg + v * KCH;
 #pragma unroll
@@ -271,18 +263,6 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
         };
         CT b[BPT];
 
-        // T X chunks (segment 0 only), unpipelined: at most a few per tile
-        for (int c = 0; c < nx; c++) {
-            CT a[G];
-            load_t(c, a);
-            load_b(x + ch.off_hhhh, H, c * KCH, b);
-            __syncthreads();            // earlier readers of buffer 0 are done
-            store_b(0, b);
-            __syncthreads();
-            mfma_chunk(0, a);
-        }
-        __syncthreads();
-
         // V chunks, software-pipelined over two register sets and two LDS
         // buffers, unrolled by two so that no copy joins the sets: while the
         // MFMAs of chunk v run, chunk v+1 is in flight, and the wait for it
@@ -331,6 +311,71 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
                     dp[ch.off_dp + (int64_t)seg * npk * H + (int64_t)(tl.row0 + row) * H + col] = (double)acc[n][q];
             }
         }
+    }
+}
+
+// 1/2 tas X, the hhhh quadratic term of D (calc.cpp:171-187 with X from
+// k_ladder_x), for 16 packed rows of one channel per wave (tile.row0/nrows,
+// all hh columns), written to slab nseg of the channel's D_p.  Waves are
+// independent: operands come straight from global memory (tas rows and the
+// small X block stay in L1/L2).
+template <typename CT, int NT>
+__device__ __forceinline__ void ladder_tx_wave(const DTile &tl, const DChan &ch, const double *__restrict__ tas,
+                                               const double *__restrict__ x, const double *__restrict__ zero,
+                                               double *__restrict__ dp) {
+    typedef typename Mma<CT>::acc Acc;
+    const int lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
+    const int H = ch.hh;
+    const bool rok = m < tl.nrows;
+    const double *at = tas + ch.off_tp + (int64_t)(tl.row0 + m) * H;
+    const double *X = x + ch.off_hhhh;
+    Acc acc[NT];
+#pragma unroll
+    for (int n = 0; n < NT; n++) acc[n] = Acc{};
+#pragma unroll
+    for (int kt = 0; kt < NT; kt++) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int k = kt * 16 + 4 * kq + j;
+            const bool kok = k < H;
+            const CT a = (CT)(0.5 * *((rok && kok) ? at + k : zero));
+#pragma unroll
+            for (int n = 0; n < NT; n++) {
+                const int col = n * 16 + m;
+                const CT b = (CT) * ((kok && col < H) ? X + k * H + col : zero);
+                acc[n] = mfma16(a, b, acc[n]);
+            }
+        }
+    }
+    double *out = dp + ch.off_dp + (int64_t)ch.nseg * ch.npk * H;
+#pragma unroll
+    for (int n = 0; n < NT; n++) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int row = Mma<CT>::row(q, lane), col = n * 16 + m;
+            if (row < tl.nrows && col < H) out[(int64_t)(tl.row0 + row) * H + col] = (double)acc[n][q];
+        }
+    }
+}
+
+template <typename CT>
+__global__ __launch_bounds__(kThreads) void k_ladder_tx(const DTile *__restrict__ tiles, int ntiles,
+                                                        const DChan *__restrict__ chans,
+                                                        const double *__restrict__ tas,
+                                                        const double *__restrict__ x,
+                                                        const double *__restrict__ zero, double *__restrict__ dp,
+                                                        const LoopCtl *__restrict__ ctl) {
+    if (ctl->done) return;
+    const int it = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+    if (it >= ntiles) return;
+    const DTile tl = tiles[it];
+    const DChan ch = chans[tl.chan];
+    switch ((ch.hh + 15) / 16) {
+        case 1: ladder_tx_wave<CT, 1>(tl, ch, tas, x, zero, dp); break;
+        case 2: ladder_tx_wave<CT, 2>(tl, ch, tas, x, zero, dp); break;
+        case 3: ladder_tx_wave<CT, 3>(tl, ch, tas, x, zero, dp); break;
+        case 4: ladder_tx_wave<CT, 4>(tl, ch, tas, x, zero, dp); break;
+        default: ladder_tx_wave<CT, 5>(tl, ch, tas, x, zero, dp); break;
     }
 }
 

@@ -22,7 +22,7 @@ struct UpdArgs {
     const int32_t *hh_swap;          // index of (j,i) within the channel
     const double *eps, *chi, *psi;   // chi [nh], psi [np]
     const double *vt;                // <ij||ab> in the pphh layout
-    const double *dp;                // ladder D on packed rows, one slab per k segment
+    const double *dp;                // ladder D on packed rows: one slab per k segment, then 1/2 T X
     const double *g;                 // ring R gathered to the pphh layout: g[(a,b),(i,j)] = R[(a,i),(j,b)]
     const double *t_old;
     double *t_new, *tas;             // new amplitudes; their (a,b)-antisymmetric packed rows
@@ -62,7 +62,7 @@ __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
             if (!diag) {
                 const int64_t slab = (int64_t)ch.npk * H;
                 const double *dq = a.dp + ch.off_dp + (int64_t)pr * H + IJ;
-                for (int q = 0; q < ch.nseg; q++) d += dq[q * slab];
+                for (int q = 0; q <= ch.nseg; q++) d += dq[q * slab];
             }
             const double ci = 0.5 * a.chi[ii], cj = 0.5 * a.chi[jj];
             const double pa = -0.5 * a.psi[ia - a.nh], pb = -0.5 * a.psi[ib - a.nh];
@@ -270,7 +270,7 @@ __global__ __launch_bounds__(kThreads) void k_expand_d(UpdArgs a, double *__rest
         const int64_t slab = (int64_t)ch.npk * H;
         const double *dq = a.dp + ch.off_dp + (int64_t)pr * H + IJ;
         double v = 0.0;
-        for (int q = 0; q < ch.nseg; q++) v += dq[q * slab];
+        for (int q = 0; q <= ch.nseg; q++) v += dq[q * slab];
         d[ch.off_t + (int64_t)r1 * H + IJ] = v;
         d[ch.off_t + (int64_t)r2 * H + IJ] = -v;
     }
