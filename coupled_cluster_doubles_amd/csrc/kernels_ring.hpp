@@ -19,6 +19,7 @@ struct RingItem {       // one wave's work: mod channel and 16-column tile
     int32_t mod, jt;
 };
 
+
 template <typename CT, int NT>
 __device__ __forceinline__ void ring_wave(const DMod &md, int jt, const double *__restrict__ tph,
                                           const CT *__restrict__ va, const CT *__restrict__ vb,
@@ -46,7 +47,7 @@ __device__ __forceinline__ void ring_wave(const DMod &md, int jt, const double *
         const int row = kt * 16 + m;
         const bool rok = row < HP;
         Acc acc = {};
-#pragma unroll
+#pragma unroll 1
         for (int pt = 0; pt < NT; pt++) {
 #pragma unroll
             for (int j = 0; j < 4; j++) {
@@ -66,7 +67,7 @@ __device__ __forceinline__ void ring_wave(const DMod &md, int jt, const double *
     }
     // R(:, jt) = T X(:, jt); step (kt, j) uses k = kt*16 + ks(j), whose X value
     // is x[kt][j] in this lane.
-#pragma unroll
+#pragma unroll 1
     for (int pt = 0; pt < NT; pt++) {
         const int row = pt * 16 + m;
         const bool rok = row < PH;
