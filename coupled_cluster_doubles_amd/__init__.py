@@ -52,6 +52,14 @@ class Model:
               "ccd_model_electron_gas")
         return cls(h)
 
+    @classmethod
+    def inf_matter(cls, density=0.08, n_shells=4, n_particle_shells=2, tz_max=1):
+        """Pure neutron matter with the Minnesota potential (host V chains)."""
+        h = C.c_void_p()
+        check(lib().ccd_model_inf_matter(density, tz_max, n_shells, n_particle_shells, C.byref(h)),
+              "ccd_model_inf_matter")
+        return cls(h)
+
     def eref(self) -> float:
         e = C.c_double()
         check(lib().ccd_model_eref(self.h, C.byref(e)), "ccd_model_eref")

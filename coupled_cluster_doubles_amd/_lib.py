@@ -59,6 +59,7 @@ SIGNATURES = [
     ("ccd_work", C.c_int, [vp, dp, dp, dp]),
     ("ccd_model_pairing", C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(vp)]),
     ("ccd_model_electron_gas", C.c_int, [C.c_double, C.c_int, C.c_int, C.POINTER(vp)]),
+    ("ccd_model_inf_matter", C.c_int, [C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]),
     ("ccd_model_destroy", C.c_int, [vp]),
     ("ccd_model_eref", C.c_int, [vp, dp]),
     ("ccd_model_sizes", C.c_int, [vp, i32p, i32p, i32p, i64p]),

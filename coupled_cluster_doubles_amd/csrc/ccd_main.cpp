@@ -129,9 +129,7 @@ int main(int argc, char **argv) {
     switch (o.basis) {
         case 0: rc = ccd_model_pairing(o.nPairStates, o.nParticles, o.g, o.xi, &model); break;
         case 2: rc = ccd_model_electron_gas(o.r_s, o.nShells, o.nParticleShells, &model); break;
-        case 1:
-            fprintf(stderr, "the infinite-matter basis is not part of this build\n");
-            return 1;
+        case 1: rc = ccd_model_inf_matter(o.density, o.tzMax, o.nShells, o.nParticleShells, &model); break;
         default:
             printf("not ready\n");
             return 1;
@@ -145,6 +143,10 @@ int main(int argc, char **argv) {
             printf("Basis 0: Pairing\n");
             printf("xi = %f, g = %f, nPairStates = %d, nParticles = %d, nSpstates = %d\n", o.xi, o.g,
                    o.nPairStates, o.nParticles, nsp);
+        } else if (o.basis == 1) {
+            printf("Basis 1: infMatter\n");
+            printf("density = %f, tzMax = %d, nShells = %d, nParticleShells = %d, nSpStates = %d, nParticles = %d\n",
+                   o.density, o.tzMax, o.nShells, o.nParticleShells, nsp, nh);
         } else {
             printf("Basis 2: electronGas\n");
             printf("r_s = %f, nShells = %d, nParticleShells = %d, nSpStates = %d, nParticles = %d\n", o.r_s,
@@ -157,7 +159,9 @@ int main(int argc, char **argv) {
     const ccd_problem *prob = nullptr;
     ccd_model_problem(model, &prob);
     ccd_ctx *ctx = nullptr;
-    const int mode = o.precision | (o.saveMemory ? CCD_MATRIX_FREE : 0);
+    // -m 1 regenerates V^pppp inside the ladder; the infinite-matter model has
+    // no on-device generator, so it always runs from stored V
+    const int mode = o.precision | (o.saveMemory && o.basis != 1 ? CCD_MATRIX_FREE : 0);
     if ((rc = ccd_ctx_create(prob, mode, &ctx))) return die("ccd_ctx_create", rc);
     int64_t dbytes = 0;
     ccd_ctx_info(ctx, nullptr, &dbytes);

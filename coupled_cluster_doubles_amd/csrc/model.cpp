@@ -5,6 +5,7 @@
 // Behaviour follows (file:line in justinlietz/coupled_cluster_doubles):
 //   src/PairingSPBasis.cpp:14-58, 127-181        pairing basis, <pq||rs>, channels
 //   src/ElectronGasSPBasis.cpp:14-210, 424-624   electron gas basis, Coulomb, channels
+//   src/InfMatterSPBasis.cpp:14-184, 355-466     pure neutron matter, Minnesota potential
 //   src/SPBasis.cpp:7-40                         E_ref and normal ordering
 //   src/SPBasis.cpp:80-819                       channel dims and sorted pair maps
 //   src/main.cpp:94-115 + src/balance.cpp:91-144 channel order at one rank
@@ -55,13 +56,16 @@ unsigned shell_energy(int k) {
 
 }  // namespace
 
+constexpr int kInfMatter = 100;      // model kind without an on-device generator
+
 struct ccd_model {
-    int kind = 0;                    // CCD_GEN_PAIRING or CCD_GEN_ELECTRON_GAS
+    int kind = 0;                    // CCD_GEN_PAIRING, CCD_GEN_ELECTRON_GAS or kInfMatter
     int32_t nsp = 0, nh = 0;
     std::vector<int32_t> qn;         // 5 per state
     std::vector<double> eps;         // normal ordered
     double eref = 0.0;
     double g = 0.0, xi = 0.0, r_s = 0.0, L = 0.0, e_sq = 0.0;
+    double density = 0.0, inf_L = 0.0;
     int32_t nmax = 0;
     std::vector<double> table;       // electron gas direct term per momentum transfer
     int32_t nch = 0;
@@ -87,8 +91,40 @@ struct ccd_model {
         return table[idx];
     }
 
-    // Antisymmetrised <pq||rs> (calcTBME of either basis).
+    // Minnesota potential <pq|v|rs>, not antisymmetrised
+    // (src/InfMatterSPBasis.cpp:361-449), in its operation order.
+    double minnesota(int p, int qq, int r, int s) const {
+        const int32_t *qi = q(p), *qj = q(qq), *qk = q(r), *ql = q(s);
+        if (qi[0] + qj[0] != qk[0] + ql[0] || qi[1] + qj[1] != qk[1] + ql[1] || qi[2] + qj[2] != qk[2] + ql[2])
+            return 0.0;
+        const double Lb = inf_L;
+        const double V_0R = 200, V_0T = 178, V_0S = 91.85;
+        const double kappa_R = 1.487, kappa_T = 0.639, kappa_S = 0.465;
+        double qSquared = 0.;
+        for (int i = 0; i < 3; i++) {
+            const double ki = 2 * M_PI * qi[i] / Lb, kj = 2 * M_PI * qj[i] / Lb;
+            const double kk = 2 * M_PI * qk[i] / Lb, kl = 2 * M_PI * ql[i] / Lb;
+            const double bra = 0.5 * (ki - kj), ket = 0.5 * (kk - kl), tr = bra - ket;
+            qSquared += tr * tr;
+        }
+        const double V_R = V_0R / (Lb * Lb * Lb) * std::pow(M_PI / kappa_R, 1.5) * std::exp(-qSquared / (4 * kappa_R));
+        const double V_T = -V_0T / (Lb * Lb * Lb) * std::pow(M_PI / kappa_T, 1.5) * std::exp(-qSquared / (4 * kappa_T));
+        const double V_S = -V_0S / (Lb * Lb * Lb) * std::pow(M_PI / kappa_S, 1.5) * std::exp(-qSquared / (4 * kappa_S));
+        auto kron = [](int a, int b) { return a != b ? 0 : 1; };
+        auto exch = [](int i, int j, int k, int l) { return (i == l && j == k) ? 1 : 0; };
+        const double spinEx = exch(qi[3], qj[3], qk[3], ql[3]);
+        const double isoSpinEx = exch(qi[4], qj[4], qk[4], ql[4]);
+        const double IsIt = kron(qi[3], qk[3]) * kron(qj[3], ql[3]) * kron(qi[4], qk[4]) * kron(qj[4], ql[4]);
+        const double PsIt = spinEx * kron(qi[4], qk[4]) * kron(qj[4], ql[4]);
+        const double PsPt = spinEx * isoSpinEx;
+        const double IsPt = kron(qi[3], qk[3]) * kron(qj[3], ql[3]) * isoSpinEx;
+        return 0.5 * (V_R + 0.5 * V_T + 0.5 * V_S) * IsIt + 0.25 * (V_T - V_S) * PsIt -
+               0.5 * (V_R + 0.5 * V_T + 0.5 * V_S) * PsPt - 0.25 * (V_T - V_S) * IsPt;
+    }
+
+    // Antisymmetrised <pq||rs> (calcTBME of each basis).
     double tbme(int p, int qq, int r, int s) const {
+        if (kind == kInfMatter) return minnesota(p, qq, r, s) - minnesota(p, qq, s, r);  // :355-359
         if (kind == CCD_GEN_PAIRING) {  // src/PairingSPBasis.cpp:127-141
             const int32_t *a = q(p), *b = q(qq), *c = q(r), *d = q(s);
             double v = 0.0;
@@ -115,6 +151,37 @@ struct ccd_model {
         const size_t w = 4 * (size_t)nmax + 1, h = w / 2;
         return ((size_t)(a[0] - b[0] + (long)h) * w * w * 3) + ((size_t)(a[1] - b[1] + (long)h) * w * 3) +
                ((size_t)(a[2] - b[2] + (long)h) * 3) + (size_t)((a[3] - b[3] + 2) / 2);
+    }
+
+    // Plane-wave momenta with n^2 up to the n_shells-th shell, two spin states,
+    // one isospin state (tz = -1), ordered by n^2, then nx, ny, nz ascending,
+    // then sz = -1, +1 (src/ElectronGasSPBasis.cpp:38-170; the same loops in
+    // src/InfMatterSPBasis.cpp:14-163 for pure neutron matter).  Holes: the
+    // states of the first n_particle_shells shells.
+    void plane_wave_states(int n_shells, int n_particle_shells) {
+        const unsigned emax = shell_energy(n_shells), ef = shell_energy(n_particle_shells);
+        const int range = (int)sqrt_at_least(emax);
+        std::vector<std::vector<int32_t>> by_shell(emax + 1);
+        int nm = 0;
+        for (int x = -range; x <= range; x++)
+            for (int y = -range; y <= range; y++)
+                for (int z = -range; z <= range; z++) {
+                    unsigned e = (unsigned)(x * x + y * y + z * z);
+                    if (e > emax) continue;
+                    nm = std::max(nm, x);
+                    for (int s = -1; s <= 1; s += 2) {
+                        auto &v = by_shell[e];
+                        v.insert(v.end(), {x, y, z, s, -1});
+                    }
+                }
+        for (auto &v : by_shell) qn.insert(qn.end(), v.begin(), v.end());
+        nsp = (int32_t)(qn.size() / 5);
+        nmax = nm;
+        nh = 0;
+        for (int p = 0; p < nsp; p++) {
+            const int32_t *a = q(p);
+            if ((unsigned)(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]) <= ef) nh++;
+        }
     }
 
     void reference_energy_and_normal_order() {  // src/SPBasis.cpp:7-40
@@ -192,7 +259,7 @@ struct ccd_model {
         prob.pp_pairs = pp_pairs.data(); prob.hh_pairs = hh_pairs.data();
         prob.n_mod = nch; prob.ph_dim = ph_dim.data(); prob.hp_dim = hp_dim.data();
         prob.ph_pairs = ph_pairs.data(); prob.hp_pairs = hp_pairs.data();
-        prob.gen_kind = kind; prob.qn = qn.data(); prob.gen_g = g;
+        prob.gen_kind = kind == kInfMatter ? CCD_GEN_NONE : kind; prob.qn = qn.data(); prob.gen_g = g;
         prob.gen_nmax = nmax; prob.gen_table = table.empty() ? nullptr : table.data();
     }
 
@@ -279,32 +346,8 @@ int ccd_model_electron_gas(double r_s, int n_shells, int n_particle_shells, ccd_
     if (!m) return ccd::fail(CCD_E_NOMEM, "ccd_model_electron_gas: out of memory");
     m->kind = CCD_GEN_ELECTRON_GAS;
     m->r_s = r_s;
-    const unsigned emax = shell_energy(n_shells), ef = shell_energy(n_particle_shells);
-    int range = (int)sqrt_at_least(emax);
-    // States ordered by n^2, then nx, ny, nz ascending, then sz = -1, +1
-    // (src/ElectronGasSPBasis.cpp:109-170); one species (tz = -1).
-    std::vector<std::vector<int32_t>> by_shell(emax + 1);
-    int nmax = 0;
-    for (int x = -range; x <= range; x++)
-        for (int y = -range; y <= range; y++)
-            for (int z = -range; z <= range; z++) {
-                unsigned e = (unsigned)(x * x + y * y + z * z);
-                if (e > emax) continue;
-                nmax = std::max(nmax, x);
-                for (int s = -1; s <= 1; s += 2) {
-                    auto &v = by_shell[e];
-                    v.insert(v.end(), {x, y, z, s, -1});
-                }
-            }
-    for (auto &v : by_shell) m->qn.insert(m->qn.end(), v.begin(), v.end());
-    m->nsp = (int32_t)(m->qn.size() / 5);
-    m->nmax = nmax;
-    int32_t nh = 0;
-    for (int p = 0; p < m->nsp; p++) {
-        const int32_t *a = m->q(p);
-        if ((unsigned)(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]) <= ef) nh++;
-    }
-    m->nh = nh;
+    m->plane_wave_states(n_shells, n_particle_shells);
+    const int nmax = m->nmax, nh = m->nh;
     // Constants and kinetic energies of src/ElectronGasSPBasis.cpp:178-207.
     double hbarc = 0.1973269788;
     double ev_per_hartree = 27.21138505;
@@ -347,6 +390,44 @@ int ccd_model_electron_gas(double r_s, int n_shells, int n_particle_shells, ccd_
     return CCD_OK;
 }
 
+int ccd_model_inf_matter(double density, int tz_max, int n_shells, int n_particle_shells, ccd_model **out) {
+    ccd::clear_error();
+    if (!out || n_shells < 1 || n_particle_shells < 1 || n_particle_shells > n_shells || !(density > 0))
+        return ccd::fail(CCD_E_INVALID, "ccd_model_inf_matter: bad arguments");
+    if (tz_max != 1)
+        return ccd::fail(CCD_E_INVALID, "ccd_model_inf_matter: only pure neutron matter (tz_max = 1) is supported");
+    ccd_model *m = new (std::nothrow) ccd_model();
+    if (!m) return ccd::fail(CCD_E_NOMEM, "ccd_model_inf_matter: out of memory");
+    m->kind = kInfMatter;
+    m->density = density;
+    m->plane_wave_states(n_shells, n_particle_shells);
+    // Kinetic energies of src/InfMatterSPBasis.cpp:166-184 (neutron mass);
+    // calc_TBME_not_antisym recomputes the same box length (:381).
+    const double hbarc = 197.3269788, massc2 = 939.565;
+    const double prefactor = hbarc * hbarc / (2. * massc2);
+    m->L = std::pow((size_t)m->nh / density, 1. / 3.);
+    m->inf_L = std::pow((size_t)m->nh / m->density, 1. / 3.);
+    m->eps.resize(m->nsp);
+    for (int p = 0; p < m->nsp; p++) {
+        const int32_t *a = m->q(p);
+        m->eps[p] = (prefactor * 4 * M_PI * M_PI / (m->L * m->L)) * (a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    }
+    // channels keyed like the electron gas: (4 nMax + 1)^3 x 3 x 1
+    // (src/InfMatterSPBasis.cpp:196-240 with chanTzMax = -2)
+    const int w = 4 * m->nmax + 1;
+    m->reference_energy_and_normal_order();
+    m->build_channels((size_t)w * w * w * 3);
+    m->fill_problem();
+    try {
+        m->build_host_v();
+    } catch (const std::bad_alloc &) {
+        delete m;
+        return ccd::fail(CCD_E_NOMEM, "ccd_model_inf_matter: out of host memory");
+    }
+    *out = m;
+    return CCD_OK;
+}
+
 int ccd_model_destroy(ccd_model *m) {
     delete m;
     return CCD_OK;
@@ -373,7 +454,7 @@ int ccd_model_sizes(const ccd_model *m, int32_t *n_sp, int32_t *n_holes, int32_t
 
 int ccd_model_problem(ccd_model *m, const ccd_problem **out) {
     if (!m || !out) return ccd::fail(CCD_E_INVALID, "ccd_model_problem: null argument");
-    *out = &m->prob;
+    *out = m->kind == kInfMatter ? &m->prob_host : &m->prob;   // no device generator: host V
     return CCD_OK;
 }
 

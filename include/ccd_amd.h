@@ -210,6 +210,13 @@ int ccd_model_pairing(int n_pair_states, int n_particles, double g, double xi,
 /* Electron gas, reference ElectronGasSPBasis(2, r_s, 1, nShells, nParticleShells). */
 int ccd_model_electron_gas(double r_s, int n_shells, int n_particle_shells,
                            ccd_model **out);
+/* Infinite nuclear matter with the Minnesota potential, reference
+ * InfMatterSPBasis(1, density, tzMax, nShells, nParticleShells); pure neutron
+ * matter (tz_max = 1) only.  There is no on-device generator: the model's
+ * descriptor carries host V chains (built here), so the matrix-free mode
+ * does not apply. */
+int ccd_model_inf_matter(double density, int tz_max, int n_shells,
+                         int n_particle_shells, ccd_model **out);
 int ccd_model_destroy(ccd_model *m);
 /* Reference energy (SPBasis::referenceEnergy, before normal ordering). */
 int ccd_model_eref(const ccd_model *m, double *eref);

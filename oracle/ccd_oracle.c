@@ -19,7 +19,7 @@ typedef struct { int32_t p, q; } opair;
 typedef struct { int32_t p, q, r; } otriple;
 
 struct orc {
-    int model;               /* 0 pairing, 2 electron gas (reference basisIndicator) */
+    int model;               /* 0 pairing, 1 infinite matter, 2 electron gas (reference basisIndicator) */
     size_t nsp, nh;
     int (*qn)[5];            /* HEG: nx ny nz sz tz; pairing: level spin */
     double *eps;             /* normal-ordered after setup */
@@ -30,6 +30,8 @@ struct orc {
     size_t emax, efermi;
     /* pairing */
     double g, xi;
+    /* infinite matter (pure neutron matter) */
+    double density, inf_L;
     /* channels, permuted order (balance.cpp ffdBalance at one rank) */
     size_t nch;
     size_t *ch_raw, *ch_inv;
@@ -123,9 +125,46 @@ static double pairing_tbme(const orc_t *o, size_t p, size_t q, size_t r, size_t 
     return v;
 }
 
-/* calcTBME: antisymmetrised matrix element (ElectronGasSPBasis.cpp:419-422). */
+static int kron_del(int i, int j) { return i != j ? 0 : 1; }          /* InfMatterSPBasis.cpp:461-466 */
+static int spin_exchange(int i, int j, int k, int l) {                  /* :452-458 */
+    return (i == l && j == k) ? 1 : 0;
+}
+
+/* Minnesota potential <pq|v|rs>, not antisymmetrised
+ * (src/InfMatterSPBasis.cpp:361-449), same operation order. */
+static double minnesota_direct(const orc_t *o, size_t p, size_t q, size_t r, size_t s) {
+    const int *qi = o->qn[p], *qj = o->qn[q], *qk = o->qn[r], *ql = o->qn[s];
+    if (qi[0] + qj[0] != qk[0] + ql[0] || qi[1] + qj[1] != qk[1] + ql[1] || qi[2] + qj[2] != qk[2] + ql[2])
+        return 0.0;
+    const double L = o->inf_L;
+    const double V_0R = 200, V_0T = 178, V_0S = 91.85;
+    const double kappa_R = 1.487, kappa_T = 0.639, kappa_S = 0.465;
+    double qSquared = 0.;
+    for (int i = 0; i < 3; i++) {
+        double ki = 2 * M_PI * qi[i] / L, kj = 2 * M_PI * qj[i] / L;
+        double kk = 2 * M_PI * qk[i] / L, kl = 2 * M_PI * ql[i] / L;
+        double bra = 0.5 * (ki - kj), ket = 0.5 * (kk - kl);
+        double tr = bra - ket;
+        qSquared += tr * tr;
+    }
+    double V_R = V_0R / (L * L * L) * pow(M_PI / kappa_R, 1.5) * exp(-qSquared / (4 * kappa_R));
+    double V_T = -V_0T / (L * L * L) * pow(M_PI / kappa_T, 1.5) * exp(-qSquared / (4 * kappa_T));
+    double V_S = -V_0S / (L * L * L) * pow(M_PI / kappa_S, 1.5) * exp(-qSquared / (4 * kappa_S));
+    double spinEx = spin_exchange(qi[3], qj[3], qk[3], ql[3]);
+    double isoSpinEx = spin_exchange(qi[4], qj[4], qk[4], ql[4]);
+    double IsIt = kron_del(qi[3], qk[3]) * kron_del(qj[3], ql[3]) * kron_del(qi[4], qk[4]) * kron_del(qj[4], ql[4]);
+    double PsIt = spinEx * kron_del(qi[4], qk[4]) * kron_del(qj[4], ql[4]);
+    double PsPt = spinEx * isoSpinEx;
+    double IsPt = kron_del(qi[3], qk[3]) * kron_del(qj[3], ql[3]) * isoSpinEx;
+    return 0.5 * (V_R + 0.5 * V_T + 0.5 * V_S) * IsIt + 0.25 * (V_T - V_S) * PsIt -
+           0.5 * (V_R + 0.5 * V_T + 0.5 * V_S) * PsPt - 0.25 * (V_T - V_S) * IsPt;
+}
+
+/* calcTBME: antisymmetrised matrix element (ElectronGasSPBasis.cpp:419-422,
+ * InfMatterSPBasis.cpp:355-359). */
 static double tbme(const orc_t *o, size_t p, size_t q, size_t r, size_t s) {
     if (o->model == 0) return pairing_tbme(o, p, q, r, s);
+    if (o->model == 1) return minnesota_direct(o, p, q, r, s) - minnesota_direct(o, p, q, s, r);
     return heg_direct(o, p, q, r, s) - heg_direct(o, p, q, s, r);
 }
 
@@ -322,10 +361,73 @@ orc_t *orc_create_pairing(int nPairStates, int nParticles, double g, double xi) 
     return o;
 }
 
+/* Plane-wave momenta n^2 <= EMax with two spin states and one isospin state
+ * (tz = -1): the index maps of ElectronGasSPBasis.cpp:38-170 and, for
+ * pure neutron matter, InfMatterSPBasis.cpp:14-163 (same loops and order). */
+static void plane_wave_basis(orc_t *o, int nShells, int nParticleShells);
+
 orc_t *orc_create_heg(double r_s, int nShells, int nParticleShells) {
     orc_t *o = calloc(1, sizeof(orc_t));
     o->model = 2;
     o->r_s = r_s;
+    plane_wave_basis(o, nShells, nParticleShells);
+    const size_t nsp = o->nsp, nh = o->nh;
+    const int nmax = o->nmax;
+    /* generateBasis (ElectronGasSPBasis.cpp:173-210): reference constants */
+    double hbarc = 0.1973269788;
+    double eVs_in_a_Hartree = 27.21138505;
+    hbarc = hbarc * 10000 / eVs_in_a_Hartree;
+    double massc2 = 0.5109989461;
+    massc2 = massc2 * 1000000 / eVs_in_a_Hartree;
+    double fine_struc = 1.0 / 137.035999139;
+    double r_bohr = hbarc / (massc2 * fine_struc);
+    double r_0 = r_s * r_bohr;
+    double prefactor = hbarc * hbarc / (2. * massc2);
+    o->L = pow((4. / 3.) * M_PI * (double)nh, 1. / 3.) * r_0;
+    o->e_sq = fine_struc * hbarc;
+    o->eps = malloc(nsp * sizeof(double));
+    for (size_t i = 0; i < nsp; i++) {
+        int *q = o->qn[i];
+        o->eps[i] = (prefactor * 4 * M_PI * M_PI / (o->L * o->L)) *
+                    (q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+    }
+    /* ElectronGasSPBasis.cpp:256-262: (4 nMax + 1)^3 x 3 channels */
+    size_t tb = 4 * (size_t)nmax + 1;
+    o->nch = tb * tb * tb * 3;
+    reference_and_normal_order(o);
+    setup_channels(o);
+    return o;
+}
+
+/* Pure neutron matter with the Minnesota potential
+ * (InfMatterSPBasis(1, density, tzMax = 1, nShells, nParticleShells)). */
+orc_t *orc_create_infmatter(double density, int nShells, int nParticleShells) {
+    orc_t *o = calloc(1, sizeof(orc_t));
+    o->model = 1;
+    o->density = density;
+    plane_wave_basis(o, nShells, nParticleShells);
+    const size_t nsp = o->nsp;
+    /* generateBasis (InfMatterSPBasis.cpp:166-184): neutron mass, L from the density */
+    const double hbarc = 197.3269788, massc2 = 939.565;
+    double prefactor = hbarc * hbarc / (2. * massc2);
+    o->L = pow(o->nh / density, 1. / 3.);
+    o->inf_L = pow(o->nh / o->density, 1. / 3.);   /* as recomputed in calc_TBME_not_antisym */
+    o->eps = malloc(nsp * sizeof(double));
+    for (size_t i = 0; i < nsp; i++) {
+        int *q = o->qn[i];
+        o->eps[i] = (prefactor * 4 * M_PI * M_PI / (o->L * o->L)) *
+                    (q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+    }
+    /* setUpChannelValues (InfMatterSPBasis.cpp:196-240) with chanTzMax = -2:
+     * (4 nMax + 1)^3 x 3 x 1 channels, keyed like the electron gas */
+    size_t tb = 4 * (size_t)o->nmax + 1;
+    o->nch = tb * tb * tb * 3;
+    reference_and_normal_order(o);
+    setup_channels(o);
+    return o;
+}
+
+static void plane_wave_basis(orc_t *o, int nShells, int nParticleShells) {
     /* ElectronGasSPBasis.cpp:38-96 */
     size_t emax = 0;
     for (size_t i = 1; i < (size_t)nShells; emax++)
@@ -374,30 +476,6 @@ orc_t *orc_create_heg(double r_s, int nShells, int nParticleShells) {
                 }
             }
     free(shells);
-    /* generateBasis (ElectronGasSPBasis.cpp:173-210): reference constants */
-    double hbarc = 0.1973269788;
-    double eVs_in_a_Hartree = 27.21138505;
-    hbarc = hbarc * 10000 / eVs_in_a_Hartree;
-    double massc2 = 0.5109989461;
-    massc2 = massc2 * 1000000 / eVs_in_a_Hartree;
-    double fine_struc = 1.0 / 137.035999139;
-    double r_bohr = hbarc / (massc2 * fine_struc);
-    double r_0 = r_s * r_bohr;
-    double prefactor = hbarc * hbarc / (2. * massc2);
-    o->L = pow((4. / 3.) * M_PI * (double)nh, 1. / 3.) * r_0;
-    o->e_sq = fine_struc * hbarc;
-    o->eps = malloc(nsp * sizeof(double));
-    for (size_t i = 0; i < nsp; i++) {
-        int *q = o->qn[i];
-        o->eps[i] = (prefactor * 4 * M_PI * M_PI / (o->L * o->L)) *
-                    (q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
-    }
-    /* ElectronGasSPBasis.cpp:256-262: (4 nMax + 1)^3 x 3 channels */
-    size_t tb = 4 * (size_t)nmax + 1;
-    o->nch = tb * tb * tb * 3;
-    reference_and_normal_order(o);
-    setup_channels(o);
-    return o;
 }
 
 /* ------------------------------------------------------------- load */

@@ -43,6 +43,9 @@ orc_t *orc_create_pairing(int nPairStates, int nParticles, double g, double xi);
 /* Electron gas: reference ElectronGasSPBasis(2, r_s, 1, nShells,
  * nParticleShells) (src/parse.cpp:239). */
 orc_t *orc_create_heg(double r_s, int nShells, int nParticleShells);
+/* InfMatterSPBasis(1, density, tzMax = 1, nShells, nParticleShells): pure
+ * neutron matter, Minnesota potential. */
+orc_t *orc_create_infmatter(double density, int nShells, int nParticleShells);
 void orc_destroy(orc_t *o);
 
 /* Build every V chain (reference loadVnn_*, src/main.cpp:293-366) and the

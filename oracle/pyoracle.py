@@ -34,6 +34,8 @@ def lib():
         L.orc_create_pairing.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double]
         L.orc_create_heg.restype = vp
         L.orc_create_heg.argtypes = [C.c_double, C.c_int, C.c_int]
+        L.orc_create_infmatter.restype = vp
+        L.orc_create_infmatter.argtypes = [C.c_double, C.c_int, C.c_int]
         L.orc_destroy.argtypes = [vp]
         L.orc_load.argtypes = [vp]
         L.orc_set_threads.argtypes = [C.c_int]
@@ -86,6 +88,11 @@ class Oracle:
     @classmethod
     def electron_gas(cls, r_s=1.0, n_shells=4, n_particle_shells=2, load=True):
         return cls(lib().orc_create_heg(r_s, n_shells, n_particle_shells), load)
+
+    @classmethod
+    def inf_matter(cls, density=0.08, n_shells=4, n_particle_shells=2, load=True):
+        """Pure neutron matter, Minnesota potential (reference --infMatter, tzMax 1)."""
+        return cls(lib().orc_create_infmatter(density, n_shells, n_particle_shells), load)
 
     def close(self):
         if self.h:

@@ -24,6 +24,8 @@ def _rel(a, b):
 def _pair(kind, args):
     if kind == "pairing":
         return Model.pairing(*args), Oracle.pairing(*args)
+    if kind == "infmatter":
+        return Model.inf_matter(*args), Oracle.inf_matter(*args)
     return Model.electron_gas(*args), Oracle.electron_gas(*args)
 
 
@@ -54,10 +56,26 @@ def test_electron_gas_n14_solve(require_gpu):
     assert np.max(np.abs(t_g - t_o)) <= 1e-9 * np.max(np.abs(t_o))
 
 
+def test_inf_matter_solve_and_published_energy(require_gpu):
+    """Pure neutron matter, Minnesota potential, density 0.08 fm^-3, 14
+    neutrons in 54 states, tolerance 1e-8: E_CCD/A = 10.155078 MeV
+    (src/README.md:8-10)."""
+    m, o = _pair("infmatter", (0.08, 4, 2))
+    ctx = Context(m, FP64)
+    assert _rel(ctx.mbpt2(), o.mbpt2()) < E_REL
+    e_g, stop_g, n_g = ctx.solve(1e-8)
+    e_o, stop_o, _ = o.solve(1e-8)
+    assert stop_g == stop_o == "converged"
+    assert n_g == len(e_o)
+    assert np.max(np.abs(e_g - e_o) / np.abs(e_o)) < E_REL
+    assert abs((m.eref() + e_g[-1]) / 14 - 10.155078) < 5e-7
+
+
 @pytest.mark.parametrize("kind,args", [
     ("pairing", (8, 8, 0.5, 1.0)),
     ("heg", (1.0, 4, 2)),
     ("heg", (1.0, 19, 2)),
+    ("infmatter", (0.08, 4, 2)),
 ])
 def test_diagrams_on_random_amplitudes(require_gpu, kind, args):
     m, o = _pair(kind, args)

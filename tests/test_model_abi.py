@@ -36,12 +36,16 @@ CONFIGS = [
     ("heg", (1.0, 3, 2)),
     ("heg", (2.0, 6, 3)),
     ("heg", (1.0, 19, 2)),
+    ("infmatter", (0.08, 4, 2)),
+    ("infmatter", (0.16, 3, 2)),
 ]
 
 
 def _both(kind, args, load=False):
     if kind == "pairing":
         return Model.pairing(*args), Oracle.pairing(*args, load=load)
+    if kind == "infmatter":
+        return Model.inf_matter(*args), Oracle.inf_matter(*args, load=load)
     return Model.electron_gas(*args), Oracle.electron_gas(*args, load=load)
 
 
@@ -57,7 +61,7 @@ def test_model_matches_oracle_bitwise(kind, args):
     assert np.array_equal(m.t2_labels(), o.t2_labels())
 
 
-@pytest.mark.parametrize("kind,args", CONFIGS[:4])
+@pytest.mark.parametrize("kind,args", CONFIGS[:4] + CONFIGS[5:6])
 def test_model_host_v_matches_oracle(kind, args):
     m, o = _both(kind, args, load=True)
     p = m.problem(host_v=True).contents
@@ -90,3 +94,9 @@ def test_context_rejects_bad_modes_before_touching_a_device():
         Context(m, 7)
     h = _lib.C.c_void_p()
     assert _lib.lib().ccd_ctx_create(m.problem(True), FP64 | MATRIX_FREE, _lib.C.byref(h)) == -1
+
+
+def test_inf_matter_model_rejects_symmetric_matter():
+    with pytest.raises(CCDError, match="pure neutron matter"):
+        Model.inf_matter(0.16, 4, 2, tz_max=2)
+

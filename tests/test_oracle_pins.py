@@ -204,3 +204,18 @@ def test_amplitudes_stay_antisymmetric():
     key = {tuple(r): k for k, r in enumerate(lab.tolist())}
     sw = np.array([key[(b, a, i, j)] for a, b, i, j in lab.tolist()])
     assert np.max(np.abs(t + t[sw])) <= 1e-14 * np.max(np.abs(t))
+
+
+def test_inf_matter_published_energy():
+    """The reference's only published result (src/README.md:8-10): pure neutron
+    matter, Minnesota potential, density 0.08 fm^-3, 14 neutrons, tolerance
+    1e-8, E_CCD/A = 10.155078 MeV.  The README's positional "3 shells" is the
+    modern -s 4 (momenta with n^2 <= 3, 54 states; -s 3 gives 10.166175)."""
+    o = Oracle.inf_matter(0.08, 4, 2)
+    assert o.nsp == 54 and o.nh == 14
+    e, stop, _ = o.solve(1e-8)
+    assert stop == "converged"
+    e_per_a = (o.eref() + e[-1]) / 14
+    assert abs(e_per_a - 10.155078) < 5e-7
+    assert e[0] == pytest.approx(0.5 * o.mbpt2(), rel=1e-13)
+
