@@ -56,6 +56,7 @@ struct orc {
     double *gat[8];
     double *chan_e;
     int loaded;
+    int free_pppp;           /* V^pppp rebuilt per channel inside the ladder (-m 1) */
 };
 
 /* ------------------------------------------------------------------ utils */
@@ -480,6 +481,17 @@ static void plane_wave_basis(orc_t *o, int nShells, int nParticleShells) {
 
 /* ------------------------------------------------------------- load */
 
+/* loadFreeVnn_pppp (load.cpp:112-318 conventions): one direct channel's
+ * V^pppp block, [(a,b),(c,d)] = <ab|v|cd>, row-major. */
+static void channel_vpppp(const orc_t *o, size_t c, double *vp) {
+    size_t P = o->pp[c];
+    for (size_t k = 0; k < P; k++)
+        for (size_t l = 0; l < P; l++)
+            vp[k * P + l] = tbme(o, o->ppl[c][k].p, o->ppl[c][k].q, o->ppl[c][l].p, o->ppl[c][l].q);
+}
+
+void orc_set_matrix_free(orc_t *o, int on) { if (!o->loaded) o->free_pppp = on != 0; }
+
 static size_t find_triple(const otriple *l, size_t n, otriple key) {
     size_t lo = 0, hi = n;
     while (lo < hi) {
@@ -497,7 +509,7 @@ int orc_load(orc_t *o) {
     size_t n = o->nch, nsp = o->nsp, nh = o->nh;
     o->v_hhhh = malloc((o->nhhhh + 1) * sizeof(double));
     o->v_hhpp = malloc((o->nt2 + 1) * sizeof(double));
-    o->v_pppp = malloc((o->npppp + 1) * sizeof(double));
+    o->v_pppp = malloc(((o->free_pppp ? 0 : o->npppp) + 1) * sizeof(double));
     o->v_phh_p = malloc((o->nt2 + 1) * sizeof(double));
     o->v_h_hpp = malloc((o->nt2 + 1) * sizeof(double));
     o->v_modA = malloc((o->off_ph[n] + 1) * sizeof(double));
@@ -515,10 +527,7 @@ int orc_load(orc_t *o) {
         for (size_t k = 0; k < H; k++)
             for (size_t l = 0; l < P; l++)
                 vhp[k * P + l] = tbme(o, o->hhl[c][k].p, o->hhl[c][k].q, o->ppl[c][l].p, o->ppl[c][l].q);
-        double *vp = o->v_pppp + o->off_pppp[c];
-        for (size_t k = 0; k < P; k++)
-            for (size_t l = 0; l < P; l++)
-                vp[k * P + l] = tbme(o, o->ppl[c][k].p, o->ppl[c][k].q, o->ppl[c][l].p, o->ppl[c][l].q);
+        if (!o->free_pppp) channel_vpppp(o, c, o->v_pppp + o->off_pppp[c]);
         /* mod channel c: V^A hp x ph = <ij|v|ab> for (i,a),(b,j) (load.cpp:275-296),
          * V^B hp x hp = <ia|v|bj> for (i,b),(j,a) (load.cpp:298-318) */
         size_t MP = o->mph[c], MH = o->mhp[c];
@@ -668,12 +677,17 @@ double orc_mbpt2(const orc_t *o) {
 }
 
 void orc_ladder(const orc_t *o, const double *t, double *d) {
-    /* calcT2Diff_pphh (calc.cpp:132-231) */
+    /* calcT2Diff_pphh (calc.cpp:132-231); with free_pppp the per-thread V^pppp
+     * buffer of calcFreeT2Diff_pphh (calc.cpp:233-262) */
     #pragma omp parallel
     {
-        size_t mx = 1;
-        for (size_t c = 0; c < o->nch; c++) if (o->hh[c] * o->hh[c] > mx) mx = o->hh[c] * o->hh[c];
+        size_t mx = 1, mp = 1;
+        for (size_t c = 0; c < o->nch; c++) {
+            if (o->hh[c] * o->hh[c] > mx) mx = o->hh[c] * o->hh[c];
+            if (o->free_pppp && o->pp[c] * o->pp[c] > mp) mp = o->pp[c] * o->pp[c];
+        }
         double *x = malloc(mx * sizeof(double));
+        double *vbuf = o->free_pppp ? malloc(mp * sizeof(double)) : NULL;
         #pragma omp for schedule(dynamic)
         for (size_t c = 0; c < o->nch; c++) {
             size_t P = o->pp[c], H = o->hh[c];
@@ -683,9 +697,12 @@ void orc_ladder(const orc_t *o, const double *t, double *d) {
             double *dc = d + o->off_t[c];
             dgemm_rm(H, H, P, 0.5, o->v_hhpp + o->off_t[c], tc, 1.0, x);
             dgemm_rm(P, H, H, 0.5, tc, x, 0.0, dc);
-            dgemm_rm(P, H, P, 0.5, o->v_pppp + o->off_pppp[c], tc, 1.0, dc);
+            const double *vp = o->v_pppp + o->off_pppp[c];
+            if (vbuf) { channel_vpppp(o, c, vbuf); vp = vbuf; }
+            dgemm_rm(P, H, P, 0.5, vp, tc, 1.0, dc);
         }
         free(x);
+        free(vbuf);
     }
 }
 

@@ -51,6 +51,9 @@ void orc_destroy(orc_t *o);
 /* Build every V chain (reference loadVnn_*, src/main.cpp:293-366) and the
  * eight T2 maps (loadMaps, src/load.cpp:389-532). */
 int orc_load(orc_t *o);
+/* Before orc_load: keep V^pppp out of memory and rebuild each channel's block
+ * inside orc_ladder (the reference's -m 1 ladder). */
+void orc_set_matrix_free(orc_t *o, int on);
 
 void orc_set_threads(int n);
 

@@ -38,6 +38,7 @@ def lib():
         L.orc_create_infmatter.argtypes = [C.c_double, C.c_int, C.c_int]
         L.orc_destroy.argtypes = [vp]
         L.orc_load.argtypes = [vp]
+        L.orc_set_matrix_free.argtypes = [vp, C.c_int]
         L.orc_set_threads.argtypes = [C.c_int]
         for f in ("orc_nsp", "orc_nholes", "orc_nchannels", "orc_nt2"):
             getattr(L, f).restype = sz
@@ -75,9 +76,11 @@ STOP = {0: "converged", 1: "max_iterations", 2: "nan"}
 class Oracle:
     """One basis + its channel-blocked chains, reference layout."""
 
-    def __init__(self, handle, load=True):
+    def __init__(self, handle, load=True, matrix_free=False):
         self.L = lib()
         self.h = handle
+        if matrix_free:
+            self.L.orc_set_matrix_free(self.h, 1)
         if load and self.L.orc_load(self.h) != 0:
             raise MemoryError("oracle: allocation failed")
 
@@ -86,8 +89,8 @@ class Oracle:
         return cls(lib().orc_create_pairing(n_levels, n_particles, g, xi), load)
 
     @classmethod
-    def electron_gas(cls, r_s=1.0, n_shells=4, n_particle_shells=2, load=True):
-        return cls(lib().orc_create_heg(r_s, n_shells, n_particle_shells), load)
+    def electron_gas(cls, r_s=1.0, n_shells=4, n_particle_shells=2, load=True, matrix_free=False):
+        return cls(lib().orc_create_heg(r_s, n_shells, n_particle_shells), load, matrix_free)
 
     @classmethod
     def inf_matter(cls, density=0.08, n_shells=4, n_particle_shells=2, load=True):

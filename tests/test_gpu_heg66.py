@@ -1,0 +1,84 @@
+"""The electron gas with 66 electrons, r_s = 0.5, n^2 <= 52 (3150 s.p. states,
+73,167 channels, 7.77e6 amplitudes; reference CLI
+--electronGas -r 0.5 -s 46 -p 5).
+
+The CPU oracle needs minutes at this size, so its solve is committed as
+tests/golden/heg66_oracle.json (written by tests/golden/make_heg66.py); the
+GPU path is held to it at the FP64 bar (energies within 1e-10 relative),
+and to size-independent properties: the stored and matrix-free modes agree
+bit for bit, the amplitudes stay antisymmetric, and the mixed-precision solve
+lands within 1e-7 Ha per particle of the FP64 one.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from coupled_cluster_doubles_amd import FP32, FP64, Context, Model
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+E_REL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def golden():
+    with open(os.path.join(HERE, "golden", "heg66_oracle.json")) as f:
+        return json.load(f)
+
+
+@pytest.fixture(scope="module")
+def model(golden):
+    m = Model.electron_gas(*golden["args"])
+    assert m.n_t2 == golden["nt2"]
+    return m
+
+
+def test_heg66_solve_matches_oracle(require_gpu, golden, model):
+    assert abs(model.eref() - golden["eref"]) <= 1e-14 * abs(golden["eref"])
+    ctx = Context(model, FP64, matrix_free=True)
+    assert abs(ctx.mbpt2() - golden["mbpt2"]) <= E_REL * abs(golden["mbpt2"])
+    e_g, stop_g, n_g = ctx.solve(golden["tol"])
+    e_o = np.array(golden["energies"])
+    assert stop_g == golden["stop"] == "converged"
+    assert abs(n_g - len(e_o)) <= 1
+    k = min(n_g, len(e_o))
+    assert np.max(np.abs(e_g[:k] - e_o[:k]) / np.abs(e_o[:k])) < E_REL
+    assert e_g[0] == pytest.approx(0.5 * golden["mbpt2"], rel=1e-12)
+    t = ctx.get_t2()
+    idx = np.array(golden["t2_sample_index"])
+    ref = np.array(golden["t2_sample_value"])
+    assert np.max(np.abs(t[idx] - ref)) <= 1e-9 * np.max(np.abs(ref))
+    assert abs(np.dot(t, t) - golden["t2_sum_sq"]) <= 1e-9 * golden["t2_sum_sq"]
+
+
+def test_heg66_stored_and_matrix_free_agree_bitwise(require_gpu, model):
+    a = Context(model, FP64)
+    b = Context(model, FP64, matrix_free=True)
+    for _ in range(3):
+        assert a.iterate() == b.iterate()
+    ta = a.get_t2()
+    assert np.array_equal(ta, b.get_t2())
+    # T2 antisymmetry after every iteration: t_ij^ab = -t_ij^ba = -t_ji^ab
+    lab = model.t2_labels().astype(np.int64)
+    n = int(lab.max()) + 1
+
+    def code(a_, b_, i, j):
+        return ((a_ * n + b_) * n + i) * n + j
+
+    keys = code(*lab.T)
+    order = np.argsort(keys)
+    for swapped in (code(lab[:, 1], lab[:, 0], lab[:, 2], lab[:, 3]),
+                    code(lab[:, 0], lab[:, 1], lab[:, 3], lab[:, 2])):
+        pos = order[np.searchsorted(keys, swapped, sorter=order)]
+        assert np.array_equal(keys[pos], swapped)
+        assert np.max(np.abs(ta + ta[pos])) <= 1e-13 * np.max(np.abs(ta))
+
+
+def test_heg66_mixed_solve_within_1e7_per_particle(require_gpu, golden, model):
+    e, stop, n, n32 = Context(model, FP32, matrix_free=True).solve_mixed(golden["tol"], 1e-6)
+    assert stop == "converged" and n32 > 0 and n > n32
+    e_o = golden["energies"][-1]
+    assert abs(e[-1] - e_o) / golden["nh"] <= 1e-7

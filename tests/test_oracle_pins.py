@@ -219,3 +219,31 @@ def test_inf_matter_published_energy():
     assert abs(e_per_a - 10.155078) < 5e-7
     assert e[0] == pytest.approx(0.5 * o.mbpt2(), rel=1e-13)
 
+
+
+def test_oracle_matrix_free_ladder_is_identical():
+    """The oracle's -m 1 ladder (V^pppp rebuilt per channel, calc.cpp:233-262)
+    gives the stored-V results bit for bit; the N=66 golden file relies on it."""
+    a = Oracle.electron_gas(1.0, 19, 2)
+    b = Oracle.electron_gas(1.0, 19, 2, matrix_free=True)
+    t = np.random.default_rng(5).uniform(-0.05, 0.05, a.nt2)
+    assert np.array_equal(a.ladder(t), b.ladder(t))
+    e1, s1, _ = a.solve(1e-10)
+    e2, s2, _ = b.solve(1e-10)
+    assert s1 == s2 == "converged" and np.array_equal(e1, e2)
+
+
+def test_heg66_golden_file_is_consistent():
+    """The committed N=66 oracle solve: protocol invariants that hold for any
+    correct run (first iterate = MBPT2/2, the stop rule at the last step)."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "heg66_oracle.json")) as f:
+        g = json.load(f)
+    e = np.array(g["energies"])
+    assert g["nsp"] == 3150 and g["nh"] == 66 and g["nt2"] == 7765428
+    assert e[0] == pytest.approx(0.5 * g["mbpt2"], rel=1e-13)
+    assert abs(e[-1] - e[-2]) <= g["tol"] < np.min(np.abs(np.diff(e[:-1])))
+    o = Oracle.electron_gas(*g["args"], load=False)
+    assert o.nsp == g["nsp"] and o.nh == g["nh"] and o.nt2 == g["nt2"]
+    assert o.eref() == g["eref"]
