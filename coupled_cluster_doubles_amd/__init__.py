@@ -20,7 +20,7 @@ FP32 = 1
 MATRIX_FREE = 0x100   # OR into the precision: V^pppp rebuilt inside the ladder (reference -m 1)
 STOP_REASONS = {0: "converged", 1: "max_iterations", 2: "nan"}
 
-__all__ = ["Model", "Context", "CCDError", "FP64", "FP32", "MATRIX_FREE", "STOP_REASONS"]
+__all__ = ["Model", "Context", "Batch", "CCDError", "FP64", "FP32", "MATRIX_FREE", "STOP_REASONS"]
 
 
 def _ptr(a: np.ndarray):
@@ -237,3 +237,67 @@ class Context:
         f, fp, b = C.c_double(), C.c_double(), C.c_double()
         check(lib().ccd_work(self.h, C.byref(f), C.byref(fp), C.byref(b)), "ccd_work")
         return {"flops": f.value, "flops_pppp": fp.value, "ladder_bytes": b.value}
+
+
+class Batch(Context):
+    """Many independent solves in one device context: every kernel of an
+    iteration covers all of them in one launch, each solve keeps its own
+    stop rule and energy log, and each one's results are bit-identical to a
+    Context of its own.  No counterpart in the reference (one solve per
+    ccd.exe process)."""
+
+    def __init__(self, models, precision: int = FP64, host_v: bool = False,
+                 matrix_free: bool = False):
+        self.models = list(models)
+        self.model = None
+        self.h = C.c_void_p()
+        n = len(self.models)
+        probs = (C.c_void_p * max(n, 1))(*[C.cast(m.problem(host_v), C.c_void_p) for m in self.models])
+        mode = precision | (MATRIX_FREE if matrix_free else 0)
+        check(lib().ccd_batch_create(probs, n, mode, C.byref(self.h)), "ccd_batch_create")
+        nt, b = C.c_int64(), C.c_int64()
+        check(lib().ccd_ctx_info(self.h, C.byref(nt), C.byref(b)), "ccd_ctx_info")
+        self.n_t2, self.device_bytes = nt.value, b.value
+        k = C.c_int()
+        self.t2_offset = np.zeros(n + 1, dtype=np.int64)
+        check(lib().ccd_batch_info(self.h, C.byref(k), _ptr(self.t2_offset)), "ccd_batch_info")
+        self.n_problems = k.value
+        self.n_holes = sum(m.n_holes for m in self.models)
+        self.n_sp = sum(m.n_sp for m in self.models)
+
+    def mbpt2(self) -> np.ndarray:
+        e = np.zeros(self.n_problems)
+        check(lib().ccd_batch_mbpt2(self.h, _ptr(e)), "ccd_batch_mbpt2")
+        return e
+
+    def iterate_n(self, n: int, mixing: float = 0.5) -> np.ndarray:
+        """n iterations of every solve; energies [n_problems, n]."""
+        en = np.zeros(max(n, 1) * self.n_problems)
+        check(lib().ccd_batch_iterate_n(self.h, n, mixing, _ptr(en)), "ccd_batch_iterate_n")
+        return en[:n * self.n_problems].reshape(self.n_problems, n).copy()
+
+    def solve(self, tol: float, max_iters: int = 100, mixing: float = 0.5, cap: int = 256):
+        """Every solve from T2 = 0: a list of (energies, stop, n_iter)."""
+        P = self.n_problems
+        en = np.zeros(cap * P)
+        n, stop = np.zeros(P, dtype=np.int32), np.zeros(P, dtype=np.int32)
+        check(lib().ccd_batch_solve(self.h, tol, max_iters, mixing, _ptr(en), cap, _ptr(n), _ptr(stop)),
+              "ccd_batch_solve")
+        return [(en[q * cap:q * cap + min(int(n[q]), cap)].copy(), STOP_REASONS[int(stop[q])], int(n[q]))
+                for q in range(P)]
+
+    def solve_mixed(self, tol: float, switch_tol: float, max_iters: int = 100, mixing: float = 0.5,
+                    cap: int = 256):
+        """Every solve in mixed precision: a list of (energies, stop, n_iter, n_fp32)."""
+        P = self.n_problems
+        en = np.zeros(cap * P)
+        n, n32, stop = (np.zeros(P, dtype=np.int32) for _ in range(3))
+        check(lib().ccd_batch_solve_mixed(self.h, tol, switch_tol, max_iters, mixing, _ptr(en), cap,
+                                          _ptr(n), _ptr(n32), _ptr(stop)), "ccd_batch_solve_mixed")
+        return [(en[q * cap:q * cap + min(int(n[q]), cap)].copy(), STOP_REASONS[int(stop[q])], int(n[q]),
+                 int(n32[q])) for q in range(P)]
+
+    def t2(self, q: int) -> np.ndarray:
+        """Current amplitudes of solve q (its own pphh layout)."""
+        return self.get_t2()[self.t2_offset[q]:self.t2_offset[q + 1]].copy()
+

@@ -57,6 +57,13 @@ SIGNATURES = [
     ("ccd_step", C.c_int, [vp, ip, ip, C.c_double, dp]),
     ("ccd_time_phases", C.c_int, [vp, C.c_int, C.c_double, ip]),
     ("ccd_work", C.c_int, [vp, dp, dp, dp]),
+    ("ccd_batch_create", C.c_int, [C.POINTER(vp), C.c_int, C.c_int, C.POINTER(vp)]),
+    ("ccd_batch_info", C.c_int, [vp, C.POINTER(C.c_int), ip]),
+    ("ccd_batch_mbpt2", C.c_int, [vp, ip]),
+    ("ccd_batch_solve", C.c_int, [vp, C.c_double, C.c_int, C.c_double, ip, C.c_int, ip, ip]),
+    ("ccd_batch_solve_mixed", C.c_int, [vp, C.c_double, C.c_double, C.c_int, C.c_double, ip, C.c_int,
+                                        ip, ip, ip]),
+    ("ccd_batch_iterate_n", C.c_int, [vp, C.c_int, C.c_double, ip]),
     ("ccd_model_pairing", C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(vp)]),
     ("ccd_model_electron_gas", C.c_int, [C.c_double, C.c_int, C.c_int, C.POINTER(vp)]),
     ("ccd_model_inf_matter", C.c_int, [C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]),

@@ -88,6 +88,8 @@ struct ccd_ctx {
     int lt_off[NTMAX + 1] = {0};  // ladder tiles grouped by 16-column tile count: [lt_off[n-1], lt_off[n])
     int64_t n_cp = 0;            // column-partial slots
     std::vector<RingItem> ritems;  // ring work items, largest channels first
+    int n_ring_wide = 0;           // leading items of channels with more than 5 tiles
+    int n_tt_narrow = 0;           // leading T X tiles of channels with hh <= 80
     size_t tr_lds = 0;
     int iter_prec = CCD_PREC_FP64;  // arithmetic of the contractions of the next iterations
     // device tables
@@ -123,7 +125,25 @@ struct ccd_ctx {
     double *d_gtab = nullptr;                   // generated-V electron gas table
     int64_t n_vp = 0, n_tp = 0, n_dp = 0, n_hhhh = 0, n_b = 0;
     double flops = 0, flops_pppp = 0;
+    // a batch of independent solves merged into this context (nprob = 1 otherwise)
+    int nprob = 1;
+    std::vector<int32_t> hlo;    // holes of solve p: [hlo[p], hlo[p+1])
+    std::vector<int64_t> t2_lo;  // its amplitudes in the pphh layout: [t2_lo[p], t2_lo[p+1])
+    std::vector<int32_t> ulo;    // its update tiles: [ulo[p], ulo[p+1])
+    int32_t *d_hlo = nullptr;
+    LoopCtl *d_pctl = nullptr;   // loop state per solve (== d_ctl for a single solve)
+    LoopCtl *h_pctl = nullptr;   // pinned staging for it (batches only)
+    GenPart *d_gpart = nullptr;
     std::vector<void *> allocs;
+};
+
+// How a merged problem splits into solves (one solve: everything).
+struct BatchInfo {
+    int nprob = 1;
+    std::vector<int32_t> chan_lo;   // direct (and mod) channels of solve p: [chan_lo[p], chan_lo[p+1])
+    std::vector<int32_t> hlo, plo;  // its holes / particles (merged state numbers)
+    std::vector<GenPart> parts;     // generator parameters per solve (gen_kind kGenBatch)
+    int64_t table_len = 0;          // entries of the merged direct-term table
 };
 
 namespace {
@@ -159,6 +179,7 @@ void free_ctx(ccd_ctx *c) {
     for (void *p : c->allocs) (void)hipFree(p);
     if (c->h_scalar) (void)hipHostFree(c->h_scalar);
     if (c->h_ctl) (void)hipHostFree(c->h_ctl);
+    if (c->h_pctl) (void)hipHostFree(c->h_pctl);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -176,22 +197,31 @@ UpdArgs upd_args(ccd_ctx *c, const double *t_old, double *t_new, double mixing) 
     a.eps = c->d_eps; a.chi = c->d_onebody; a.psi = c->d_onebody + c->nh;
     a.vt = c->d_vt; a.dp = c->d_dp; a.g = c->d_g; a.t_old = t_old; a.t_new = t_new;
     a.tas = c->d_tas; a.red = c->d_red; a.cpos = c->d_cpos; a.rpos = c->d_rpos; a.mixing = mixing; a.nh = c->nh; a.ctl = c->d_ctl;
+    a.pctl = c->d_pctl;
     return a;
 }
 
 // --- launch helpers, one per phase of the iteration; each takes the stream
 // it launches on
 
-int launch_ring(ccd_ctx *c, hipStream_t st) {
-    if (c->ritems.empty()) return CCD_OK;
-    const int n = (int)c->ritems.size();
+template <typename CT, bool WIDE>
+void launch_ring_ct(ccd_ctx *c, const CT *va, const CT *vb, int lo, int hi, hipStream_t st) {
+    const int n = hi - lo;
+    if (n <= 0) return;
     const dim3 grid((unsigned)((n + kThreads / 64 - 1) / (kThreads / 64)));
-    if (c->iter_prec == CCD_PREC_FP32)
-        hipLaunchKernelGGL(k_ring<float>, grid, dim3(kThreads), 0, st, c->d_ritems, n, c->d_mods, c->d_tph,
-                           c->d_va32, c->d_vb32, c->d_zero, c->d_r, c->d_ctl);
-    else
-        hipLaunchKernelGGL(k_ring<double>, grid, dim3(kThreads), 0, st, c->d_ritems, n, c->d_mods, c->d_tph,
-                           c->d_va, c->d_vb, c->d_zero, c->d_r, c->d_ctl);
+    hipLaunchKernelGGL((k_ring<CT, WIDE>), grid, dim3(kThreads), 0, st, c->d_ritems + lo, n, c->d_mods, c->d_tph,
+                       va, vb, c->d_zero, c->d_r, c->d_ctl);
+}
+
+int launch_ring(ccd_ctx *c, hipStream_t st) {
+    const int n = (int)c->ritems.size(), nw = c->n_ring_wide;
+    if (c->iter_prec == CCD_PREC_FP32) {
+        launch_ring_ct<float, true>(c, c->d_va32, c->d_vb32, 0, nw, st);
+        launch_ring_ct<float, false>(c, c->d_va32, c->d_vb32, nw, n, st);
+    } else {
+        launch_ring_ct<double, true>(c, c->d_va, c->d_vb, 0, nw, st);
+        launch_ring_ct<double, false>(c, c->d_va, c->d_vb, nw, n, st);
+    }
     return check_launch("k_ring");
 }
 
@@ -242,6 +272,9 @@ void launch_ladder_v_src(ccd_ctx *c, const CT *vp, int lo, int hi, hipStream_t s
     if (lo <= 3 && 3 <= hi) launch_ladder_v_nt<CT, 3, SRC>(c, vp, st);
     if (lo <= 4 && 4 <= hi) launch_ladder_v_nt<CT, 4, SRC>(c, vp, st);
     if (lo <= 5 && 5 <= hi) launch_ladder_v_nt<CT, 5, SRC>(c, vp, st);
+    if (lo <= 6 && 6 <= hi) launch_ladder_v_nt<CT, 6, SRC>(c, vp, st);
+    if (lo <= 7 && 7 <= hi) launch_ladder_v_nt<CT, 7, SRC>(c, vp, st);
+    if (lo <= 8 && 8 <= hi) launch_ladder_v_nt<CT, 8, SRC>(c, vp, st);
 }
 
 template <typename CT>
@@ -252,16 +285,24 @@ void launch_ladder_v_ct(ccd_ctx *c, const CT *vp, int lo, int hi, hipStream_t st
 }
 
 // 1/2 T X into the last D_p slab of every channel
-int launch_ladder_tx(ccd_ctx *c, hipStream_t st) {
-    if (c->ttiles.empty()) return CCD_OK;
-    const int n = (int)c->ttiles.size();
+template <typename CT, bool WIDE>
+void launch_ladder_tx_ct(ccd_ctx *c, int lo, int hi, hipStream_t st) {
+    const int n = hi - lo;
+    if (n <= 0) return;
     const dim3 grid((unsigned)((n + kThreads / 64 - 1) / (kThreads / 64)));
-    if (c->iter_prec == CCD_PREC_FP32)
-        hipLaunchKernelGGL(k_ladder_tx<float>, grid, dim3(kThreads), 0, st, c->d_ttiles, n, c->d_chans, c->d_tas,
-                           c->d_x, c->d_zero, c->d_dp, c->d_ctl);
-    else
-        hipLaunchKernelGGL(k_ladder_tx<double>, grid, dim3(kThreads), 0, st, c->d_ttiles, n, c->d_chans, c->d_tas,
-                           c->d_x, c->d_zero, c->d_dp, c->d_ctl);
+    hipLaunchKernelGGL((k_ladder_tx<CT, WIDE>), grid, dim3(kThreads), 0, st, c->d_ttiles + lo, n, c->d_chans,
+                       c->d_tas, c->d_x, c->d_zero, c->d_dp, c->d_ctl);
+}
+
+int launch_ladder_tx(ccd_ctx *c, hipStream_t st) {
+    const int n = (int)c->ttiles.size(), nn = c->n_tt_narrow;
+    if (c->iter_prec == CCD_PREC_FP32) {
+        launch_ladder_tx_ct<float, false>(c, 0, nn, st);
+        launch_ladder_tx_ct<float, true>(c, nn, n, st);
+    } else {
+        launch_ladder_tx_ct<double, false>(c, 0, nn, st);
+        launch_ladder_tx_ct<double, true>(c, nn, n, st);
+    }
     return check_launch("k_ladder_tx");
 }
 
@@ -282,7 +323,8 @@ int launch_update(ccd_ctx *c, const double *t_old, double *t_new, double mixing,
 
 int launch_reduce(ccd_ctx *c, hipStream_t st) {
     hipLaunchKernelGGL(k_reduce, dim3((unsigned)c->nsp), dim3(kThreads), 0, st, c->d_red_ptr, c->nh, c->d_red,
-                       c->d_onebody, c->d_ticket, c->d_scalar, c->d_ctl, c->d_elog);
+                       c->d_onebody, c->d_ticket, c->d_scalar, c->d_ctl, c->d_elog, c->nprob, c->d_hlo,
+                       c->d_pctl);
     return check_launch("k_reduce");
 }
 
@@ -388,6 +430,129 @@ int validate(const ccd_problem *p) {
     return CCD_OK;
 }
 
+// One problem as a batch of one.
+BatchInfo single_batch(const ccd_problem *p) {
+    BatchInfo bi;
+    bi.chan_lo = {0, p->n_chan};
+    bi.hlo = {0, p->n_holes};
+    bi.plo = {p->n_holes, p->n_sp};
+    return bi;
+}
+
+// Several problems merged into one: states renumbered holes first (the holes
+// of solve 0, of solve 1, ..., then the particles in the same order), the
+// channel lists and the caller's V chains back to back in solve order (direct
+// and mod channels alike), and for generated V the solve's index in qn[5 s + 4]
+// with its parameters in BatchInfo::parts.
+struct Merged {
+    ccd_problem p{};
+    BatchInfo bi;
+    std::vector<double> eps, v_hhhh, v_hhpp, v_pppp, v_modA, v_modB, table;
+    std::vector<int32_t> pp_dim, hh_dim, ph_dim, hp_dim, pp_pairs, hh_pairs, ph_pairs, hp_pairs, qn;
+};
+
+int merge_problems(const ccd_problem *const *probs, int n, Merged &m) {
+    int nh = 0, nsp = 0, gen = -1;
+    for (int q = 0; q < n; q++) {
+        int rc = validate(probs[q]);
+        if (rc) return rc;
+        const bool g = probs[q]->gen_kind != CCD_GEN_NONE;
+        if (gen >= 0 && g != (gen == 1))
+            return ccd::fail(CCD_E_INVALID, "a batch needs generated V for every problem or for none");
+        gen = g ? 1 : 0;
+        nh += probs[q]->n_holes;
+        nsp += probs[q]->n_sp;
+        if ((int64_t)nsp >= INT32_MAX / 8) return ccd::fail(CCD_E_INVALID, "batch too large");
+    }
+    BatchInfo &bi = m.bi;
+    bi.nprob = n;
+    bi.chan_lo.assign(1, 0);
+    bi.hlo.assign(1, 0);
+    bi.plo.assign(1, nh);
+    m.eps.assign(nsp, 0.0);
+    if (gen) m.qn.assign(5 * (size_t)nsp, 0);
+    int h0 = 0, p0 = nh, nchan = 0, nmod = 0;
+    for (int q = 0; q < n; q++) {
+        const ccd_problem &p = *probs[q];
+        const int qh = p.n_holes;
+        auto id = [&](int s) { return s < qh ? h0 + s : p0 + (s - qh); };
+        for (int s = 0; s < p.n_sp; s++) {
+            m.eps[id(s)] = p.eps[s];
+            if (gen) {
+                for (int f = 0; f < 4; f++) m.qn[5 * (size_t)id(s) + f] = p.qn[5 * (size_t)s + f];
+                m.qn[5 * (size_t)id(s) + 4] = q;
+            }
+        }
+        int64_t npp = 0, nhh = 0, nph = 0, nhp = 0, nhhhh = 0, npppp = 0, nt = 0, nmb = 0;
+        for (int k = 0; k < p.n_chan; k++) {
+            const int64_t P = p.pp_dim[k], H = p.hh_dim[k];
+            m.pp_dim.push_back((int32_t)P);
+            m.hh_dim.push_back((int32_t)H);
+            npp += P; nhh += H; nhhhh += H * H; npppp += P * P; nt += P * H;
+        }
+        for (int k = 0; k < p.n_mod; k++) {
+            const int64_t PH = p.ph_dim[k], HP = p.hp_dim[k];
+            m.ph_dim.push_back((int32_t)PH);
+            m.hp_dim.push_back((int32_t)HP);
+            nph += PH; nhp += HP; nmb += HP * HP;
+        }
+        auto add_pairs = [&](std::vector<int32_t> &dst, const int32_t *src, int64_t cnt) {
+            for (int64_t k = 0; k < 2 * cnt; k++) {
+                const int s = src[k];
+                if (s < 0 || s >= p.n_sp) return false;
+                dst.push_back(id(s));
+            }
+            return true;
+        };
+        if (!add_pairs(m.pp_pairs, p.pp_pairs, npp) || !add_pairs(m.hh_pairs, p.hh_pairs, nhh) ||
+            !add_pairs(m.ph_pairs, p.ph_pairs, nph) || !add_pairs(m.hp_pairs, p.hp_pairs, nhp))
+            return ccd::fail(CCD_E_INVALID, "batch problem %d: pair entry out of range", q);
+        if (gen) {
+            GenPart gp{p.gen_kind, p.gen_nmax, p.gen_g, (int64_t)m.table.size()};
+            if (p.gen_kind == CCD_GEN_ELECTRON_GAS) {
+                for (int s = 0; s < p.n_sp; s++)
+                    for (int f = 0; f < 3; f++)
+                        if (std::abs(p.qn[5 * s + f]) > p.gen_nmax)
+                            return ccd::fail(CCD_E_INVALID, "batch problem %d: state %d has |n| > gen_nmax", q, s);
+                const size_t w = 4 * (size_t)p.gen_nmax + 1;
+                m.table.insert(m.table.end(), p.gen_table, p.gen_table + w * w * w);
+            }
+            bi.parts.push_back(gp);
+        } else {
+            m.v_hhhh.insert(m.v_hhhh.end(), p.v_hhhh, p.v_hhhh + nhhhh);
+            m.v_hhpp.insert(m.v_hhpp.end(), p.v_hhpp, p.v_hhpp + nt);
+            m.v_pppp.insert(m.v_pppp.end(), p.v_pppp, p.v_pppp + npppp);
+            m.v_modA.insert(m.v_modA.end(), p.v_modA, p.v_modA + nt);
+            m.v_modB.insert(m.v_modB.end(), p.v_modB, p.v_modB + nmb);
+        }
+        nchan += p.n_chan;
+        nmod += p.n_mod;
+        h0 += qh;
+        p0 += p.n_sp - qh;
+        bi.chan_lo.push_back(nchan);
+        bi.hlo.push_back(h0);
+        bi.plo.push_back(p0);
+    }
+    bi.table_len = (int64_t)m.table.size();
+    if (m.table.empty()) m.table.push_back(0.0);
+    ccd_problem &o = m.p;
+    o.n_sp = nsp; o.n_holes = nh; o.eps = m.eps.data();
+    o.n_chan = nchan; o.pp_dim = m.pp_dim.data(); o.hh_dim = m.hh_dim.data();
+    o.pp_pairs = m.pp_pairs.data(); o.hh_pairs = m.hh_pairs.data();
+    o.n_mod = nmod; o.ph_dim = m.ph_dim.data(); o.hp_dim = m.hp_dim.data();
+    o.ph_pairs = m.ph_pairs.data(); o.hp_pairs = m.hp_pairs.data();
+    if (gen) {
+        o.gen_kind = kGenBatch;
+        o.qn = m.qn.data();
+        o.gen_table = m.table.data();
+    } else {
+        o.gen_kind = CCD_GEN_NONE;
+        o.v_hhhh = m.v_hhhh.data(); o.v_hhpp = m.v_hhpp.data(); o.v_pppp = m.v_pppp.data();
+        o.v_modA = m.v_modA.data(); o.v_modB = m.v_modB.data();
+    }
+    return CCD_OK;
+}
+
 // Pair records of the generated-V ladder (LadderGen).  The electron-gas form
 // needs, and checks here: two spin values, conserved total momentum and spin
 // in every direct channel, and a direct-term table even in each component;
@@ -478,9 +643,12 @@ int build_generated_ladder(ccd_ctx *c, const ccd_problem *p, const std::vector<i
 // Host tables: channel descriptors, swaps, packed rows, tiles, transposition
 // plans and reduction lists.  Every index a kernel dereferences is produced
 // and range-checked here.
-int build(ccd_ctx *c, const ccd_problem *p) {
+int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
     const int nsp = p->n_sp, nh = p->n_holes, np = nsp - nh;
     c->nsp = nsp; c->nh = nh; c->np = np;
+    const int nprob = bi.nprob;
+    c->nprob = nprob;
+    c->hlo = bi.hlo;
     int64_t npp = 0, nhh = 0, nphp = 0, nhpp = 0;
     for (int k = 0; k < p->n_chan; k++) {
         if (p->pp_dim[k] < 0 || p->hh_dim[k] < 0) return ccd::fail(CCD_E_INVALID, "negative dim");
@@ -516,8 +684,11 @@ int build(ccd_ctx *c, const ccd_problem *p) {
     std::vector<int32_t> prow, prow_sw;
     int64_t off_t = 0, off_vp = 0, off_tp = 0, off_dp = 0, off_hhhh = 0, ipp = 0, ihh = 0;
     c->flops = 0; c->flops_pppp = 0;
+    c->t2_lo.assign(nprob + 1, 0);
     std::unordered_map<int64_t, int32_t> pos;
+    int prob = 0;
     for (int k = 0; k < p->n_chan; k++) {
+        while (prob + 1 < nprob && k >= bi.chan_lo[prob + 1]) c->t2_lo[++prob] = off_t;
         const int P = p->pp_dim[k], H = p->hh_dim[k];
         if (P > 0 && H > 0) {
             if (H > NTMAX * 16) return ccd::fail(CCD_E_INVALID, "channel %d has hh=%d > %d", k, H, NTMAX * 16);
@@ -554,7 +725,7 @@ int build(ccd_ctx *c, const ccd_problem *p) {
             const int ldv = (npk + 15) / 16 * 16;
             const int nseg = ladder_segments(npk, (H + 15) / 16);
             DChan ch{off_t, off_vp, off_tp, off_hhhh, P, H, (int32_t)ipp, (int32_t)ihh, npk, ldv, pr_off, ndiag,
-                     off_dp, nseg, 0};
+                     off_dp, nseg, prob};
             c->chans.push_back(ch);
             off_vp += (int64_t)npk * ldv;
             off_tp += (int64_t)npk * H;
@@ -567,25 +738,45 @@ int build(ccd_ctx *c, const ccd_problem *p) {
         ipp += P; ihh += H;
     }
     c->nt2 = off_t;
+    while (prob + 1 < nprob) c->t2_lo[++prob] = off_t;
+    c->t2_lo[nprob] = off_t;
     c->n_vp = off_vp; c->n_tp = off_tp; c->n_dp = off_dp; c->n_hhhh = off_hhhh;
     if (c->nt2 >= (int64_t)INT32_MAX)
         return ccd::fail(CCD_E_INVALID, "T2 has %lld elements (limit 2^31)", (long long)c->nt2);
 
+    // ---- (particle, hole) slots of the position tables: solve q owns a block
+    // of np_q x nh_q slots (a pair of two solves has none)
+    std::vector<int32_t> sprob(nsp, 0);
+    std::vector<int64_t> sbase(nprob + 1, 0);
+    for (int q = 0; q < nprob; q++) {
+        for (int x = bi.hlo[q]; x < bi.hlo[q + 1]; x++) sprob[x] = q;
+        for (int x = bi.plo[q]; x < bi.plo[q + 1]; x++) sprob[x] = q;
+        sbase[q + 1] = sbase[q] + (int64_t)(bi.hlo[q + 1] - bi.hlo[q]) * (bi.plo[q + 1] - bi.plo[q]);
+    }
+    auto slot = [&](int a, int i) -> int64_t {   // particle a, hole i
+        const int q = sprob[a];
+        if (sprob[i] != q) return -1;
+        return sbase[q] + (int64_t)(a - bi.plo[q]) * (bi.hlo[q + 1] - bi.hlo[q]) + (i - bi.hlo[q]);
+    };
+    const size_t nslot = (size_t)sbase[nprob];
+
     // ---- mod channels and position tables (host only)
-    std::vector<int32_t> colidx((size_t)nh * np, -1);
-    std::vector<int32_t> mod_of_ph((size_t)np * nh, -1), mod_of_hp((size_t)nh * np, -1);
+    std::vector<int32_t> colidx(nslot, -1);
+    std::vector<int32_t> mod_of_ph(nslot, -1), mod_of_hp(nslot, -1);
     int64_t off_ph = 0, off_b = 0, iph = 0, ihp = 0;
     for (int k = 0; k < p->n_mod; k++) {
         const int PH = p->ph_dim[k], HP = p->hp_dim[k];
         for (int q = 0; q < PH; q++) {
             int a = phl[iph + q].x, i = phl[iph + q].y;
-            size_t s = (size_t)(a - nh) * nh + i;
+            const int64_t s = slot(a, i);
+            if (s < 0) return ccd::fail(CCD_E_INVALID, "ph pair (%d,%d) spans two solves", a, i);
             if (mod_of_ph[s] >= 0) return ccd::fail(CCD_E_INVALID, "ph pair (%d,%d) listed twice", a, i);
             mod_of_ph[s] = k;
         }
         for (int q = 0; q < HP; q++) {
             int j = hpl[ihp + q].x, b = hpl[ihp + q].y;
-            size_t s = (size_t)j * np + (b - nh);
+            const int64_t s = slot(b, j);
+            if (s < 0) return ccd::fail(CCD_E_INVALID, "hp pair (%d,%d) spans two solves", j, b);
             if (mod_of_hp[s] >= 0) return ccd::fail(CCD_E_INVALID, "hp pair (%d,%d) listed twice", j, b);
             colidx[s] = q;
             mod_of_hp[s] = k;
@@ -611,13 +802,15 @@ int build(ccd_ctx *c, const ccd_problem *p) {
         };
         std::stable_sort(c->ritems.begin(), c->ritems.end(),
                          [&](const RingItem &a, const RingItem &b) { return tiles_of(a) > tiles_of(b); });
+        c->n_ring_wide = 0;
+        while (c->n_ring_wide < (int)c->ritems.size() && tiles_of(c->ritems[c->n_ring_wide]) > 5) c->n_ring_wide++;
     }
 
     // ---- per-particle transposition plans between the two layouts:
     // particle a's block holds the elements (a,b,i,j); its pphh slots run over
     // the rows (a,b) in channel order, its phhp slots over the rows (a,i) in
     // mod-channel order
-    std::vector<int32_t> blk(np, 0), blk2(np, 0), row_start_pp(npp, -1), ph_start((size_t)np * nh, -1);
+    std::vector<int32_t> blk(np, 0), blk2(np, 0), row_start_pp(npp, -1), ph_start(nslot, -1);
     for (const DChan &ch : c->chans)
         for (int AB = 0; AB < ch.pp; AB++) {
             const int a = ppl[ch.pp_off + AB].x - nh;
@@ -627,7 +820,7 @@ int build(ccd_ctx *c, const ccd_problem *p) {
     for (const DMod &m : c->mods)
         for (int q = 0; q < m.ph; q++) {
             const int a = phl[m.ph_off + q].x - nh, i = phl[m.ph_off + q].y;
-            ph_start[(size_t)a * nh + i] = blk2[a];
+            ph_start[slot(a + nh, i)] = blk2[a];
             blk2[a] += m.hp;
         }
     std::vector<int32_t> tr_base(np + 1, 0);
@@ -653,7 +846,7 @@ int build(ccd_ctx *c, const ccd_problem *p) {
         for (int q = 0; q < m.ph; q++) {
             const int a = phl[m.ph_off + q].x - nh, i = phl[m.ph_off + q].y;
             for (int JB = 0; JB < m.hp; JB++)
-                addr_ph[tr_base[a] + ph_start[(size_t)a * nh + i] + JB] = (int32_t)(m.off_ph + (int64_t)q * m.hp + JB);
+                addr_ph[tr_base[a] + ph_start[slot(a + nh, i)] + JB] = (int32_t)(m.off_ph + (int64_t)q * m.hp + JB);
         }
     std::vector<uint16_t> perm_ph(c->nt2), perm_pp(c->nt2);
     std::vector<uint8_t> seen(c->nt2, 0);
@@ -662,11 +855,12 @@ int build(ccd_ctx *c, const ccd_problem *p) {
             const int a = ppl[ch.pp_off + AB].x, b = ppl[ch.pp_off + AB].y, pa = a - nh;
             for (int IJ = 0; IJ < ch.hh; IJ++) {
                 const int i = hhl[ch.hh_off + IJ].x, j = hhl[ch.hh_off + IJ].y;
-                const int32_t m1 = mod_of_ph[(size_t)pa * nh + i], m2 = mod_of_hp[(size_t)j * np + (b - nh)];
+                const int64_t s1 = slot(a, i), s2 = slot(b, j);
+                const int32_t m1 = s1 < 0 ? -1 : mod_of_ph[s1], m2 = s2 < 0 ? -2 : mod_of_hp[s2];
                 if (m1 < 0 || m1 != m2)
                     return ccd::fail(CCD_E_INVALID, "T2 element (%d,%d,%d,%d) has no consistent mod channel", a, b,
                                      i, j);
-                const int32_t sph = ph_start[(size_t)pa * nh + i] + colidx[(size_t)j * np + (b - nh)];
+                const int32_t sph = ph_start[s1] + colidx[s2];
                 const int32_t spp = row_start_pp[ch.pp_off + AB] + IJ;
                 if (sph < 0 || sph >= blk[pa] || spp < 0 || spp >= blk[pa] || seen[tr_base[pa] + sph])
                     return ccd::fail(CCD_E_INVALID, "inconsistent pphh/phhp correspondence at (%d,%d,%d,%d)", a, b,
@@ -694,9 +888,13 @@ int build(ccd_ctx *c, const ccd_problem *p) {
         }
         c->lt_off[nt] = (int)c->ltiles.size();
     }
-    for (int k = 0; k < (int)c->chans.size(); k++)
-        for (int r = 0; r < c->chans[k].npk; r += WROWS)
-            c->ttiles.push_back(DTile{k, r, std::min(WROWS, c->chans[k].npk - r), 0});
+    for (int wide = 0; wide < 2; wide++) {   // channels with hh <= 80 first
+        for (int k = 0; k < (int)c->chans.size(); k++)
+            if ((c->chans[k].hh > 80) == (wide == 1))
+                for (int r = 0; r < c->chans[k].npk; r += WROWS)
+                    c->ttiles.push_back(DTile{k, r, std::min(WROWS, c->chans[k].npk - r), 0});
+        if (wide == 0) c->n_tt_narrow = (int)c->ttiles.size();
+    }
     for (int k = 0; k < (int)c->chans.size(); k++) {
         const DChan &ch = c->chans[k];
         for (int r = 0; r < ch.hh; r += 16)
@@ -708,6 +906,9 @@ int build(ccd_ctx *c, const ccd_problem *p) {
         }
     }
     if (c->n_cp >= (int64_t)INT32_MAX) return ccd::fail(CCD_E_INVALID, "too many column partials");
+    c->ulo.assign(nprob + 1, 0);   // update tiles follow the channels, so each solve's are contiguous
+    for (const DTile &t : c->utiles) c->ulo[c->chans[t.chan].prob + 1]++;
+    for (int q = 0; q < nprob; q++) c->ulo[q + 1] += c->ulo[q];
 
     // ---- reduction lists: chi_i over column partials, psi_a over row sums
     std::vector<std::vector<int32_t>> lists((size_t)nsp);
@@ -756,11 +957,18 @@ int build(ccd_ctx *c, const ccd_problem *p) {
         (rc = dalloc(c, &c->d_vh, (size_t)c->n_hhhh)) ||
         (rc = dalloc(c, &c->d_vp, c->matrix_free ? (size_t)1 : (size_t)c->n_vp)) ||
         (rc = dalloc(c, &c->d_vb, (size_t)c->n_b)) || (rc = dalloc(c, &c->d_red, (size_t)std::max<int64_t>(c->n_red, 1))) ||
-        (rc = dalloc(c, &c->d_partials, std::max(c->utiles.size(), (size_t)1))) || (rc = dalloc(c, &c->d_scalar, 4)) ||
+        (rc = dalloc(c, &c->d_partials, std::max(c->utiles.size(), (size_t)1))) ||
+        (rc = dalloc(c, &c->d_scalar, (size_t)std::max(4, nprob))) ||
         (rc = dalloc(c, &c->d_zero, (size_t)kZeroDoubles)) || (rc = dalloc(c, &c->d_ctl, 1)) || (rc = dalloc(c, &c->d_ticket, 1)) ||
-        (rc = dalloc(c, &c->d_elog, (size_t)kElogCap)))
+        (rc = dalloc(c, &c->d_elog, (size_t)kElogCap * nprob)) || (rc = upload(c, &c->d_hlo, bi.hlo)))
         return rc;
     HIP_TRY(hipHostMalloc((void **)&c->h_ctl, sizeof(LoopCtl), hipHostMallocDefault));
+    if (nprob > 1) {
+        if ((rc = dalloc(c, &c->d_pctl, (size_t)nprob))) return rc;
+        HIP_TRY(hipHostMalloc((void **)&c->h_pctl, sizeof(LoopCtl) * nprob, hipHostMallocDefault));
+    } else {
+        c->d_pctl = c->d_ctl;
+    }
     HIP_TRY(hipMemset(c->d_ticket, 0, sizeof(int)));
     HIP_TRY(hipMemset(c->d_zero, 0, sizeof(double) * kZeroDoubles));
     HIP_TRY(hipHostMalloc((void **)&c->h_scalar, 4 * sizeof(double), hipHostMallocDefault));
@@ -769,8 +977,13 @@ int build(ccd_ctx *c, const ccd_problem *p) {
     if (p->gen_kind != CCD_GEN_NONE) {
         std::vector<int32_t> qn(p->qn, p->qn + 5 * (size_t)nsp);
         if ((rc = upload(c, &c->d_qn, qn))) return rc;
-        Gen G{p->gen_kind, p->gen_nmax, p->gen_g, c->d_qn, nullptr};
-        if (p->gen_kind == CCD_GEN_ELECTRON_GAS) {
+        Gen G{p->gen_kind, p->gen_nmax, p->gen_g, c->d_qn, nullptr, nullptr};
+        if (p->gen_kind == kGenBatch) {   // per-solve parameters; merge_problems checked the states
+            std::vector<double> tab(p->gen_table, p->gen_table + bi.table_len);
+            if ((rc = upload(c, &c->d_table, tab)) || (rc = upload(c, &c->d_gpart, bi.parts))) return rc;
+            G.table = c->d_table;
+            G.part = c->d_gpart;
+        } else if (p->gen_kind == CCD_GEN_ELECTRON_GAS) {
             for (int s = 0; s < nsp; s++)
                 for (int k = 0; k < 3; k++)
                     if (std::abs(p->qn[5 * s + k]) > p->gen_nmax)
@@ -865,18 +1078,55 @@ int build(ccd_ctx *c, const ccd_problem *p) {
 // Arm the device loop state.  fresh: iteration count 0 and E_prev = 100.0
 // (src/main.cpp:494); otherwise the count and E_prev carry on.  tol < 0 and
 // max_iters = INT_MAX give a free run that never stops by itself.
+// A batch arms every solve this way; a solve that stopped other than by
+// converging stays stopped when the loop carries on.  The batch's own state
+// counts iterations and stops once every solve has.
 int arm_loop(ccd_ctx *c, double tol, int max_iters, bool fresh) {
-    HIP_TRY(hipStreamSynchronize(c->stream));   // the staging buffer may still be in use
-    if (fresh) {
-        *c->h_ctl = LoopCtl{0, 0, CCD_STOP_CONVERGED, max_iters, 100.0, tol, kElogCap, 0};
-    } else {
-        HIP_TRY(hipMemcpy(c->h_ctl, c->d_ctl, sizeof(LoopCtl), hipMemcpyDeviceToHost));
-        c->h_ctl->done = 0;
-        c->h_ctl->reason = CCD_STOP_CONVERGED;
-        c->h_ctl->tol = tol;
-        c->h_ctl->max_iters = max_iters;
+    HIP_TRY(hipStreamSynchronize(c->stream));   // the staging buffers may still be in use
+    const LoopCtl start{0, 0, CCD_STOP_CONVERGED, max_iters, 100.0, tol, kElogCap, 0};
+    if (c->nprob == 1) {
+        if (fresh) {
+            *c->h_ctl = start;
+        } else {
+            HIP_TRY(hipMemcpy(c->h_ctl, c->d_ctl, sizeof(LoopCtl), hipMemcpyDeviceToHost));
+            c->h_ctl->done = 0;
+            c->h_ctl->reason = CCD_STOP_CONVERGED;
+            c->h_ctl->tol = tol;
+            c->h_ctl->max_iters = max_iters;
+        }
+        HIP_TRY(hipMemcpyAsync(c->d_ctl, c->h_ctl, sizeof(LoopCtl), hipMemcpyHostToDevice, c->stream));
+        return CCD_OK;
     }
+    const size_t pb = sizeof(LoopCtl) * (size_t)c->nprob;
+    if (fresh) {
+        for (int q = 0; q < c->nprob; q++) c->h_pctl[q] = start;
+        *c->h_ctl = LoopCtl{0, 0, CCD_STOP_CONVERGED, INT_MAX, 100.0, -1.0, kElogCap, 0};
+    } else {
+        HIP_TRY(hipMemcpy(c->h_pctl, c->d_pctl, pb, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(c->h_ctl, c->d_ctl, sizeof(LoopCtl), hipMemcpyDeviceToHost));
+        bool running = false;
+        for (int q = 0; q < c->nprob; q++) {
+            LoopCtl &l = c->h_pctl[q];
+            if (l.reason == CCD_STOP_CONVERGED) {
+                l.done = 0;
+                l.tol = tol;
+                l.max_iters = max_iters;
+            }
+            running |= !l.done;
+        }
+        c->h_ctl->done = running ? 0 : 1;
+    }
+    HIP_TRY(hipMemcpyAsync(c->d_pctl, c->h_pctl, pb, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->d_ctl, c->h_ctl, sizeof(LoopCtl), hipMemcpyHostToDevice, c->stream));
+    return CCD_OK;
+}
+
+// Per-solve loop state of a batch into h_pctl (after a run_loop).
+int read_pctl(ccd_ctx *c) {
+    if (c->nprob == 1) return CCD_OK;
+    HIP_TRY(hipMemcpyAsync(c->h_pctl, c->d_pctl, sizeof(LoopCtl) * (size_t)c->nprob, hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return CCD_OK;
 }
 
@@ -904,10 +1154,13 @@ int run_loop(ccd_ctx *c, double mixing, int batch) {
     return CCD_OK;
 }
 
-int copy_log(ccd_ctx *c, double *energies, int cap, int count) {
+// Energies of solve q (0 for a single solve): the first min(cap, count)
+// entries of its log.
+int copy_log(ccd_ctx *c, double *energies, int cap, int count, int q = 0) {
     const int n = std::min(std::min(cap, count), kElogCap);
     if (energies && n > 0) {
-        HIP_TRY(hipMemcpyAsync(energies, c->d_elog, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(energies, c->d_elog + (size_t)q * kElogCap, sizeof(double) * n,
+                               hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
     }
     return CCD_OK;
@@ -943,12 +1196,12 @@ int set_state(ccd_ctx *c, const double *t2) {
 
 extern "C" {
 
-int ccd_ctx_create(const ccd_problem *prob, int precision, ccd_ctx **out) {
-    ccd::clear_error();
-    if (!out) return ccd::fail(CCD_E_INVALID, "ccd_ctx_create: out is null");
-    *out = nullptr;
-    int rc = validate(prob);
-    if (rc) return rc;
+}  // extern "C"
+
+namespace {
+
+int create_ctx(const ccd_problem *prob, const BatchInfo &bi, int precision, ccd_ctx **out) {
+    int rc;
     const bool matrix_free = (precision & CCD_MATRIX_FREE) != 0;
     precision &= ~CCD_MATRIX_FREE;
     if (precision != CCD_PREC_FP64 && precision != CCD_PREC_FP32)
@@ -976,7 +1229,7 @@ int ccd_ctx_create(const ccd_problem *prob, int precision, ccd_ctx **out) {
             return ccd::fail(CCD_E_HIP, "could not create HIP events");
         }
     try {
-        rc = build(c, prob);
+        rc = build(c, prob, bi);
     } catch (const std::bad_alloc &) {
         rc = ccd::fail(CCD_E_NOMEM, "out of host memory while building tables");
     }
@@ -990,6 +1243,142 @@ int ccd_ctx_create(const ccd_problem *prob, int precision, ccd_ctx **out) {
         return rc;
     }
     *out = c;
+    return CCD_OK;
+}
+
+int batch_only(const ccd_ctx *c, const char *what) {
+    return ccd::fail(CCD_E_STATE, "%s: context holds a batch of %d solves (use the ccd_batch_* calls)", what,
+                     c->nprob);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ccd_ctx_create(const ccd_problem *prob, int precision, ccd_ctx **out) {
+    ccd::clear_error();
+    if (!out) return ccd::fail(CCD_E_INVALID, "ccd_ctx_create: out is null");
+    *out = nullptr;
+    int rc = validate(prob);
+    if (rc) return rc;
+    return create_ctx(prob, single_batch(prob), precision, out);
+}
+
+int ccd_batch_create(const ccd_problem *const *probs, int n, int precision, ccd_ctx **out) {
+    ccd::clear_error();
+    if (!out) return ccd::fail(CCD_E_INVALID, "ccd_batch_create: out is null");
+    *out = nullptr;
+    if (!probs || n <= 0) return ccd::fail(CCD_E_INVALID, "ccd_batch_create: no problems");
+    Merged m;
+    int rc;
+    try {
+        rc = merge_problems(probs, n, m);
+    } catch (const std::bad_alloc &) {
+        rc = ccd::fail(CCD_E_NOMEM, "out of host memory while merging the batch");
+    }
+    if (rc) return rc;
+    return create_ctx(&m.p, m.bi, precision, out);
+}
+
+int ccd_batch_info(const ccd_ctx *c, int *n_problems, int64_t *t2_offset) {
+    ccd::clear_error();
+    if (!c) return ccd::fail(CCD_E_INVALID, "null context");
+    if (n_problems) *n_problems = c->nprob;
+    if (t2_offset)
+        for (int q = 0; q <= c->nprob; q++) t2_offset[q] = c->t2_lo[q];
+    return CCD_OK;
+}
+
+int ccd_batch_mbpt2(ccd_ctx *c, double *corr) {
+    ccd::clear_error();
+    if (!c || !corr) return ccd::fail(CCD_E_INVALID, "ccd_batch_mbpt2: null argument");
+    int rc;
+    if (!c->utiles.empty()) {
+        UpdArgs a = upd_args(c, nullptr, nullptr, 0.0);
+        hipLaunchKernelGGL(k_mbpt2, dim3((unsigned)c->utiles.size()), dim3(kThreads), 0, c->stream, a,
+                           c->d_partials);
+        if ((rc = check_launch("k_mbpt2"))) return rc;
+    }
+    for (int q = 0; q < c->nprob; q++) {
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kThreads), 0, c->stream, c->d_partials + c->ulo[q],
+                           c->ulo[q + 1] - c->ulo[q], 0.25, c->d_scalar + q);
+        if ((rc = check_launch("k_sum_partials"))) return rc;
+    }
+    HIP_TRY(hipMemcpyAsync(corr, c->d_scalar, sizeof(double) * c->nprob, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CCD_OK;
+}
+
+int ccd_batch_solve(ccd_ctx *c, double tol, int max_iters, double mixing, double *energies, int cap, int *n_iter,
+                    int *stop) {
+    ccd::clear_error();
+    if (!c) return ccd::fail(CCD_E_INVALID, "null context");
+    if (max_iters < 0 || cap < 0) return ccd::fail(CCD_E_INVALID, "max_iters or cap < 0");
+    int rc = zero_state(c);
+    if (rc || (rc = arm_loop(c, tol, max_iters, true)) || (rc = run_loop(c, mixing, kSolveBatch)) ||
+        (rc = read_pctl(c)))
+        return rc;
+    for (int q = 0; q < c->nprob; q++) {
+        const LoopCtl st = c->nprob == 1 ? *c->h_ctl : c->h_pctl[q];
+        if ((rc = copy_log(c, energies ? energies + (size_t)q * cap : nullptr, cap, st.count, q))) return rc;
+        if (n_iter) n_iter[q] = st.count;
+        if (stop) stop[q] = st.reason;
+    }
+    return arm_free_run(c);
+}
+
+int ccd_batch_solve_mixed(ccd_ctx *c, double tol, double switch_tol, int max_iters, double mixing,
+                          double *energies, int cap, int *n_iter, int *n_fp32, int *stop) {
+    ccd::clear_error();
+    if (!c) return ccd::fail(CCD_E_INVALID, "null context");
+    if (c->precision != CCD_PREC_FP32)
+        return ccd::fail(CCD_E_STATE, "ccd_batch_solve_mixed needs a context created with CCD_PREC_FP32");
+    if (max_iters < 0 || cap < 0) return ccd::fail(CCD_E_INVALID, "max_iters or cap < 0");
+    const int saved = c->iter_prec;
+    std::vector<LoopCtl> phase1(c->nprob);
+    int rc = zero_state(c);
+    c->iter_prec = CCD_PREC_FP32;
+    if (!rc) rc = arm_loop(c, std::max(tol, switch_tol), max_iters, true);
+    if (!rc) rc = run_loop(c, mixing, kSolveBatch);
+    if (!rc) rc = read_pctl(c);
+    if (!rc)
+        for (int q = 0; q < c->nprob; q++) phase1[q] = c->nprob == 1 ? *c->h_ctl : c->h_pctl[q];
+    if (!rc && switch_tol > tol) {
+        c->iter_prec = CCD_PREC_FP64;
+        rc = arm_loop(c, tol, max_iters, false);
+        if (!rc) rc = run_loop(c, mixing, kSolveBatch);
+        if (!rc) rc = read_pctl(c);
+    }
+    c->iter_prec = saved;
+    if (rc) return rc;
+    for (int q = 0; q < c->nprob; q++) {
+        const LoopCtl st = c->nprob == 1 ? *c->h_ctl : c->h_pctl[q];
+        if ((rc = copy_log(c, energies ? energies + (size_t)q * cap : nullptr, cap, st.count, q))) return rc;
+        if (n_iter) n_iter[q] = st.count;
+        if (n_fp32) n_fp32[q] = phase1[q].count;
+        if (stop) stop[q] = st.reason;
+    }
+    return arm_free_run(c);
+}
+
+int ccd_batch_iterate_n(ccd_ctx *c, int n, double mixing, double *energies) {
+    ccd::clear_error();
+    if (!c || n < 0) return ccd::fail(CCD_E_INVALID, "ccd_batch_iterate_n: bad argument");
+    int rc = arm_free_run(c);
+    if (!rc) rc = fork_aux(c);
+    for (int k = 0; k < n && !rc; k++) rc = run_iteration(c, mixing, nullptr);
+    if (rc || (rc = join_aux(c))) return rc;
+    HIP_TRY(hipMemcpyAsync(c->h_ctl, c->d_ctl, sizeof(LoopCtl), hipMemcpyDeviceToHost, c->stream));
+    if ((rc = read_pctl(c))) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int q = 0; q < c->nprob; q++) {
+        const LoopCtl st = c->nprob == 1 ? *c->h_ctl : c->h_pctl[q];
+        if (st.count != n) {   // a NaN energy stops the solve even in a free run
+            if (c->nprob == 1) c->cur = (c->cur + st.count - n) & 1;
+            return ccd::fail(CCD_E_STATE, "solve %d: iteration %d produced a NaN energy", q, st.count);
+        }
+        if ((rc = copy_log(c, energies ? energies + (size_t)q * n : nullptr, n, n, q))) return rc;
+    }
     return CCD_OK;
 }
 
@@ -1026,6 +1415,7 @@ int ccd_work(const ccd_ctx *ctx, double *flops, double *flops_pppp, double *ladd
 int ccd_mbpt2(ccd_ctx *c, double *corr) {
     ccd::clear_error();
     if (!c || !corr) return ccd::fail(CCD_E_INVALID, "ccd_mbpt2: null argument");
+    if (c->nprob > 1) return batch_only(c, "ccd_mbpt2");
     if (c->utiles.empty()) {
         *corr = 0.0;
         return CCD_OK;
@@ -1052,6 +1442,7 @@ int ccd_reset(ccd_ctx *c) {
 int ccd_iterate(ccd_ctx *c, double mixing, double *corr) {
     ccd::clear_error();
     if (!c) return ccd::fail(CCD_E_INVALID, "null context");
+    if (c->nprob > 1) return batch_only(c, "ccd_iterate");
     int rc = arm_free_run(c);
     if (rc || (rc = fork_aux(c)) || (rc = run_iteration(c, mixing, nullptr)) || (rc = join_aux(c))) return rc;
     double e = 0.0;
@@ -1064,6 +1455,7 @@ int ccd_solve(ccd_ctx *c, double tol, int max_iters, double mixing, double *ener
               int *stop) {
     ccd::clear_error();
     if (!c) return ccd::fail(CCD_E_INVALID, "null context");
+    if (c->nprob > 1) return batch_only(c, "ccd_solve");
     if (max_iters < 0) return ccd::fail(CCD_E_INVALID, "max_iters < 0");
     int rc = zero_state(c);
     if (rc || (rc = arm_loop(c, tol, max_iters, true)) || (rc = run_loop(c, mixing, kSolveBatch))) return rc;
@@ -1089,6 +1481,7 @@ int ccd_solve_mixed(ccd_ctx *c, double tol, double switch_tol, int max_iters, do
                     int cap, int *n_iter, int *n_fp32, int *stop) {
     ccd::clear_error();
     if (!c) return ccd::fail(CCD_E_INVALID, "null context");
+    if (c->nprob > 1) return batch_only(c, "ccd_solve_mixed");
     if (c->precision != CCD_PREC_FP32)
         return ccd::fail(CCD_E_STATE, "ccd_solve_mixed needs a context created with CCD_PREC_FP32");
     if (max_iters < 0) return ccd::fail(CCD_E_INVALID, "max_iters < 0");
@@ -1118,6 +1511,7 @@ int ccd_solve_mixed(ccd_ctx *c, double tol, double switch_tol, int max_iters, do
 int ccd_iterate_n(ccd_ctx *c, int n, double mixing, double *energies) {
     ccd::clear_error();
     if (!c || n < 0) return ccd::fail(CCD_E_INVALID, "ccd_iterate_n: bad argument");
+    if (c->nprob > 1) return batch_only(c, "ccd_iterate_n");
     int rc = arm_free_run(c);
     if (!rc) rc = fork_aux(c);
     for (int k = 0; k < n && !rc; k++) rc = run_iteration(c, mixing, nullptr);
@@ -1207,6 +1601,7 @@ int ccd_onebody(ccd_ctx *c, const double *t, double *chi, double *psi) {
 int ccd_step(ccd_ctx *c, const double *t_old, double *t_new, double mixing, double *corr) {
     ccd::clear_error();
     if (!c || !t_old || !t_new) return ccd::fail(CCD_E_INVALID, "ccd_step: null argument");
+    if (c->nprob > 1) return batch_only(c, "ccd_step");
     int rc = set_state(c, t_old);
     if (rc) return rc;
     if ((rc = fork_aux(c)) || (rc = run_iteration(c, mixing, nullptr)) || (rc = join_aux(c))) return rc;

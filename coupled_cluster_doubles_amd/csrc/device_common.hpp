@@ -22,7 +22,8 @@ struct DChan {          // active direct channel (pp > 0 and hh > 0)
     int32_t npk, ldv;        // number of pairs a<b; leading dimension (multiple of 8)
     int32_t pr_off, ndiag;   // first entry in the row-pair lists; number of pairs (a,a)
     int64_t off_dp;     // ladder output: nseg + 1 slabs of npk x hh partial sums
-    int32_t nseg, pad;  // number of k segments of the pppp ladder (slab nseg: 1/2 T X)
+    int32_t nseg;       // number of k segments of the pppp ladder (slab nseg: 1/2 T X)
+    int32_t prob;       // solve the channel belongs to (0 outside a batch)
 };
 
 struct DMod {           // active mod channel (ph > 0 and hp > 0)
@@ -58,12 +59,25 @@ struct LoopCtl {
     int pad;
 };
 
+// A batch of independent solves merged into one problem: every s.p. state
+// carries its solve's index in qn[5 s + 4], and the solve's generator
+// parameters are looked up per element.
+constexpr int kGenBatch = 0x40;
+
+struct GenPart {        // generator parameters of one solve of a batch
+    int kind;
+    int nmax;
+    double g;
+    int64_t tab_off;    // its direct-term table inside Gen::table
+};
+
 struct Gen {            // on-device interaction
     int kind;
     int nmax;
     double g;
     const int32_t *qn;
     const double *table;
+    const GenPart *part;  // kind == kGenBatch: per-solve parameters
 };
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
@@ -90,22 +104,29 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
 // src/ElectronGasSPBasis.cpp:419-471 with the direct term tabulated).
 __device__ inline double gen_tbme(const Gen &G, int p, int q, int r, int s) {
     const int32_t *a = G.qn + 5 * p, *b = G.qn + 5 * q, *c = G.qn + 5 * r, *d = G.qn + 5 * s;
-    if (G.kind == CCD_GEN_PAIRING) {
+    int kind = G.kind, nmax = G.nmax;
+    double g = G.g;
+    const double *table = G.table;
+    if (kind == kGenBatch) {   // the four states belong to one solve
+        const GenPart P = G.part[a[4]];
+        kind = P.kind; nmax = P.nmax; g = P.g; table = G.table + P.tab_off;
+    }
+    if (kind == CCD_GEN_PAIRING) {
         double v = 0.0;
         if (a[0] == b[0] && c[0] == d[0] && a[1] == -b[1] && -c[1] == d[1]) {
-            v = -0.5 * G.g;
+            v = -0.5 * g;
             if (a[1] == -c[1]) v = -v;
         }
         return v;
     }
     const bool cons = a[0] + b[0] == c[0] + d[0] && a[1] + b[1] == c[1] + d[1] && a[2] + b[2] == c[2] + d[2];
     if (!cons) return 0.0;
-    const int m2 = 2 * G.nmax, w = 4 * G.nmax + 1;
+    const int m2 = 2 * nmax, w = 4 * nmax + 1;
     double d1 = 0.0, d2 = 0.0;
     if (a[3] == c[3] && b[3] == d[3])
-        d1 = G.table[((a[0] - c[0] + m2) * w + (a[1] - c[1] + m2)) * w + (a[2] - c[2] + m2)];
+        d1 = table[((a[0] - c[0] + m2) * w + (a[1] - c[1] + m2)) * w + (a[2] - c[2] + m2)];
     if (a[3] == d[3] && b[3] == c[3])
-        d2 = G.table[((a[0] - d[0] + m2) * w + (a[1] - d[1] + m2)) * w + (a[2] - d[2] + m2)];
+        d2 = table[((a[0] - d[0] + m2) * w + (a[1] - d[1] + m2)) * w + (a[2] - d[2] + m2)];
     return d1 - d2;
 }
 

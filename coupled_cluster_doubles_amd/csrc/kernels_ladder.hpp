@@ -13,7 +13,7 @@
 #include "device_common.hpp"
 
 constexpr int KC = 32;                 // k per chunk of k_ladder_x (k-slot q of step s: k = 8q + s)
-constexpr int NTMAX = 5;               // 16-column tiles: hh <= 80
+constexpr int NTMAX = 8;               // 16-column tiles: hh <= 128
 constexpr int LROWS = 64;              // packed rows per ladder block (16 per wave)
 constexpr int WROWS = 16;              // packed rows per wave tile (T X term)
 constexpr int kGasTableMax = 4096;     // entries of the generated-V gas table (LDS, 32 KB)
@@ -152,7 +152,9 @@ __device__ __forceinline__ int kslot(int q, int s) {
 
 template <int NT>
 struct LadderShape {
-    static constexpr int KCH = 32;                  // k per pipeline stage
+    // k per pipeline stage: the B tile (KCH x 16 NT, double-buffered) stays
+    // within 40 KB of LDS
+    static constexpr int KCH = NT <= 5 ? 32 : 16;
     // packed pairs per k segment: whole channels for the narrow classes (many
     // tiles, one prologue each), 256 for the few wide channels
     static constexpr int KSEG = NT <= 2 ? 1024 : 256;
@@ -175,6 +177,7 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
     typedef typename Mma<CT>::acc Acc;
     typedef CT vec __attribute__((ext_vector_type(VW)));
     static_assert(G <= 16, "one k-slot record per lane of a 16-lane row");
+    static_assert(KSEG % KCH == 0 && G % VW == 0, "k segments of whole chunks, chunks of whole 16-byte loads");
     __shared__ CT Bs[2][KCH][STR];
     extern __shared__ double gtab[];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
@@ -358,7 +361,9 @@ __device__ __forceinline__ void ladder_tx_wave(const DTile &tl, const DChan &ch,
     }
 }
 
-template <typename CT>
+// WIDE instances take the tiles of channels with hh > 80 (NT = 6..8), the
+// others NT = 1..5; the tile list holds the narrow channels first.
+template <typename CT, bool WIDE>
 __global__ __launch_bounds__(kThreads) void k_ladder_tx(const DTile *__restrict__ tiles, int ntiles,
                                                         const DChan *__restrict__ chans,
                                                         const double *__restrict__ tas,
@@ -370,12 +375,21 @@ __global__ __launch_bounds__(kThreads) void k_ladder_tx(const DTile *__restrict_
     if (it >= ntiles) return;
     const DTile tl = tiles[it];
     const DChan ch = chans[tl.chan];
-    switch ((ch.hh + 15) / 16) {
-        case 1: ladder_tx_wave<CT, 1>(tl, ch, tas, x, zero, dp); break;
-        case 2: ladder_tx_wave<CT, 2>(tl, ch, tas, x, zero, dp); break;
-        case 3: ladder_tx_wave<CT, 3>(tl, ch, tas, x, zero, dp); break;
-        case 4: ladder_tx_wave<CT, 4>(tl, ch, tas, x, zero, dp); break;
-        default: ladder_tx_wave<CT, 5>(tl, ch, tas, x, zero, dp); break;
+    const int nt = (ch.hh + 15) / 16;
+    if (WIDE) {
+        switch (nt) {
+            case 6: ladder_tx_wave<CT, 6>(tl, ch, tas, x, zero, dp); break;
+            case 7: ladder_tx_wave<CT, 7>(tl, ch, tas, x, zero, dp); break;
+            default: ladder_tx_wave<CT, 8>(tl, ch, tas, x, zero, dp); break;
+        }
+    } else {
+        switch (nt) {
+            case 1: ladder_tx_wave<CT, 1>(tl, ch, tas, x, zero, dp); break;
+            case 2: ladder_tx_wave<CT, 2>(tl, ch, tas, x, zero, dp); break;
+            case 3: ladder_tx_wave<CT, 3>(tl, ch, tas, x, zero, dp); break;
+            case 4: ladder_tx_wave<CT, 4>(tl, ch, tas, x, zero, dp); break;
+            default: ladder_tx_wave<CT, 5>(tl, ch, tas, x, zero, dp); break;
+        }
     }
 }
 
@@ -385,6 +399,9 @@ inline int ladder_segments(int npk, int nt) {
                      : nt == 2 ? LadderShape<2>::KSEG
                      : nt == 3 ? LadderShape<3>::KSEG
                      : nt == 4 ? LadderShape<4>::KSEG
-                               : LadderShape<5>::KSEG;
+                     : nt == 5 ? LadderShape<5>::KSEG
+                     : nt == 6 ? LadderShape<6>::KSEG
+                     : nt == 7 ? LadderShape<7>::KSEG
+                               : LadderShape<8>::KSEG;
     return npk > 0 ? (npk + kseg - 1) / kseg : 1;
 }

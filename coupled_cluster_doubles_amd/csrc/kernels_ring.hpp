@@ -91,7 +91,9 @@ __device__ __forceinline__ void ring_wave(const DMod &md, int jt, const double *
 
 // Items are sorted by tile count, largest first; wave w of block b takes item
 // 4b + w.  A channel with NT = ceil(max(ph, hp) / 16) has ceil(hp / 16) items.
-template <typename CT>
+// WIDE instances take the items with NT = 6..8, the others NT = 1..5 (the
+// wide register footprint would lower the occupancy of every item).
+template <typename CT, bool WIDE>
 __global__ __launch_bounds__(kThreads) void k_ring(const RingItem *__restrict__ items, int nitems,
                                                    const DMod *__restrict__ mods,
                                                    const double *__restrict__ tph,
@@ -104,11 +106,20 @@ __global__ __launch_bounds__(kThreads) void k_ring(const RingItem *__restrict__ 
     if (it >= nitems) return;
     const RingItem w = items[it];
     const DMod md = mods[w.mod];
-    switch ((max(md.ph, md.hp) + 15) / 16) {
-        case 1: ring_wave<CT, 1>(md, w.jt, tph, va, vb, zero, r); break;
-        case 2: ring_wave<CT, 2>(md, w.jt, tph, va, vb, zero, r); break;
-        case 3: ring_wave<CT, 3>(md, w.jt, tph, va, vb, zero, r); break;
-        case 4: ring_wave<CT, 4>(md, w.jt, tph, va, vb, zero, r); break;
-        default: ring_wave<CT, 5>(md, w.jt, tph, va, vb, zero, r); break;
+    const int nt = (max(md.ph, md.hp) + 15) / 16;
+    if (WIDE) {
+        switch (nt) {
+            case 6: ring_wave<CT, 6>(md, w.jt, tph, va, vb, zero, r); break;
+            case 7: ring_wave<CT, 7>(md, w.jt, tph, va, vb, zero, r); break;
+            default: ring_wave<CT, 8>(md, w.jt, tph, va, vb, zero, r); break;
+        }
+    } else {
+        switch (nt) {
+            case 1: ring_wave<CT, 1>(md, w.jt, tph, va, vb, zero, r); break;
+            case 2: ring_wave<CT, 2>(md, w.jt, tph, va, vb, zero, r); break;
+            case 3: ring_wave<CT, 3>(md, w.jt, tph, va, vb, zero, r); break;
+            case 4: ring_wave<CT, 4>(md, w.jt, tph, va, vb, zero, r); break;
+            default: ring_wave<CT, 5>(md, w.jt, tph, va, vb, zero, r); break;
+        }
     }
 }

@@ -31,17 +31,21 @@ struct UpdArgs {
     const int32_t *rpos;             //   row sum (pp_off + row) -> place (psi_a lists)
     double mixing;
     int nh;
-    const LoopCtl *ctl;
+    const LoopCtl *ctl;              // loop state of the context
+    const LoopCtl *pctl;             // per solve (a batch); pctl == ctl for a single solve
 };
 
 // UPDATE = true: one Jacobi step for both rows of each pair; UPDATE = false:
 // only the W sums of the current amplitudes (to derive chi, psi, E from them).
+// A solve of a batch that has stopped keeps its amplitudes: its new T is the
+// old one.
 template <bool UPDATE>
 __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
     if (a.ctl->done) return;
     __shared__ double w1s[kThreads], w2s[kThreads];
     const DTile tl = a.tiles[blockIdx.x];
     const DChan ch = a.chans[tl.chan];
+    const bool frozen = UPDATE && a.pctl[ch.prob].done;
     const int H = ch.hh, n = tl.nrows * H;
     const int l = threadIdx.x;
     if (l < n) {
@@ -79,6 +83,7 @@ __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
             x1 -= pa * t2;
             x1 = x1 / (a.eps[ii] + a.eps[jj] - a.eps[ia] - a.eps[ib]);
             x1 = a.mixing * x1 + (1. - a.mixing) * t1;
+            if (frozen) x1 = t1;
             a.t_new[b1 + IJ] = x1;
             w1 = v1 * x1;
             w2 = 0.0;
@@ -96,6 +101,7 @@ __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
                 x2 -= pb * t1;
                 x2 = x2 / (a.eps[ii] + a.eps[jj] - a.eps[ib] - a.eps[ia]);
                 x2 = a.mixing * x2 + (1. - a.mixing) * t2;
+                if (frozen) x2 = t2;
                 a.t_new[b2 + IJ] = x2;
                 a.tas[ch.off_tp + (int64_t)pr * H + IJ] = 0.5 * (x1 - x2);
                 w2 = v2 * x2;
@@ -154,19 +160,23 @@ __global__ __launch_bounds__(kThreads) void k_transpose(const int32_t *__restric
 // chi_i (blocks 0..nh-1) and psi_a (blocks nh..nsp-1) as fixed-order sums of
 // their column partials / row sums, which the update stored contiguously
 // (parts[ptr[k] .. ptr[k+1])).  The chi block that finishes last
-// (ticket counter) then forms E = 1/4 sum_i chi_i = 1/4 sum V o T (every element has
-// one first hole index) in a fixed order and runs the loop bookkeeping of
-// src/main.cpp:511-519, 667-670: dE = E - E_prev, E_prev = E, iterations++,
-// and the loop goes on while |dE| > tol unless more than max_iters iterations
-// have run.  The last block resets the ticket counter for the next launch.
+// (ticket counter) then forms, for every solve p still running, E_p = 1/4
+// sum of chi_i over its holes [hlo[p], hlo[p+1]) (= 1/4 sum V o T: every
+// element has one first hole index) in a fixed order, and runs the loop
+// bookkeeping of src/main.cpp:511-519, 667-670 on pctl[p]: dE = E - E_prev,
+// E_prev = E, iterations++, and the loop goes on while |dE| > tol unless
+// more than max_iters iterations have run.  A single solve has pctl == ctl;
+// a batch counts its iterations in ctl and stops once every solve has.  The
+// last block resets the ticket counter for the next launch.
 __global__ __launch_bounds__(kThreads) void k_reduce(const int32_t *__restrict__ ptr, int nh,
                                                      const double *__restrict__ parts,
                                                      double *__restrict__ out, int *__restrict__ ticket,
-                                                     double *__restrict__ energy, LoopCtl *__restrict__ ctl,
-                                                     double *__restrict__ log) {
+                                                     double *__restrict__ energy, LoopCtl *ctl,
+                                                     double *__restrict__ log, int nprob,
+                                                     const int32_t *__restrict__ hlo, LoopCtl *pctl) {
     if (ctl->done) return;
     __shared__ double red[kThreads / 64];
-    __shared__ int last;
+    __shared__ int last, pdone;
     const int k = blockIdx.x;
     double s = 0.0;
     for (int q = ptr[k] + threadIdx.x; q < ptr[k + 1]; q += kThreads) s += parts[q];
@@ -183,25 +193,41 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const int32_t *__restrict__
     __syncthreads();
     if (!last) return;
     __threadfence();
-    double c = 0.0;
-    for (int i = threadIdx.x; i < nh; i += kThreads)
-        c += __hip_atomic_load(out + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const double tot = block_sum(c, red);
+    for (int p = 0; p < nprob; p++) {
+        if (threadIdx.x == 0) pdone = pctl[p].done;
+        __syncthreads();
+        const int skip = pdone;
+        __syncthreads();
+        if (skip) continue;
+        double c = 0.0;
+        for (int i = hlo[p] + threadIdx.x; i < hlo[p + 1]; i += kThreads)
+            c += __hip_atomic_load(out + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const double tot = block_sum(c, red);
+        if (threadIdx.x == 0) {
+            LoopCtl *pc = pctl + p;
+            const double e = 0.25 * tot;
+            energy[p] = e;
+            const double diff = e - pc->prev;
+            pc->prev = e;
+            const int it = pc->count;
+            if (it < pc->cap) log[(int64_t)p * pc->cap + it] = e;
+            pc->count = it + 1;
+            if (!(fabs(diff) > pc->tol)) {
+                pc->reason = isnan(diff) ? CCD_STOP_NAN : CCD_STOP_CONVERGED;
+                pc->done = 1;
+            } else if (it + 1 > pc->max_iters) {
+                pc->reason = CCD_STOP_MAX_ITERS;
+                pc->done = 1;
+            }
+        }
+    }
     if (threadIdx.x == 0) {
         *ticket = 0;
-        const double e = 0.25 * tot;
-        energy[0] = e;
-        const double diff = e - ctl->prev;
-        ctl->prev = e;
-        const int it = ctl->count;
-        if (it < ctl->cap) log[it] = e;
-        ctl->count = it + 1;
-        if (!(fabs(diff) > ctl->tol)) {
-            ctl->reason = isnan(diff) ? CCD_STOP_NAN : CCD_STOP_CONVERGED;
-            ctl->done = 1;
-        } else if (it + 1 > ctl->max_iters) {
-            ctl->reason = CCD_STOP_MAX_ITERS;
-            ctl->done = 1;
+        if (pctl != ctl) {
+            int all = 1;
+            for (int p = 0; p < nprob; p++) all &= pctl[p].done;
+            ctl->count += 1;
+            if (all) ctl->done = 1;
         }
     }
 }

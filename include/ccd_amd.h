@@ -202,6 +202,42 @@ int ccd_time_phases(ccd_ctx *ctx, int iters, double mixing, double *ms);
 int ccd_work(const ccd_ctx *ctx, double *flops, double *flops_pppp,
              double *bytes_vpppp);
 
+/* ---------------------------------------------------------------- batches
+ *
+ * Many independent solves in one context -- the parameter sweep, which the
+ * reference runs as one ccd.exe process per basis (src/main.cpp is one
+ * solve).  No counterpart in the reference.  The problems are merged into
+ * one channel-blocked problem: states renumbered holes first (the holes of
+ * problem 0, of problem 1, ..., then the particles in the same order),
+ * channels and their chains back to back in problem order.  Every kernel of
+ * an iteration then covers all solves in one launch.  Each solve keeps its
+ * own loop state (E_prev, the stop rule, its energy log); a solve that has
+ * stopped keeps its amplitudes while the others go on, and the iteration
+ * ends once all have stopped.  Each solve's energies and amplitudes are
+ * bit-identical to those of a context of its own.  The problems must all
+ * generate V on the device (gen_kind != CCD_GEN_NONE; pairing and electron
+ * gas may be mixed) or all supply it.  ccd_get_t2 / ccd_set_t2 and the
+ * per-diagram entry points work on the merged layout; ccd_mbpt2,
+ * ccd_iterate, ccd_iterate_n, ccd_solve, ccd_solve_mixed and ccd_step fail
+ * with CCD_E_STATE on a batch of more than one solve. */
+int ccd_batch_create(const ccd_problem *const *probs, int n, int precision,
+                     ccd_ctx **out);
+/* Number of solves; t2_offset (if not NULL, n+1 entries): solve q's
+ * amplitudes are elements [t2_offset[q], t2_offset[q+1]) of ccd_get_t2. */
+int ccd_batch_info(const ccd_ctx *ctx, int *n_problems, int64_t *t2_offset);
+/* MBPT2 of every solve (corr[n]). */
+int ccd_batch_mbpt2(ccd_ctx *ctx, double *corr);
+/* ccd_solve for every solve at once: energies[q*cap + k], n_iter[q], stop[q]. */
+int ccd_batch_solve(ccd_ctx *ctx, double tol, int max_iters, double mixing,
+                    double *energies, int cap, int *n_iter, int *stop);
+/* ccd_solve_mixed for every solve at once (same layout, n_fp32[q]); the
+ * FP32 phase lasts until every solve has stopped in it. */
+int ccd_batch_solve_mixed(ccd_ctx *ctx, double tol, double switch_tol,
+                          int max_iters, double mixing, double *energies,
+                          int cap, int *n_iter, int *n_fp32, int *stop);
+/* ccd_iterate_n for every solve at once: energies[q*n + k]. */
+int ccd_batch_iterate_n(ccd_ctx *ctx, int n, double mixing, double *energies);
+
 /* ---------------------------------------------------------------- model */
 
 /* Pairing model, reference PairingSPBasis(0, xi, g, nPairStates, nParticles). */

@@ -1,0 +1,128 @@
+"""Batches of independent solves in one context (ccd_batch_*).
+
+Each solve of a batch must give exactly what a context of its own gives:
+the same MBPT2, the same energy in every iteration, the same stop reason
+and iteration count, and the same final amplitudes, bit for bit -- also
+when the solves stop at different iterations (a stopped solve keeps its
+amplitudes while the others go on).  Single contexts are held to the
+oracle in test_gpu_parity.py; here a few solves are also checked against
+the oracle directly.  The 64-solve seeded sweep (electron gas N = 14/38/54
+and pairing models) runs in full.
+"""
+import numpy as np
+import pytest
+
+from coupled_cluster_doubles_amd import FP32, FP64, Batch, Context, Model
+from coupled_cluster_doubles_amd.sweep import sweep_configs, sweep_models
+from oracle.pyoracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _small():
+    cfg = [("pairing", (8, 8, 0.5, 1.0)), ("heg", (1.0, 19, 2)), ("pairing", (12, 12, 0.3, 1.0)),
+           ("heg", (3.0, 4, 2)), ("heg", (0.5, 19, 2))]
+    return cfg, [Model.pairing(*a) if k == "pairing" else Model.electron_gas(*a) for k, a in cfg]
+
+
+def _same_as_single(models, results, batch, mbpt2=None, tol=1e-10):
+    for q, m in enumerate(models):
+        c = Context(m, FP64)
+        if mbpt2 is not None:
+            assert mbpt2[q] == c.mbpt2(), q
+        e, stop, n = c.solve(tol)
+        eb, stop_b, n_b = results[q]
+        assert (stop_b, n_b) == (stop, n), q
+        assert np.array_equal(eb, e, equal_nan=True), q     # a diverging solve ends in NaN
+        assert np.array_equal(batch.t2(q), c.get_t2(), equal_nan=True), q
+        c.close()
+
+
+def test_batch_solves_match_single_contexts_bitwise(require_gpu):
+    cfg, models = _small()
+    b = Batch(models, FP64)
+    assert b.n_problems == len(models)
+    assert b.n_t2 == sum(m.n_t2 for m in models)
+    mb = b.mbpt2()
+    res = b.solve(1e-10)
+    counts = {r[2] for r in res}
+    assert len(counts) > 1            # the solves stop at different iterations
+    _same_as_single(models, res, b, mb)
+    # a second solve from T2 = 0 repeats the first
+    res2 = b.solve(1e-10)
+    for r, r2 in zip(res, res2):
+        assert r[1:] == r2[1:] and np.array_equal(r[0], r2[0])
+
+
+def test_batch_against_oracle(require_gpu):
+    cfg, models = _small()
+    b = Batch(models, FP64)
+    res = b.solve(1e-10)
+    for q, (kind, args) in enumerate(cfg[:3]):
+        o = Oracle.pairing(*args) if kind == "pairing" else Oracle.electron_gas(*args)
+        e_o, stop_o, _ = o.solve(1e-10)
+        e_g, stop_g, n_g = res[q]
+        assert stop_g == stop_o == "converged"
+        assert abs(n_g - len(e_o)) <= 1
+        k = min(n_g, len(e_o))
+        assert np.max(np.abs(e_g[:k] - e_o[:k]) / np.abs(e_o[:k])) < 1e-10
+        o.close()
+
+
+def test_batch_modes_and_iterate_n(require_gpu):
+    _, models = _small()
+    stored = Batch(models, FP64)
+    free = Batch(models, FP64, matrix_free=True)
+    es, ef = stored.iterate_n(4), free.iterate_n(4)
+    assert es.shape == (len(models), 4)
+    assert np.array_equal(es, ef)
+    assert np.array_equal(stored.get_t2(), free.get_t2())
+    for q, m in enumerate(models):
+        assert np.array_equal(Context(m, FP64).iterate_n(4), es[q]), q
+
+
+def test_batch_rejects_single_solve_calls(require_gpu):
+    from coupled_cluster_doubles_amd import CCDError
+    _, models = _small()
+    b = Batch(models[:2], FP64)
+    with pytest.raises(CCDError, match="batch"):
+        b.iterate()
+    with pytest.raises(CCDError, match="batch"):
+        Context.solve(b, 1e-10)
+
+
+def test_batch_mixed_precision_within_1e7_per_particle(require_gpu):
+    """A solve that finishes its FP32 phase early waits, frozen, for the
+    others; its iterates are still exactly those of a context of its own."""
+    _, models = _small()
+    e64 = [r[0][-1] for r in Batch(models, FP64).solve(1e-10)]
+    res = Batch(models, FP32).solve_mixed(1e-10, 1e-6)
+    for q, m in enumerate(models):
+        e, stop, n, n32 = res[q]
+        assert stop == "converged" and 0 < n32 <= n
+        assert abs(e[-1] - e64[q]) / m.n_holes <= 1e-7
+        e1, stop1, n1, n321 = Context(m, FP32).solve_mixed(1e-10, 1e-6)
+        assert (stop1, n1, n321) == (stop, n, n32) and np.array_equal(e1, e), q
+
+
+def test_sweep64_batch(require_gpu):
+    """The seeded 64-solve sweep as one batch: FP64 results bit-identical to
+    64 separate contexts, mixed precision within 1e-7 per particle."""
+    cfg = sweep_configs()
+    models = sweep_models()
+    b = Batch(models, FP64)
+    mb = b.mbpt2()
+    res = b.solve(1e-10)
+    _same_as_single(models, res, b, mb)
+    e64 = [r[0][-1] for r in res]
+    stops = [r[1] for r in res]
+    b.close()
+    print("sweep stop reasons:", {k: stops.count(k) for k in set(stops)})
+    mixed = Batch(models, FP32).solve_mixed(1e-10, 1e-6)
+    for q, m in enumerate(models):
+        e, stop, n, n32 = mixed[q]
+        if stops[q] == "converged":
+            assert stop == "converged", (q, cfg[q])
+            assert abs(e[-1] - e64[q]) / m.n_holes <= 1e-7, (q, cfg[q])
+        else:                              # the FP64 solve diverges: so does the mixed one
+            assert stop != "converged", (q, cfg[q])

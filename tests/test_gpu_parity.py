@@ -76,6 +76,8 @@ def test_inf_matter_solve_and_published_energy(require_gpu):
     ("heg", (1.0, 4, 2)),
     ("heg", (1.0, 19, 2)),
     ("infmatter", (0.08, 4, 2)),
+    ("pairing", (14, 14, 0.4, 1.0)),      # hh = 98: 7 column tiles
+    ("pairing", (16, 16, 0.6, 1.0)),      # hh = 128: 8 column tiles
 ])
 def test_diagrams_on_random_amplitudes(require_gpu, kind, args):
     m, o = _pair(kind, args)

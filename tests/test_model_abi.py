@@ -100,3 +100,23 @@ def test_inf_matter_model_rejects_symmetric_matter():
     with pytest.raises(CCDError, match="pure neutron matter"):
         Model.inf_matter(0.16, 4, 2, tz_max=2)
 
+
+
+def test_batch_rejects_mixed_interaction_sources_before_touching_a_device():
+    from coupled_cluster_doubles_amd import Batch
+    with pytest.raises(CCDError, match="generated V for every problem or for none"):
+        Batch([Model.pairing(8, 8, 0.5, 1.0), Model.inf_matter(0.08, 3, 2)])
+    with pytest.raises(CCDError, match="no problems"):
+        Batch([])
+
+
+def test_sweep_is_seeded_and_covers_the_bases():
+    from coupled_cluster_doubles_amd.sweep import sweep_configs
+    a, b = sweep_configs(), sweep_configs()
+    assert a == b and len(a) == 64
+    heg = [x for k, x in a if k == "heg"]
+    pair = [x for k, x in a if k == "pairing"]
+    assert len(heg) == len(pair) == 32
+    assert {x[1:] for x in heg} == {(19, 2), (25, 3), (32, 4)}
+    assert all(0.5 <= x[0] <= 5.0 for x in heg)
+    assert all(x[0] == x[1] and x[0] % 2 == 0 and 8 <= x[0] <= 16 and 0.2 <= x[2] <= 0.8 for x in pair)
