@@ -2,7 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
                     [--precision mixed|fp64|fp32] [--mode free|stored]
-                    [--no-cpu-baseline] [--config heg54|heg66|heg14|pairing]
+                    [--no-cpu-baseline] [--config heg54|heg66|heg14|pairing|sweep64]
 
 A step is one full Jacobi iteration of the CCD amplitude equations over the
 whole problem (both ladders, ring, one-body terms, remaps, update, energy),
@@ -14,9 +14,14 @@ FP64 sweeps to |dE| <= 1e-10) and a full FP64 solve are timed from T2 = 0
 and their energies compared per particle.  fp64 / fp32: steps in that
 arithmetic only.
 
---mode free (default) is the reference's matrix-free mode (-m 1): V^pppp is
-rebuilt inside the pppp ladder every iteration instead of being read from
-HBM; the iterates are bit-identical to --mode stored.  Prints one JSON line.
+--mode free (the default for the electron gas) is the reference's
+matrix-free mode (-m 1): V^pppp is rebuilt inside the pppp ladder every
+iteration instead of being read from HBM; the iterates are bit-identical to
+--mode stored (the default for the pairing model and the sweep).
+
+--config sweep64: the seeded sweep of 64 independent solves (electron gas
+N=14/38/54 and pairing models, coupled_cluster_doubles_amd/sweep.py) as one
+batch; a step advances every solve by one iteration.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -44,6 +49,9 @@ CONFIGS = {
     # reference CLI: --pairing -l 8 -n 8 -g 0.5 -x 1.0 -t 1e-12
     "pairing": dict(kind="pairing", args=(8, 8, 0.5, 1.0),
                     name="pairing model, 8 levels, 8 particles, g=0.5, CCD iteration"),
+    "sweep64": dict(kind="sweep", args=(),
+                    name="64 seeded solves in one batch (electron gas N=14/38/54, r_s in [0.5,5]; "
+                         "pairing 8-16 levels, g in [0.2,0.8]), one iteration of every solve"),
 }
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
@@ -57,30 +65,39 @@ def make_model(cfg):
     return Model.pairing(*cfg["args"])
 
 
-def packed_ladder_flops(model):
+def make_context(cfg, prec, free, models=None):
+    from coupled_cluster_doubles_amd import Batch, Context
+    if cfg["kind"] == "sweep":
+        return Batch(models, prec, matrix_free=free)
+    return Context(models[0], prec, matrix_free=free)
+
+
+def packed_ladder_flops(models):
     """MFMA-side flops of one pass of the packed pppp ladder: per direct
     channel 2 npk^2 hh (V t over pairs c<d) + 2 npk hh^2 (T X), npk = pairs a<b."""
-    d = model.dims()
-    pairs = model.pairs("pp")
-    f, o = 0.0, 0
-    for pp, hh in zip(d["pp_dim"], d["hh_dim"]):
-        blk = pairs[o:o + pp]
-        o += pp
-        if pp == 0 or hh == 0:
-            continue
-        npk = int(np.count_nonzero(blk[:, 0] < blk[:, 1]))
-        f += 2.0 * npk * npk * hh + 2.0 * npk * hh * hh
+    f = 0.0
+    for model in models:
+        d = model.dims()
+        pairs = model.pairs("pp")
+        o = 0
+        for pp, hh in zip(d["pp_dim"], d["hh_dim"]):
+            blk = pairs[o:o + pp]
+            o += pp
+            if pp == 0 or hh == 0:
+                continue
+            npk = int(np.count_nonzero(blk[:, 0] < blk[:, 1]))
+            f += 2.0 * npk * npk * hh + 2.0 * npk * hh * hh
     return f
 
 
-def cpu_baseline(cfg, iters, threads, matrix_free=False):
-    """The FP64 CPU oracle (plain C + OpenMP) timed on this host."""
+def cpu_baseline(kind, args, iters, threads, matrix_free=False):
+    """The FP64 CPU oracle (plain C + OpenMP) timed on this host: iterations/s."""
     from oracle.pyoracle import Oracle, set_threads
     set_threads(threads)
-    if cfg["kind"] == "heg":
-        o = Oracle.electron_gas(*cfg["args"], matrix_free=matrix_free)
+    if kind == "heg":
+        o = Oracle.electron_gas(*args, matrix_free=matrix_free)
     else:
-        o = Oracle.pairing(*cfg["args"])
+        o = Oracle.pairing(*args)
     t = np.zeros(o.nt2)
     t, _ = o.step(t)                      # first iteration untimed (page-in)
     t0 = time.perf_counter()
@@ -89,6 +106,22 @@ def cpu_baseline(cfg, iters, threads, matrix_free=False):
     dt = time.perf_counter() - t0
     o.close()
     return iters / dt, e
+
+
+def cpu_baseline_sweep(iters, threads):
+    """Seconds per sweep iteration on the oracle, from one representative solve
+    per basis (an iteration's cost depends on the basis, not on r_s or g),
+    weighted by the basis' count in the sweep."""
+    from coupled_cluster_doubles_amd.sweep import sweep_configs
+    count = {}
+    for kind, args in sweep_configs():
+        key = (kind, args[1:] if kind == "heg" else args[:2])
+        count.setdefault(key, [0, (kind, args)])[0] += 1
+    sec = 0.0
+    for key, (n, (kind, args)) in sorted(count.items()):
+        rate, _ = cpu_baseline(kind, args, iters, threads)
+        sec += n / rate
+    return 1.0 / sec
 
 
 def main():
@@ -100,8 +133,9 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64", "fp32"])
     ap.add_argument("--switch-tol", type=float, default=1e-6,
                     help="mixed solve: |dE| below which FP64 sweeps replace FP32 iterations")
-    ap.add_argument("--mode", default="free", choices=["stored", "free"],
-                    help="stored: V^pppp read from HBM; free: rebuilt in the ladder (reference -m 1)")
+    ap.add_argument("--mode", default=None, choices=["stored", "free"],
+                    help="stored: V^pppp read from HBM; free: rebuilt in the ladder (reference -m 1); "
+                         "default free for the electron gas, stored otherwise")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -126,10 +160,17 @@ def main():
 
     cfg = CONFIGS[args.config]
     prec = FP64 if args.precision == "fp64" else FP32
+    if args.mode is None:
+        args.mode = "free" if cfg["kind"] == "heg" else "stored"
     free = args.mode == "free"
-    model = make_model(cfg)
+    sweep = cfg["kind"] == "sweep"
+    if sweep:
+        from coupled_cluster_doubles_amd.sweep import sweep_models
+        models = sweep_models()
+    else:
+        models = [make_model(cfg)]
     t_setup = time.perf_counter()
-    ctx = Context(model, prec, matrix_free=free)
+    ctx = make_context(cfg, prec, free, models)
     setup_s = time.perf_counter() - t_setup
     mbpt2 = ctx.mbpt2()
     work = ctx.work()
@@ -138,8 +179,9 @@ def main():
     # the call returns after the last one has finished on the device
     ctx.iterate_n(args.warmup)
     t0 = time.perf_counter()
-    e = float(ctx.iterate_n(args.steps)[-1])
+    e = ctx.iterate_n(args.steps)
     dt = time.perf_counter() - t0
+    e = e[:, -1].tolist() if sweep else float(e[-1])
     ms_step = 1e3 * dt / args.steps
     iters_per_s = args.steps / dt
 
@@ -153,7 +195,7 @@ def main():
         roof = {"bound": "hbm", "kernel": "k_ladder_v", "achieved": achieved, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes": lb}
     else:
-        lf = packed_ladder_flops(model)
+        lf = packed_ladder_flops(models)
         peak = FP64_PEAK_TFLOPS if prec == FP64 else FP32_PEAK_TFLOPS
         achieved = lf / (ladder_ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "kernel": "k_ladder_v", "achieved": achieved, "peak": peak,
@@ -168,8 +210,9 @@ def main():
             traffic = json.load(f).get("bytes_per_launch")
 
     mixed = None
-    if args.precision == "mixed":
+    if args.precision == "mixed" and not sweep:
         # full solves from T2 = 0, timed on the host clock around the call
+        model = models[0]
         n_part = model.n_holes
         t0 = time.perf_counter()
         e_m, stop_m, n_m, n32 = ctx.solve_mixed(1e-10, args.switch_tol)
@@ -187,6 +230,25 @@ def main():
                  "abs_dE_per_particle": abs(float(e_m[-1]) - float(e_d[-1])) / n_part,
                  "speedup_vs_fp64_solve": t_d / t_m}
         ctx64.close()
+    elif args.precision == "mixed":
+        # every solve of the sweep from T2 = 0, mixed and FP64, one batch each
+        t0 = time.perf_counter()
+        rm = ctx.solve_mixed(1e-10, args.switch_tol)
+        t_m = time.perf_counter() - t0
+        ctx.close()
+        ctx64 = make_context(cfg, FP64, free, models)
+        t0 = time.perf_counter()
+        rd = ctx64.solve(1e-10)
+        t_d = time.perf_counter() - t0
+        conv = [q for q in range(len(models)) if rd[q][1] == "converged"]
+        mixed = {"tol": 1e-10, "switch_tol": args.switch_tol,
+                 "fp64_stops": {k: sum(r[1] == k for r in rd) for k in ("converged", "max_iterations", "nan")},
+                 "mixed_converged_where_fp64_did": all(rm[q][1] == "converged" for q in conv),
+                 "max_abs_dE_per_particle": max(abs(float(rm[q][0][-1]) - float(rd[q][0][-1]))
+                                                / models[q].n_holes for q in conv),
+                 "batch_iters_mixed": max(r[2] for r in rm), "batch_iters_fp64": max(r[2] for r in rd),
+                 "mixed_solve_s": t_m, "fp64_solve_s": t_d, "speedup_vs_fp64_solve": t_d / t_m}
+        ctx64.close()
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -194,12 +256,16 @@ def main():
         # V^pppp in the reference's layout is 54 GB at N=66: the oracle then
         # rebuilds it per channel (the reference's -m 1), else it reads it
         cpu_free = args.config == "heg66"
-        v, _ = cpu_baseline(cfg, args.cpu_iters, threads, cpu_free)
-        cpu = {"value": v, "unit": "iter/s", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_iters} FP64 iterations of the same workload by the C oracle "
-                         f"(OpenMP over channels, naive dgemm, V^pppp "
-                         f"{'rebuilt per channel' if cpu_free else 'stored'}), "
-                         f"after one untimed iteration"}
+        if sweep:
+            v = cpu_baseline_sweep(args.cpu_iters, threads)
+            sample = (f"{args.cpu_iters} FP64 iterations by the C oracle of one solve per basis "
+                      f"of the sweep (8 bases), weighted by the basis counts")
+        else:
+            v, _ = cpu_baseline(cfg["kind"], cfg["args"], args.cpu_iters, threads, cpu_free)
+            sample = (f"{args.cpu_iters} FP64 iterations of the same workload by the C oracle "
+                      f"(OpenMP over channels, naive dgemm, V^pppp "
+                      f"{'rebuilt per channel' if cpu_free else 'stored'}), after one untimed iteration")
+        cpu = {"value": v, "unit": "iter/s", "cores": threads, "kind": "port", "sample": sample}
 
     out = {
         "metric": "CCD iter/s + channel-GEMM TFLOP/s, HEG N=54; mixed vs FP64 at |dE|/N<=1e-7",
@@ -219,7 +285,7 @@ def main():
                    "n_t2": ctx.n_t2, "device_bytes": ctx.device_bytes},
         "gemm_tflops": gemm_tflops,
         "gemm_flops_per_iter": work["flops"],
-        "mbpt2": mbpt2,
+        "mbpt2": mbpt2.tolist() if sweep else mbpt2,
         "corr_after_steps": e,
         "phase_ms": {"ring": phases[0], "ladder_x": phases[1], "ladder_d": phases[2],
                      "update": phases[3], "reductions": phases[4], "iteration": phases[5]},

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <limits>
 #include <climits>
 #include <cmath>
 #include <cstdarg>
@@ -1371,13 +1372,18 @@ int ccd_batch_iterate_n(ccd_ctx *c, int n, double mixing, double *energies) {
     HIP_TRY(hipMemcpyAsync(c->h_ctl, c->d_ctl, sizeof(LoopCtl), hipMemcpyDeviceToHost, c->stream));
     if ((rc = read_pctl(c))) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->nprob == 1 && c->h_ctl->count != n) {   // a NaN energy stops even a free run
+        c->cur = (c->cur + c->h_ctl->count - n) & 1;
+        return ccd::fail(CCD_E_STATE, "iteration %d produced a NaN energy", c->h_ctl->count);
+    }
     for (int q = 0; q < c->nprob; q++) {
-        const LoopCtl st = c->nprob == 1 ? *c->h_ctl : c->h_pctl[q];
-        if (st.count != n) {   // a NaN energy stops the solve even in a free run
-            if (c->nprob == 1) c->cur = (c->cur + st.count - n) & 1;
-            return ccd::fail(CCD_E_STATE, "solve %d: iteration %d produced a NaN energy", q, st.count);
-        }
-        if ((rc = copy_log(c, energies ? energies + (size_t)q * n : nullptr, n, n, q))) return rc;
+        // a solve of a batch whose energy turned NaN stopped there (its
+        // amplitudes kept); the entries after its last iteration are NaN
+        const int cnt = c->nprob == 1 ? n : c->h_pctl[q].count;
+        double *eq = energies ? energies + (size_t)q * n : nullptr;
+        if ((rc = copy_log(c, eq, n, cnt, q))) return rc;
+        if (eq)
+            for (int k = std::min(cnt, kElogCap); k < n; k++) eq[k] = std::numeric_limits<double>::quiet_NaN();
     }
     return CCD_OK;
 }

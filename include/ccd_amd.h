@@ -235,7 +235,9 @@ int ccd_batch_solve(ccd_ctx *ctx, double tol, int max_iters, double mixing,
 int ccd_batch_solve_mixed(ccd_ctx *ctx, double tol, double switch_tol,
                           int max_iters, double mixing, double *energies,
                           int cap, int *n_iter, int *n_fp32, int *stop);
-/* ccd_iterate_n for every solve at once: energies[q*n + k]. */
+/* ccd_iterate_n for every solve at once: energies[q*n + k].  A solve whose
+ * energy turns NaN stops there (its later entries are NaN) while the others
+ * go on; this is not an error for a batch. */
 int ccd_batch_iterate_n(ccd_ctx *ctx, int n, double mixing, double *energies);
 
 /* ---------------------------------------------------------------- model */

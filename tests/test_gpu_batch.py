@@ -126,3 +126,17 @@ def test_sweep64_batch(require_gpu):
             assert abs(e[-1] - e64[q]) / m.n_holes <= 1e-7, (q, cfg[q])
         else:                              # the FP64 solve diverges: so does the mixed one
             assert stop != "converged", (q, cfg[q])
+
+
+def test_batch_iterate_n_keeps_going_past_a_diverging_solve(require_gpu):
+    """Electron gas N=38 at r_s = 4.72 diverges under the reference's Jacobi
+    iteration (NaN after 18 iterations); in a batch it stops there and the
+    other solve is unaffected."""
+    div = Model.electron_gas(4.722872261101834, 25, 3)
+    ok = Model.pairing(10, 10, 0.45, 1.0)
+    en = Batch([div, ok], FP64).iterate_n(25)
+    e_div, stop, n = Context(div, FP64).solve(0.0, max_iters=30)
+    assert stop == "nan" and n < 25
+    assert np.array_equal(en[0, :n], e_div, equal_nan=True)
+    assert np.all(np.isnan(en[0, n:]))
+    assert np.array_equal(en[1], Context(ok, FP64).iterate_n(25))
