@@ -350,10 +350,10 @@ int join_aux(ccd_ctx *c) {
 // stream, the events bracketing ring + R remap | X | pppp ladder | update |
 // T remap + reductions.  Untimed: the independent phases run concurrently
 // on four streams, ordered by events --
-//   main: pppp ladder (1 tile/side)          | update
+//   main: pppp ladder (1 and 2 tiles)        | update
 //   aux1: ring, R remap                      | after the update: T remap (which
 //                                            |   the next iteration's ring follows)
-//   aux2: X, 1/2 T X, pppp ladder (2 tiles)  |
+//   aux2: X, 1/2 T X                         |
 //   aux3: pppp ladder (3+ tiles)             | after the update: reductions +
 //                                            |   energy (which the next update waits for)
 // (the pppp ladder does not need X; aux2 starts after the previous update).
@@ -382,12 +382,11 @@ int run_iteration(ccd_ctx *c, double mixing, hipEvent_t *ev) {
         // e[4] still holds the previous iteration's update (or nothing in the
         // first iteration of a batch, which fork_aux orders instead)
         HIP_TRY(hipStreamWaitEvent(s2, e[4], 0));
-        if ((rc = launch_ladder_v(c, 1, 1, s0))) return rc;
+        if ((rc = launch_ladder_x(c, s2)) || (rc = launch_ladder_tx(c, s2))) return rc;
+        HIP_TRY(hipEventRecord(e[2], s2));
         if ((rc = launch_ring(c, s1)) || (rc = launch_transpose(c, false, c->d_r, c->d_g, s1))) return rc;
         HIP_TRY(hipEventRecord(e[0], s1));
-        if ((rc = launch_ladder_x(c, s2)) || (rc = launch_ladder_tx(c, s2)) || (rc = launch_ladder_v(c, 2, 2, s2)))
-            return rc;
-        HIP_TRY(hipEventRecord(e[2], s2));
+        if ((rc = launch_ladder_v(c, 1, 2, s0))) return rc;
         if ((rc = launch_ladder_v(c, 3, NTMAX, s3))) return rc;
         HIP_TRY(hipEventRecord(e[3], s3));
         HIP_TRY(hipStreamWaitEvent(s0, e[0], 0));

@@ -42,7 +42,7 @@ struct UpdArgs {
 template <bool UPDATE>
 __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
     if (a.ctl->done) return;
-    __shared__ double w1s[kThreads], w2s[kThreads];
+    __shared__ double w1s[kThreads], w2s[kThreads], cs[kThreads];
     const DTile tl = a.tiles[blockIdx.x];
     const DChan ch = a.chans[tl.chan];
     const bool frozen = UPDATE && a.pctl[ch.prob].done;
@@ -112,25 +112,31 @@ __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
         }
         w1s[l] = w1;
         w2s[l] = w2;
+        cs[l] = w1 + w2;
     }
     __syncthreads();
-    // deterministic row sums (one thread per pair) and column partials (one per column)
-    if (l < tl.nrows) {
-        const int pr = tl.row0 + l;
-        double s1 = 0.0, s2 = 0.0;
-        for (int q = 0; q < H; q++) {
-            s1 += w1s[l * H + q];
-            s2 += w2s[l * H + q];
+    // deterministic pairwise trees: row sums of w1 and w2 over the columns of
+    // each pair (slot IJ = 0 ends with the sum) and column partials of
+    // w1 + w2 over the pairs of the tile (slot pr = 0); in step d a slot adds
+    // the one d further on, which does not change in that step
+    const int pq = l / H, cq = l - pq * H;
+    for (int d = 1; d < H || d < tl.nrows; d <<= 1) {
+        if (l < n) {
+            if ((cq & (2 * d - 1)) == 0 && cq + d < H) {
+                w1s[l] += w1s[l + d];
+                w2s[l] += w2s[l + d];
+            }
+            if ((pq & (2 * d - 1)) == 0 && pq + d < tl.nrows) cs[l] += cs[l + d * H];
         }
+        __syncthreads();
+    }
+    if (l < n && cq == 0) {
+        const int pr = tl.row0 + pq;
         const int r1 = a.prow[ch.pr_off + pr], r2 = a.prow_sw[ch.pr_off + pr];
-        a.red[a.rpos[ch.pp_off + r1]] = s1;
-        if (r1 != r2) a.red[a.rpos[ch.pp_off + r2]] = s2;
+        a.red[a.rpos[ch.pp_off + r1]] = w1s[l];
+        if (r1 != r2) a.red[a.rpos[ch.pp_off + r2]] = w2s[l];
     }
-    if (l < H) {
-        double s = 0.0;
-        for (int q = 0; q < tl.nrows; q++) s += w1s[q * H + l] + w2s[q * H + l];
-        a.red[a.cpos[tl.aux + l]] = s;
-    }
+    if (l < n && pq == 0) a.red[a.cpos[tl.aux + cq]] = cs[l];
 }
 
 // Per-particle transposition between the pphh and phhp layouts (both hold
