@@ -171,7 +171,11 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
                                                        const LoopCtl *__restrict__ ctl) {
     if (ctl->done) return;
     constexpr int KCH = LadderShape<NT>::KCH, KSEG = LadderShape<NT>::KSEG, G = KCH / 4;
-    constexpr int W = NT * 16, STR = W + 1, BPT = (KCH * W + kThreads - 1) / kThreads;
+    // LDS row stride of the B tile: an MFMA step reads rows kslot(kq, s) for
+    // the four lane groups kq; for FP32 the two groups of a half-wave then
+    // sit 16 banks apart (4 STR = 16 mod 32), so they do not conflict
+    constexpr int W = NT * 16, STR = sizeof(CT) == 8 ? W + 1 : W + 4;
+    constexpr int BPT = (KCH * W + kThreads - 1) / kThreads;
     constexpr bool GEN = SRC != LV_STORED;
     constexpr int VW = 16 / sizeof(CT);   // elements per 16-byte load of stored V
     typedef typename Mma<CT>::acc Acc;
