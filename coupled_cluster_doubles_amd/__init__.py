@@ -54,7 +54,7 @@ class Model:
 
     @classmethod
     def inf_matter(cls, density=0.08, n_shells=4, n_particle_shells=2, tz_max=1):
-        """Pure neutron matter with the Minnesota potential (host V chains)."""
+        """Pure neutron matter with the Minnesota potential (V generated on the device)."""
         h = C.c_void_p()
         check(lib().ccd_model_inf_matter(density, tz_max, n_shells, n_particle_shells, C.byref(h)),
               "ccd_model_inf_matter")

@@ -420,10 +420,13 @@ int validate(const ccd_problem *p) {
     if (p->gen_kind == CCD_GEN_NONE) {
         if (!p->v_hhhh || !p->v_hhpp || !p->v_pppp || !p->v_modA || !p->v_modB)
             return ccd::fail(CCD_E_INVALID, "gen_kind NONE needs all five V chains");
-    } else if (p->gen_kind == CCD_GEN_PAIRING || p->gen_kind == CCD_GEN_ELECTRON_GAS) {
+    } else if (p->gen_kind == CCD_GEN_PAIRING || p->gen_kind == CCD_GEN_ELECTRON_GAS ||
+               p->gen_kind == CCD_GEN_INF_MATTER) {
         if (!p->qn) return ccd::fail(CCD_E_INVALID, "generator needs qn");
         if (p->gen_kind == CCD_GEN_ELECTRON_GAS && (!p->gen_table || p->gen_nmax < 0))
             return ccd::fail(CCD_E_INVALID, "electron-gas generator needs gen_table and gen_nmax");
+        if (p->gen_kind == CCD_GEN_INF_MATTER && (!p->gen_table || !(p->gen_g > 0)))
+            return ccd::fail(CCD_E_INVALID, "infinite-matter generator needs gen_table (3 prefactors) and gen_g = L");
     } else {
         return ccd::fail(CCD_E_INVALID, "unknown gen_kind %d", p->gen_kind);
     }
@@ -516,6 +519,13 @@ int merge_problems(const ccd_problem *const *probs, int n, Merged &m) {
                             return ccd::fail(CCD_E_INVALID, "batch problem %d: state %d has |n| > gen_nmax", q, s);
                 const size_t w = 4 * (size_t)p.gen_nmax + 1;
                 m.table.insert(m.table.end(), p.gen_table, p.gen_table + w * w * w);
+            } else if (p.gen_kind == CCD_GEN_INF_MATTER) {
+                // qn[5 s + 4] becomes the solve index: one isospin for every
+                // state of a pure-neutron-matter problem, so the element is unchanged
+                for (int s = 1; s < p.n_sp; s++)
+                    if (p.qn[5 * s + 4] != p.qn[4])
+                        return ccd::fail(CCD_E_INVALID, "batch problem %d: infinite matter with two isospins", q);
+                m.table.insert(m.table.end(), p.gen_table, p.gen_table + 3);
             }
             bi.parts.push_back(gp);
         } else {
@@ -990,6 +1000,10 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
                         return ccd::fail(CCD_E_INVALID, "state %d has |n| > gen_nmax", s);
             const size_t w = 4 * (size_t)p->gen_nmax + 1;
             std::vector<double> tab(p->gen_table, p->gen_table + w * w * w);
+            if ((rc = upload(c, &c->d_table, tab))) return rc;
+            G.table = c->d_table;
+        } else if (p->gen_kind == CCD_GEN_INF_MATTER) {
+            std::vector<double> tab(p->gen_table, p->gen_table + 3);
             if ((rc = upload(c, &c->d_table, tab))) return rc;
             G.table = c->d_table;
         }

@@ -159,9 +159,8 @@ int main(int argc, char **argv) {
     const ccd_problem *prob = nullptr;
     ccd_model_problem(model, &prob);
     ccd_ctx *ctx = nullptr;
-    // -m 1 regenerates V^pppp inside the ladder; the infinite-matter model has
-    // no on-device generator, so it always runs from stored V
-    const int mode = o.precision | (o.saveMemory && o.basis != 1 ? CCD_MATRIX_FREE : 0);
+    // -m 1 regenerates V^pppp inside the ladder
+    const int mode = o.precision | (o.saveMemory ? CCD_MATRIX_FREE : 0);
     if ((rc = ccd_ctx_create(prob, mode, &ctx))) return die("ccd_ctx_create", rc);
     int64_t dbytes = 0;
     ccd_ctx_info(ctx, nullptr, &dbytes);

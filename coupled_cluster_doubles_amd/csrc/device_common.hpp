@@ -99,9 +99,39 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
     return s;
 }
 
+// Minnesota <ij|v|kl> of pure neutron matter, not antisymmetrised
+// (src/InfMatterSPBasis.cpp:361-449), in the host model's operation order
+// (no contraction); pref = the three prefactors +-V_0X / L^3 (pi/kappa_X)^1.5.
+__device__ __attribute__((noinline)) double minnesota_direct(const int32_t *qi, const int32_t *qj,
+                                                             const int32_t *qk, const int32_t *ql, double Lb,
+                                                             const double *pref) {
+#pragma clang fp contract(off)
+    if (qi[0] + qj[0] != qk[0] + ql[0] || qi[1] + qj[1] != qk[1] + ql[1] || qi[2] + qj[2] != qk[2] + ql[2])
+        return 0.0;
+    const double kappa_R = 1.487, kappa_T = 0.639, kappa_S = 0.465;
+    double qSquared = 0.;
+    for (int i = 0; i < 3; i++) {
+        const double ki = 2 * M_PI * qi[i] / Lb, kj = 2 * M_PI * qj[i] / Lb;
+        const double kk = 2 * M_PI * qk[i] / Lb, kl = 2 * M_PI * ql[i] / Lb;
+        const double bra = 0.5 * (ki - kj), ket = 0.5 * (kk - kl), tr = bra - ket;
+        qSquared += tr * tr;
+    }
+    const double V_R = pref[0] * exp(-qSquared / (4 * kappa_R));
+    const double V_T = pref[1] * exp(-qSquared / (4 * kappa_T));
+    const double V_S = pref[2] * exp(-qSquared / (4 * kappa_S));
+    const double spinEx = (qi[3] == ql[3] && qj[3] == qk[3]) ? 1 : 0;
+    const double isoSpinEx = (qi[4] == ql[4] && qj[4] == qk[4]) ? 1 : 0;
+    const double ks = (qi[3] == qk[3] ? 1 : 0) * (qj[3] == ql[3] ? 1 : 0);
+    const double kt = (qi[4] == qk[4] ? 1 : 0) * (qj[4] == ql[4] ? 1 : 0);
+    const double IsIt = ks * kt, PsIt = spinEx * kt, PsPt = spinEx * isoSpinEx, IsPt = ks * isoSpinEx;
+    return 0.5 * (V_R + 0.5 * V_T + 0.5 * V_S) * IsIt + 0.25 * (V_T - V_S) * PsIt -
+           0.5 * (V_R + 0.5 * V_T + 0.5 * V_S) * PsPt - 0.25 * (V_T - V_S) * IsPt;
+}
+
 // Antisymmetrised <pq||rs> generated from quantum numbers; bitwise equal to
-// the host model (pairing: src/PairingSPBasis.cpp:127-141; electron gas:
-// src/ElectronGasSPBasis.cpp:419-471 with the direct term tabulated).
+// the host model for the pairing model (src/PairingSPBasis.cpp:127-141) and
+// the electron gas (src/ElectronGasSPBasis.cpp:419-471 with the direct term
+// tabulated); infinite matter up to the device exp (src/InfMatterSPBasis.cpp:355-359).
 __device__ inline double gen_tbme(const Gen &G, int p, int q, int r, int s) {
     const int32_t *a = G.qn + 5 * p, *b = G.qn + 5 * q, *c = G.qn + 5 * r, *d = G.qn + 5 * s;
     int kind = G.kind, nmax = G.nmax;
@@ -111,6 +141,7 @@ __device__ inline double gen_tbme(const Gen &G, int p, int q, int r, int s) {
         const GenPart P = G.part[a[4]];
         kind = P.kind; nmax = P.nmax; g = P.g; table = G.table + P.tab_off;
     }
+    if (kind == CCD_GEN_INF_MATTER) return minnesota_direct(a, b, c, d, g, table) - minnesota_direct(a, b, d, c, g, table);
     if (kind == CCD_GEN_PAIRING) {
         double v = 0.0;
         if (a[0] == b[0] && c[0] == d[0] && a[1] == -b[1] && -c[1] == d[1]) {

@@ -56,10 +56,10 @@ unsigned shell_energy(int k) {
 
 }  // namespace
 
-constexpr int kInfMatter = 100;      // model kind without an on-device generator
+constexpr int kInfMatter = CCD_GEN_INF_MATTER;
 
 struct ccd_model {
-    int kind = 0;                    // CCD_GEN_PAIRING, CCD_GEN_ELECTRON_GAS or kInfMatter
+    int kind = 0;                    // CCD_GEN_PAIRING, CCD_GEN_ELECTRON_GAS or CCD_GEN_INF_MATTER
     int32_t nsp = 0, nh = 0;
     std::vector<int32_t> qn;         // 5 per state
     std::vector<double> eps;         // normal ordered
@@ -67,7 +67,8 @@ struct ccd_model {
     double g = 0.0, xi = 0.0, r_s = 0.0, L = 0.0, e_sq = 0.0;
     double density = 0.0, inf_L = 0.0;
     int32_t nmax = 0;
-    std::vector<double> table;       // electron gas direct term per momentum transfer
+    std::vector<double> table;       // electron gas direct term per momentum transfer;
+                                     // infinite matter: the 3 Minnesota prefactors
     int32_t nch = 0;
     std::vector<int32_t> pp_dim, hh_dim, ph_dim, hp_dim;
     std::vector<int32_t> pp_pairs, hh_pairs, ph_pairs, hp_pairs;
@@ -259,7 +260,7 @@ struct ccd_model {
         prob.pp_pairs = pp_pairs.data(); prob.hh_pairs = hh_pairs.data();
         prob.n_mod = nch; prob.ph_dim = ph_dim.data(); prob.hp_dim = hp_dim.data();
         prob.ph_pairs = ph_pairs.data(); prob.hp_pairs = hp_pairs.data();
-        prob.gen_kind = kind == kInfMatter ? CCD_GEN_NONE : kind; prob.qn = qn.data(); prob.gen_g = g;
+        prob.gen_kind = kind; prob.qn = qn.data(); prob.gen_g = kind == kInfMatter ? inf_L : g;
         prob.gen_nmax = nmax; prob.gen_table = table.empty() ? nullptr : table.data();
     }
 
@@ -417,13 +418,17 @@ int ccd_model_inf_matter(double density, int tz_max, int n_shells, int n_particl
     const int w = 4 * m->nmax + 1;
     m->reference_energy_and_normal_order();
     m->build_channels((size_t)w * w * w * 3);
-    m->fill_problem();
-    try {
-        m->build_host_v();
-    } catch (const std::bad_alloc &) {
-        delete m;
-        return ccd::fail(CCD_E_NOMEM, "ccd_model_inf_matter: out of host memory");
+    // the Minnesota prefactors of src/InfMatterSPBasis.cpp:412-414, for the
+    // on-device generator (the Gaussians are evaluated per element there)
+    {
+        const double Lb = m->inf_L;
+        const double V_0R = 200, V_0T = 178, V_0S = 91.85;
+        const double kappa_R = 1.487, kappa_T = 0.639, kappa_S = 0.465;
+        m->table = {V_0R / (Lb * Lb * Lb) * std::pow(M_PI / kappa_R, 1.5),
+                    -V_0T / (Lb * Lb * Lb) * std::pow(M_PI / kappa_T, 1.5),
+                    -V_0S / (Lb * Lb * Lb) * std::pow(M_PI / kappa_S, 1.5)};
     }
+    m->fill_problem();
     *out = m;
     return CCD_OK;
 }
@@ -454,7 +459,7 @@ int ccd_model_sizes(const ccd_model *m, int32_t *n_sp, int32_t *n_holes, int32_t
 
 int ccd_model_problem(ccd_model *m, const ccd_problem **out) {
     if (!m || !out) return ccd::fail(CCD_E_INVALID, "ccd_model_problem: null argument");
-    *out = m->kind == kInfMatter ? &m->prob_host : &m->prob;   // no device generator: host V
+    *out = &m->prob;
     return CCD_OK;
 }
 

@@ -63,6 +63,8 @@ extern "C" {
 #define CCD_GEN_NONE 0           /* V blocks supplied by the caller (v_* below) */
 #define CCD_GEN_PAIRING 1        /* generate <pq||rs> of the pairing model on device */
 #define CCD_GEN_ELECTRON_GAS 2   /* generate the antisymmetrised Coulomb element on device */
+#define CCD_GEN_INF_MATTER 3     /* generate the antisymmetrised Minnesota element of pure
+                                    neutron matter on device */
 
 /* Solve stop reasons (ccd_solve). */
 #define CCD_STOP_CONVERGED 0
@@ -96,11 +98,14 @@ typedef struct ccd_problem {
     /* On-device generation (gen_kind != CCD_GEN_NONE). */
     int32_t gen_kind;
     const int32_t *qn;      /* [n_sp*5] electron gas: nx ny nz sz tz; pairing: level spin - - - */
-    double gen_g;           /* pairing strength g */
+    double gen_g;           /* pairing strength g; infinite matter: the box length L */
     int32_t gen_nmax;       /* electron gas: max |n_x| of the basis */
     const double *gen_table;/* electron gas: direct Coulomb term per momentum transfer,
                                (4*gen_nmax+1)^3 entries, index
-                               ((dx+2m)*(4m+1) + (dy+2m))*(4m+1) + (dz+2m), m = gen_nmax */
+                               ((dx+2m)*(4m+1) + (dy+2m))*(4m+1) + (dz+2m), m = gen_nmax;
+                               infinite matter: the 3 Minnesota prefactors
+                               +-V_0X / L^3 * (pi / kappa_X)^1.5 for X = R, T, S
+                               (src/InfMatterSPBasis.cpp:412-414 up to the Gaussian) */
 } ccd_problem;
 
 typedef struct ccd_ctx ccd_ctx;
@@ -250,9 +255,9 @@ int ccd_model_electron_gas(double r_s, int n_shells, int n_particle_shells,
                            ccd_model **out);
 /* Infinite nuclear matter with the Minnesota potential, reference
  * InfMatterSPBasis(1, density, tzMax, nShells, nParticleShells); pure neutron
- * matter (tz_max = 1) only.  There is no on-device generator: the model's
- * descriptor carries host V chains (built here), so the matrix-free mode
- * does not apply. */
+ * matter (tz_max = 1) only.  The descriptor generates V on the device
+ * (CCD_GEN_INF_MATTER; the device exp can differ from the host's in the last
+ * bit, so its elements match the host V chains to about 1e-16 relative). */
 int ccd_model_inf_matter(double density, int tz_max, int n_shells,
                          int n_particle_shells, ccd_model **out);
 int ccd_model_destroy(ccd_model *m);

@@ -105,8 +105,9 @@ def test_driver_electron_gas_stored_and_matrix_free(require_gpu, tmp_path):
 
 
 @pytest.mark.gpu
-def test_driver_inf_matter_published_energy(require_gpu):
-    r = run("--infMatter", "-d", "0.08", "-z", "1", "-s", "4", "-p", "2", "-t", "1e-8")
+@pytest.mark.parametrize("save_memory", ["0", "1"])
+def test_driver_inf_matter_published_energy(require_gpu, save_memory):
+    r = run("--infMatter", "-d", "0.08", "-z", "1", "-s", "4", "-p", "2", "-t", "1e-8", "-m", save_memory)
     assert r.returncode == 0, r.stderr
     v = report(r.stdout)
     assert abs(float(v["E_CCD/A"]) - 10.155078) < 5e-7

@@ -21,8 +21,9 @@ pytestmark = pytest.mark.gpu
 
 def _small():
     cfg = [("pairing", (8, 8, 0.5, 1.0)), ("heg", (1.0, 19, 2)), ("pairing", (12, 12, 0.3, 1.0)),
-           ("heg", (3.0, 4, 2)), ("heg", (0.5, 19, 2))]
-    return cfg, [Model.pairing(*a) if k == "pairing" else Model.electron_gas(*a) for k, a in cfg]
+           ("heg", (3.0, 4, 2)), ("heg", (0.5, 19, 2)), ("infmatter", (0.08, 4, 2))]
+    make = {"pairing": Model.pairing, "heg": Model.electron_gas, "infmatter": Model.inf_matter}
+    return cfg, [make[k](*a) for k, a in cfg]
 
 
 def _same_as_single(models, results, batch, mbpt2=None, tol=1e-10):

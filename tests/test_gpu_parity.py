@@ -123,6 +123,19 @@ def test_host_v_matches_generated_v(require_gpu):
     assert np.array_equal(a, b) and ea == eb
 
 
+def test_inf_matter_generated_v_matches_host_v(require_gpu):
+    """The device Minnesota generator against the host chains (which equal the
+    oracle's bit for bit): the device exp may differ in the last bit."""
+    m = Model.inf_matter(0.08, 4, 2)
+    c_gen = Context(m, FP64)
+    c_host = Context(m, FP64, host_v=True)
+    assert abs(c_gen.mbpt2() - c_host.mbpt2()) <= 1e-14 * abs(c_host.mbpt2())
+    t = np.random.default_rng(5).uniform(-0.05, 0.05, m.n_t2)
+    a, ea = c_gen.step(t)
+    b, eb = c_host.step(t)
+    assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(b)) and abs(ea - eb) <= 1e-13 * abs(eb)
+
+
 def test_headline_n54_matches_oracle_two_iterations(require_gpu):
     m, o = _pair("heg", (1.0, 32, 4))
     ctx = Context(m, FP64)
@@ -159,11 +172,12 @@ def test_headline_n54_amplitudes_stay_antisymmetric(require_gpu):
     ("pairing", (8, 8, 0.5, 1.0)),       # generic per-element generator
     ("heg", (1.0, 19, 2)),               # electron-gas table generator
     ("heg", (1.0, 32, 4)),               # headline shape
+    ("infmatter", (0.08, 4, 2)),         # Minnesota generator
 ])
 def test_matrix_free_is_bit_identical_to_stored(require_gpu, kind, args):
     """-m 1 twin: V^pppp rebuilt inside the ladder gives the stored-V iterates
     bit for bit (same elements, same summation order)."""
-    m = Model.pairing(*args) if kind == "pairing" else Model.electron_gas(*args)
+    m = _pair(kind, args)[0]
     stored = Context(m, FP64)
     free = Context(m, FP64, matrix_free=True)
     assert free.device_bytes < stored.device_bytes or kind == "pairing"

@@ -103,9 +103,14 @@ def test_inf_matter_model_rejects_symmetric_matter():
 
 
 def test_batch_rejects_mixed_interaction_sources_before_touching_a_device():
+    import ctypes as C
     from coupled_cluster_doubles_amd import Batch
+    from coupled_cluster_doubles_amd._lib import check, lib
+    a, b = Model.pairing(8, 8, 0.5, 1.0), Model.inf_matter(0.08, 3, 2)
+    probs = (C.c_void_p * 2)(C.cast(a.problem(), C.c_void_p), C.cast(b.problem(host_v=True), C.c_void_p))
+    h = C.c_void_p()
     with pytest.raises(CCDError, match="generated V for every problem or for none"):
-        Batch([Model.pairing(8, 8, 0.5, 1.0), Model.inf_matter(0.08, 3, 2)])
+        check(lib().ccd_batch_create(probs, 2, 0, C.byref(h)), "ccd_batch_create")
     with pytest.raises(CCDError, match="no problems"):
         Batch([])
 
