@@ -90,6 +90,9 @@ struct ccd_ctx {
     int64_t n_cp = 0;            // column-partial slots
     std::vector<RingItem> ritems;  // ring work items, largest channels first
     int n_ring_wide = 0;           // leading items of channels with more than 5 tiles
+    std::vector<RingGroup> rgroups;  // staged-ring blocks: 1-4 tile channels, then 5 tile ones
+    int n_rg_five = 0;               // leading groups: the 5-tile channels
+    size_t rg_lds[2] = {0, 0};       // LDS of the narrow launch, FP64 / FP32
     int n_tt_narrow = 0;           // leading T X tiles of channels with hh <= 80
     size_t tr_lds = 0;
     int iter_prec = CCD_PREC_FP64;  // arithmetic of the contractions of the next iterations
@@ -122,6 +125,7 @@ struct ccd_ctx {
     double *d_table = nullptr;
     int2 *d_rec1 = nullptr, *d_rec2 = nullptr;  // generated-V pair records
     RingItem *d_ritems = nullptr;
+    RingGroup *d_rgroups = nullptr;
     float *d_vp32 = nullptr, *d_va32 = nullptr, *d_vb32 = nullptr;  // FP32 copies (mixed precision)
     double *d_gtab = nullptr;                   // generated-V electron gas table
     int64_t n_vp = 0, n_tp = 0, n_dp = 0, n_hhhh = 0, n_b = 0;
@@ -214,14 +218,60 @@ void launch_ring_ct(ccd_ctx *c, const CT *va, const CT *vb, int lo, int hi, hipS
                        va, vb, c->d_zero, c->d_r, c->d_ctl);
 }
 
+template <typename CT>
+size_t ring_stage_bytes(int nt) {
+    switch (nt) {
+        case 1: return RingStage<CT, 1>::BYTES;
+        case 2: return RingStage<CT, 2>::BYTES;
+        case 3: return RingStage<CT, 3>::BYTES;
+        case 4: return RingStage<CT, 4>::BYTES;
+        default: return RingStage<CT, 5>::BYTES;
+    }
+}
+
+// LDS-staged ring: one launch over the groups of 1-4 tile channels, one over
+// the 5-tile channels (whose register and LDS footprint would lower the
+// occupancy of the others)
+template <typename CT>
+void launch_ring_staged(ccd_ctx *c, const CT *va, const CT *vb, hipStream_t st) {
+    const int n5 = c->n_rg_five, n = (int)c->rgroups.size();
+    if (n5 > 0) {
+        const size_t lds = RingStage<CT, 5>::BYTES;
+        hipLaunchKernelGGL((k_ring_staged<CT, 5, 5>), dim3((unsigned)n5), dim3(kThreads), lds, st, c->d_rgroups,
+                           c->d_mods, c->d_tph, va, vb, c->d_zero, c->d_r, c->d_ctl);
+    }
+    if (n > n5) {
+        const size_t lds = c->rg_lds[sizeof(CT) == 8 ? 0 : 1];
+        hipLaunchKernelGGL((k_ring_staged<CT, 1, 4>), dim3((unsigned)(n - n5)), dim3(kThreads), lds, st,
+                           c->d_rgroups + n5, c->d_mods, c->d_tph, va, vb, c->d_zero, c->d_r, c->d_ctl);
+    }
+}
+
+// The staged instances that may use more than 64 KB of LDS need the limit raised.
+template <typename CT, int LO, int HI>
+int allow_ring_lds(size_t bytes) {
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_ring_staged<CT, LO, HI>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    return CCD_OK;
+}
+
+int allow_ring_lds_all() {
+    int rc;
+    if ((rc = allow_ring_lds<double, 1, 4>(RingStage<double, 4>::BYTES)) ||
+        (rc = allow_ring_lds<double, 5, 5>(RingStage<double, 5>::BYTES)) ||
+        (rc = allow_ring_lds<float, 5, 5>(RingStage<float, 5>::BYTES)))
+        return rc;
+    return CCD_OK;
+}
+
 int launch_ring(ccd_ctx *c, hipStream_t st) {
-    const int n = (int)c->ritems.size(), nw = c->n_ring_wide;
+    const int nw = c->n_ring_wide;
     if (c->iter_prec == CCD_PREC_FP32) {
         launch_ring_ct<float, true>(c, c->d_va32, c->d_vb32, 0, nw, st);
-        launch_ring_ct<float, false>(c, c->d_va32, c->d_vb32, nw, n, st);
+        launch_ring_staged<float>(c, c->d_va32, c->d_vb32, st);
     } else {
         launch_ring_ct<double, true>(c, c->d_va, c->d_vb, 0, nw, st);
-        launch_ring_ct<double, false>(c, c->d_va, c->d_vb, nw, n, st);
+        launch_ring_staged<double>(c, c->d_va, c->d_vb, st);
     }
     return check_launch("k_ring");
 }
@@ -814,6 +864,31 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
                          [&](const RingItem &a, const RingItem &b) { return tiles_of(a) > tiles_of(b); });
         c->n_ring_wide = 0;
         while (c->n_ring_wide < (int)c->ritems.size() && tiles_of(c->ritems[c->n_ring_wide]) > 5) c->n_ring_wide++;
+        // blocks of the LDS-staged kernel: channels of one tile count per
+        // block (4 of 1 tile, 2 of 2, else 1), more tiles first
+        c->rgroups.clear();
+        c->rg_lds[0] = c->rg_lds[1] = 0;
+        for (int nt = 5; nt >= 1; nt--) {
+            const int per = nt == 1 ? 4 : nt == 2 ? 2 : 1;
+            const size_t before = c->rgroups.size();
+            RingGroup g{{-1, -1, -1, -1}};
+            int fill = 0;
+            for (size_t k = 0; k < c->mods.size(); k++) {
+                if ((std::max(c->mods[k].ph, c->mods[k].hp) + 15) / 16 != nt) continue;
+                g.mod[fill++] = (int32_t)k;
+                if (fill == per) {
+                    c->rgroups.push_back(g);
+                    g = RingGroup{{-1, -1, -1, -1}};
+                    fill = 0;
+                }
+            }
+            if (fill) c->rgroups.push_back(g);
+            if (nt == 5) c->n_rg_five = (int)c->rgroups.size();   // the 5-tile groups lead
+            if (nt <= 4 && c->rgroups.size() > before) {
+                c->rg_lds[0] = std::max(c->rg_lds[0], ring_stage_bytes<double>(nt));
+                c->rg_lds[1] = std::max(c->rg_lds[1], ring_stage_bytes<float>(nt));
+            }
+        }
     }
 
     // ---- per-particle transposition plans between the two layouts:
@@ -948,6 +1023,7 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
         (rc = upload(c, &c->d_rtiles, c->rtiles)) || (rc = upload(c, &c->d_ltiles, c->ltiles)) ||
         (rc = upload(c, &c->d_utiles, c->utiles)) ||
         (rc = upload(c, &c->d_xtiles, c->xtiles)) || (rc = upload(c, &c->d_ttiles, c->ttiles)) || (rc = upload(c, &c->d_ritems, c->ritems)) ||
+        (rc = upload(c, &c->d_rgroups, c->rgroups)) ||
         (rc = upload(c, &c->d_ppl, ppl)) ||
         (rc = upload(c, &c->d_hhl, hhl)) || (rc = upload(c, &c->d_phl, phl)) || (rc = upload(c, &c->d_hpl, hpl)) ||
         (rc = upload(c, &c->d_hh_swap, hh_swap)) || (rc = upload(c, &c->d_prow, prow)) ||
@@ -1242,6 +1318,10 @@ int create_ctx(const ccd_problem *prob, const BatchInfo &bi, int precision, ccd_
             free_ctx(c);
             return ccd::fail(CCD_E_HIP, "could not create HIP events");
         }
+    if ((rc = allow_ring_lds_all())) {
+        free_ctx(c);
+        return rc;
+    }
     try {
         rc = build(c, prob, bi);
     } catch (const std::bad_alloc &) {

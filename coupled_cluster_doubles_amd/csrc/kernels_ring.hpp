@@ -89,6 +89,201 @@ __device__ __forceinline__ void ring_wave(const DMod &md, int jt, const double *
     }
 }
 
+// LDS-staged ring for mod channels with NT = ceil(max(ph, hp) / 16) <= 5.
+// A block of 4 waves takes a group of channels of one NT: four channels of
+// 1 tile (a wave each), two of 2 tiles (two waves each), or one of 3, 4 or 5
+// tiles (wave w computes the column tiles jt = w, w + 4).  V^A (hp x ph) and
+// T^ph (ph x hp) of every channel of the group are staged once in LDS with
+// coalesced loads (the per-wave kernel below reads both from global memory
+// once per column tile, 16 rows per load instruction); each column tile is
+// then computed as in ring_wave, with the same MFMA sequence, so both
+// kernels give identical results.  Staged blocks are D x D (D = 16 NT), zero
+// outside the channel's block.  Row strides: FP64 odd, so a 16-lane pass of
+// b64 reads down a column hits 16 distinct bank pairs; FP32 = 4 mod 32, so
+// the 16-byte row reads (4 consecutive k per lane) of 8 lanes and the 4-byte
+// column reads of a half-wave hit distinct banks.
+template <typename CT, int NT>
+struct RingStage {
+    static constexpr int D = 16 * NT;
+    static constexpr int STR = sizeof(CT) == 8 ? D + 1 : (D + 31) / 32 * 32 + 4;
+    static constexpr int G = NT == 1 ? 4 : NT == 2 ? 2 : 1;   // channels per block
+    static constexpr size_t CH_BYTES = 2ull * D * STR * sizeof(CT);
+    static constexpr size_t BYTES = G * CH_BYTES;
+};
+
+struct RingGroup {      // the mod channels of one block of the staged ring (-1: none)
+    int32_t mod[4];
+};
+
+// Column tile jt of a staged channel (As = V^A, Ts = T^ph in LDS; bv = V^B(:, jt)):
+// X(:, jt) = V^B(:, jt) + 1/2 V^A T(:, jt), then R(:, jt) = T X(:, jt), with the
+// k-slot of MFMA step j for lane group kq as in ring_wave.
+template <typename CT, int NT, int STR>
+__device__ __forceinline__ void ring_staged_tile(CT (*As)[STR], CT (*Ts)[STR], const DMod &my, int jt,
+                                                 const CT (&bv)[NT][4], double *__restrict__ r) {
+    typedef typename Mma<CT>::acc Acc;
+    constexpr bool F64 = sizeof(CT) == 8;
+    const int lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
+    const int PH = my.ph, HP = my.hp, col = jt * 16 + m;
+    const bool cok = col < HP;
+    Acc x[NT];
+#pragma unroll
+    for (int kt = 0; kt < NT; kt++) {
+        Acc acc = {};
+#pragma unroll
+        for (int pt = 0; pt < NT; pt++) {
+            if constexpr (F64) {
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    acc = mfma16(As[kt * 16 + m][pt * 16 + 4 * j + kq], Ts[pt * 16 + 4 * j + kq][col], acc);
+            } else {
+                const flt4 a4 = *reinterpret_cast<const flt4 *>(&As[kt * 16 + m][pt * 16 + 4 * kq]);
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc = mfma16(a4[j], Ts[pt * 16 + 4 * kq + j][col], acc);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) x[kt][q] = (CT)0.5 * acc[q] + bv[kt][q];
+    }
+#pragma unroll
+    for (int pt = 0; pt < NT; pt++) {
+        Acc acc = {};
+#pragma unroll
+        for (int kt = 0; kt < NT; kt++) {
+            if constexpr (F64) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc = mfma16(Ts[pt * 16 + m][kt * 16 + 4 * j + kq], x[kt][j], acc);
+            } else {
+                const flt4 a4 = *reinterpret_cast<const flt4 *>(&Ts[pt * 16 + m][kt * 16 + 4 * kq]);
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc = mfma16(a4[j], x[kt][j], acc);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int i = pt * 16 + Mma<CT>::row(q, lane);
+            if (i < PH && cok) r[my.off_ph + i * HP + col] = (double)acc[q];
+        }
+    }
+}
+
+template <typename CT, int NT>
+__device__ __forceinline__ void ring_staged_block(const RingGroup &grp, const DMod *__restrict__ mods,
+                                                  const double *__restrict__ tph, const CT *__restrict__ va,
+                                                  const CT *__restrict__ vb, const double *__restrict__ zero,
+                                                  double *__restrict__ r, unsigned char *lds) {
+    typedef RingStage<CT, NT> S;
+    typedef typename Mma<CT>::acc Acc;
+    constexpr int D = S::D, STR = S::STR, G = S::G;
+    constexpr bool F64 = sizeof(CT) == 8;
+    // the wave index is uniform across the wave: kept in scalar registers, so
+    // are the channel descriptor and the column-tile loop below
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
+    const CT *zc = reinterpret_cast<const CT *>(zero);
+    DMod md[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        const int k = grp.mod[g];
+        md[g] = k >= 0 ? mods[k] : DMod{0, 0, 0, 0, 0, 0};
+    }
+    // this wave's channel and first column tile
+    const int g0 = NT == 1 ? wave : NT == 2 ? wave >> 1 : 0;
+    const int jt0 = NT == 1 ? 0 : NT == 2 ? (wave & 1) : wave;
+    DMod my = md[0];
+#pragma unroll
+    for (int g = 1; g < G; g++)
+        if (g0 == g) my = md[g];
+    const int PH = my.ph, HP = my.hp;
+
+    // V^B(:, jt0), the initial value of X(:, jt0): issued first, consumed after the first product
+    CT bv[NT][4];
+    {
+        const CT *B = vb + my.off_b;
+        const int col = jt0 * 16 + m;
+#pragma unroll
+        for (int kt = 0; kt < NT; kt++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int i = kt * 16 + Mma<CT>::row(q, lane);
+                bv[kt][q] = *((i < HP && col < HP) ? B + i * HP + col : zc);
+            }
+    }
+
+    // staging in rounds of up to 8 elements per thread (every load of a round
+    // is issued before its LDS stores): element e = thread + 256 q of the group
+    // lies in channel e / D^2 = q / NT^2 (D^2 = 256 NT^2), a constant of the
+    // unrolled loops
+    constexpr int PER = G * D * D / kThreads;
+    static_assert(G * D * D % kThreads == 0, "whole staging rounds");
+    constexpr int RND = PER < 8 ? PER : 8;
+#pragma unroll
+    for (int q0 = 0; q0 < PER; q0 += RND) {
+        CT sa[RND];
+        double st[RND];
+#pragma unroll
+        for (int u = 0; u < RND; u++) {
+            const int q = q0 + u;
+            if (q < PER) {
+                const int g = q / (NT * NT), rem = threadIdx.x + (q - g * NT * NT) * kThreads;
+                const int row = rem / D, c = rem - row * D;
+                const DMod &h = md[g];
+                sa[u] = *((row < h.hp && c < h.ph) ? va + h.off_ph + row * h.ph + c : zc);
+                st[u] = *((row < h.ph && c < h.hp) ? tph + h.off_ph + row * h.hp + c : zero);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < RND; u++) {
+            const int q = q0 + u;
+            if (q < PER) {
+                const int g = q / (NT * NT), rem = threadIdx.x + (q - g * NT * NT) * kThreads;
+                const int row = rem / D, c = rem - row * D;
+                CT(*As)[STR] = reinterpret_cast<CT(*)[STR]>(lds + g * S::CH_BYTES);
+                CT(*Ts)[STR] = reinterpret_cast<CT(*)[STR]>(lds + g * S::CH_BYTES + sizeof(CT) * D * STR);
+                As[row][c] = sa[u];
+                Ts[row][c] = (CT)st[u];
+            }
+        }
+    }
+    __syncthreads();
+
+    CT(*As)[STR] = reinterpret_cast<CT(*)[STR]>(lds + g0 * S::CH_BYTES);
+    CT(*Ts)[STR] = reinterpret_cast<CT(*)[STR]>(lds + g0 * S::CH_BYTES + sizeof(CT) * D * STR);
+    if (jt0 * 16 < HP) ring_staged_tile<CT, NT, STR>(As, Ts, my, jt0, bv, r);
+    if (NT == 5 && wave == 0 && 4 * 16 < HP) {   // the fifth column tile
+        const CT *B = vb + my.off_b;
+        const int col = 4 * 16 + m;
+#pragma unroll
+        for (int kt = 0; kt < NT; kt++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int i = kt * 16 + Mma<CT>::row(q, lane);
+                bv[kt][q] = *((i < HP && col < HP) ? B + i * HP + col : zc);
+            }
+        ring_staged_tile<CT, NT, STR>(As, Ts, my, 4, bv, r);
+    }
+}
+
+// Groups of channels with LO..HI tiles (largest first), one group per block.
+template <typename CT, int LO, int HI>
+__global__ __launch_bounds__(kThreads) void k_ring_staged(const RingGroup *__restrict__ groups,
+                                                          const DMod *__restrict__ mods,
+                                                          const double *__restrict__ tph,
+                                                          const CT *__restrict__ va, const CT *__restrict__ vb,
+                                                          const double *__restrict__ zero,
+                                                          double *__restrict__ r,
+                                                          const LoopCtl *__restrict__ ctl) {
+    if (ctl->done) return;
+    extern __shared__ __align__(16) unsigned char ring_lds[];
+    const RingGroup grp = groups[blockIdx.x];
+    const DMod m0 = mods[grp.mod[0]];
+    const int nt = (max(m0.ph, m0.hp) + 15) / 16;
+    static_for<LO, HI + 1>([&](auto N) {
+        constexpr int NT = decltype(N)::value;
+        if (nt == NT) ring_staged_block<CT, NT>(grp, mods, tph, va, vb, zero, r, ring_lds);
+    });
+}
+
 // Items are sorted by tile count, largest first; wave w of block b takes item
 // 4b + w.  A channel with NT = ceil(max(ph, hp) / 16) has ceil(hp / 16) items.
 // WIDE instances take the items with NT = 6..8, the others NT = 1..5 (the

@@ -226,6 +226,27 @@ def test_fp32_diagrams_close_to_oracle(require_gpu):
     assert np.array_equal(ctx.ladder(ta), Context(m, FP64).ladder(ta))
 
 
+@pytest.mark.parametrize("kind,args", [
+    ("pairing", (12, 12, 0.5, 1.0)),      # mod channels of 3 and 5 column tiles
+    ("pairing", (10, 10, 0.5, 1.0)),      # 2 and 4 tiles
+    ("pairing", (16, 8, 0.5, 1.0)),       # 3 and 6 tiles
+    ("heg", (1.0, 10, 3)),                # 1 and 2 tiles, many channels
+    ("heg", (1.0, 32, 4)),                # headline shape: 1 to 4 tiles
+])
+def test_ring_every_tile_class(require_gpu, kind, args):
+    """calcT2Diff_phhp on random T^ph for every mod-channel width the
+    LDS-staged (1-5 tiles) and per-wave (6-8 tiles) ring kernels take, in
+    FP64 (1e-12) and FP32 (2e-5, relative to the largest element)."""
+    from coupled_cluster_doubles_amd import FP32
+    m, o = _pair(kind, args)
+    tph = np.random.default_rng(5).uniform(-0.05, 0.05, m.n_t2)
+    r_o = o.ring(tph)
+    r64 = Context(m, FP64).ring(tph)
+    assert np.max(np.abs(r64 - r_o)) <= A_REL * np.max(np.abs(r_o))
+    r32 = Context(m, FP32).ring(tph)
+    assert np.max(np.abs(r32 - r_o)) <= F32_REL * np.max(np.abs(r_o))
+
+
 @pytest.mark.parametrize("free", [False, True])
 def test_fp32_stored_and_generated_v_agree_bitwise(require_gpu, free):
     from coupled_cluster_doubles_amd import FP32
