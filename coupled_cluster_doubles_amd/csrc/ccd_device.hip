@@ -215,7 +215,7 @@ void launch_ring_ct(ccd_ctx *c, const CT *va, const CT *vb, int lo, int hi, hipS
     if (n <= 0) return;
     const dim3 grid((unsigned)((n + kThreads / 64 - 1) / (kThreads / 64)));
     hipLaunchKernelGGL((k_ring<CT, WIDE>), grid, dim3(kThreads), 0, st, c->d_ritems + lo, n, c->d_mods, c->d_tph,
-                       va, vb, c->d_zero, c->d_r, c->d_ctl);
+                       va, vb, c->d_zero, reinterpret_cast<CT *>(c->d_r), c->d_ctl);
 }
 
 template <typename CT>
@@ -238,12 +238,13 @@ void launch_ring_staged(ccd_ctx *c, const CT *va, const CT *vb, hipStream_t st) 
     if (n5 > 0) {
         const size_t lds = RingStage<CT, 5>::BYTES;
         hipLaunchKernelGGL((k_ring_staged<CT, 5, 5>), dim3((unsigned)n5), dim3(kThreads), lds, st, c->d_rgroups,
-                           c->d_mods, c->d_tph, va, vb, c->d_zero, c->d_r, c->d_ctl);
+                           c->d_mods, c->d_tph, va, vb, c->d_zero, reinterpret_cast<CT *>(c->d_r), c->d_ctl);
     }
     if (n > n5) {
         const size_t lds = c->rg_lds[sizeof(CT) == 8 ? 0 : 1];
         hipLaunchKernelGGL((k_ring_staged<CT, 1, 4>), dim3((unsigned)(n - n5)), dim3(kThreads), lds, st,
-                           c->d_rgroups + n5, c->d_mods, c->d_tph, va, vb, c->d_zero, c->d_r, c->d_ctl);
+                           c->d_rgroups + n5, c->d_mods, c->d_tph, va, vb, c->d_zero,
+                           reinterpret_cast<CT *>(c->d_r), c->d_ctl);
     }
 }
 
@@ -276,15 +277,28 @@ int launch_ring(ccd_ctx *c, hipStream_t st) {
     return check_launch("k_ring");
 }
 
-// pphh -> phhp (T to T^ph) when to_ph, else phhp -> pphh (R to G)
+// pphh -> phhp: the amplitudes T to T^ph (FP64)
 int launch_transpose(ccd_ctx *c, bool to_ph, const double *src, double *dst, hipStream_t st) {
     if (c->np == 0 || c->nt2 == 0) return CCD_OK;
     if (to_ph)
-        hipLaunchKernelGGL(k_transpose, dim3((unsigned)c->np), dim3(kThreads), c->tr_lds, st, c->d_tr_base,
+        hipLaunchKernelGGL(k_transpose<double>, dim3((unsigned)c->np), dim3(kThreads), c->tr_lds, st, c->d_tr_base,
                            c->d_addr_pp, c->d_addr_ph, c->d_perm_ph, src, dst, c->d_ctl);
     else
-        hipLaunchKernelGGL(k_transpose, dim3((unsigned)c->np), dim3(kThreads), c->tr_lds, st, c->d_tr_base,
+        hipLaunchKernelGGL(k_transpose<double>, dim3((unsigned)c->np), dim3(kThreads), c->tr_lds, st, c->d_tr_base,
                            c->d_addr_ph, c->d_addr_pp, c->d_perm_pp, src, dst, c->d_ctl);
+    return check_launch("k_transpose");
+}
+
+// phhp -> pphh: the ring output R to G, in the contractions' arithmetic type
+int launch_remap_r(ccd_ctx *c, hipStream_t st) {
+    if (c->np == 0 || c->nt2 == 0) return CCD_OK;
+    if (c->iter_prec == CCD_PREC_FP32)
+        hipLaunchKernelGGL(k_transpose<float>, dim3((unsigned)c->np), dim3(kThreads), c->tr_lds, st, c->d_tr_base,
+                           c->d_addr_ph, c->d_addr_pp, c->d_perm_pp, reinterpret_cast<const float *>(c->d_r),
+                           reinterpret_cast<float *>(c->d_g), c->d_ctl);
+    else
+        hipLaunchKernelGGL(k_transpose<double>, dim3((unsigned)c->np), dim3(kThreads), c->tr_lds, st, c->d_tr_base,
+                           c->d_addr_ph, c->d_addr_pp, c->d_perm_pp, c->d_r, c->d_g, c->d_ctl);
     return check_launch("k_transpose");
 }
 
@@ -312,7 +326,7 @@ void launch_ladder_v_nt(ccd_ctx *c, const CT *vp, hipStream_t st) {
     const size_t lds = SRC == LV_GAS ? sizeof(double) * (size_t)c->lgen.ntab : 0;
     hipLaunchKernelGGL((k_ladder_v<CT, NT, SRC>), dim3((unsigned)grid), dim3(kThreads), lds, st,
                        c->d_ltiles + c->lt_off[NT - 1], n, c->d_chans, vp, c->lgen, c->d_tas, c->d_x,
-                       c->d_zero, c->d_dp, c->d_ctl);
+                       c->d_zero, reinterpret_cast<CT *>(c->d_dp), c->d_ctl);
 }
 
 // NT classes [lo, hi] of the pppp ladder
@@ -342,7 +356,7 @@ void launch_ladder_tx_ct(ccd_ctx *c, int lo, int hi, hipStream_t st) {
     if (n <= 0) return;
     const dim3 grid((unsigned)((n + kThreads / 64 - 1) / (kThreads / 64)));
     hipLaunchKernelGGL((k_ladder_tx<CT, WIDE>), grid, dim3(kThreads), 0, st, c->d_ttiles + lo, n, c->d_chans,
-                       c->d_tas, c->d_x, c->d_zero, c->d_dp, c->d_ctl);
+                       c->d_tas, c->d_x, c->d_zero, reinterpret_cast<CT *>(c->d_dp), c->d_ctl);
 }
 
 int launch_ladder_tx(ccd_ctx *c, hipStream_t st) {
@@ -368,7 +382,10 @@ template <bool UPDATE>
 int launch_update(ccd_ctx *c, const double *t_old, double *t_new, double mixing, hipStream_t st) {
     if (c->utiles.empty()) return CCD_OK;
     UpdArgs a = upd_args(c, t_old, t_new, mixing);
-    hipLaunchKernelGGL(k_update<UPDATE>, dim3((unsigned)c->utiles.size()), dim3(kThreads), 0, st, a);
+    if (UPDATE && c->iter_prec == CCD_PREC_FP32)
+        hipLaunchKernelGGL((k_update<UPDATE, float>), dim3((unsigned)c->utiles.size()), dim3(kThreads), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_update<UPDATE, double>), dim3((unsigned)c->utiles.size()), dim3(kThreads), 0, st, a);
     return check_launch("k_update");
 }
 
@@ -416,7 +433,7 @@ int run_iteration(ccd_ctx *c, double mixing, hipEvent_t *ev) {
     if (ev) {
         const hipStream_t st = c->stream;
         HIP_TRY(hipEventRecord(ev[0], st));
-        if ((rc = launch_ring(c, st)) || (rc = launch_transpose(c, false, c->d_r, c->d_g, st))) return rc;
+        if ((rc = launch_ring(c, st)) || (rc = launch_remap_r(c, st))) return rc;
         HIP_TRY(hipEventRecord(ev[1], st));
         if ((rc = launch_ladder_x(c, st))) return rc;
         HIP_TRY(hipEventRecord(ev[2], st));
@@ -434,7 +451,7 @@ int run_iteration(ccd_ctx *c, double mixing, hipEvent_t *ev) {
         HIP_TRY(hipStreamWaitEvent(s2, e[4], 0));
         if ((rc = launch_ladder_x(c, s2)) || (rc = launch_ladder_tx(c, s2))) return rc;
         HIP_TRY(hipEventRecord(e[2], s2));
-        if ((rc = launch_ring(c, s1)) || (rc = launch_transpose(c, false, c->d_r, c->d_g, s1))) return rc;
+        if ((rc = launch_ring(c, s1)) || (rc = launch_remap_r(c, s1))) return rc;
         HIP_TRY(hipEventRecord(e[0], s1));
         if ((rc = launch_ladder_v(c, 1, 2, s0))) return rc;
         if ((rc = launch_ladder_v(c, 3, NTMAX, s3))) return rc;
@@ -1653,7 +1670,12 @@ int ccd_ladder(ccd_ctx *c, const double *t, double *d) {
         return rc;
     if (!c->utiles.empty()) {
         UpdArgs a = upd_args(c, nullptr, nullptr, 0.0);
-        hipLaunchKernelGGL(k_expand_d, dim3((unsigned)c->utiles.size()), dim3(kThreads), 0, c->stream, a, c->d_g);
+        if (c->iter_prec == CCD_PREC_FP32)
+            hipLaunchKernelGGL(k_expand_d<float>, dim3((unsigned)c->utiles.size()), dim3(kThreads), 0, c->stream, a,
+                               c->d_g);
+        else
+            hipLaunchKernelGGL(k_expand_d<double>, dim3((unsigned)c->utiles.size()), dim3(kThreads), 0, c->stream, a,
+                               c->d_g);
         if ((rc = check_launch("k_expand_d"))) return rc;
     }
     HIP_TRY(hipMemcpyAsync(d, c->d_g, sizeof(double) * c->nt2, hipMemcpyDeviceToHost, c->stream));
@@ -1668,6 +1690,13 @@ int ccd_ring(ccd_ctx *c, const double *tph, double *r) {
     HIP_TRY(hipMemcpyAsync(c->d_tph, tph, sizeof(double) * c->nt2, hipMemcpyHostToDevice, c->stream));
     int rc = launch_ring(c, c->stream);
     if (rc) return rc;
+    if (c->iter_prec == CCD_PREC_FP32) {   // R in FP32 (exactly the FP32 sums)
+        std::vector<float> r32((size_t)c->nt2);
+        HIP_TRY(hipMemcpyAsync(r32.data(), c->d_r, sizeof(float) * c->nt2, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        for (int64_t k = 0; k < c->nt2; k++) r[k] = r32[k];
+        return CCD_OK;
+    }
     HIP_TRY(hipMemcpyAsync(r, c->d_r, sizeof(double) * c->nt2, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return CCD_OK;

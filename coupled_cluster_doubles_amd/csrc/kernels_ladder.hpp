@@ -142,7 +142,8 @@ __device__ __forceinline__ void static_for(F &&f) {
 //
 // Arithmetic type CT: double, or float for the mixed-precision iterations
 // (V^pppp stored in FP32, or generated in FP64 and rounded; amplitudes and X
-// rounded on load; FP32 accumulation; D_p written in FP64).  For float the
+// rounded on load; FP32 accumulation; D_p written in FP32, which holds the
+// FP32 sums exactly).  For float the
 // k-slot order is k = 16 (s/4) + 4q + s%4, again 64 contiguous bytes per row
 // and load instruction.
 template <typename CT>
@@ -167,7 +168,7 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
                                                        const double *__restrict__ tas,
                                                        const double *__restrict__ x,
                                                        const double *__restrict__ zero,
-                                                       double *__restrict__ dp,
+                                                       CT *__restrict__ dp,
                                                        const LoopCtl *__restrict__ ctl) {
     if (ctl->done) return;
     constexpr int KCH = LadderShape<NT>::KCH, KSEG = LadderShape<NT>::KSEG, G = KCH / 4;
@@ -315,7 +316,7 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
             for (int q = 0; q < 4; q++) {
                 const int row = wave * 16 + Mma<CT>::row(q, lane), col = n * 16 + m;
                 if (row < tl.nrows && col < H)
-                    dp[ch.off_dp + (int64_t)seg * npk * H + (int64_t)(tl.row0 + row) * H + col] = (double)acc[n][q];
+                    dp[ch.off_dp + (int64_t)seg * npk * H + (int64_t)(tl.row0 + row) * H + col] = acc[n][q];
             }
         }
     }
@@ -329,7 +330,7 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
 template <typename CT, int NT>
 __device__ __forceinline__ void ladder_tx_wave(const DTile &tl, const DChan &ch, const double *__restrict__ tas,
                                                const double *__restrict__ x, const double *__restrict__ zero,
-                                               double *__restrict__ dp) {
+                                               CT *__restrict__ dp) {
     typedef typename Mma<CT>::acc Acc;
     const int lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
     const int H = ch.hh;
@@ -354,13 +355,13 @@ __device__ __forceinline__ void ladder_tx_wave(const DTile &tl, const DChan &ch,
             }
         }
     }
-    double *out = dp + ch.off_dp + (int64_t)ch.nseg * ch.npk * H;
+    CT *out = dp + ch.off_dp + (int64_t)ch.nseg * ch.npk * H;
 #pragma unroll
     for (int n = 0; n < NT; n++) {
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const int row = Mma<CT>::row(q, lane), col = n * 16 + m;
-            if (row < tl.nrows && col < H) out[(int64_t)(tl.row0 + row) * H + col] = (double)acc[n][q];
+            if (row < tl.nrows && col < H) out[(int64_t)(tl.row0 + row) * H + col] = acc[n][q];
         }
     }
 }
@@ -372,7 +373,7 @@ __global__ __launch_bounds__(kThreads) void k_ladder_tx(const DTile *__restrict_
                                                         const DChan *__restrict__ chans,
                                                         const double *__restrict__ tas,
                                                         const double *__restrict__ x,
-                                                        const double *__restrict__ zero, double *__restrict__ dp,
+                                                        const double *__restrict__ zero, CT *__restrict__ dp,
                                                         const LoopCtl *__restrict__ ctl) {
     if (ctl->done) return;
     const int it = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);

@@ -9,7 +9,7 @@
 //
 // Arithmetic type CT: double (v_mfma_f64_16x16x4f64) or float
 // (v_mfma_f32_16x16x4f32, V^A and V^B stored in FP32, T^ph rounded on load);
-// R is written in FP64 either way.  Every load is unconditional: elements
+// R is written in CT (FP32 holds the FP32 sums exactly).  Every load is unconditional: elements
 // outside the block read the zero buffer.
 #pragma once
 
@@ -23,7 +23,7 @@ struct RingItem {       // one wave's work: mod channel and 16-column tile
 template <typename CT, int NT>
 __device__ __forceinline__ void ring_wave(const DMod &md, int jt, const double *__restrict__ tph,
                                           const CT *__restrict__ va, const CT *__restrict__ vb,
-                                          const double *__restrict__ zero, double *__restrict__ r) {
+                                          const double *__restrict__ zero, CT *__restrict__ r) {
     typedef typename Mma<CT>::acc Acc;
     constexpr bool F64 = sizeof(CT) == 8;
     const int lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
@@ -84,7 +84,7 @@ __device__ __forceinline__ void ring_wave(const DMod &md, int jt, const double *
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const int i = pt * 16 + Mma<CT>::row(q, lane);
-            if (i < PH && cok) r[md.off_ph + i * HP + col] = (double)acc[q];
+            if (i < PH && cok) r[md.off_ph + i * HP + col] = acc[q];
         }
     }
 }
@@ -120,7 +120,7 @@ struct RingGroup {      // the mod channels of one block of the staged ring (-1:
 // k-slot of MFMA step j for lane group kq as in ring_wave.
 template <typename CT, int NT, int STR>
 __device__ __forceinline__ void ring_staged_tile(CT (*As)[STR], CT (*Ts)[STR], const DMod &my, int jt,
-                                                 const CT (&bv)[NT][4], double *__restrict__ r) {
+                                                 const CT (&bv)[NT][4], CT *__restrict__ r) {
     typedef typename Mma<CT>::acc Acc;
     constexpr bool F64 = sizeof(CT) == 8;
     const int lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
@@ -162,7 +162,7 @@ __device__ __forceinline__ void ring_staged_tile(CT (*As)[STR], CT (*Ts)[STR], c
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const int i = pt * 16 + Mma<CT>::row(q, lane);
-            if (i < PH && cok) r[my.off_ph + i * HP + col] = (double)acc[q];
+            if (i < PH && cok) r[my.off_ph + i * HP + col] = acc[q];
         }
     }
 }
@@ -171,7 +171,7 @@ template <typename CT, int NT>
 __device__ __forceinline__ void ring_staged_block(const RingGroup &grp, const DMod *__restrict__ mods,
                                                   const double *__restrict__ tph, const CT *__restrict__ va,
                                                   const CT *__restrict__ vb, const double *__restrict__ zero,
-                                                  double *__restrict__ r, unsigned char *lds) {
+                                                  CT *__restrict__ r, unsigned char *lds) {
     typedef RingStage<CT, NT> S;
     typedef typename Mma<CT>::acc Acc;
     constexpr int D = S::D, STR = S::STR, G = S::G;
@@ -271,7 +271,7 @@ __global__ __launch_bounds__(kThreads) void k_ring_staged(const RingGroup *__res
                                                           const double *__restrict__ tph,
                                                           const CT *__restrict__ va, const CT *__restrict__ vb,
                                                           const double *__restrict__ zero,
-                                                          double *__restrict__ r,
+                                                          CT *__restrict__ r,
                                                           const LoopCtl *__restrict__ ctl) {
     if (ctl->done) return;
     extern __shared__ __align__(16) unsigned char ring_lds[];
@@ -294,7 +294,7 @@ __global__ __launch_bounds__(kThreads) void k_ring(const RingItem *__restrict__ 
                                                    const double *__restrict__ tph,
                                                    const CT *__restrict__ va, const CT *__restrict__ vb,
                                                    const double *__restrict__ zero,
-                                                   double *__restrict__ r,
+                                                   CT *__restrict__ r,
                                                    const LoopCtl *__restrict__ ctl) {
     if (ctl->done) return;
     const int it = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);

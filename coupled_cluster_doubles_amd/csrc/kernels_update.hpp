@@ -22,8 +22,9 @@ struct UpdArgs {
     const int32_t *hh_swap;          // index of (j,i) within the channel
     const double *eps, *chi, *psi;   // chi [nh], psi [np]
     const double *vt;                // <ij||ab> in the pphh layout
-    const double *dp;                // ladder D on packed rows: one slab per k segment, then 1/2 T X
-    const double *g;                 // ring R gathered to the pphh layout: g[(a,b),(i,j)] = R[(a,i),(j,b)]
+    const void *dp;                  // ladder D on packed rows: one slab per k segment, then 1/2 T X
+    const void *g;                   // ring R gathered to the pphh layout: g[(a,b),(i,j)] = R[(a,i),(j,b)]
+                                     // (both in the contractions' arithmetic type)
     const double *t_old;
     double *t_new, *tas;             // new amplitudes; their (a,b)-antisymmetric packed rows
     double *red;                     // W = V o T partial sums in reduction-list order:
@@ -38,10 +39,12 @@ struct UpdArgs {
 // UPDATE = true: one Jacobi step for both rows of each pair; UPDATE = false:
 // only the W sums of the current amplitudes (to derive chi, psi, E from them).
 // A solve of a batch that has stopped keeps its amplitudes: its new T is the
-// old one.
-template <bool UPDATE>
+// old one.  OT: type of the ladder and ring outputs (float after FP32
+// contractions, which they hold exactly); the update itself runs in FP64.
+template <bool UPDATE, typename OT>
 __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
     if (a.ctl->done) return;
+    const OT *gg = static_cast<const OT *>(a.g);
     __shared__ double w1s[kThreads], w2s[kThreads], cs[kThreads];
     const DTile tl = a.tiles[blockIdx.x];
     const DChan ch = a.chans[tl.chan];
@@ -61,12 +64,12 @@ __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
             const int2 ab = a.ppl[ch.pp_off + r1], ij = a.hhl[ch.hh_off + IJ];
             const int ia = ab.x, ib = ab.y, ii = ij.x, jj = ij.y;
             const double t1s = a.t_old[b1 + JI], t2s = a.t_old[b2 + JI];
-            const double g1 = a.g[b1 + IJ], g1s = a.g[b1 + JI], g2 = a.g[b2 + IJ], g2s = a.g[b2 + JI];
+            const double g1 = gg[b1 + IJ], g1s = gg[b1 + JI], g2 = gg[b2 + IJ], g2s = gg[b2 + JI];
             double d = 0.0;
             if (!diag) {
                 const int64_t slab = (int64_t)ch.npk * H;
-                const double *dq = a.dp + ch.off_dp + (int64_t)pr * H + IJ;
-                for (int q = 0; q <= ch.nseg; q++) d += dq[q * slab];
+                const OT *dq = static_cast<const OT *>(a.dp) + ch.off_dp + (int64_t)pr * H + IJ;
+                for (int q = 0; q <= ch.nseg; q++) d += (double)dq[q * slab];
             }
             const double ci = 0.5 * a.chi[ii], cj = 0.5 * a.chi[jj];
             const double pa = -0.5 * a.psi[ia - a.nh], pb = -0.5 * a.psi[ib - a.nh];
@@ -147,15 +150,17 @@ __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
 // dst[dst_addr[base+l]] from LDS slot perm[base+l].  With src = pphh it is the
 // reference's mapToDistributed(ai_jb); with src = phhp it gathers R into the
 // pphh layout.
+template <typename E>
 __global__ __launch_bounds__(kThreads) void k_transpose(const int32_t *__restrict__ base,
                                                         const int32_t *__restrict__ src_addr,
                                                         const int32_t *__restrict__ dst_addr,
                                                         const uint16_t *__restrict__ perm,
-                                                        const double *__restrict__ src,
-                                                        double *__restrict__ dst,
+                                                        const E *__restrict__ src,
+                                                        E *__restrict__ dst,
                                                         const LoopCtl *__restrict__ ctl) {
     if (ctl->done) return;
-    extern __shared__ double buf[];
+    extern __shared__ double tr_lds[];
+    E *buf = reinterpret_cast<E *>(tr_lds);
     const int a = blockIdx.x;
     const int b0 = base[a], n = base[a + 1] - b0;
     for (int l = threadIdx.x; l < n; l += kThreads) buf[l] = src[src_addr[b0 + l]];
@@ -289,6 +294,7 @@ __global__ __launch_bounds__(kThreads) void k_antisym(const DTile *__restrict__ 
 }
 
 // D in the ordered pphh layout from the packed ladder output (ladder entry point).
+template <typename OT>
 __global__ __launch_bounds__(kThreads) void k_expand_d(UpdArgs a, double *__restrict__ d) {
     const DTile tl = a.tiles[blockIdx.x];
     const DChan ch = a.chans[tl.chan];
@@ -300,9 +306,9 @@ __global__ __launch_bounds__(kThreads) void k_expand_d(UpdArgs a, double *__rest
         d[ch.off_t + (int64_t)r1 * H + IJ] = 0.0;
     } else {
         const int64_t slab = (int64_t)ch.npk * H;
-        const double *dq = a.dp + ch.off_dp + (int64_t)pr * H + IJ;
+        const OT *dq = static_cast<const OT *>(a.dp) + ch.off_dp + (int64_t)pr * H + IJ;
         double v = 0.0;
-        for (int q = 0; q <= ch.nseg; q++) v += dq[q * slab];
+        for (int q = 0; q <= ch.nseg; q++) v += (double)dq[q * slab];
         d[ch.off_t + (int64_t)r1 * H + IJ] = v;
         d[ch.off_t + (int64_t)r2 * H + IJ] = -v;
     }
