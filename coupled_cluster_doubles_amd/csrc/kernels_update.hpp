@@ -163,9 +163,42 @@ __global__ __launch_bounds__(kThreads) void k_transpose(const int32_t *__restric
     E *buf = reinterpret_cast<E *>(tr_lds);
     const int a = blockIdx.x;
     const int b0 = base[a], n = base[a + 1] - b0;
-    for (int l = threadIdx.x; l < n; l += kThreads) buf[l] = src[src_addr[b0 + l]];
+    // rounds of U slots per thread: every address load of a round is issued,
+    // then every data load, so a thread waits for two memory round trips per
+    // round instead of two per slot (a masked-off slot reads element 0 of its
+    // array and discards it)
+    constexpr int U = 8;
+    for (int l0 = threadIdx.x; l0 < n; l0 += kThreads * U) {
+        int ad[U];
+        E v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int l = l0 + u * kThreads;
+            ad[u] = src_addr[b0 + (l < n ? l : 0)];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) v[u] = src[ad[u]];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int l = l0 + u * kThreads;
+            if (l < n) buf[l] = v[u];
+        }
+    }
     __syncthreads();
-    for (int l = threadIdx.x; l < n; l += kThreads) dst[dst_addr[b0 + l]] = buf[perm[b0 + l]];
+    for (int l0 = threadIdx.x; l0 < n; l0 += kThreads * U) {
+        int ad[U], pm[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int l = l0 + u * kThreads, q = b0 + (l < n ? l : 0);
+            ad[u] = dst_addr[q];
+            pm[u] = perm[q];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int l = l0 + u * kThreads;
+            if (l < n) dst[ad[u]] = buf[pm[u]];
+        }
+    }
 }
 
 // chi_i (blocks 0..nh-1) and psi_a (blocks nh..nsp-1) as fixed-order sums of
