@@ -1515,14 +1515,15 @@ int ccd_work(const ccd_ctx *ctx, double *flops, double *flops_pppp, double *ladd
     if (flops) *flops = ctx->flops;
     if (flops_pppp) *flops_pppp = ctx->flops_pppp;
     if (ladder_bytes) {
-        // one k_ladder_v launch reads the packed V^pppp (unless generated), the
-        // antisymmetrised amplitudes and X once and writes D_p once
-        // (V^pppp in the arithmetic type of the iteration, the rest in FP64)
+        // one pass of the pppp ladder reads the packed V^pppp (unless
+        // generated), the antisymmetrised amplitudes and X once and writes D_p
+        // once (V^pppp and D_p in the arithmetic type of the iteration,
+        // amplitudes and X in FP64)
         const double vb = ctx->iter_prec == CCD_PREC_FP64 ? 8.0 : 4.0;
         double b = 0.0;
         for (const DChan &ch : ctx->chans)
-            b += (ctx->matrix_free ? 0.0 : vb * ch.npk * ch.npk) +
-                 8.0 * (2.0 * ch.npk * ch.hh + (double)ch.hh * ch.hh);
+            b += (ctx->matrix_free ? 0.0 : vb * ch.npk * ch.npk) + vb * ch.npk * ch.hh +
+                 8.0 * ((double)ch.npk * ch.hh + (double)ch.hh * ch.hh);
         *ladder_bytes = b;
     }
     return CCD_OK;
