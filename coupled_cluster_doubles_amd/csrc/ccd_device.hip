@@ -105,6 +105,9 @@ struct ccd_ctx {
     int32_t *d_red_ptr = nullptr, *d_cpos = nullptr, *d_rpos = nullptr;
     int64_t n_red = 0;
     int32_t *d_tr_base = nullptr, *d_addr_pp = nullptr, *d_addr_ph = nullptr;
+    // internal row order of the pphh layout (see build): element maps to and
+    // from the caller's order, and the caller's row of each internal pp row
+    int32_t *d_t2_in = nullptr, *d_t2_out = nullptr, *d_pp_old = nullptr;
     uint16_t *d_perm_ph = nullptr, *d_perm_pp = nullptr;
     // device state and interaction
     double *d_eps = nullptr, *d_onebody = nullptr;  // chi [nh] then psi [np]
@@ -470,6 +473,27 @@ int run_iteration(ccd_ctx *c, double mixing, hipEvent_t *ev) {
     return CCD_OK;
 }
 
+// Caller amplitudes (the reference's pphh order) to an internal-order device
+// array, through a device scratch array of nt2 doubles; and back.
+int t2_upload(ccd_ctx *c, const double *host, double *dst, double *scratch) {
+    HIP_TRY(hipMemcpyAsync(scratch, host, sizeof(double) * c->nt2, hipMemcpyHostToDevice, c->stream));
+    if (c->nt2 > 0) {
+        const unsigned g = (unsigned)std::min<int64_t>((c->nt2 + kThreads - 1) / kThreads, 8192);
+        hipLaunchKernelGGL(k_gather, dim3(g), dim3(kThreads), 0, c->stream, scratch, c->d_t2_in, dst, c->nt2);
+    }
+    return check_launch("k_gather");
+}
+
+int t2_download(ccd_ctx *c, const double *src, double *host, double *scratch) {
+    if (c->nt2 > 0) {
+        const unsigned g = (unsigned)std::min<int64_t>((c->nt2 + kThreads - 1) / kThreads, 8192);
+        hipLaunchKernelGGL(k_gather, dim3(g), dim3(kThreads), 0, c->stream, src, c->d_t2_out, scratch, c->nt2);
+    }
+    if (int rc = check_launch("k_gather")) return rc;
+    HIP_TRY(hipMemcpyAsync(host, scratch, sizeof(double) * c->nt2, hipMemcpyDeviceToHost, c->stream));
+    return CCD_OK;
+}
+
 int read_scalar(ccd_ctx *c, double *out) {
     HIP_TRY(hipMemcpyAsync(c->h_scalar, c->d_scalar, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -763,6 +787,10 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
     c->flops = 0; c->flops_pppp = 0;
     c->t2_lo.assign(nprob + 1, 0);
     std::unordered_map<int64_t, int32_t> pos;
+    std::vector<int32_t> order, t2_in, t2_out, pp_old(npp);
+    std::vector<int2> seg;
+    for (int64_t k = 0, o = 0; k < p->n_chan; k++)
+        for (int q = 0; q < p->pp_dim[k]; q++) pp_old[o++] = q;
     int prob = 0;
     for (int k = 0; k < p->n_chan; k++) {
         while (prob + 1 < nprob && k >= bi.chan_lo[prob + 1]) c->t2_lo[++prob] = off_t;
@@ -772,6 +800,34 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
             pos.clear();
             for (int q = 0; q < P; q++) pos[(int64_t)ppl[ipp + q].x * nsp + ppl[ipp + q].y] = q;
             if ((int)pos.size() != P) return ccd::fail(CCD_E_INVALID, "pp channel %d lists a pair twice", k);
+            // internal row order: each pair (a,b) with a<b, in the caller's
+            // order, directly followed by its partner (b,a); then the pairs
+            // (a,a).  The update reads both rows of a pair, so its loads cover
+            // one contiguous run of rows.
+            order.clear();
+            for (int q = 0; q < P; q++)
+                if (ppl[ipp + q].x < ppl[ipp + q].y) {
+                    auto it = pos.find((int64_t)ppl[ipp + q].y * nsp + ppl[ipp + q].x);
+                    if (it == pos.end()) return ccd::fail(CCD_E_INVALID, "pp channel %d lacks a swapped pair", k);
+                    order.push_back(q);
+                    order.push_back(it->second);
+                }
+            for (int q = 0; q < P; q++)
+                if (ppl[ipp + q].x == ppl[ipp + q].y) order.push_back(q);
+            if ((int)order.size() != P) return ccd::fail(CCD_E_INVALID, "pp channel %d lacks a swapped pair", k);
+            seg.assign(ppl.begin() + ipp, ppl.begin() + ipp + P);
+            t2_in.resize((size_t)(off_t + (int64_t)P * H));
+            t2_out.resize((size_t)(off_t + (int64_t)P * H));
+            for (int q = 0; q < P; q++) {
+                ppl[ipp + q] = seg[order[q]];
+                pp_old[ipp + q] = order[q];
+                for (int ij = 0; ij < H; ij++) {
+                    t2_in[off_t + (int64_t)q * H + ij] = (int32_t)(off_t + (int64_t)order[q] * H + ij);
+                    t2_out[off_t + (int64_t)order[q] * H + ij] = (int32_t)(off_t + (int64_t)q * H + ij);
+                }
+            }
+            pos.clear();
+            for (int q = 0; q < P; q++) pos[(int64_t)ppl[ipp + q].x * nsp + ppl[ipp + q].y] = q;
             for (int q = 0; q < P; q++) {
                 auto it = pos.find((int64_t)ppl[ipp + q].y * nsp + ppl[ipp + q].x);
                 if (it == pos.end()) return ccd::fail(CCD_E_INVALID, "pp channel %d lacks a swapped pair", k);
@@ -1041,6 +1097,8 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
         (rc = upload(c, &c->d_utiles, c->utiles)) ||
         (rc = upload(c, &c->d_xtiles, c->xtiles)) || (rc = upload(c, &c->d_ttiles, c->ttiles)) || (rc = upload(c, &c->d_ritems, c->ritems)) ||
         (rc = upload(c, &c->d_rgroups, c->rgroups)) ||
+        (rc = upload(c, &c->d_t2_in, t2_in)) || (rc = upload(c, &c->d_t2_out, t2_out)) ||
+        (rc = upload(c, &c->d_pp_old, pp_old)) ||
         (rc = upload(c, &c->d_ppl, ppl)) ||
         (rc = upload(c, &c->d_hhl, hhl)) || (rc = upload(c, &c->d_phl, phl)) || (rc = upload(c, &c->d_hpl, hpl)) ||
         (rc = upload(c, &c->d_hh_swap, hh_swap)) || (rc = upload(c, &c->d_prow, prow)) ||
@@ -1128,20 +1186,22 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
             const DChan &ch = c->chans[ci++];
             // The packed ladder needs V^pppp antisymmetric in both pairs (exactly,
             // as calcTBME produces it); check, then keep the a<b, c<d block.
+            // (V is in the caller's row order; r, q, rs and prow are internal rows)
             const double *V = p->v_pppp + o_pppp[k];
             const int P = ch.pp;
+            const int32_t *old = pp_old.data() + ch.pp_off;
             for (int r = 0; r < P; r++) {
                 const int rs = pp_swap[ch.pp_off + r];
                 for (int q = 0; q < P; q++) {
-                    const double v = V[(size_t)r * P + q];
-                    if (V[(size_t)rs * P + q] != -v || V[(size_t)r * P + pp_swap[ch.pp_off + q]] != -v)
+                    const double v = V[(size_t)old[r] * P + old[q]];
+                    if (V[(size_t)old[rs] * P + old[q]] != -v || V[(size_t)old[r] * P + old[pp_swap[ch.pp_off + q]]] != -v)
                         return ccd::fail(CCD_E_INVALID, "V^pppp of channel %d is not antisymmetric", k);
                 }
             }
             blkv.assign((size_t)ch.npk * ch.ldv, 0.0);
             for (int u = 0; u < ch.npk; u++)
                 for (int w = 0; w < ch.npk; w++)
-                    blkv[(size_t)u * ch.ldv + w] = V[(size_t)prow[ch.pr_off + u] * P + prow[ch.pr_off + w]];
+                    blkv[(size_t)u * ch.ldv + w] = V[(size_t)old[prow[ch.pr_off + u]] * P + old[prow[ch.pr_off + w]]];
             HIP_TRY(hipMemcpy(c->d_vp + ch.off_vp, blkv.data(), sizeof(double) * blkv.size(), hipMemcpyHostToDevice));
             HIP_TRY(hipMemcpy(c->d_vh + ch.off_hhhh, p->v_hhhh + o_hhhh[k], sizeof(double) * ch.hh * (size_t)ch.hh,
                               hipMemcpyHostToDevice));
@@ -1150,7 +1210,7 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
         HIP_TRY(hipMemcpy(c->d_g, p->v_hhpp, sizeof(double) * nt, hipMemcpyHostToDevice));
         if (!c->chans.empty())
             hipLaunchKernelGGL(k_transpose_hhpp, dim3((unsigned)c->chans.size()), dim3(kThreads), 0, c->stream,
-                               c->d_chans, c->d_g, c->d_vt);
+                               c->d_chans, c->d_pp_old, c->d_g, c->d_vt);
         if ((rc = check_launch("k_transpose_hhpp"))) return rc;
         HIP_TRY(hipMemcpy(c->d_va, p->v_modA, sizeof(double) * nt, hipMemcpyHostToDevice));
         int mi = 0;
@@ -1289,8 +1349,8 @@ int zero_state(ccd_ctx *c) {
 int set_state(ccd_ctx *c, const double *t2) {
     if (int rc0 = arm_free_run(c)) return rc0;
     double *t = c->d_t[c->cur];
-    HIP_TRY(hipMemcpyAsync(t, t2, sizeof(double) * c->nt2, hipMemcpyHostToDevice, c->stream));
     int rc;
+    if ((rc = t2_upload(c, t2, t, c->d_g))) return rc;
     if ((rc = launch_transpose(c, true, t, c->d_tph, c->stream)) || (rc = launch_antisym(c, t)) ||
         (rc = launch_update<false>(c, t, nullptr, 0.0, c->stream)) || (rc = launch_reduce(c, c->stream)))
         return rc;
@@ -1645,7 +1705,7 @@ int ccd_iterate_n(ccd_ctx *c, int n, double mixing, double *energies) {
 int ccd_get_t2(ccd_ctx *c, double *t2) {
     ccd::clear_error();
     if (!c || !t2) return ccd::fail(CCD_E_INVALID, "ccd_get_t2: null argument");
-    HIP_TRY(hipMemcpyAsync(t2, c->d_t[c->cur], sizeof(double) * c->nt2, hipMemcpyDeviceToHost, c->stream));
+    if (int rc = t2_download(c, c->d_t[c->cur], t2, c->d_g)) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
     return CCD_OK;
 }
@@ -1664,8 +1724,8 @@ int ccd_ladder(ccd_ctx *c, const double *t, double *d) {
     if (!c || !t || !d) return ccd::fail(CCD_E_INVALID, "ccd_ladder: null argument");    if (int rc0 = arm_free_run(c)) return rc0;
 
     double *dt = c->d_t[1 - c->cur];  // scratch: the non-current T buffer
-    HIP_TRY(hipMemcpyAsync(dt, t, sizeof(double) * c->nt2, hipMemcpyHostToDevice, c->stream));
     int rc;
+    if ((rc = t2_upload(c, t, dt, c->d_g))) return rc;
     if ((rc = launch_antisym(c, dt)) || (rc = launch_ladder_x(c, c->stream)) ||
         (rc = launch_ladder_tx(c, c->stream)) || (rc = launch_ladder_v(c, 1, NTMAX, c->stream)))
         return rc;
@@ -1679,7 +1739,7 @@ int ccd_ladder(ccd_ctx *c, const double *t, double *d) {
                                c->d_g);
         if ((rc = check_launch("k_expand_d"))) return rc;
     }
-    HIP_TRY(hipMemcpyAsync(d, c->d_g, sizeof(double) * c->nt2, hipMemcpyDeviceToHost, c->stream));
+    if ((rc = t2_download(c, c->d_g, d, c->d_r))) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
     return CCD_OK;
 }
@@ -1708,8 +1768,8 @@ int ccd_remap_ph(ccd_ctx *c, const double *t, double *tph) {
     if (!c || !t || !tph) return ccd::fail(CCD_E_INVALID, "ccd_remap_ph: null argument");    if (int rc0 = arm_free_run(c)) return rc0;
 
     double *dt = c->d_t[1 - c->cur];
-    HIP_TRY(hipMemcpyAsync(dt, t, sizeof(double) * c->nt2, hipMemcpyHostToDevice, c->stream));
-    int rc = launch_transpose(c, true, dt, c->d_tph, c->stream);
+    int rc = t2_upload(c, t, dt, c->d_g);
+    if (!rc) rc = launch_transpose(c, true, dt, c->d_tph, c->stream);
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(tph, c->d_tph, sizeof(double) * c->nt2, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1736,7 +1796,7 @@ int ccd_step(ccd_ctx *c, const double *t_old, double *t_new, double mixing, doub
     if ((rc = fork_aux(c)) || (rc = run_iteration(c, mixing, nullptr)) || (rc = join_aux(c))) return rc;
     double e = 0.0;
     if ((rc = read_scalar(c, &e))) return rc;
-    HIP_TRY(hipMemcpyAsync(t_new, c->d_t[c->cur], sizeof(double) * c->nt2, hipMemcpyDeviceToHost, c->stream));
+    if ((rc = t2_download(c, c->d_t[c->cur], t_new, c->d_g))) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (corr) *corr = e;
     return CCD_OK;

@@ -71,14 +71,22 @@ __global__ __launch_bounds__(kThreads) void k_gen_mod(const DMod *__restrict__ m
 
 // Caller V^hhpp (hh x pp) -> pphh layout.
 __global__ __launch_bounds__(kThreads) void k_transpose_hhpp(const DChan *__restrict__ chans,
+                                                             const int32_t *__restrict__ pp_old,
                                                              const double *__restrict__ vhhpp,
                                                              double *__restrict__ vt) {
     const DChan ch = chans[blockIdx.x];
     const int P = ch.pp, H = ch.hh;
     for (int l = threadIdx.x; l < P * H; l += kThreads) {
-        const int AB = l / H, IJ = l - AB * H;
-        vt[ch.off_t + l] = vhhpp[ch.off_t + (int64_t)IJ * P + AB];
+        const int AB = l / H, IJ = l - AB * H;   // internal row AB is the caller's row pp_old[AB]
+        vt[ch.off_t + l] = vhhpp[ch.off_t + (int64_t)IJ * P + pp_old[ch.pp_off + AB]];
     }
+}
+
+// dst[i] = src[idx[i]]: amplitudes between the caller's and the internal row order.
+__global__ __launch_bounds__(kThreads) void k_gather(const double *__restrict__ src, const int32_t *__restrict__ idx,
+                                                     double *__restrict__ dst, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+        dst[i] = src[idx[i]];
 }
 
 // Packed rows of V^hhpp^T: vtp[u][kl] = vt[(a_u,b_u)][kl] (tile = packed rows).
