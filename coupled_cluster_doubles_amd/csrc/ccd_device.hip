@@ -96,6 +96,7 @@ struct ccd_ctx {
     int n_tt_narrow = 0;           // leading T X tiles of channels with hh <= 80
     size_t tr_lds = 0;
     int iter_prec = CCD_PREC_FP64;  // arithmetic of the contractions of the next iterations
+    int tph_prec = CCD_PREC_FP64;   // element type T^ph was last written in (the ring reads it in iter_prec)
     // device tables
     DChan *d_chans = nullptr;
     DMod *d_mods = nullptr;
@@ -217,8 +218,8 @@ void launch_ring_ct(ccd_ctx *c, const CT *va, const CT *vb, int lo, int hi, hipS
     const int n = hi - lo;
     if (n <= 0) return;
     const dim3 grid((unsigned)((n + kThreads / 64 - 1) / (kThreads / 64)));
-    hipLaunchKernelGGL((k_ring<CT, WIDE>), grid, dim3(kThreads), 0, st, c->d_ritems + lo, n, c->d_mods, c->d_tph,
-                       va, vb, c->d_zero, reinterpret_cast<CT *>(c->d_r), c->d_ctl);
+    hipLaunchKernelGGL((k_ring<CT, WIDE>), grid, dim3(kThreads), 0, st, c->d_ritems + lo, n, c->d_mods,
+                       reinterpret_cast<const CT *>(c->d_tph), va, vb, c->d_zero, reinterpret_cast<CT *>(c->d_r), c->d_ctl);
 }
 
 template <typename CT>
@@ -241,12 +242,13 @@ void launch_ring_staged(ccd_ctx *c, const CT *va, const CT *vb, hipStream_t st) 
     if (n5 > 0) {
         const size_t lds = RingStage<CT, 5>::BYTES;
         hipLaunchKernelGGL((k_ring_staged<CT, 5, 5>), dim3((unsigned)n5), dim3(kThreads), lds, st, c->d_rgroups,
-                           c->d_mods, c->d_tph, va, vb, c->d_zero, reinterpret_cast<CT *>(c->d_r), c->d_ctl);
+                           c->d_mods, reinterpret_cast<const CT *>(c->d_tph), va, vb, c->d_zero,
+                           reinterpret_cast<CT *>(c->d_r), c->d_ctl);
     }
     if (n > n5) {
         const size_t lds = c->rg_lds[sizeof(CT) == 8 ? 0 : 1];
         hipLaunchKernelGGL((k_ring_staged<CT, 1, 4>), dim3((unsigned)(n - n5)), dim3(kThreads), lds, st,
-                           c->d_rgroups + n5, c->d_mods, c->d_tph, va, vb, c->d_zero,
+                           c->d_rgroups + n5, c->d_mods, reinterpret_cast<const CT *>(c->d_tph), va, vb, c->d_zero,
                            reinterpret_cast<CT *>(c->d_r), c->d_ctl);
     }
 }
@@ -280,7 +282,7 @@ int launch_ring(ccd_ctx *c, hipStream_t st) {
     return check_launch("k_ring");
 }
 
-// pphh -> phhp: the amplitudes T to T^ph (FP64)
+// pphh -> phhp: amplitudes to T^ph in FP64 (caller arrays of the per-diagram entry points)
 int launch_transpose(ccd_ctx *c, bool to_ph, const double *src, double *dst, hipStream_t st) {
     if (c->np == 0 || c->nt2 == 0) return CCD_OK;
     if (to_ph)
@@ -290,6 +292,32 @@ int launch_transpose(ccd_ctx *c, bool to_ph, const double *src, double *dst, hip
         hipLaunchKernelGGL(k_transpose<double>, dim3((unsigned)c->np), dim3(kThreads), c->tr_lds, st, c->d_tr_base,
                            c->d_addr_ph, c->d_addr_pp, c->d_perm_pp, src, dst, c->d_ctl);
     return check_launch("k_transpose");
+}
+
+// The T remap of an iteration: amplitudes t to the context's T^ph, in the
+// arithmetic type of the contractions (FP32 holds the rounding the FP32 ring
+// applies anyway)
+// force: run even when the loop has stopped (the zero buffer stands in for a
+// loop state that is not done)
+int launch_tremap(ccd_ctx *c, const double *t, hipStream_t st, bool force = false) {
+    c->tph_prec = c->iter_prec;
+    if (c->np == 0 || c->nt2 == 0) return CCD_OK;
+    const LoopCtl *ctl = force ? reinterpret_cast<const LoopCtl *>(c->d_zero) : c->d_ctl;
+    if (c->iter_prec == CCD_PREC_FP32)
+        hipLaunchKernelGGL((k_transpose<double, float>), dim3((unsigned)c->np), dim3(kThreads), c->tr_lds, st,
+                           c->d_tr_base, c->d_addr_pp, c->d_addr_ph, c->d_perm_ph, t,
+                           reinterpret_cast<float *>(c->d_tph), ctl);
+    else
+        hipLaunchKernelGGL(k_transpose<double>, dim3((unsigned)c->np), dim3(kThreads), c->tr_lds, st, c->d_tr_base,
+                           c->d_addr_pp, c->d_addr_ph, c->d_perm_ph, t, c->d_tph, ctl);
+    return check_launch("k_transpose");
+}
+
+// Rebuild T^ph from the current amplitudes when its type is not the one the
+// next iterations read (after a change of precision).
+int sync_tph(ccd_ctx *c) {
+    if (c->tph_prec == c->iter_prec) return CCD_OK;
+    return launch_tremap(c, c->d_t[c->cur], c->stream, true);
 }
 
 // phhp -> pphh: the ring output R to G, in the contractions' arithmetic type
@@ -444,7 +472,7 @@ int run_iteration(ccd_ctx *c, double mixing, hipEvent_t *ev) {
         HIP_TRY(hipEventRecord(ev[3], st));
         if ((rc = launch_update<true>(c, t_old, t_new, mixing, st))) return rc;
         HIP_TRY(hipEventRecord(ev[4], st));
-        if ((rc = launch_transpose(c, true, t_new, c->d_tph, st)) || (rc = launch_reduce(c, st))) return rc;
+        if ((rc = launch_tremap(c, t_new, st)) || (rc = launch_reduce(c, st))) return rc;
         HIP_TRY(hipEventRecord(ev[5], st));
     } else {
         const hipStream_t s0 = c->stream, s1 = c->aux[0], s2 = c->aux[1], s3 = c->aux[2];
@@ -465,7 +493,7 @@ int run_iteration(ccd_ctx *c, double mixing, hipEvent_t *ev) {
         if ((rc = launch_update<true>(c, t_old, t_new, mixing, s0))) return rc;
         HIP_TRY(hipEventRecord(e[4], s0));
         HIP_TRY(hipStreamWaitEvent(s1, e[4], 0));
-        if ((rc = launch_transpose(c, true, t_new, c->d_tph, s1))) return rc;
+        if ((rc = launch_tremap(c, t_new, s1))) return rc;
         HIP_TRY(hipStreamWaitEvent(s3, e[4], 0));
         if ((rc = launch_reduce(c, s3))) return rc;
     }
@@ -1297,7 +1325,10 @@ int read_pctl(ccd_ctx *c) {
     return CCD_OK;
 }
 
-int arm_free_run(ccd_ctx *c) { return arm_loop(c, -1.0, INT_MAX, true); }
+int arm_free_run(ccd_ctx *c) {
+    int rc = arm_loop(c, -1.0, INT_MAX, true);
+    return rc ? rc : sync_tph(c);   // T^ph in the type the next iterations read
+}
 
 // Queue iterations in batches without waiting for each energy; after each
 // batch read the loop state.  Iterations queued past the stopping point do
@@ -1337,7 +1368,8 @@ int zero_state(ccd_ctx *c) {
     const size_t nt = sizeof(double) * (size_t)c->nt2;
     HIP_TRY(hipMemsetAsync(c->d_t[0], 0, nt, c->stream));
     HIP_TRY(hipMemsetAsync(c->d_t[1], 0, nt, c->stream));
-    HIP_TRY(hipMemsetAsync(c->d_tph, 0, nt, c->stream));
+    HIP_TRY(hipMemsetAsync(c->d_tph, 0, nt, c->stream));   // zero in either element type
+    c->tph_prec = c->iter_prec;
     HIP_TRY(hipMemsetAsync(c->d_tas, 0, sizeof(double) * (size_t)std::max<int64_t>(c->n_tp, 1), c->stream));
     HIP_TRY(hipMemsetAsync(c->d_onebody, 0, sizeof(double) * c->nsp, c->stream));
     c->cur = 0;
@@ -1351,7 +1383,7 @@ int set_state(ccd_ctx *c, const double *t2) {
     double *t = c->d_t[c->cur];
     int rc;
     if ((rc = t2_upload(c, t2, t, c->d_g))) return rc;
-    if ((rc = launch_transpose(c, true, t, c->d_tph, c->stream)) || (rc = launch_antisym(c, t)) ||
+    if ((rc = launch_tremap(c, t, c->stream)) || (rc = launch_antisym(c, t)) ||
         (rc = launch_update<false>(c, t, nullptr, 0.0, c->stream)) || (rc = launch_reduce(c, c->stream)))
         return rc;
     return CCD_OK;
@@ -1509,6 +1541,7 @@ int ccd_batch_solve_mixed(ccd_ctx *c, double tol, double switch_tol, int max_ite
     std::vector<LoopCtl> phase1(c->nprob);
     int rc = zero_state(c);
     c->iter_prec = CCD_PREC_FP32;
+    c->tph_prec = CCD_PREC_FP32;   // T^ph is zero: the same in either type
     if (!rc) rc = arm_loop(c, std::max(tol, switch_tol), max_iters, true);
     if (!rc) rc = run_loop(c, mixing, kSolveBatch);
     if (!rc) rc = read_pctl(c);
@@ -1517,11 +1550,14 @@ int ccd_batch_solve_mixed(ccd_ctx *c, double tol, double switch_tol, int max_ite
     if (!rc && switch_tol > tol) {
         c->iter_prec = CCD_PREC_FP64;
         rc = arm_loop(c, tol, max_iters, false);
+        if (!rc) rc = sync_tph(c);
         if (!rc) rc = run_loop(c, mixing, kSolveBatch);
         if (!rc) rc = read_pctl(c);
     }
     c->iter_prec = saved;
+    const int rs = sync_tph(c);
     if (rc) return rc;
+    if (rs) return rs;
     for (int q = 0; q < c->nprob; q++) {
         const LoopCtl st = c->nprob == 1 ? *c->h_ctl : c->h_pctl[q];
         if ((rc = copy_log(c, energies ? energies + (size_t)q * cap : nullptr, cap, st.count, q))) return rc;
@@ -1651,7 +1687,7 @@ int ccd_set_precision(ccd_ctx *c, int precision) {
     if (precision == CCD_PREC_FP32 && c->precision != CCD_PREC_FP32)
         return ccd::fail(CCD_E_STATE, "FP32 iterations need a context created with CCD_PREC_FP32");
     c->iter_prec = precision;
-    return CCD_OK;
+    return sync_tph(c);
 }
 
 int ccd_solve_mixed(ccd_ctx *c, double tol, double switch_tol, int max_iters, double mixing, double *energies,
@@ -1666,6 +1702,7 @@ int ccd_solve_mixed(ccd_ctx *c, double tol, double switch_tol, int max_iters, do
     int rc = zero_state(c);
     // FP32 contractions while |dE| > max(tol, switch_tol) ...
     c->iter_prec = CCD_PREC_FP32;
+    c->tph_prec = CCD_PREC_FP32;   // T^ph is zero: the same in either type
     if (!rc) rc = arm_loop(c, std::max(tol, switch_tol), max_iters, true);
     if (!rc) rc = run_loop(c, mixing, kSolveBatch);
     int n32 = rc ? 0 : c->h_ctl->count;
@@ -1673,10 +1710,13 @@ int ccd_solve_mixed(ccd_ctx *c, double tol, double switch_tol, int max_iters, do
     if (!rc && c->h_ctl->reason == CCD_STOP_CONVERGED && switch_tol > tol) {
         c->iter_prec = CCD_PREC_FP64;
         rc = arm_loop(c, tol, max_iters, false);
+        if (!rc) rc = sync_tph(c);
         if (!rc) rc = run_loop(c, mixing, kSolveBatch);
     }
     c->iter_prec = saved;
+    const int rs = sync_tph(c);
     if (rc) return rc;
+    if (rs) return rs;
     const LoopCtl st = *c->h_ctl;
     if ((rc = copy_log(c, energies, cap, st.count))) return rc;
     if (n_iter) *n_iter = st.count;
@@ -1748,7 +1788,18 @@ int ccd_ring(ccd_ctx *c, const double *tph, double *r) {
     ccd::clear_error();
     if (!c || !tph || !r) return ccd::fail(CCD_E_INVALID, "ccd_ring: null argument");    if (int rc0 = arm_free_run(c)) return rc0;
 
-    HIP_TRY(hipMemcpyAsync(c->d_tph, tph, sizeof(double) * c->nt2, hipMemcpyHostToDevice, c->stream));
+    if (c->iter_prec == CCD_PREC_FP32) {   // T^ph in FP32, as the FP32 iterations hold it
+        HIP_TRY(hipMemcpyAsync(c->d_g, tph, sizeof(double) * c->nt2, hipMemcpyHostToDevice, c->stream));
+        if (c->nt2 > 0) {
+            const unsigned g = (unsigned)std::min<int64_t>((c->nt2 + kThreads - 1) / kThreads, 4096);
+            hipLaunchKernelGGL(k_to_f32, dim3(g), dim3(kThreads), 0, c->stream, c->d_g,
+                               reinterpret_cast<float *>(c->d_tph), c->nt2);
+            if (int rc1 = check_launch("k_to_f32")) return rc1;
+        }
+    } else {
+        HIP_TRY(hipMemcpyAsync(c->d_tph, tph, sizeof(double) * c->nt2, hipMemcpyHostToDevice, c->stream));
+    }
+    c->tph_prec = c->iter_prec;
     int rc = launch_ring(c, c->stream);
     if (rc) return rc;
     if (c->iter_prec == CCD_PREC_FP32) {   // R in FP32 (exactly the FP32 sums)
@@ -1770,6 +1821,7 @@ int ccd_remap_ph(ccd_ctx *c, const double *t, double *tph) {
     double *dt = c->d_t[1 - c->cur];
     int rc = t2_upload(c, t, dt, c->d_g);
     if (!rc) rc = launch_transpose(c, true, dt, c->d_tph, c->stream);
+    c->tph_prec = CCD_PREC_FP64;
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(tph, c->d_tph, sizeof(double) * c->nt2, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));

@@ -8,7 +8,7 @@
 // operand of the second product by ordering the k-slots to match it.
 //
 // Arithmetic type CT: double (v_mfma_f64_16x16x4f64) or float
-// (v_mfma_f32_16x16x4f32, V^A and V^B stored in FP32, T^ph rounded on load);
+// (v_mfma_f32_16x16x4f32, V^A, V^B and T^ph in FP32);
 // R is written in CT (FP32 holds the FP32 sums exactly).  Every load is unconditional: elements
 // outside the block read the zero buffer.
 #pragma once
@@ -21,14 +21,14 @@ struct RingItem {       // one wave's work: mod channel and 16-column tile
 
 
 template <typename CT, int NT>
-__device__ __forceinline__ void ring_wave(const DMod &md, int jt, const double *__restrict__ tph,
+__device__ __forceinline__ void ring_wave(const DMod &md, int jt, const CT *__restrict__ tph,
                                           const CT *__restrict__ va, const CT *__restrict__ vb,
                                           const double *__restrict__ zero, CT *__restrict__ r) {
     typedef typename Mma<CT>::acc Acc;
     constexpr bool F64 = sizeof(CT) == 8;
     const int lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
     const int PH = md.ph, HP = md.hp;
-    const double *T = tph + md.off_ph;         // PH x HP, [(a,i),(j,b)]
+    const CT *T = tph + md.off_ph;             // PH x HP, [(a,i),(j,b)]
     const CT *A = va + md.off_ph;              // HP x PH, V^A [(i,a),(b,j)]
     const CT *B = vb + md.off_b;               // HP x HP, V^B
     const CT *zc = reinterpret_cast<const CT *>(zero);
@@ -54,7 +54,7 @@ __device__ __forceinline__ void ring_wave(const DMod &md, int jt, const double *
                 const int k = pt * 16 + ks(j);
                 const bool kok = k < PH;
                 const CT a = *((rok && kok) ? A + row * PH + k : zc);
-                const CT b = (CT) * ((kok && cok) ? T + k * HP + col : zero);
+                const CT b = *((kok && cok) ? T + k * HP + col : zc);
                 acc = mfma16(a, b, acc);
             }
         }
@@ -77,7 +77,7 @@ __device__ __forceinline__ void ring_wave(const DMod &md, int jt, const double *
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 const int k = kt * 16 + ks(j);
-                const CT a = (CT) * ((rok && k < HP) ? T + row * HP + k : zero);
+                const CT a = *((rok && k < HP) ? T + row * HP + k : zc);
                 acc = mfma16(a, x[kt][j], acc);
             }
         }
@@ -169,7 +169,7 @@ __device__ __forceinline__ void ring_staged_tile(CT (*As)[STR], CT (*Ts)[STR], c
 
 template <typename CT, int NT>
 __device__ __forceinline__ void ring_staged_block(const RingGroup &grp, const DMod *__restrict__ mods,
-                                                  const double *__restrict__ tph, const CT *__restrict__ va,
+                                                  const CT *__restrict__ tph, const CT *__restrict__ va,
                                                   const CT *__restrict__ vb, const double *__restrict__ zero,
                                                   CT *__restrict__ r, unsigned char *lds) {
     typedef RingStage<CT, NT> S;
@@ -219,8 +219,7 @@ __device__ __forceinline__ void ring_staged_block(const RingGroup &grp, const DM
     constexpr int RND = PER < 8 ? PER : 8;
 #pragma unroll
     for (int q0 = 0; q0 < PER; q0 += RND) {
-        CT sa[RND];
-        double st[RND];
+        CT sa[RND], st[RND];
 #pragma unroll
         for (int u = 0; u < RND; u++) {
             const int q = q0 + u;
@@ -229,7 +228,7 @@ __device__ __forceinline__ void ring_staged_block(const RingGroup &grp, const DM
                 const int row = rem / D, c = rem - row * D;
                 const DMod &h = md[g];
                 sa[u] = *((row < h.hp && c < h.ph) ? va + h.off_ph + row * h.ph + c : zc);
-                st[u] = *((row < h.ph && c < h.hp) ? tph + h.off_ph + row * h.hp + c : zero);
+                st[u] = *((row < h.ph && c < h.hp) ? tph + h.off_ph + row * h.hp + c : zc);
             }
         }
 #pragma unroll
@@ -241,7 +240,7 @@ __device__ __forceinline__ void ring_staged_block(const RingGroup &grp, const DM
                 CT(*As)[STR] = reinterpret_cast<CT(*)[STR]>(lds + g * S::CH_BYTES);
                 CT(*Ts)[STR] = reinterpret_cast<CT(*)[STR]>(lds + g * S::CH_BYTES + sizeof(CT) * D * STR);
                 As[row][c] = sa[u];
-                Ts[row][c] = (CT)st[u];
+                Ts[row][c] = st[u];
             }
         }
     }
@@ -268,7 +267,7 @@ __device__ __forceinline__ void ring_staged_block(const RingGroup &grp, const DM
 template <typename CT, int LO, int HI>
 __global__ __launch_bounds__(kThreads) void k_ring_staged(const RingGroup *__restrict__ groups,
                                                           const DMod *__restrict__ mods,
-                                                          const double *__restrict__ tph,
+                                                          const CT *__restrict__ tph,
                                                           const CT *__restrict__ va, const CT *__restrict__ vb,
                                                           const double *__restrict__ zero,
                                                           CT *__restrict__ r,
@@ -291,7 +290,7 @@ __global__ __launch_bounds__(kThreads) void k_ring_staged(const RingGroup *__res
 template <typename CT, bool WIDE>
 __global__ __launch_bounds__(kThreads) void k_ring(const RingItem *__restrict__ items, int nitems,
                                                    const DMod *__restrict__ mods,
-                                                   const double *__restrict__ tph,
+                                                   const CT *__restrict__ tph,
                                                    const CT *__restrict__ va, const CT *__restrict__ vb,
                                                    const double *__restrict__ zero,
                                                    CT *__restrict__ r,

@@ -149,13 +149,14 @@ __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
 // coalesce), then slot l of the destination order is written to
 // dst[dst_addr[base+l]] from LDS slot perm[base+l].  With src = pphh it is the
 // reference's mapToDistributed(ai_jb); with src = phhp it gathers R into the
-// pphh layout.
-template <typename E>
+// pphh layout.  Elements are converted from S to E on the way (T^ph for FP32
+// contractions is the FP32 rounding of T, which is all the FP32 ring reads).
+template <typename S, typename E = S>
 __global__ __launch_bounds__(kThreads) void k_transpose(const int32_t *__restrict__ base,
                                                         const int32_t *__restrict__ src_addr,
                                                         const int32_t *__restrict__ dst_addr,
                                                         const uint16_t *__restrict__ perm,
-                                                        const E *__restrict__ src,
+                                                        const S *__restrict__ src,
                                                         E *__restrict__ dst,
                                                         const LoopCtl *__restrict__ ctl) {
     if (ctl->done) return;
@@ -177,7 +178,7 @@ __global__ __launch_bounds__(kThreads) void k_transpose(const int32_t *__restric
             ad[u] = src_addr[b0 + (l < n ? l : 0)];
         }
 #pragma unroll
-        for (int u = 0; u < U; u++) v[u] = src[ad[u]];
+        for (int u = 0; u < U; u++) v[u] = (E)src[ad[u]];
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int l = l0 + u * kThreads;
