@@ -9,10 +9,11 @@ whole problem (both ladders, ring, one-body terms, remaps, update, energy),
 with the amplitudes and interaction resident in HBM.
 
 --precision mixed (default): the steps are FP32-contraction iterations (the
-bulk of a mixed solve); afterwards a full mixed solve (FP32 iterations, then
-FP64 sweeps to |dE| <= 1e-10) and a full FP64 solve are timed from T2 = 0
-and their energies compared per particle.  fp64 / fp32: steps in that
-arithmetic only.
+bulk of a mixed solve); afterwards a full mixed solve (FP32 iterations while
+|dE| > --switch-tol, then FP64 sweeps to |dE| <= 1e-10) and a full FP64 solve
+are timed from T2 = 0 and their energies compared per particle; the mixed
+solve is also timed with the switch at 1e-6 and with no FP64 sweeps
+(switch_sweep).  fp64 / fp32: steps in that arithmetic only.
 
 --mode free (the default for the electron gas) is the reference's
 matrix-free mode (-m 1): V^pppp is rebuilt inside the pppp ladder every
@@ -131,7 +132,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="heg54", choices=sorted(CONFIGS))
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64", "fp32"])
-    ap.add_argument("--switch-tol", type=float, default=1e-6,
+    ap.add_argument("--switch-tol", type=float, default=1e-8,
                     help="mixed solve: |dE| below which FP64 sweeps replace FP32 iterations")
     ap.add_argument("--mode", default=None, choices=["stored", "free"],
                     help="stored: V^pppp read from HBM; free: rebuilt in the ladder (reference -m 1); "
@@ -229,6 +230,17 @@ def main():
                  "fp64_energy": float(e_d[-1]),
                  "abs_dE_per_particle": abs(float(e_m[-1]) - float(e_d[-1])) / n_part,
                  "speedup_vs_fp64_solve": t_d / t_m}
+        # the same solve with other switch points (1e-10 = tol: FP32 iterations only)
+        sw_runs = []
+        for sw in (1e-6, 1e-8, 1e-10):
+            ctx.solve_mixed(1e-10, sw)                 # warm
+            t0 = time.perf_counter()
+            e_s, stop_s, n_s, n32_s = ctx.solve_mixed(1e-10, sw)
+            t_s = time.perf_counter() - t0
+            sw_runs.append({"switch_tol": sw, "iters": n_s, "fp32_iters": n32_s, "stop": stop_s,
+                          "solve_s": t_s, "speedup_vs_fp64_solve": t_d / t_s,
+                          "abs_dE_per_particle": abs(float(e_s[-1]) - float(e_d[-1])) / n_part})
+        mixed["switch_sweep"] = sw_runs
         ctx64.close()
     elif args.precision == "mixed":
         # every solve of the sweep from T2 = 0, mixed and FP64, one batch each

@@ -248,6 +248,31 @@ def test_ring_every_tile_class(require_gpu, kind, args):
 
 
 @pytest.mark.parametrize("free", [False, True])
+def test_precision_switch_continues_the_same_iteration(require_gpu, free):
+    """FP32 iterations keep T^ph in FP32; after switching a context to FP64
+    it must iterate exactly as an FP64 context started from the same
+    amplitudes (T^ph rebuilt in the new type, nothing stale)."""
+    from coupled_cluster_doubles_amd import FP32
+    m = Model.electron_gas(1.0, 19, 2)
+    a = Context(m, FP32, matrix_free=free)
+    a.iterate_n(3)
+    t3 = a.get_t2()
+    a.set_precision(FP64)
+    ea = a.iterate_n(2)
+    b = Context(m, FP64, matrix_free=free)
+    b.set_t2(t3)
+    eb = b.iterate_n(2)
+    assert np.array_equal(ea, eb)
+    assert np.array_equal(a.get_t2(), b.get_t2())
+    # and back: FP32 again continues as an FP32 context from those amplitudes
+    a.set_precision(FP32)
+    ea32 = a.iterate_n(2)
+    c = Context(m, FP32, matrix_free=free)
+    c.set_t2(b.get_t2())
+    assert np.array_equal(ea32, c.iterate_n(2))
+
+
+@pytest.mark.parametrize("free", [False, True])
 def test_fp32_stored_and_generated_v_agree_bitwise(require_gpu, free):
     from coupled_cluster_doubles_amd import FP32
     m = Model.electron_gas(1.0, 19, 2)
