@@ -212,7 +212,8 @@ __global__ __launch_bounds__(kThreads) void k_transpose(const int32_t *__restric
 // E_prev = E, iterations++, and the loop goes on while |dE| > tol unless
 // more than max_iters iterations have run.  A single solve has pctl == ctl;
 // a batch counts its iterations in ctl and stops once every solve has.  The
-// last block resets the ticket counter for the next launch.
+// last block resets the ticket counter for the next launch.  Without holes
+// (an empty problem) block 0 does that part.
 __global__ __launch_bounds__(kThreads) void k_reduce(const int32_t *__restrict__ ptr, int nh,
                                                      const double *__restrict__ parts,
                                                      double *__restrict__ out, int *__restrict__ ticket,
@@ -232,7 +233,7 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const int32_t *__restrict__
             __threadfence();
             last = atomicAdd(ticket, 1) == nh - 1;
         } else {
-            last = 0;
+            last = nh == 0 && k == 0;   // no holes (no amplitudes): block 0 runs the loop step, E = 0
         }
     }
     __syncthreads();

@@ -112,6 +112,51 @@ def test_diagrams_on_random_amplitudes(require_gpu, kind, args):
     assert _rel(e_g, e_o) < E_REL
 
 
+@pytest.mark.parametrize("kind,args", [
+    ("pairing", (2, 2, 0.5, 1.0)),       # one pair in two levels: 6 amplitudes
+    ("pairing", (3, 2, 0.3, 1.0)),       # odd level count
+    ("pairing", (5, 6, 0.4, 1.0)),       # more holes than particles
+    ("pairing", (10, 4, 0.5, 1.0)),      # more particles than holes
+    ("heg", (1.0, 2, 1)),                # 2 electrons in 14 states
+    ("heg", (2.0, 3, 1)),                # 2 electrons in 38 states, r_s = 2
+])
+def test_small_and_ragged_problems(require_gpu, kind, args):
+    """Full solves on the smallest and most uneven bases: the reference's
+    protocol (energies, iteration count, stop reason) in both V modes, and
+    the mixed solve within 1e-7 per particle."""
+    from coupled_cluster_doubles_amd import FP32
+    m, o = _pair(kind, args)
+    e_o, stop_o, _ = o.solve(1e-12)
+    for free in (False, True):
+        ctx = Context(m, FP64, matrix_free=free)
+        assert _rel(ctx.mbpt2(), o.mbpt2()) < E_REL
+        e_g, stop_g, n_g = ctx.solve(1e-12)
+        assert stop_g == stop_o == "converged"
+        assert abs(n_g - len(e_o)) <= 1
+        k = min(n_g, len(e_o))
+        assert np.max(np.abs(e_g[:k] - e_o[:k]) / np.abs(e_o[:k])) < E_REL
+    e_m, stop_m, _, _ = Context(m, FP32).solve_mixed(1e-12, 1e-8)
+    assert stop_m == "converged" and abs(e_m[-1] - e_o[-1]) / m.n_holes <= 1e-7
+
+
+@pytest.mark.parametrize("kind,args", [
+    ("heg", (1.0, 1, 1)),                # every state a hole: no amplitudes
+    ("pairing", (4, 0, 0.5, 1.0)),       # no particles: no holes, no amplitudes
+])
+def test_empty_problems(require_gpu, kind, args):
+    """No amplitudes at all: MBPT2 = 0, and the loop runs the reference's two
+    iterations (E = 0 after E_prev = 100, then dE = 0) and stops converged."""
+    m, o = _pair(kind, args)
+    assert m.n_t2 == 0
+    e_o, stop_o, _ = o.solve(1e-10)
+    ctx = Context(m, FP64)
+    assert ctx.mbpt2() == 0.0
+    e_g, stop_g, n_g = ctx.solve(1e-10)
+    assert (stop_g, n_g) == (stop_o, len(e_o)) == ("converged", 2)
+    assert np.all(e_g == 0.0)
+    assert ctx.get_t2().size == 0
+
+
 def test_host_v_matches_generated_v(require_gpu):
     m = Model.electron_gas(1.0, 6, 2)
     c_gen = Context(m, FP64)
