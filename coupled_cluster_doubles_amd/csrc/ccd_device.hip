@@ -87,6 +87,10 @@ struct ccd_ctx {
     std::vector<DMod> mods;
     std::vector<DTile> rtiles, ltiles, utiles, xtiles, ttiles;  // packed-row, ladder, update, X, T X tiles
     int lt_off[NTMAX + 1] = {0};  // ladder tiles grouped by 16-column tile count: [lt_off[n-1], lt_off[n])
+    // blocks of the ladder launch of class n: lb_grid[n] blocks, block b runs the
+    // class's tiles lbptr[lb_base[n] + b] .. lbptr[lb_base[n] + b + 1] - 1
+    int lb_base[NTMAX + 1] = {0}, lb_grid[NTMAX + 1] = {0};
+    int32_t *d_lbptr = nullptr;
     int64_t n_cp = 0;            // column-partial slots
     std::vector<RingItem> ritems;  // ring work items, largest channels first
     int n_ring_wide = 0;           // leading items of channels with more than 5 tiles
@@ -347,16 +351,16 @@ int launch_ladder_x(ccd_ctx *c, hipStream_t st) {
     return check_launch("k_ladder_x");
 }
 
-// Stored V: one block per tile.  Generated V: at most gen_blocks blocks
-// looping over the tiles, the gas table in dynamic LDS.
+// Stored V: one block per tile.  Generated V: at most gen_blocks blocks, each
+// running its own run of tiles (lbptr), the gas table in dynamic LDS.
 template <typename CT, int NT, int SRC>
 void launch_ladder_v_nt(ccd_ctx *c, const CT *vp, hipStream_t st) {
-    const int n = c->lt_off[NT] - c->lt_off[NT - 1];
-    if (n <= 0) return;
-    const int grid = SRC == LV_STORED ? n : std::min(n, c->gen_blocks);
+    const int grid = c->lb_grid[NT];
+    if (grid <= 0) return;
     const size_t lds = SRC == LV_GAS ? sizeof(double) * (size_t)c->lgen.ntab : 0;
     hipLaunchKernelGGL((k_ladder_v<CT, NT, SRC>), dim3((unsigned)grid), dim3(kThreads), lds, st,
-                       c->d_ltiles + c->lt_off[NT - 1], n, c->d_chans, vp, c->lgen, c->d_tas, c->d_x,
+                       c->d_ltiles + c->lt_off[NT - 1], c->d_lbptr + c->lb_base[NT], c->d_chans, vp, c->lgen,
+                       c->d_tas, c->d_x,
                        c->d_zero, reinterpret_cast<CT *>(c->d_dp), c->d_ctl);
 }
 
@@ -763,9 +767,6 @@ int build_generated_ladder(ccd_ctx *c, const ccd_problem *p, const std::vector<i
     } else {
         c->lsrc = LV_GENERIC;
     }
-    int ncu = 0;
-    HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
-    c->gen_blocks = std::max(1, ncu) * 4;
     return CCD_OK;
 }
 
@@ -1074,6 +1075,66 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
         }
         c->lt_off[nt] = (int)c->ltiles.size();
     }
+    // Block assignment.  Stored V: one block per tile.  Generated V: a
+    // persistent grid of gen_blocks blocks (each stages the generator table
+    // once), block b running a contiguous run of the class's tiles.  Two
+    // assignments are costed per class with a model of the CU loads (block b
+    // on CU b mod ncu, cost of a tile = its k chunks + 1): round robin in
+    // channel order (block b takes tiles b, b+G, ...), and longest tile first
+    // to the least loaded block.  The second is taken when its most loaded
+    // CU carries at least 10% less (the large electron gases); otherwise the
+    // channel order is kept (it measured faster where the two are close).
+    // Tiles write disjoint outputs: the assignment changes no result.
+    std::vector<int32_t> lbptr;
+    {
+        int ncu = 0;
+        HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+        ncu = std::max(1, ncu);
+        c->gen_blocks = ncu * 4;
+        std::vector<DTile> cls;
+        for (int nt = 1; nt <= NTMAX; nt++) {
+            const int lo = c->lt_off[nt - 1], n = c->lt_off[nt] - lo;
+            const int G = c->matrix_free ? std::min(n, c->gen_blocks) : n;
+            c->lb_base[nt] = (int)lbptr.size();
+            c->lb_grid[nt] = G;
+            lbptr.push_back(0);
+            if (G <= 0) continue;
+            const int kch = nt <= 5 ? LadderShape<1>::KCH : LadderShape<8>::KCH;
+            const int kseg = nt <= 2 ? LadderShape<1>::KSEG : LadderShape<3>::KSEG;
+            std::vector<int64_t> cost(n);
+            for (int i = 0; i < n; i++) {   // k chunks of the tile, plus one for its prologue
+                const DTile &t = c->ltiles[lo + i];
+                const int klen = std::min(c->chans[t.chan].npk, (t.aux + 1) * kseg) - t.aux * kseg;
+                cost[i] = (klen + kch - 1) / kch + 1;
+            }
+            auto cu_max = [&](const std::vector<std::vector<int>> &own) {
+                std::vector<int64_t> cu(ncu, 0);
+                for (int b = 0; b < G; b++)
+                    for (int i : own[b]) cu[b % ncu] += cost[i];
+                return *std::max_element(cu.begin(), cu.end());
+            };
+            std::vector<std::vector<int>> rr(G), lpt(G);
+            for (int i = 0; i < n; i++) rr[i % G].push_back(i);
+            std::vector<int> idx(n);
+            for (int i = 0; i < n; i++) idx[i] = i;
+            std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+            std::vector<int64_t> load(G, 0);
+            for (int i : idx) {   // least loaded block, lowest index on ties
+                int b = 0;
+                for (int q = 1; q < G; q++)
+                    if (load[q] < load[b]) b = q;
+                load[b] += cost[i];
+                lpt[b].push_back(i);
+            }
+            const auto &own = 10 * cu_max(lpt) <= 9 * cu_max(rr) ? lpt : rr;
+            cls.clear();
+            for (int b = 0; b < G; b++) {
+                for (int i : own[b]) cls.push_back(c->ltiles[lo + i]);
+                lbptr.push_back((int32_t)cls.size());
+            }
+            std::copy(cls.begin(), cls.end(), c->ltiles.begin() + lo);
+        }
+    }
     for (int wide = 0; wide < 2; wide++) {   // channels with hh <= 80 first
         for (int k = 0; k < (int)c->chans.size(); k++)
             if ((c->chans[k].hh > 80) == (wide == 1))
@@ -1124,7 +1185,7 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
         (rc = upload(c, &c->d_rtiles, c->rtiles)) || (rc = upload(c, &c->d_ltiles, c->ltiles)) ||
         (rc = upload(c, &c->d_utiles, c->utiles)) ||
         (rc = upload(c, &c->d_xtiles, c->xtiles)) || (rc = upload(c, &c->d_ttiles, c->ttiles)) || (rc = upload(c, &c->d_ritems, c->ritems)) ||
-        (rc = upload(c, &c->d_rgroups, c->rgroups)) ||
+        (rc = upload(c, &c->d_rgroups, c->rgroups)) || (rc = upload(c, &c->d_lbptr, lbptr)) ||
         (rc = upload(c, &c->d_t2_in, t2_in)) || (rc = upload(c, &c->d_t2_out, t2_out)) ||
         (rc = upload(c, &c->d_pp_old, pp_old)) ||
         (rc = upload(c, &c->d_ppl, ppl)) ||

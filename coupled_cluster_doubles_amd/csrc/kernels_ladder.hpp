@@ -136,9 +136,9 @@ __device__ __forceinline__ void static_for(F &&f) {
 // the chunk (G <= 16), prefetched one chunk ahead; MFMA step s broadcasts
 // lane s of each row, and each lane builds its A value from that record and
 // its row's records.  Every source gives the same values in the same
-// summation order, so the modes agree bit for bit.  Blocks loop over tiles
-// (the generated modes launch fewer blocks than tiles, so the gas table is
-// staged into LDS once per block).
+// summation order, so the modes agree bit for bit.  Block b runs the tiles
+// bptr[b] .. bptr[b+1]-1 (the generated modes launch fewer blocks than tiles,
+// so the gas table is staged into LDS once per block).
 //
 // Arithmetic type CT: double, or float for the mixed-precision iterations
 // (V^pppp stored in FP32, or generated in FP64 and rounded; amplitudes and X
@@ -162,7 +162,8 @@ struct LadderShape {
 };
 
 template <typename CT, int NT, int SRC>
-__global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__ tiles, int ntiles,
+__global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__ tiles,
+                                                       const int32_t *__restrict__ bptr,
                                                        const DChan *__restrict__ chans,
                                                        const CT *__restrict__ vp, LadderGen lg,
                                                        const double *__restrict__ tas,
@@ -193,7 +194,7 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
     const uint32_t tbase = (uint32_t)(uintptr_t)(lds_double *)gtab;  // LDS byte address
     const uint32_t zaddr = tbase + lg.zoff;
 
-    for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+    for (int ti = bptr[blockIdx.x]; ti < bptr[blockIdx.x + 1]; ti++) {
         const DTile tl = tiles[ti];
         const DChan ch = chans[tl.chan];
         const int H = ch.hh, npk = ch.npk, ldv = ch.ldv, seg = tl.aux;
