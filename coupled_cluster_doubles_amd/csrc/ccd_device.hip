@@ -744,17 +744,41 @@ int build_generated_ladder(ccd_ctx *c, const ccd_problem *p, const std::vector<i
         const uint32_t sp = a[3] == s_hi && s_hi != s_lo ? zoff : 0u;
         return int2{(int)(x | y << 16), (int)(z | sp << 16)};
     };
-    for (const DChan &ch : c->chans)
+    // Channels whose pairs all have one particle of each spin (total spin 0):
+    // there <ab||cd> = s(ab) s(cd) T(k_up(ab) - k_up(cd)), with k_up the
+    // momentum of the pair's up-spin particle and s = +1 when the first
+    // particle is the up one, -1 otherwise (exactly one of the two terms
+    // survives, and T is even).  Their records: rec1 = k_up, scaled as above
+    // without a spin field; rec2 = (sign bit of s as a float/double sign
+    // mask, 0).
+    auto scaled_up = [&](int s) {
+        const int32_t *a = qn + 5 * (size_t)s;
+        const uint32_t x = 8u * D * D * (a[0] + m), y = 8u * D * (a[1] + m), z = 8u * (a[2] + m);
+        return int2{(int)(x | y << 16), (int)z};
+    };
+    for (DChan &ch : c->chans) {
+        ch.spin0 = 0;
+        if (gas && s_hi != s_lo && ch.ndiag == 0 && ch.npk > 0) {
+            const int2 f = ppl[ch.pp_off + prow[ch.pr_off]];
+            ch.spin0 = qn[5 * f.x + 3] != qn[5 * f.y + 3];
+        }
         for (int u = 0; u < ch.npk + ch.ndiag; u++) {
             const int2 ab = ppl[ch.pp_off + prow[ch.pr_off + u]];
-            if (gas) {
+            if (ch.spin0) {
+                const bool first_up = qn[5 * ab.x + 3] == s_hi;
+                rec1[ch.pr_off + u] = scaled_up(first_up ? ab.x : ab.y);
+                rec2[ch.pr_off + u] = int2{first_up ? 0 : (int)0x80000000u, 0};
+            } else if (gas) {
                 rec1[ch.pr_off + u] = scaled(ab.x);
                 rec2[ch.pr_off + u] = scaled(ab.y);
             } else {
                 rec1[ch.pr_off + u] = int2{ab.x, ab.y};
             }
         }
+    }
     int rc;
+    if (!c->chans.empty())   // the spin0 flags
+        HIP_TRY(hipMemcpy(c->d_chans, c->chans.data(), sizeof(DChan) * c->chans.size(), hipMemcpyHostToDevice));
     if ((rc = upload(c, &c->d_rec1, rec1)) || (rc = upload(c, &c->d_rec2, rec2))) return rc;
     c->lgen.rec1 = c->d_rec1;
     c->lgen.rec2 = c->d_rec2;
@@ -887,7 +911,7 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
             const int ldv = (npk + 15) / 16 * 16;
             const int nseg = ladder_segments(npk, (H + 15) / 16);
             DChan ch{off_t, off_vp, off_tp, off_hhhh, P, H, (int32_t)ipp, (int32_t)ihh, npk, ldv, pr_off, ndiag,
-                     off_dp, nseg, prob};
+                     off_dp, nseg, prob, 0, 0};
             c->chans.push_back(ch);
             off_vp += (int64_t)npk * ldv;
             off_tp += (int64_t)npk * H;

@@ -24,6 +24,9 @@ struct DChan {          // active direct channel (pp > 0 and hh > 0)
     int64_t off_dp;     // ladder output: nseg + 1 slabs of npk x hh partial sums
     int32_t nseg;       // number of k segments of the pppp ladder (slab nseg: 1/2 T X)
     int32_t prob;       // solve the channel belongs to (0 outside a batch)
+    int32_t spin0;      // generated electron-gas V: 1 when every pair has one particle of each
+                        // spin (one table term per element, see k_ladder_v)
+    int32_t pad;
 };
 
 struct DMod {           // active mod channel (ph > 0 and hp > 0)

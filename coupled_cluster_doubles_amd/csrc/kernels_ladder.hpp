@@ -102,6 +102,19 @@ __device__ __forceinline__ uint32_t gas_addr(int2 r, int2 c, uint32_t base, uint
                zaddr);
 }
 
+// LDS address of T(|k_up(ab) - k_up(cd)|) for the records of a total-spin-0
+// channel (no spin field: the address stays inside the table).
+__device__ __forceinline__ uint32_t gas_addr1(int2 r, int2 c, uint32_t base) {
+    return __builtin_amdgcn_sad_u16((uint32_t)r.x, (uint32_t)c.x,
+                                    __builtin_amdgcn_sad_u16((uint32_t)r.y, (uint32_t)c.y, base));
+}
+
+// v with its sign bit flipped where the mask m (0 or 0x80000000) says so.
+__device__ __forceinline__ float flip_sign(float v, uint32_t m) { return __uint_as_float(__float_as_uint(v) ^ m); }
+__device__ __forceinline__ double flip_sign(double v, uint32_t m) {
+    return __hiloint2double(__double2hiint(v) ^ (int)m, __double2loint(v));
+}
+
 __device__ __forceinline__ double lds_load(uint32_t addr) {
     return *reinterpret_cast<lds_double *>(static_cast<uintptr_t>(addr));
 }
@@ -197,129 +210,148 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
     for (int ti = bptr[blockIdx.x]; ti < bptr[blockIdx.x + 1]; ti++) {
         const DTile tl = tiles[ti];
         const DChan ch = chans[tl.chan];
-        const int H = ch.hh, npk = ch.npk, ldv = ch.ldv, seg = tl.aux;
-        const int kbeg = seg * KSEG, kend = min(npk, kbeg + KSEG);
-        const int nv = (kend - kbeg + KCH - 1) / KCH;
-        const int lrow = wave * 16 + m;
-        const bool rok = lrow < tl.nrows;
-        const int64_t u = tl.row0 + lrow;
-        const CT *av = GEN ? nullptr : vp + ch.off_vp + u * ldv;
-        int2 ra = {0, 0}, rb = {0, 0};
-        if (GEN) {
-            const int2 *z2 = reinterpret_cast<const int2 *>(zero);
-            ra = *(rok ? lg.rec1 + ch.pr_off + u : z2);
-            if (SRC == LV_GAS) rb = *(rok ? lg.rec2 + ch.pr_off + u : z2);
-        }
-
-        // Every load is unconditional: an element outside the block reads the
-        // zero buffer instead, so no load sits behind a branch and the
-        // prefetches stay in flight across the MFMAs.
-        auto load_a = [&](int v, CT (&a)[G]) {   // stored V: V chunk v
-            const int kc0 = kbeg + v * KCH;
-#pragma unroll
-            for (int h = 0; h < G / VW; h++) {
-                // k-slots (kq, VW h .. VW h + VW-1) are VW consecutive k; k and
-                // ldv are multiples of VW, so the group lies inside the row
-                const int k = kc0 + kslot<CT>(kq, VW * h);
-                const vec w = *reinterpret_cast<const vec *>((rok && k < kend && k < ldv)
-                                                                 ? av + k
-                                                                 : reinterpret_cast<const CT *>(zero));
-#pragma unroll
-                for (int e = 0; e < VW; e++) a[VW * h + e] = w[e];
+        // ONE: a total-spin-0 channel of the generated electron gas, V from one
+        // table term (records of k_up; the signs s(cd) applied to the B rows,
+        // s(ab) to the output rows, so every product equals the two-term one)
+        auto tile = [&](auto one_) {
+            constexpr bool ONE = decltype(one_)::value;
+            const int H = ch.hh, npk = ch.npk, ldv = ch.ldv, seg = tl.aux;
+            const int kbeg = seg * KSEG, kend = min(npk, kbeg + KSEG);
+            const int nv = (kend - kbeg + KCH - 1) / KCH;
+            const int lrow = wave * 16 + m;
+            const bool rok = lrow < tl.nrows;
+            const int64_t u = tl.row0 + lrow;
+            const CT *av = GEN ? nullptr : vp + ch.off_vp + u * ldv;
+            int2 ra = {0, 0}, rb = {0, 0};
+            if (GEN) {
+                const int2 *z2 = reinterpret_cast<const int2 *>(zero);
+                ra = *(rok ? lg.rec1 + ch.pr_off + u : z2);
+                if (SRC == LV_GAS) rb = *(rok ? lg.rec2 + ch.pr_off + u : z2);
             }
-        };
-        auto load_r = [&](int v, int2 &rr) {  // generated V: record of k-slot (kq, m) of V chunk v
-            const int k = kbeg + v * KCH + kslot<CT>(kq, m);
-            rr = *((m < G && k < kend) ? lg.rec1 + ch.pr_off + k : reinterpret_cast<const int2 *>(zero));
-        };
-        auto gen_a = [&](int2 rr, CT (&a)[G]) {  // generated V from the chunk's records
-            static_for<0, G>([&](auto S) {
-                constexpr int s = decltype(S)::value;
-                const int2 r = {row_bcast<s>(rr.x), row_bcast<s>(rr.y)};
-                if (SRC == LV_GAS) {
-                    a[s] = (CT)(lds_load(gas_addr(ra, r, tbase, zaddr)) - lds_load(gas_addr(rb, r, tbase, zaddr)));
-                } else {
-                    a[s] = (CT)gen_tbme(lg.G, ra.x, ra.y, r.x, r.y);
+
+            // Every load is unconditional: an element outside the block reads the
+            // zero buffer instead, so no load sits behind a branch and the
+            // prefetches stay in flight across the MFMAs.
+            auto load_a = [&](int v, CT (&a)[G]) {   // stored V: V chunk v
+                const int kc0 = kbeg + v * KCH;
+#pragma unroll
+                for (int h = 0; h < G / VW; h++) {
+                    // k-slots (kq, VW h .. VW h + VW-1) are VW consecutive k; k and
+                    // ldv are multiples of VW, so the group lies inside the row
+                    const int k = kc0 + kslot<CT>(kq, VW * h);
+                    const vec w = *reinterpret_cast<const vec *>((rok && k < kend && k < ldv)
+                                                                     ? av + k
+                                                                     : reinterpret_cast<const CT *>(zero));
+#pragma unroll
+                    for (int e = 0; e < VW; e++) a[VW * h + e] = w[e];
                 }
-            });
-        };
-        // B tile of k rows k0.. of a K x H row-major block
-        auto load_b = [&](const double *B, int K, int k0, CT (&b)[BPT]) {
+            };
+            auto load_r = [&](int v, int2 &rr) {  // generated V: record of k-slot (kq, m) of V chunk v
+                const int k = kbeg + v * KCH + kslot<CT>(kq, m);
+                rr = *((m < G && k < kend) ? lg.rec1 + ch.pr_off + k : reinterpret_cast<const int2 *>(zero));
+            };
+            auto gen_a = [&](int2 rr, CT (&a)[G]) {  // generated V from the chunk's records
+                static_for<0, G>([&](auto S) {
+                    constexpr int s = decltype(S)::value;
+                    const int2 r = {row_bcast<s>(rr.x), row_bcast<s>(rr.y)};
+                    if (SRC == LV_GAS && ONE) {
+                        a[s] = (CT)lds_load(gas_addr1(ra, r, tbase));
+                    } else if (SRC == LV_GAS) {
+                        a[s] = (CT)(lds_load(gas_addr(ra, r, tbase, zaddr)) - lds_load(gas_addr(rb, r, tbase, zaddr)));
+                    } else {
+                        a[s] = (CT)gen_tbme(lg.G, ra.x, ra.y, r.x, r.y);
+                    }
+                });
+            };
+            // B tile of k rows k0.. of a K x H row-major block
+            // (ONE: with the sign mask of row k, applied when the tile is stored)
+            auto load_b = [&](const double *B, int K, int k0, CT (&b)[BPT], uint32_t (&sm)[BPT]) {
 #pragma unroll
-            for (int q = 0; q < BPT; q++) {
-                const int idx = threadIdx.x + kThreads * q, kk = idx / W, col = idx - kk * W, k = k0 + kk;
-                b[q] = (CT) * ((idx < KCH * W && k < K && col < H) ? B + (int64_t)k * H + col : zero);
-            }
-        };
-        auto store_b = [&](int buf, const CT (&b)[BPT]) {
+                for (int q = 0; q < BPT; q++) {
+                    const int idx = threadIdx.x + kThreads * q, kk = idx / W, col = idx - kk * W, k = k0 + kk;
+                    b[q] = (CT) * ((idx < KCH * W && k < K && col < H) ? B + (int64_t)k * H + col : zero);
+                    if (ONE)
+                        sm[q] = *((idx < KCH * W && k < K) ? reinterpret_cast<const uint32_t *>(lg.rec2 + ch.pr_off + k)
+                                                           : reinterpret_cast<const uint32_t *>(zero));
+                }
+            };
+            auto store_b = [&](int buf, const CT (&b)[BPT], const uint32_t (&sm)[BPT]) {
 #pragma unroll
-            for (int q = 0; q < BPT; q++) {
-                const int idx = threadIdx.x + kThreads * q, kk = idx / W, col = idx - kk * W;
-                if (idx < KCH * W) Bs[buf][kk][col] = b[q];
-            }
-        };
+                for (int q = 0; q < BPT; q++) {
+                    const int idx = threadIdx.x + kThreads * q, kk = idx / W, col = idx - kk * W;
+                    if (idx < KCH * W) Bs[buf][kk][col] = ONE ? flip_sign(b[q], sm[q]) : b[q];
+                }
+            };
 
-        Acc acc[NT];
+            Acc acc[NT];
 #pragma unroll
-        for (int n = 0; n < NT; n++) acc[n] = Acc{};
-        auto mfma_chunk = [&](int buf, const CT (&a)[G]) {
+            for (int n = 0; n < NT; n++) acc[n] = Acc{};
+            auto mfma_chunk = [&](int buf, const CT (&a)[G]) {
 #pragma unroll
-            for (int s = 0; s < G; s++) {
+                for (int s = 0; s < G; s++) {
 #pragma unroll
-                for (int n = 0; n < NT; n++)
-                    acc[n] = mfma16(a[s], Bs[buf][kslot<CT>(kq, s)][n * 16 + m], acc[n]);
-            }
-        };
-        CT b[BPT];
+                    for (int n = 0; n < NT; n++)
+                        acc[n] = mfma16(a[s], Bs[buf][kslot<CT>(kq, s)][n * 16 + m], acc[n]);
+                }
+            };
+            CT b[BPT];
+            uint32_t sm[BPT] = {};
 
-        // V chunks, software-pipelined over two register sets and two LDS
-        // buffers, unrolled by two so that no copy joins the sets: while the
-        // MFMAs of chunk v run, chunk v+1 is in flight, and the wait for it
-        // comes only before its own MFMAs, with chunk v+2 already issued.
-        // Prefetches past the last chunk read the zero buffer.  B is loaded
-        // before A: loads complete in issue order, so the LDS store of the B
-        // tile does not wait for the A prefetch.  With a generated V the A
-        // values of chunk v+1 are built between the (dependent) MFMAs of chunk
-        // v, from records loaded two chunks ahead.
-        const double *tb0 = tas + ch.off_tp;
-        CT a0[G], a1[G];
-        int2 r0 = {0, 0}, r1 = {0, 0};
-        auto stage = [&](int v, int buf, CT (&acur)[G], CT (&anext)[G], int2 &rnext, int2 &rnext2) {
-            load_b(tb0, kend, kbeg + (v + 1) * KCH, b);
+            // V chunks, software-pipelined over two register sets and two LDS
+            // buffers, unrolled by two so that no copy joins the sets: while the
+            // MFMAs of chunk v run, chunk v+1 is in flight, and the wait for it
+            // comes only before its own MFMAs, with chunk v+2 already issued.
+            // Prefetches past the last chunk read the zero buffer.  B is loaded
+            // before A: loads complete in issue order, so the LDS store of the B
+            // tile does not wait for the A prefetch.  With a generated V the A
+            // values of chunk v+1 are built between the (dependent) MFMAs of chunk
+            // v, from records loaded two chunks ahead.
+            const double *tb0 = tas + ch.off_tp;
+            CT a0[G], a1[G];
+            int2 r0 = {0, 0}, r1 = {0, 0};
+            auto stage = [&](int v, int buf, CT (&acur)[G], CT (&anext)[G], int2 &rnext, int2 &rnext2) {
+                load_b(tb0, kend, kbeg + (v + 1) * KCH, b, sm);
+                if (!GEN) {
+                    load_a(v + 1, anext);
+                } else {
+                    load_r(v + 2, rnext2);
+                    gen_a(rnext, anext);
+                }
+                mfma_chunk(buf, acur);
+                store_b(buf ^ 1, b, sm);
+                __syncthreads();
+            };
+            load_b(tb0, kend, kbeg, b, sm);
             if (!GEN) {
-                load_a(v + 1, anext);
+                load_a(0, a0);
             } else {
-                load_r(v + 2, rnext2);
-                gen_a(rnext, anext);
+                load_r(0, r0);
+                load_r(1, r1);
+                gen_a(r0, a0);
             }
-            mfma_chunk(buf, acur);
-            store_b(buf ^ 1, b);
+            store_b(0, b, sm);
             __syncthreads();
-        };
-        load_b(tb0, kend, kbeg, b);
-        if (!GEN) {
-            load_a(0, a0);
-        } else {
-            load_r(0, r0);
-            load_r(1, r1);
-            gen_a(r0, a0);
-        }
-        store_b(0, b);
-        __syncthreads();
-        for (int v = 0; v < nv; v += 2) {
-            stage(v, 0, a0, a1, r1, r0);
-            if (v + 1 >= nv) break;
-            stage(v + 1, 1, a1, a0, r0, r1);
-        }
-#pragma unroll
-        for (int n = 0; n < NT; n++) {
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int row = wave * 16 + Mma<CT>::row(q, lane), col = n * 16 + m;
-                if (row < tl.nrows && col < H)
-                    dp[ch.off_dp + (int64_t)seg * npk * H + (int64_t)(tl.row0 + row) * H + col] = acc[n][q];
+            for (int v = 0; v < nv; v += 2) {
+                stage(v, 0, a0, a1, r1, r0);
+                if (v + 1 >= nv) break;
+                stage(v + 1, 1, a1, a0, r0, r1);
             }
-        }
+#pragma unroll
+            for (int n = 0; n < NT; n++) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int row = wave * 16 + Mma<CT>::row(q, lane), col = n * 16 + m;
+                    if (row < tl.nrows && col < H) {
+                        CT val = acc[n][q];
+                        if (ONE)
+                            val = flip_sign(val, *reinterpret_cast<const uint32_t *>(lg.rec2 + ch.pr_off + tl.row0 + row));
+                        dp[ch.off_dp + (int64_t)seg * npk * H + (int64_t)(tl.row0 + row) * H + col] = val;
+                    }
+                }
+            }
+        };
+        if (SRC == LV_GAS && ch.spin0) tile(std::true_type{});
+        else tile(std::false_type{});
     }
 }
 
