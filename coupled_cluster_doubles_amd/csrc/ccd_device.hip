@@ -756,18 +756,24 @@ int build_generated_ladder(ccd_ctx *c, const ccd_problem *p, const std::vector<i
         const uint32_t x = 8u * D * D * (a[0] + m), y = 8u * D * (a[1] + m), z = 8u * (a[2] + m);
         return int2{(int)(x | y << 16), (int)z};
     };
+    // Channels whose pairs have two particles of the same spin (or a basis
+    // of one spin): both terms survive for every element, no spin mask, so
+    // the records carry no spin field either (spin0 = 2).
     for (DChan &ch : c->chans) {
         ch.spin0 = 0;
-        if (gas && s_hi != s_lo && ch.ndiag == 0 && ch.npk > 0) {
+        if (gas && ch.npk + ch.ndiag > 0) {
             const int2 f = ppl[ch.pp_off + prow[ch.pr_off]];
-            ch.spin0 = qn[5 * f.x + 3] != qn[5 * f.y + 3];
+            ch.spin0 = qn[5 * f.x + 3] != qn[5 * f.y + 3] && ch.ndiag == 0 ? 1 : 2;
         }
         for (int u = 0; u < ch.npk + ch.ndiag; u++) {
             const int2 ab = ppl[ch.pp_off + prow[ch.pr_off + u]];
-            if (ch.spin0) {
+            if (ch.spin0 == 1) {
                 const bool first_up = qn[5 * ab.x + 3] == s_hi;
                 rec1[ch.pr_off + u] = scaled_up(first_up ? ab.x : ab.y);
                 rec2[ch.pr_off + u] = int2{first_up ? 0 : (int)0x80000000u, 0};
+            } else if (ch.spin0 == 2) {
+                rec1[ch.pr_off + u] = scaled_up(ab.x);
+                rec2[ch.pr_off + u] = scaled_up(ab.y);
             } else if (gas) {
                 rec1[ch.pr_off + u] = scaled(ab.x);
                 rec2[ch.pr_off + u] = scaled(ab.y);

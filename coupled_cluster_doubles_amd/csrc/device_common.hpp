@@ -25,7 +25,8 @@ struct DChan {          // active direct channel (pp > 0 and hh > 0)
     int32_t nseg;       // number of k segments of the pppp ladder (slab nseg: 1/2 T X)
     int32_t prob;       // solve the channel belongs to (0 outside a batch)
     int32_t spin0;      // generated electron-gas V: 1 when every pair has one particle of each
-                        // spin (one table term per element, see k_ladder_v)
+                        // spin (one table term per element), 2 when both have the same spin
+                        // (two terms, no spin mask), see k_ladder_v
     int32_t pad;
 };
 

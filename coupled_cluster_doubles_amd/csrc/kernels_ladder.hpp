@@ -210,11 +210,14 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
     for (int ti = bptr[blockIdx.x]; ti < bptr[blockIdx.x + 1]; ti++) {
         const DTile tl = tiles[ti];
         const DChan ch = chans[tl.chan];
-        // ONE: a total-spin-0 channel of the generated electron gas, V from one
-        // table term (records of k_up; the signs s(cd) applied to the B rows,
-        // s(ab) to the output rows, so every product equals the two-term one)
-        auto tile = [&](auto one_) {
-            constexpr bool ONE = decltype(one_)::value;
+        // MODE 1 (ONE): a total-spin-0 channel of the generated electron gas,
+        // V from one table term (records of k_up; the signs s(cd) applied to
+        // the B rows, s(ab) to the output rows, so every product equals the
+        // two-term one).  MODE 2 (SAME): pairs of one spin, two terms without
+        // the spin mask (same addresses, same values).
+        auto tile = [&](auto mode_) {
+            constexpr int MODE = decltype(mode_)::value;
+            constexpr bool ONE = MODE == 1, SAME = MODE == 2;
             const int H = ch.hh, npk = ch.npk, ldv = ch.ldv, seg = tl.aux;
             const int kbeg = seg * KSEG, kend = min(npk, kbeg + KSEG);
             const int nv = (kend - kbeg + KCH - 1) / KCH;
@@ -256,6 +259,8 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
                     const int2 r = {row_bcast<s>(rr.x), row_bcast<s>(rr.y)};
                     if (SRC == LV_GAS && ONE) {
                         a[s] = (CT)lds_load(gas_addr1(ra, r, tbase));
+                    } else if (SRC == LV_GAS && SAME) {
+                        a[s] = (CT)(lds_load(gas_addr1(ra, r, tbase)) - lds_load(gas_addr1(rb, r, tbase)));
                     } else if (SRC == LV_GAS) {
                         a[s] = (CT)(lds_load(gas_addr(ra, r, tbase, zaddr)) - lds_load(gas_addr(rb, r, tbase, zaddr)));
                     } else {
@@ -350,8 +355,9 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
                 }
             }
         };
-        if (SRC == LV_GAS && ch.spin0) tile(std::true_type{});
-        else tile(std::false_type{});
+        if (SRC == LV_GAS && ch.spin0 == 1) tile(std::integral_constant<int, 1>{});
+        else if (SRC == LV_GAS && ch.spin0 == 2) tile(std::integral_constant<int, 2>{});
+        else tile(std::integral_constant<int, 0>{});
     }
 }
 
