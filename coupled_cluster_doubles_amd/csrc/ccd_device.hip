@@ -914,6 +914,9 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
                     prow_sw.push_back(q);
                 }
             const int ndiag = (int)prow.size() - pr_off - npk;
+            if ((int64_t)npk * H * 8 >= (int64_t)INT32_MAX)   // the ladder's buffer loads use 32-bit offsets
+                return ccd::fail(CCD_E_INVALID, "channel %d has %lld packed amplitudes (limit 2^28)", k,
+                                 (long long)npk * H);
             const int ldv = (npk + 15) / 16 * 16;
             const int nseg = ladder_segments(npk, (H + 15) / 16);
             DChan ch{off_t, off_vp, off_tp, off_hhhh, P, H, (int32_t)ipp, (int32_t)ihh, npk, ldv, pr_off, ndiag,

@@ -12,6 +12,32 @@ constexpr int kZeroDoubles = 32;  // size of the zero buffer masked-off loads re
 constexpr int kElogCap = 1024;    // energies logged by a device-run iteration loop
 constexpr int kSolveBatch = 4;    // iterations queued between two reads of the loop state
 
+// Raw (unswizzled) buffer loads.  A lane whose byte offset is at or beyond the
+// buffer's size reads 0 without touching memory (the hardware range check), so
+// an element outside a tile needs no select: its offset is kOOB (the sum of
+// kOOB and an in-buffer offset below 2^31 stays out of range).  Buffers hold
+// fewer than 2^31 bytes.
+constexpr uint32_t kOOB = 0x80000000u;
+typedef __amdgpu_buffer_rsrc_t BufRsrc;
+// base and bytes are wave-uniform; the descriptor is built from lane 0's values
+// so that it lives in scalar registers.
+__device__ __forceinline__ BufRsrc make_rsrc(const void *base, uint32_t bytes) {
+    const uint64_t p = reinterpret_cast<uintptr_t>(base);
+    const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)p);
+    const uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(p >> 32));
+    return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void *>(hi << 32 | lo), (short)0,
+                                             __builtin_amdgcn_readfirstlane((int)bytes), 0x00020000);
+}
+__device__ __forceinline__ double buf_f64(BufRsrc r, uint32_t off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ int2 buf_i2(BufRsrc r, uint32_t off) {
+    return __builtin_bit_cast(int2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ uint32_t buf_u32(BufRsrc r, uint32_t off) {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+}
+
 struct DChan {          // active direct channel (pp > 0 and hh > 0)
     int64_t off_t;      // block offset in the pphh layout
     int64_t off_vp;     // packed V^pppp block: npk x ldv, rows/cols = pairs a<b

@@ -199,9 +199,13 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
     static_assert(KSEG % KCH == 0 && G % VW == 0, "k segments of whole chunks, chunks of whole 16-byte loads");
     __shared__ CT Bs[2][KCH][STR];
     extern __shared__ double gtab[];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
+    // tid < kThreads (the launch bound) as a value the compiler can see, so that
+    // the B-tile slot bounds (KCH * W is a multiple of kThreads) fold away and
+    // the tile loads stay ahead of their stores, outside any branch
+    const int tid = threadIdx.x & (kThreads - 1);
+    const int wave = tid >> 6, lane = tid & 63, m = lane & 15, kq = lane >> 4;
     if (SRC == LV_GAS) {
-        for (int i = threadIdx.x; i < lg.ntab; i += kThreads) gtab[i] = lg.tab[i];
+        for (int i = tid; i < lg.ntab; i += kThreads) gtab[i] = lg.tab[i];
         __syncthreads();
     }
     const uint32_t tbase = (uint32_t)(uintptr_t)(lds_double *)gtab;  // LDS byte address
@@ -249,9 +253,12 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
                     for (int e = 0; e < VW; e++) a[VW * h + e] = w[e];
                 }
             };
+            // Buffer loads of the records (generated V) and of the B tiles: rows
+            // at or past kend lie outside their buffers and read 0.
+            const BufRsrc rsrc = GEN ? make_rsrc(lg.rec1 + ch.pr_off, (uint32_t)kend * 8u) : make_rsrc(zero, 0);
+            const uint32_t roff = m < G ? (uint32_t)(kbeg + kslot<CT>(kq, m)) * 8u : kOOB;
             auto load_r = [&](int v, int2 &rr) {  // generated V: record of k-slot (kq, m) of V chunk v
-                const int k = kbeg + v * KCH + kslot<CT>(kq, m);
-                rr = *((m < G && k < kend) ? lg.rec1 + ch.pr_off + k : reinterpret_cast<const int2 *>(zero));
+                rr = buf_i2(rsrc, roff + (uint32_t)(v * KCH) * 8u);
             };
             auto gen_a = [&](int2 rr, CT (&a)[G]) {  // generated V from the chunk's records
                 static_for<0, G>([&](auto S) {
@@ -268,22 +275,29 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
                     }
                 });
             };
-            // B tile of k rows k0.. of a K x H row-major block
+            // B tile: k rows k0.. of the channel's npk x H amplitude rows
             // (ONE: with the sign mask of row k, applied when the tile is stored)
-            auto load_b = [&](const double *B, int K, int k0, CT (&b)[BPT], uint32_t (&sm)[BPT]) {
+            const BufRsrc bsrc = make_rsrc(tas + ch.off_tp, (uint32_t)kend * (uint32_t)H * 8u);
+            const BufRsrc ssrc = ONE ? make_rsrc(lg.rec2 + ch.pr_off, (uint32_t)kend * 8u) : make_rsrc(zero, 0);
+            uint32_t boff[BPT], soff[BPT];
+#pragma unroll
+            for (int q = 0; q < BPT; q++) {
+                const int idx = tid + kThreads * q, kk = idx / W, col = idx - kk * W;
+                boff[q] = (idx < KCH * W && col < H) ? (uint32_t)(kk * H + col) * 8u : kOOB;
+                soff[q] = idx < KCH * W ? (uint32_t)kk * 8u : kOOB;
+            }
+            auto load_b = [&](int k0, CT (&b)[BPT], uint32_t (&sm)[BPT]) {
+                const uint32_t ob = (uint32_t)k0 * (uint32_t)H * 8u, os = (uint32_t)k0 * 8u;
 #pragma unroll
                 for (int q = 0; q < BPT; q++) {
-                    const int idx = threadIdx.x + kThreads * q, kk = idx / W, col = idx - kk * W, k = k0 + kk;
-                    b[q] = (CT) * ((idx < KCH * W && k < K && col < H) ? B + (int64_t)k * H + col : zero);
-                    if (ONE)
-                        sm[q] = *((idx < KCH * W && k < K) ? reinterpret_cast<const uint32_t *>(lg.rec2 + ch.pr_off + k)
-                                                           : reinterpret_cast<const uint32_t *>(zero));
+                    b[q] = (CT)buf_f64(bsrc, boff[q] + ob);
+                    if (ONE) sm[q] = buf_u32(ssrc, soff[q] + os);
                 }
             };
             auto store_b = [&](int buf, const CT (&b)[BPT], const uint32_t (&sm)[BPT]) {
 #pragma unroll
                 for (int q = 0; q < BPT; q++) {
-                    const int idx = threadIdx.x + kThreads * q, kk = idx / W, col = idx - kk * W;
+                    const int idx = tid + kThreads * q, kk = idx / W, col = idx - kk * W;
                     if (idx < KCH * W) Bs[buf][kk][col] = ONE ? flip_sign(b[q], sm[q]) : b[q];
                 }
             };
@@ -299,23 +313,25 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
                         acc[n] = mfma16(a[s], Bs[buf][kslot<CT>(kq, s)][n * 16 + m], acc[n]);
                 }
             };
-            CT b[BPT];
-            uint32_t sm[BPT] = {};
+            CT b0[BPT], b1[BPT];
+            uint32_t sm0[BPT] = {}, sm1[BPT] = {};
 
             // V chunks, software-pipelined over two register sets and two LDS
             // buffers, unrolled by two so that no copy joins the sets: while the
             // MFMAs of chunk v run, chunk v+1 is in flight, and the wait for it
             // comes only before its own MFMAs, with chunk v+2 already issued.
-            // Prefetches past the last chunk read the zero buffer.  B is loaded
-            // before A: loads complete in issue order, so the LDS store of the B
-            // tile does not wait for the A prefetch.  With a generated V the A
-            // values of chunk v+1 are built between the (dependent) MFMAs of chunk
-            // v, from records loaded two chunks ahead.
-            const double *tb0 = tas + ch.off_tp;
+            // The B tile of chunk v+2 is loaded into registers during chunk v and
+            // goes to LDS at the end of chunk v+1, so each tile load has two chunks
+            // of work behind it.  Loads past the last chunk lie outside their
+            // buffers and read 0.  With a generated V the A values of chunk v+1 are
+            // built between the (dependent) MFMAs of chunk v, from records loaded
+            // two chunks ahead.
             CT a0[G], a1[G];
             int2 r0 = {0, 0}, r1 = {0, 0};
-            auto stage = [&](int v, int buf, CT (&acur)[G], CT (&anext)[G], int2 &rnext, int2 &rnext2) {
-                load_b(tb0, kend, kbeg + (v + 1) * KCH, b, sm);
+            auto stage = [&](int v, int buf, CT (&acur)[G], CT (&anext)[G], int2 &rnext, int2 &rnext2,
+                             CT (&bload)[BPT], uint32_t (&smload)[BPT], const CT (&bnext)[BPT],
+                             const uint32_t (&smnext)[BPT]) {
+                load_b(kbeg + (v + 2) * KCH, bload, smload);
                 if (!GEN) {
                     load_a(v + 1, anext);
                 } else {
@@ -323,10 +339,11 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
                     gen_a(rnext, anext);
                 }
                 mfma_chunk(buf, acur);
-                store_b(buf ^ 1, b, sm);
+                store_b(buf ^ 1, bnext, smnext);
                 __syncthreads();
             };
-            load_b(tb0, kend, kbeg, b, sm);
+            load_b(kbeg, b0, sm0);
+            load_b(kbeg + KCH, b1, sm1);
             if (!GEN) {
                 load_a(0, a0);
             } else {
@@ -334,12 +351,12 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
                 load_r(1, r1);
                 gen_a(r0, a0);
             }
-            store_b(0, b, sm);
+            store_b(0, b0, sm0);
             __syncthreads();
             for (int v = 0; v < nv; v += 2) {
-                stage(v, 0, a0, a1, r1, r0);
+                stage(v, 0, a0, a1, r1, r0, b0, sm0, b1, sm1);
                 if (v + 1 >= nv) break;
-                stage(v + 1, 1, a1, a0, r0, r1);
+                stage(v + 1, 1, a1, a0, r0, r1, b1, sm1, b0, sm0);
             }
 #pragma unroll
             for (int n = 0; n < NT; n++) {
