@@ -43,15 +43,18 @@ struct UpdArgs {
 // contractions, which they hold exactly); the update itself runs in FP64.
 template <bool UPDATE, typename OT>
 __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
+    const DTile tl = a.tiles[blockIdx.x];
     if (a.ctl->done) return;
     const OT *gg = static_cast<const OT *>(a.g);
     __shared__ double w1s[kThreads], w2s[kThreads], cs[kThreads];
-    const DTile tl = a.tiles[blockIdx.x];
     const DChan ch = a.chans[tl.chan];
     const bool frozen = UPDATE && a.pctl[ch.prob].done;
     const int H = ch.hh, n = tl.nrows * H;
-    const int l = threadIdx.x;
+    const int l = threadIdx.x & (kThreads - 1);
     if (l < n) {
+        // every load of the element pair is issued before the first store (the
+        // stores alias nothing the loads read, but the compiler cannot know
+        // that)
         const int pr = tl.row0 + l / H, IJ = l % H;
         const int JI = a.hh_swap[ch.hh_off + IJ];
         const int r1 = a.prow[ch.pr_off + pr], r2 = a.prow_sw[ch.pr_off + pr];
@@ -65,14 +68,24 @@ __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
             const int ia = ab.x, ib = ab.y, ii = ij.x, jj = ij.y;
             const double t1s = a.t_old[b1 + JI], t2s = a.t_old[b2 + JI];
             const double g1 = gg[b1 + IJ], g1s = gg[b1 + JI], g2 = gg[b2 + IJ], g2s = gg[b2 + JI];
+            // D_p: the slabs 0..nseg added in order, loaded four at a time
+            // (slab indices clamped to nseg; the clamped loads are not added)
             double d = 0.0;
             if (!diag) {
                 const int64_t slab = (int64_t)ch.npk * H;
                 const OT *dq = static_cast<const OT *>(a.dp) + ch.off_dp + (int64_t)pr * H + IJ;
-                for (int q = 0; q <= ch.nseg; q++) d += (double)dq[q * slab];
+                for (int q0 = 0; q0 <= ch.nseg; q0 += 4) {
+                    OT dv[4];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) dv[u] = dq[min(q0 + u, ch.nseg) * slab];
+#pragma unroll
+                    for (int u = 0; u < 4; u++)
+                        if (q0 + u <= ch.nseg) d += (double)dv[u];
+                }
             }
             const double ci = 0.5 * a.chi[ii], cj = 0.5 * a.chi[jj];
             const double pa = -0.5 * a.psi[ia - a.nh], pb = -0.5 * a.psi[ib - a.nh];
+            const double ei = a.eps[ii], ej = a.eps[jj], ea = a.eps[ia], eb = a.eps[ib];
             // element (a,b,i,j), in the order of src/calc.cpp:393-408
             double x1 = v1;
             x1 += d;
@@ -84,15 +97,15 @@ __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
             x1 -= cj * t1;
             x1 += pb * t1;
             x1 -= pa * t2;
-            x1 = x1 / (a.eps[ii] + a.eps[jj] - a.eps[ia] - a.eps[ib]);
+            x1 = x1 / (ei + ej - ea - eb);
             x1 = a.mixing * x1 + (1. - a.mixing) * t1;
             if (frozen) x1 = t1;
-            a.t_new[b1 + IJ] = x1;
             w1 = v1 * x1;
             w2 = 0.0;
+            double x2 = 0.0;
             if (!diag) {
                 // element (b,a,i,j)
-                double x2 = v2;
+                x2 = v2;
                 x2 -= d;
                 x2 += g2;
                 x2 -= g2s;
@@ -102,12 +115,15 @@ __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
                 x2 -= cj * t2;
                 x2 += pa * t2;
                 x2 -= pb * t1;
-                x2 = x2 / (a.eps[ii] + a.eps[jj] - a.eps[ib] - a.eps[ia]);
+                x2 = x2 / (ei + ej - eb - ea);
                 x2 = a.mixing * x2 + (1. - a.mixing) * t2;
                 if (frozen) x2 = t2;
+                w2 = v2 * x2;
+            }
+            a.t_new[b1 + IJ] = x1;
+            if (!diag) {
                 a.t_new[b2 + IJ] = x2;
                 a.tas[ch.off_tp + (int64_t)pr * H + IJ] = 0.5 * (x1 - x2);
-                w2 = v2 * x2;
             }
         } else {
             w1 = v1 * t1;
