@@ -1052,9 +1052,9 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
         tr_base[a + 1] = tr_base[a] + blk[a];
     }
     if (tr_base[np] != c->nt2) return ccd::fail(CCD_E_INVALID, "particle blocks do not cover T2");
-    if (max_blk > 65535 || (size_t)max_blk * sizeof(double) > 64 * 1024)
+    if (max_blk > 65535 || (size_t)(max_blk + 1) * sizeof(double) > 64 * 1024)
         return ccd::fail(CCD_E_INVALID, "a particle block of %d elements exceeds the transpose kernel's LDS", max_blk);
-    c->tr_lds = (size_t)std::max(max_blk, 1) * sizeof(double);
+    c->tr_lds = (size_t)(max_blk + 1) * sizeof(double);   // plus the spare slot
     std::vector<int32_t> addr_pp(c->nt2, -1), addr_ph(c->nt2, -1);
     for (const DChan &ch : c->chans)
         for (int AB = 0; AB < ch.pp; AB++) {

@@ -396,19 +396,26 @@ __device__ __forceinline__ void ladder_tx_wave(const DTile &tl, const DChan &ch,
     Acc acc[NT];
 #pragma unroll
     for (int n = 0; n < NT; n++) acc[n] = Acc{};
+    // per 16-wide k step: every operand load first, then the MFMAs (loaded
+    // inside the MFMA loop, each product waited for its own loads)
 #pragma unroll
     for (int kt = 0; kt < NT; kt++) {
+        CT av[4], bv[4][NT];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int k = kt * 16 + 4 * kq + j;
             const bool kok = k < H;
-            const CT a = (CT)(0.5 * *((rok && kok) ? at + k : zero));
+            av[j] = (CT)(0.5 * *((rok && kok) ? at + k : zero));
 #pragma unroll
             for (int n = 0; n < NT; n++) {
                 const int col = n * 16 + m;
-                const CT b = (CT) * ((kok && col < H) ? X + k * H + col : zero);
-                acc[n] = mfma16(a, b, acc[n]);
+                bv[j][n] = (CT) * ((kok && col < H) ? X + k * H + col : zero);
             }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+#pragma unroll
+            for (int n = 0; n < NT; n++) acc[n] = mfma16(av[j], bv[j][n], acc[n]);
         }
     }
     CT *out = dp + ch.off_dp + (int64_t)ch.nseg * ch.npk * H;

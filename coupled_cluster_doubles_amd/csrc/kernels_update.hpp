@@ -167,6 +167,7 @@ __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
 // reference's mapToDistributed(ai_jb); with src = phhp it gathers R into the
 // pphh layout.  Elements are converted from S to E on the way (T^ph for FP32
 // contractions is the FP32 rounding of T, which is all the FP32 ring reads).
+// The LDS block holds the largest particle block plus one spare slot.
 template <typename S, typename E = S>
 __global__ __launch_bounds__(kThreads) void k_transpose(const int32_t *__restrict__ base,
                                                         const int32_t *__restrict__ src_addr,
@@ -195,10 +196,12 @@ __global__ __launch_bounds__(kThreads) void k_transpose(const int32_t *__restric
         }
 #pragma unroll
         for (int u = 0; u < U; u++) v[u] = (E)src[ad[u]];
+        // slots past the block go to the spare LDS slot n (never read), so
+        // that no load sits behind a branch
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int l = l0 + u * kThreads;
-            if (l < n) buf[l] = v[u];
+            buf[l < n ? l : n] = v[u];
         }
     }
     __syncthreads();
