@@ -70,6 +70,8 @@ extern "C" const char *ccd_last_error(void) { return ccd::g_err.c_str(); }
 
 // ------------------------------------------------------------------ context
 
+constexpr int kStale = -1;   // tph_prec / tas_prec: holds a caller's term, rebuild before iterating
+
 struct ccd_ctx {
     int device = 0;
     int precision = CCD_PREC_FP64;
@@ -100,7 +102,11 @@ struct ccd_ctx {
     int n_tt_narrow = 0;           // leading T X tiles of channels with hh <= 80
     size_t tr_lds = 0;
     int iter_prec = CCD_PREC_FP64;  // arithmetic of the contractions of the next iterations
-    int tph_prec = CCD_PREC_FP64;   // element type T^ph was last written in (the ring reads it in iter_prec)
+    // element type T^ph and the antisymmetrised rows (d_tas) were last written
+    // in (the ring and the ladder read them in iter_prec); kStale: not derived
+    // from the current amplitudes (an entry point computed a caller's term)
+    int tph_prec = CCD_PREC_FP64;
+    int tas_prec = CCD_PREC_FP64;
     // device tables
     DChan *d_chans = nullptr;
     DMod *d_mods = nullptr;
@@ -317,9 +323,14 @@ int launch_tremap(ccd_ctx *c, const double *t, hipStream_t st, bool force = fals
     return check_launch("k_transpose");
 }
 
-// Rebuild T^ph from the current amplitudes when its type is not the one the
-// next iterations read (after a change of precision).
-int sync_tph(ccd_ctx *c) {
+int launch_antisym(ccd_ctx *c, const double *t);
+
+// Rebuild T^ph and the antisymmetrised rows from the current amplitudes when
+// their type is not the one the next iterations read (after a change of
+// precision) or they hold a caller's term (kStale).
+int sync_state(ccd_ctx *c) {
+    if (c->tas_prec != c->iter_prec)
+        if (int rc = launch_antisym(c, c->d_t[c->cur])) return rc;
     if (c->tph_prec == c->iter_prec) return CCD_OK;
     return launch_tremap(c, c->d_t[c->cur], c->stream, true);
 }
@@ -337,17 +348,28 @@ int launch_remap_r(ccd_ctx *c, hipStream_t st) {
     return check_launch("k_transpose");
 }
 
+// The antisymmetrised packed rows of t, in the arithmetic type of the ladder
 int launch_antisym(ccd_ctx *c, const double *t) {
+    c->tas_prec = c->iter_prec;
     if (c->rtiles.empty()) return CCD_OK;
-    hipLaunchKernelGGL(k_antisym, dim3((unsigned)c->rtiles.size()), dim3(kThreads), 0, c->stream, c->d_rtiles,
-                       c->d_chans, c->d_prow, c->d_prow_sw, t, c->d_tas);
+    if (c->iter_prec == CCD_PREC_FP32)
+        hipLaunchKernelGGL(k_antisym<float>, dim3((unsigned)c->rtiles.size()), dim3(kThreads), 0, c->stream,
+                           c->d_rtiles, c->d_chans, c->d_prow, c->d_prow_sw, t, reinterpret_cast<float *>(c->d_tas));
+    else
+        hipLaunchKernelGGL(k_antisym<double>, dim3((unsigned)c->rtiles.size()), dim3(kThreads), 0, c->stream,
+                           c->d_rtiles, c->d_chans, c->d_prow, c->d_prow_sw, t, c->d_tas);
     return check_launch("k_antisym");
 }
 
 int launch_ladder_x(ccd_ctx *c, hipStream_t st) {
     if (c->xtiles.empty()) return CCD_OK;
-    hipLaunchKernelGGL(k_ladder_x, dim3((unsigned)c->xtiles.size()), dim3(kThreads), 0, st, c->d_xtiles,
-                       c->d_chans, c->d_vtp, c->d_tas, c->d_vh, c->d_x, c->d_ctl);
+    if (c->iter_prec == CCD_PREC_FP32)
+        hipLaunchKernelGGL(k_ladder_x<float>, dim3((unsigned)c->xtiles.size()), dim3(kThreads), 0, st, c->d_xtiles,
+                           c->d_chans, c->d_vtp, reinterpret_cast<const float *>(c->d_tas), c->d_vh, c->d_zero,
+                           c->d_x, c->d_ctl);
+    else
+        hipLaunchKernelGGL(k_ladder_x<double>, dim3((unsigned)c->xtiles.size()), dim3(kThreads), 0, st, c->d_xtiles,
+                           c->d_chans, c->d_vtp, c->d_tas, c->d_vh, c->d_zero, c->d_x, c->d_ctl);
     return check_launch("k_ladder_x");
 }
 
@@ -360,7 +382,7 @@ void launch_ladder_v_nt(ccd_ctx *c, const CT *vp, hipStream_t st) {
     const size_t lds = SRC == LV_GAS ? sizeof(double) * (size_t)c->lgen.ntab : 0;
     hipLaunchKernelGGL((k_ladder_v<CT, NT, SRC>), dim3((unsigned)grid), dim3(kThreads), lds, st,
                        c->d_ltiles + c->lt_off[NT - 1], c->d_lbptr + c->lb_base[NT], c->d_chans, vp, c->lgen,
-                       c->d_tas, c->d_x,
+                       reinterpret_cast<const CT *>(c->d_tas), c->d_x,
                        c->d_zero, reinterpret_cast<CT *>(c->d_dp), c->d_ctl);
 }
 
@@ -391,7 +413,8 @@ void launch_ladder_tx_ct(ccd_ctx *c, int lo, int hi, hipStream_t st) {
     if (n <= 0) return;
     const dim3 grid((unsigned)((n + kThreads / 64 - 1) / (kThreads / 64)));
     hipLaunchKernelGGL((k_ladder_tx<CT, WIDE>), grid, dim3(kThreads), 0, st, c->d_ttiles + lo, n, c->d_chans,
-                       c->d_tas, c->d_x, c->d_zero, reinterpret_cast<CT *>(c->d_dp), c->d_ctl);
+                       reinterpret_cast<const CT *>(c->d_tas), c->d_x, c->d_zero, reinterpret_cast<CT *>(c->d_dp),
+                       c->d_ctl);
 }
 
 int launch_ladder_tx(ccd_ctx *c, hipStream_t st) {
@@ -415,6 +438,7 @@ int launch_ladder_v(ccd_ctx *c, int lo, int hi, hipStream_t st) {
 
 template <bool UPDATE>
 int launch_update(ccd_ctx *c, const double *t_old, double *t_new, double mixing, hipStream_t st) {
+    if (UPDATE) c->tas_prec = c->iter_prec;   // the update writes them in OT
     if (c->utiles.empty()) return CCD_OK;
     UpdArgs a = upd_args(c, t_old, t_new, mixing);
     if (UPDATE && c->iter_prec == CCD_PREC_FP32)
@@ -1421,7 +1445,7 @@ int read_pctl(ccd_ctx *c) {
 
 int arm_free_run(ccd_ctx *c) {
     int rc = arm_loop(c, -1.0, INT_MAX, true);
-    return rc ? rc : sync_tph(c);   // T^ph in the type the next iterations read
+    return rc ? rc : sync_state(c);   // T^ph in the type the next iterations read
 }
 
 // Queue iterations in batches without waiting for each energy; after each
@@ -1465,6 +1489,7 @@ int zero_state(ccd_ctx *c) {
     HIP_TRY(hipMemsetAsync(c->d_tph, 0, nt, c->stream));   // zero in either element type
     c->tph_prec = c->iter_prec;
     HIP_TRY(hipMemsetAsync(c->d_tas, 0, sizeof(double) * (size_t)std::max<int64_t>(c->n_tp, 1), c->stream));
+    c->tas_prec = c->iter_prec;
     HIP_TRY(hipMemsetAsync(c->d_onebody, 0, sizeof(double) * c->nsp, c->stream));
     c->cur = 0;
     return arm_free_run(c);
@@ -1635,7 +1660,8 @@ int ccd_batch_solve_mixed(ccd_ctx *c, double tol, double switch_tol, int max_ite
     std::vector<LoopCtl> phase1(c->nprob);
     int rc = zero_state(c);
     c->iter_prec = CCD_PREC_FP32;
-    c->tph_prec = CCD_PREC_FP32;   // T^ph is zero: the same in either type
+    c->tph_prec = CCD_PREC_FP32;   // T^ph and tas are zero: the same in either type
+    c->tas_prec = CCD_PREC_FP32;
     if (!rc) rc = arm_loop(c, std::max(tol, switch_tol), max_iters, true);
     if (!rc) rc = run_loop(c, mixing, kSolveBatch);
     if (!rc) rc = read_pctl(c);
@@ -1644,12 +1670,12 @@ int ccd_batch_solve_mixed(ccd_ctx *c, double tol, double switch_tol, int max_ite
     if (!rc && switch_tol > tol) {
         c->iter_prec = CCD_PREC_FP64;
         rc = arm_loop(c, tol, max_iters, false);
-        if (!rc) rc = sync_tph(c);
+        if (!rc) rc = sync_state(c);
         if (!rc) rc = run_loop(c, mixing, kSolveBatch);
         if (!rc) rc = read_pctl(c);
     }
     c->iter_prec = saved;
-    const int rs = sync_tph(c);
+    const int rs = sync_state(c);
     if (rc) return rc;
     if (rs) return rs;
     for (int q = 0; q < c->nprob; q++) {
@@ -1781,7 +1807,7 @@ int ccd_set_precision(ccd_ctx *c, int precision) {
     if (precision == CCD_PREC_FP32 && c->precision != CCD_PREC_FP32)
         return ccd::fail(CCD_E_STATE, "FP32 iterations need a context created with CCD_PREC_FP32");
     c->iter_prec = precision;
-    return sync_tph(c);
+    return sync_state(c);
 }
 
 int ccd_solve_mixed(ccd_ctx *c, double tol, double switch_tol, int max_iters, double mixing, double *energies,
@@ -1796,7 +1822,8 @@ int ccd_solve_mixed(ccd_ctx *c, double tol, double switch_tol, int max_iters, do
     int rc = zero_state(c);
     // FP32 contractions while |dE| > max(tol, switch_tol) ...
     c->iter_prec = CCD_PREC_FP32;
-    c->tph_prec = CCD_PREC_FP32;   // T^ph is zero: the same in either type
+    c->tph_prec = CCD_PREC_FP32;   // T^ph and tas are zero: the same in either type
+    c->tas_prec = CCD_PREC_FP32;
     if (!rc) rc = arm_loop(c, std::max(tol, switch_tol), max_iters, true);
     if (!rc) rc = run_loop(c, mixing, kSolveBatch);
     int n32 = rc ? 0 : c->h_ctl->count;
@@ -1804,11 +1831,11 @@ int ccd_solve_mixed(ccd_ctx *c, double tol, double switch_tol, int max_iters, do
     if (!rc && c->h_ctl->reason == CCD_STOP_CONVERGED && switch_tol > tol) {
         c->iter_prec = CCD_PREC_FP64;
         rc = arm_loop(c, tol, max_iters, false);
-        if (!rc) rc = sync_tph(c);
+        if (!rc) rc = sync_state(c);
         if (!rc) rc = run_loop(c, mixing, kSolveBatch);
     }
     c->iter_prec = saved;
-    const int rs = sync_tph(c);
+    const int rs = sync_state(c);
     if (rc) return rc;
     if (rs) return rs;
     const LoopCtl st = *c->h_ctl;
@@ -1860,8 +1887,10 @@ int ccd_ladder(ccd_ctx *c, const double *t, double *d) {
     double *dt = c->d_t[1 - c->cur];  // scratch: the non-current T buffer
     int rc;
     if ((rc = t2_upload(c, t, dt, c->d_g))) return rc;
-    if ((rc = launch_antisym(c, dt)) || (rc = launch_ladder_x(c, c->stream)) ||
-        (rc = launch_ladder_tx(c, c->stream)) || (rc = launch_ladder_v(c, 1, NTMAX, c->stream)))
+    rc = launch_antisym(c, dt);
+    c->tas_prec = kStale;   // the caller's rows, not those of the current amplitudes
+    if (rc || (rc = launch_ladder_x(c, c->stream)) || (rc = launch_ladder_tx(c, c->stream)) ||
+        (rc = launch_ladder_v(c, 1, NTMAX, c->stream)))
         return rc;
     if (!c->utiles.empty()) {
         UpdArgs a = upd_args(c, nullptr, nullptr, 0.0);
@@ -1893,7 +1922,7 @@ int ccd_ring(ccd_ctx *c, const double *tph, double *r) {
     } else {
         HIP_TRY(hipMemcpyAsync(c->d_tph, tph, sizeof(double) * c->nt2, hipMemcpyHostToDevice, c->stream));
     }
-    c->tph_prec = c->iter_prec;
+    c->tph_prec = kStale;   // the caller's T^ph, not that of the current amplitudes
     int rc = launch_ring(c, c->stream);
     if (rc) return rc;
     if (c->iter_prec == CCD_PREC_FP32) {   // R in FP32 (exactly the FP32 sums)
@@ -1915,7 +1944,7 @@ int ccd_remap_ph(ccd_ctx *c, const double *t, double *tph) {
     double *dt = c->d_t[1 - c->cur];
     int rc = t2_upload(c, t, dt, c->d_g);
     if (!rc) rc = launch_transpose(c, true, dt, c->d_tph, c->stream);
-    c->tph_prec = CCD_PREC_FP64;
+    c->tph_prec = kStale;   // the caller's T^ph
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(tph, c->d_tph, sizeof(double) * c->nt2, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));

@@ -37,6 +37,11 @@ __device__ __forceinline__ int2 buf_i2(BufRsrc r, uint32_t off) {
 __device__ __forceinline__ uint32_t buf_u32(BufRsrc r, uint32_t off) {
     return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
 }
+template <typename T>
+__device__ __forceinline__ T buf_load(BufRsrc r, uint32_t off) {
+    if constexpr (sizeof(T) == 8) return buf_f64(r, off);
+    else return __builtin_bit_cast(float, buf_u32(r, off));
+}
 
 struct DChan {          // active direct channel (pp > 0 and hh > 0)
     int64_t off_t;      // block offset in the pphh layout

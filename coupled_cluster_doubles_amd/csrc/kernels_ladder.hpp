@@ -20,12 +20,15 @@ constexpr int kGasTableMax = 4096;     // entries of the generated-V gas table (
 
 // X = V^hhhh + sum_{c<d} V^hhpp[kl][cd] tas[cd][ij], one 16x16 tile of X per
 // block; the four waves take every fourth k chunk and their partial tiles are
-// added in wave order through LDS.
+// added in wave order through LDS.  FP64 arithmetic; tas in TT (FP32 in the
+// FP32 iterations, whose ladder reads it so).
+template <typename TT>
 __global__ __launch_bounds__(kThreads) void k_ladder_x(const DTile *__restrict__ tiles,
                                                        const DChan *__restrict__ chans,
                                                        const double *__restrict__ vtp,
-                                                       const double *__restrict__ tas,
+                                                       const TT *__restrict__ tas,
                                                        const double *__restrict__ vh,
+                                                       const double *__restrict__ zero,
                                                        double *__restrict__ x,
                                                        const LoopCtl *__restrict__ ctl) {
     if (ctl->done) return;
@@ -35,15 +38,18 @@ __global__ __launch_bounds__(kThreads) void k_ladder_x(const DTile *__restrict__
     const DChan ch = chans[tl.chan];
     const int H = ch.hh, npk = ch.npk, kl0 = tl.row0, ij0 = tl.aux;
     const bool aok = kl0 + m < H, bok = ij0 + m < H;
+    const TT *zt = reinterpret_cast<const TT *>(zero);
     dbl4 acc = {0.0, 0.0, 0.0, 0.0};
     for (int k0 = wave * KC; k0 < npk; k0 += KC * (kThreads / 64)) {
+        // every load unconditional (elements outside the block read the zero
+        // buffer), so that the 16 loads of a chunk are in flight together
         double a[8], b[8];
 #pragma unroll
         for (int s = 0; s < 8; s++) {
             const int w = k0 + kq * 8 + s;
             const bool ok = w < npk;
-            a[s] = (ok && aok) ? vtp[ch.off_tp + (int64_t)w * H + kl0 + m] : 0.0;
-            b[s] = (ok && bok) ? tas[ch.off_tp + (int64_t)w * H + ij0 + m] : 0.0;
+            a[s] = *((ok && aok) ? vtp + ch.off_tp + (int64_t)w * H + kl0 + m : zero);
+            b[s] = (double)*((ok && bok) ? tas + ch.off_tp + (int64_t)w * H + ij0 + m : zt);
         }
 #pragma unroll
         for (int s = 0; s < 8; s++) acc = mfma16(a[s], b[s], acc);
@@ -154,9 +160,10 @@ __device__ __forceinline__ void static_for(F &&f) {
 // so the gas table is staged into LDS once per block).
 //
 // Arithmetic type CT: double, or float for the mixed-precision iterations
-// (V^pppp stored in FP32, or generated in FP64 and rounded; amplitudes and X
-// rounded on load; FP32 accumulation; D_p written in FP32, which holds the
-// FP32 sums exactly).  For float the
+// (V^pppp stored in FP32, or generated in FP64 and rounded; the antisymmetrised
+// amplitudes stored in FP32 by the update, X rounded on load; FP32
+// accumulation; D_p written in FP32, which holds the FP32 sums exactly).  For
+// float the
 // k-slot order is k = 16 (s/4) + 4q + s%4, again 64 contiguous bytes per row
 // and load instruction.
 template <typename CT>
@@ -179,7 +186,7 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
                                                        const int32_t *__restrict__ bptr,
                                                        const DChan *__restrict__ chans,
                                                        const CT *__restrict__ vp, LadderGen lg,
-                                                       const double *__restrict__ tas,
+                                                       const CT *__restrict__ tas,
                                                        const double *__restrict__ x,
                                                        const double *__restrict__ zero,
                                                        CT *__restrict__ dp,
@@ -277,20 +284,20 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
             };
             // B tile: k rows k0.. of the channel's npk x H amplitude rows
             // (ONE: with the sign mask of row k, applied when the tile is stored)
-            const BufRsrc bsrc = make_rsrc(tas + ch.off_tp, (uint32_t)kend * (uint32_t)H * 8u);
+            const BufRsrc bsrc = make_rsrc(tas + ch.off_tp, (uint32_t)kend * (uint32_t)H * (uint32_t)sizeof(CT));
             const BufRsrc ssrc = ONE ? make_rsrc(lg.rec2 + ch.pr_off, (uint32_t)kend * 8u) : make_rsrc(zero, 0);
             uint32_t boff[BPT], soff[BPT];
 #pragma unroll
             for (int q = 0; q < BPT; q++) {
                 const int idx = tid + kThreads * q, kk = idx / W, col = idx - kk * W;
-                boff[q] = (idx < KCH * W && col < H) ? (uint32_t)(kk * H + col) * 8u : kOOB;
+                boff[q] = (idx < KCH * W && col < H) ? (uint32_t)(kk * H + col) * (uint32_t)sizeof(CT) : kOOB;
                 soff[q] = idx < KCH * W ? (uint32_t)kk * 8u : kOOB;
             }
             auto load_b = [&](int k0, CT (&b)[BPT], uint32_t (&sm)[BPT]) {
-                const uint32_t ob = (uint32_t)k0 * (uint32_t)H * 8u, os = (uint32_t)k0 * 8u;
+                const uint32_t ob = (uint32_t)k0 * (uint32_t)H * (uint32_t)sizeof(CT), os = (uint32_t)k0 * 8u;
 #pragma unroll
                 for (int q = 0; q < BPT; q++) {
-                    b[q] = (CT)buf_f64(bsrc, boff[q] + ob);
+                    b[q] = buf_load<CT>(bsrc, boff[q] + ob);
                     if (ONE) sm[q] = buf_u32(ssrc, soff[q] + os);
                 }
             };
@@ -384,14 +391,15 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
 // independent: operands come straight from global memory (tas rows and the
 // small X block stay in L1/L2).
 template <typename CT, int NT>
-__device__ __forceinline__ void ladder_tx_wave(const DTile &tl, const DChan &ch, const double *__restrict__ tas,
+__device__ __forceinline__ void ladder_tx_wave(const DTile &tl, const DChan &ch, const CT *__restrict__ tas,
                                                const double *__restrict__ x, const double *__restrict__ zero,
                                                CT *__restrict__ dp) {
     typedef typename Mma<CT>::acc Acc;
     const int lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
     const int H = ch.hh;
     const bool rok = m < tl.nrows;
-    const double *at = tas + ch.off_tp + (int64_t)(tl.row0 + m) * H;
+    const CT *at = tas + ch.off_tp + (int64_t)(tl.row0 + m) * H;
+    const CT *zc = reinterpret_cast<const CT *>(zero);
     const double *X = x + ch.off_hhhh;
     Acc acc[NT];
 #pragma unroll
@@ -405,7 +413,7 @@ __device__ __forceinline__ void ladder_tx_wave(const DTile &tl, const DChan &ch,
         for (int j = 0; j < 4; j++) {
             const int k = kt * 16 + 4 * kq + j;
             const bool kok = k < H;
-            av[j] = (CT)(0.5 * *((rok && kok) ? at + k : zero));
+            av[j] = (CT)0.5 * *((rok && kok) ? at + k : zc);
 #pragma unroll
             for (int n = 0; n < NT; n++) {
                 const int col = n * 16 + m;
@@ -434,7 +442,7 @@ __device__ __forceinline__ void ladder_tx_wave(const DTile &tl, const DChan &ch,
 template <typename CT, bool WIDE>
 __global__ __launch_bounds__(kThreads) void k_ladder_tx(const DTile *__restrict__ tiles, int ntiles,
                                                         const DChan *__restrict__ chans,
-                                                        const double *__restrict__ tas,
+                                                        const CT *__restrict__ tas,
                                                         const double *__restrict__ x,
                                                         const double *__restrict__ zero, CT *__restrict__ dp,
                                                         const LoopCtl *__restrict__ ctl) {

@@ -26,7 +26,8 @@ struct UpdArgs {
     const void *g;                   // ring R gathered to the pphh layout: g[(a,b),(i,j)] = R[(a,i),(j,b)]
                                      // (both in the contractions' arithmetic type)
     const double *t_old;
-    double *t_new, *tas;             // new amplitudes; their (a,b)-antisymmetric packed rows
+    double *t_new;                   // new amplitudes
+    void *tas;                       // their (a,b)-antisymmetric packed rows, in OT
     double *red;                     // W = V o T partial sums in reduction-list order:
     const int32_t *cpos;             //   column partial (tile aux + IJ) -> place (chi_i lists)
     const int32_t *rpos;             //   row sum (pp_off + row) -> place (psi_a lists)
@@ -123,7 +124,7 @@ __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
             a.t_new[b1 + IJ] = x1;
             if (!diag) {
                 a.t_new[b2 + IJ] = x2;
-                a.tas[ch.off_tp + (int64_t)pr * H + IJ] = 0.5 * (x1 - x2);
+                static_cast<OT *>(a.tas)[ch.off_tp + (int64_t)pr * H + IJ] = (OT)(0.5 * (x1 - x2));
             }
         } else {
             w1 = v1 * t1;
@@ -329,13 +330,15 @@ __global__ __launch_bounds__(kThreads) void k_sum_partials(const double *__restr
     if (threadIdx.x == 0) out[0] = scale * b;
 }
 
-// Antisymmetrised packed rows of caller amplitudes: tas[u] = (t[(a,b)] - t[(b,a)]) / 2.
+// Antisymmetrised packed rows of amplitudes: tas[u] = (t[(a,b)] - t[(b,a)]) / 2,
+// stored in TT (the arithmetic type of the ladder that reads them).
+template <typename TT>
 __global__ __launch_bounds__(kThreads) void k_antisym(const DTile *__restrict__ tiles,
                                                       const DChan *__restrict__ chans,
                                                       const int32_t *__restrict__ prow,
                                                       const int32_t *__restrict__ prow_sw,
                                                       const double *__restrict__ t,
-                                                      double *__restrict__ tas) {
+                                                      TT *__restrict__ tas) {
     const DTile tl = tiles[blockIdx.x];
     const DChan ch = chans[tl.chan];
     const int H = ch.hh;
@@ -343,7 +346,7 @@ __global__ __launch_bounds__(kThreads) void k_antisym(const DTile *__restrict__ 
         const int u = tl.row0 + l / H, ij = l % H;
         const double x = t[ch.off_t + (int64_t)prow[ch.pr_off + u] * H + ij];
         const double y = t[ch.off_t + (int64_t)prow_sw[ch.pr_off + u] * H + ij];
-        tas[ch.off_tp + (int64_t)u * H + ij] = 0.5 * (x - y);
+        tas[ch.off_tp + (int64_t)u * H + ij] = (TT)(0.5 * (x - y));
     }
 }
 

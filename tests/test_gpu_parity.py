@@ -317,6 +317,28 @@ def test_precision_switch_continues_the_same_iteration(require_gpu, free):
     assert np.array_equal(ea32, c.iterate_n(2))
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_entry_points_leave_the_iteration_state_alone(require_gpu, prec):
+    """ccd_ladder, ccd_ring and ccd_remap_ph compute a caller's term in the
+    context's work arrays (the antisymmetrised rows, T^ph); the next
+    iterations must still start from the context's own amplitudes."""
+    from coupled_cluster_doubles_amd import FP32
+    p = FP32 if prec == "fp32" else FP64
+    m = Model.electron_gas(1.0, 19, 2)
+    a = Context(m, p, matrix_free=True)
+    a.iterate_n(2)
+    t = a.get_t2()
+    b = Context(m, p, matrix_free=True)
+    b.set_t2(t)
+    rng = np.random.default_rng(11)
+    a.ladder(rng.uniform(-0.05, 0.05, m.n_t2))
+    a.ring(rng.uniform(-0.05, 0.05, m.n_t2))
+    a.remap_ph(rng.uniform(-0.05, 0.05, m.n_t2))
+    assert np.array_equal(a.get_t2(), t)
+    assert np.array_equal(a.iterate_n(2), b.iterate_n(2))
+    assert np.array_equal(a.get_t2(), b.get_t2())
+
+
 @pytest.mark.parametrize("free", [False, True])
 def test_fp32_stored_and_generated_v_agree_bitwise(require_gpu, free):
     from coupled_cluster_doubles_amd import FP32
