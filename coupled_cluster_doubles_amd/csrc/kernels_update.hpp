@@ -52,6 +52,10 @@ __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
     const bool frozen = UPDATE && a.pctl[ch.prob].done;
     const int H = ch.hh, n = tl.nrows * H;
     const int l = threadIdx.x & (kThreads - 1);
+    // places of this slot's sums in the reduction order, loaded with the
+    // element's indices so that the final stores need no further round trip
+    int rp1 = 0, rp2 = 0, cp = 0;
+    bool pdiag = true;
     if (l < n) {
         // every load of the element pair is issued before the first store (the
         // stores alias nothing the loads read, but the compiler cannot know
@@ -60,6 +64,10 @@ __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
         const int JI = a.hh_swap[ch.hh_off + IJ];
         const int r1 = a.prow[ch.pr_off + pr], r2 = a.prow_sw[ch.pr_off + pr];
         const bool diag = r1 == r2;
+        cp = a.cpos[tl.aux + IJ];
+        rp1 = a.rpos[ch.pp_off + r1];
+        rp2 = a.rpos[ch.pp_off + r2];
+        pdiag = diag;
         const int64_t b1 = ch.off_t + (int64_t)r1 * H, b2 = ch.off_t + (int64_t)r2 * H;
         const double v1 = a.vt[b1 + IJ], v2 = a.vt[b2 + IJ];
         const double t1 = a.t_old[b1 + IJ], t2 = a.t_old[b2 + IJ];
@@ -151,12 +159,10 @@ __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
         __syncthreads();
     }
     if (l < n && cq == 0) {
-        const int pr = tl.row0 + pq;
-        const int r1 = a.prow[ch.pr_off + pr], r2 = a.prow_sw[ch.pr_off + pr];
-        a.red[a.rpos[ch.pp_off + r1]] = w1s[l];
-        if (r1 != r2) a.red[a.rpos[ch.pp_off + r2]] = w2s[l];
+        a.red[rp1] = w1s[l];
+        if (!pdiag) a.red[rp2] = w2s[l];
     }
-    if (l < n && pq == 0) a.red[a.cpos[tl.aux + cq]] = cs[l];
+    if (l < n && pq == 0) a.red[cp] = cs[l];
 }
 
 // Per-particle transposition between the pphh and phhp layouts (both hold
