@@ -1733,13 +1733,13 @@ int ccd_work(const ccd_ctx *ctx, double *flops, double *flops_pppp, double *ladd
     if (ladder_bytes) {
         // one pass of the pppp ladder reads the packed V^pppp (unless
         // generated), the antisymmetrised amplitudes and X once and writes D_p
-        // once (V^pppp and D_p in the arithmetic type of the iteration,
-        // amplitudes and X in FP64)
+        // once (V^pppp, the amplitude rows and D_p in the arithmetic type of
+        // the iteration, X in FP64)
         const double vb = ctx->iter_prec == CCD_PREC_FP64 ? 8.0 : 4.0;
         double b = 0.0;
         for (const DChan &ch : ctx->chans)
-            b += (ctx->matrix_free ? 0.0 : vb * ch.npk * ch.npk) + vb * ch.npk * ch.hh +
-                 8.0 * ((double)ch.npk * ch.hh + (double)ch.hh * ch.hh);
+            b += (ctx->matrix_free ? 0.0 : vb * ch.npk * ch.npk) + 2.0 * vb * ch.npk * ch.hh +
+                 8.0 * (double)ch.hh * ch.hh;
         *ladder_bytes = b;
     }
     return CCD_OK;

@@ -203,8 +203,9 @@ int ccd_time_phases(ccd_ctx *ctx, int iters, double mixing, double *ms);
  * flops F = sum_c (2hh^2 pp + 2pp hh^2 + 2pp^2 hh) + sum_m (2hp^2 ph + 2ph hp^2)
  * and the pppp-ladder part of it; bytes one pass of the packed pppp ladder
  * moves: V^pppp for pairs a<b, c<d (none in the matrix-free mode), the
- * antisymmetrised amplitudes and X read once, D_p written once (V^pppp and
- * D_p in the arithmetic type of the iterations, the rest in FP64). */
+ * antisymmetrised amplitudes and X read once, D_p written once (V^pppp, the
+ * amplitude rows and D_p in the arithmetic type of the iterations, X in
+ * FP64). */
 int ccd_work(const ccd_ctx *ctx, double *flops, double *flops_pppp,
              double *bytes_vpppp);
 
