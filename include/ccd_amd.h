@@ -173,9 +173,10 @@ int ccd_get_t2(ccd_ctx *ctx, double *t2);
 int ccd_set_t2(ccd_ctx *ctx, const double *t2);
 
 /* Per-diagram entry points on caller amplitudes (host arrays), for parity
- * tests of each piece against calc.cpp.  They reuse the context's device
- * buffers, so the iteration state is undefined afterwards: call ccd_set_t2
- * or ccd_reset before iterating again.  ccd_ladder computes the ladder of the
+ * tests of each piece against calc.cpp.  ccd_ladder, ccd_ring and
+ * ccd_remap_ph leave the current amplitudes in place (work arrays they
+ * overwrite are rebuilt from them before the next iteration); ccd_onebody
+ * and ccd_step make t (t_new) the current amplitudes.  ccd_ladder computes the ladder of the
  * part of t antisymmetric in (a,b) (V^pppp is stored for pairs a<b only);
  * the reference's amplitudes are always antisymmetric, so on them it is the
  * reference's calcT2Diff_pphh.
