@@ -938,6 +938,10 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
                     prow_sw.push_back(q);
                 }
             const int ndiag = (int)prow.size() - pr_off - npk;
+            for (int u = 0; u < npk + ndiag; u++)   // the update computes the rows from u
+                if (prow[pr_off + u] != (u < npk ? 2 * u : npk + u) ||
+                    prow_sw[pr_off + u] != (u < npk ? 2 * u + 1 : npk + u))
+                    return ccd::fail(CCD_E_INVALID, "pp channel %d: unexpected internal row order", k);
             if ((int64_t)npk * H * 8 >= (int64_t)INT32_MAX)   // the ladder's buffer loads use 32-bit offsets
                 return ccd::fail(CCD_E_INVALID, "channel %d has %lld packed amplitudes (limit 2^28)", k,
                                  (long long)npk * H);

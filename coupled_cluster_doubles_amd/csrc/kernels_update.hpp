@@ -62,8 +62,10 @@ __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
         // that)
         const int pr = tl.row0 + l / H, IJ = l % H;
         const int JI = a.hh_swap[ch.hh_off + IJ];
-        const int r1 = a.prow[ch.pr_off + pr], r2 = a.prow_sw[ch.pr_off + pr];
-        const bool diag = r1 == r2;
+        // rows of the pair in the internal order: packed pair u at 2u (a,b) and
+        // 2u+1 (b,a), the pairs (a,a) after them (the prow / prow_sw lists)
+        const bool diag = pr >= ch.npk;
+        const int r1 = diag ? ch.npk + pr : 2 * pr, r2 = diag ? r1 : r1 + 1;
         cp = a.cpos[tl.aux + IJ];
         rp1 = a.rpos[ch.pp_off + r1];
         rp2 = a.rpos[ch.pp_off + r2];
