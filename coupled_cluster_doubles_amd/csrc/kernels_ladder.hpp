@@ -244,7 +244,8 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
             }
 
             // Every load is unconditional: an element outside the block reads the
-            // zero buffer instead, so no load sits behind a branch and the
+            // zero buffer (stored V) or lies outside its buffer and reads 0 (the
+            // buffer loads below), so no load sits behind a branch and the
             // prefetches stay in flight across the MFMAs.
             auto load_a = [&](int v, CT (&a)[G]) {   // stored V: V chunk v
                 const int kc0 = kbeg + v * KCH;
