@@ -382,21 +382,39 @@ void launch_ladder_v_nt(ccd_ctx *c, const CT *vp, hipStream_t st) {
     const size_t lds = SRC == LV_GAS ? sizeof(double) * (size_t)c->lgen.ntab : 0;
     hipLaunchKernelGGL((k_ladder_v<CT, NT, SRC>), dim3((unsigned)grid), dim3(kThreads), lds, st,
                        c->d_ltiles + c->lt_off[NT - 1], c->d_lbptr + c->lb_base[NT], c->d_chans, vp, c->lgen,
-                       reinterpret_cast<const CT *>(c->d_tas), c->d_x,
-                       c->d_zero, reinterpret_cast<CT *>(c->d_dp), c->d_ctl);
+                       reinterpret_cast<const CT *>(c->d_tas), c->d_zero, reinterpret_cast<CT *>(c->d_dp),
+                       c->d_ctl);
 }
 
-// NT classes [lo, hi] of the pppp ladder
+// Classes NT1 and NT2 in one launch (either may be empty)
+template <typename CT, int NT1, int NT2, int SRC>
+void launch_ladder_v_pair(ccd_ctx *c, const CT *vp, hipStream_t st) {
+    const int g1 = c->lb_grid[NT1], g2 = c->lb_grid[NT2];
+    if (g1 + g2 <= 0) return;
+    const size_t lds = SRC == LV_GAS ? sizeof(double) * (size_t)c->lgen.ntab : 0;
+    hipLaunchKernelGGL((k_ladder_v2<CT, NT1, NT2, SRC>), dim3((unsigned)(g1 + g2)), dim3(kThreads), lds, st,
+                       c->d_ltiles + c->lt_off[NT1 - 1], c->d_lbptr + c->lb_base[NT1], g1,
+                       c->d_ltiles + c->lt_off[NT2 - 1], c->d_lbptr + c->lb_base[NT2], c->d_chans, vp, c->lgen,
+                       reinterpret_cast<const CT *>(c->d_tas), c->d_zero, reinterpret_cast<CT *>(c->d_dp),
+                       c->d_ctl);
+}
+
+// NT classes [lo, hi] of the pppp ladder: classes 1 and 2 (many blocks each)
+// launched on their own, the wider classes (few long blocks) in pairs
 template <typename CT, int SRC>
 void launch_ladder_v_src(ccd_ctx *c, const CT *vp, int lo, int hi, hipStream_t st) {
     if (lo <= 1 && 1 <= hi) launch_ladder_v_nt<CT, 1, SRC>(c, vp, st);
     if (lo <= 2 && 2 <= hi) launch_ladder_v_nt<CT, 2, SRC>(c, vp, st);
-    if (lo <= 3 && 3 <= hi) launch_ladder_v_nt<CT, 3, SRC>(c, vp, st);
-    if (lo <= 4 && 4 <= hi) launch_ladder_v_nt<CT, 4, SRC>(c, vp, st);
-    if (lo <= 5 && 5 <= hi) launch_ladder_v_nt<CT, 5, SRC>(c, vp, st);
-    if (lo <= 6 && 6 <= hi) launch_ladder_v_nt<CT, 6, SRC>(c, vp, st);
-    if (lo <= 7 && 7 <= hi) launch_ladder_v_nt<CT, 7, SRC>(c, vp, st);
-    if (lo <= 8 && 8 <= hi) launch_ladder_v_nt<CT, 8, SRC>(c, vp, st);
+    static_assert(NTMAX == 8, "pairs (3,4), (5,6), (7,8)");
+    if (lo <= 3 && 4 <= hi) launch_ladder_v_pair<CT, 3, 4, SRC>(c, vp, st);
+    else if (lo <= 3 && 3 <= hi) launch_ladder_v_nt<CT, 3, SRC>(c, vp, st);
+    else if (lo <= 4 && 4 <= hi) launch_ladder_v_nt<CT, 4, SRC>(c, vp, st);
+    if (lo <= 5 && 6 <= hi) launch_ladder_v_pair<CT, 5, 6, SRC>(c, vp, st);
+    else if (lo <= 5 && 5 <= hi) launch_ladder_v_nt<CT, 5, SRC>(c, vp, st);
+    else if (lo <= 6 && 6 <= hi) launch_ladder_v_nt<CT, 6, SRC>(c, vp, st);
+    if (lo <= 7 && 8 <= hi) launch_ladder_v_pair<CT, 7, 8, SRC>(c, vp, st);
+    else if (lo <= 7 && 7 <= hi) launch_ladder_v_nt<CT, 7, SRC>(c, vp, st);
+    else if (lo <= 8 && 8 <= hi) launch_ladder_v_nt<CT, 8, SRC>(c, vp, st);
 }
 
 template <typename CT>

@@ -181,17 +181,12 @@ struct LadderShape {
     static constexpr int KSEG = NT <= 2 ? 1024 : 256;
 };
 
+// Tiles b0 .. b1-1 of one class (NT) by one block; the gas table is already in LDS.
 template <typename CT, int NT, int SRC>
-__global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__ tiles,
-                                                       const int32_t *__restrict__ bptr,
-                                                       const DChan *__restrict__ chans,
-                                                       const CT *__restrict__ vp, LadderGen lg,
-                                                       const CT *__restrict__ tas,
-                                                       const double *__restrict__ x,
-                                                       const double *__restrict__ zero,
-                                                       CT *__restrict__ dp,
-                                                       const LoopCtl *__restrict__ ctl) {
-    if (ctl->done) return;
+__device__ __forceinline__ void ladder_run(const DTile *__restrict__ tiles, int b0, int b1,
+                                           const DChan *__restrict__ chans, const CT *__restrict__ vp,
+                                           const LadderGen &lg, const CT *__restrict__ tas,
+                                           const double *__restrict__ zero, CT *__restrict__ dp) {
     constexpr int KCH = LadderShape<NT>::KCH, KSEG = LadderShape<NT>::KSEG, G = KCH / 4;
     // LDS row stride of the B tile: an MFMA step reads rows kslot(kq, s) for
     // the four lane groups kq; for FP32 the two groups of a half-wave then
@@ -211,14 +206,10 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
     // the tile loads stay ahead of their stores, outside any branch
     const int tid = threadIdx.x & (kThreads - 1);
     const int wave = tid >> 6, lane = tid & 63, m = lane & 15, kq = lane >> 4;
-    if (SRC == LV_GAS) {
-        for (int i = tid; i < lg.ntab; i += kThreads) gtab[i] = lg.tab[i];
-        __syncthreads();
-    }
     const uint32_t tbase = (uint32_t)(uintptr_t)(lds_double *)gtab;  // LDS byte address
     const uint32_t zaddr = tbase + lg.zoff;
 
-    for (int ti = bptr[blockIdx.x]; ti < bptr[blockIdx.x + 1]; ti++) {
+    for (int ti = b0; ti < b1; ti++) {
         const DTile tl = tiles[ti];
         const DChan ch = chans[tl.chan];
         // MODE 1 (ONE): a total-spin-0 channel of the generated electron gas,
@@ -384,6 +375,53 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
         else if (SRC == LV_GAS && ch.spin0 == 2) tile(std::integral_constant<int, 2>{});
         else tile(std::integral_constant<int, 0>{});
     }
+}
+
+// The gas table into LDS, once per block
+template <int SRC>
+__device__ __forceinline__ void stage_gas_table(const LadderGen &lg) {
+    extern __shared__ double gtab[];
+    if (SRC == LV_GAS) {
+        for (int i = threadIdx.x & (kThreads - 1); i < lg.ntab; i += kThreads) gtab[i] = lg.tab[i];
+        __syncthreads();
+    }
+}
+
+// One class: block b runs the class's tiles bptr[b] .. bptr[b+1]-1.
+template <typename CT, int NT, int SRC>
+__global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__ tiles,
+                                                       const int32_t *__restrict__ bptr,
+                                                       const DChan *__restrict__ chans,
+                                                       const CT *__restrict__ vp, LadderGen lg,
+                                                       const CT *__restrict__ tas,
+                                                       const double *__restrict__ zero,
+                                                       CT *__restrict__ dp,
+                                                       const LoopCtl *__restrict__ ctl) {
+    if (ctl->done) return;
+    stage_gas_table<SRC>(lg);
+    ladder_run<CT, NT, SRC>(tiles, bptr[blockIdx.x], bptr[blockIdx.x + 1], chans, vp, lg, tas, zero, dp);
+}
+
+// Two classes in one launch, so that the few long blocks of both run side by
+// side: blocks 0 .. g1-1 run class NT1 (tiles1, bptr1), the others class NT2.
+template <typename CT, int NT1, int NT2, int SRC>
+__global__ __launch_bounds__(kThreads) void k_ladder_v2(const DTile *__restrict__ tiles1,
+                                                        const int32_t *__restrict__ bptr1, int g1,
+                                                        const DTile *__restrict__ tiles2,
+                                                        const int32_t *__restrict__ bptr2,
+                                                        const DChan *__restrict__ chans,
+                                                        const CT *__restrict__ vp, LadderGen lg,
+                                                        const CT *__restrict__ tas,
+                                                        const double *__restrict__ zero,
+                                                        CT *__restrict__ dp,
+                                                        const LoopCtl *__restrict__ ctl) {
+    if (ctl->done) return;
+    stage_gas_table<SRC>(lg);
+    const int b = blockIdx.x;
+    if (b < g1)
+        ladder_run<CT, NT1, SRC>(tiles1, bptr1[b], bptr1[b + 1], chans, vp, lg, tas, zero, dp);
+    else
+        ladder_run<CT, NT2, SRC>(tiles2, bptr2[b - g1], bptr2[b - g1 + 1], chans, vp, lg, tas, zero, dp);
 }
 
 // 1/2 tas X, the hhhh quadratic term of D (calc.cpp:171-187 with X from
