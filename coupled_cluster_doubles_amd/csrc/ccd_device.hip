@@ -70,7 +70,7 @@ extern "C" const char *ccd_last_error(void) { return ccd::g_err.c_str(); }
 
 // ------------------------------------------------------------------ context
 
-constexpr int kStale = -1;   // tph_prec / tas_prec: holds a caller's term, rebuild before iterating
+constexpr int kStale = -1;   // tph_prec / tas_prec: not derived from the current amplitudes, rebuild before iterating
 
 struct ccd_ctx {
     int device = 0;
@@ -104,9 +104,13 @@ struct ccd_ctx {
     int iter_prec = CCD_PREC_FP64;  // arithmetic of the contractions of the next iterations
     // element type T^ph and the antisymmetrised rows (d_tas) were last written
     // in (the ring and the ladder read them in iter_prec); kStale: not derived
-    // from the current amplitudes (an entry point computed a caller's term)
+    // from the current amplitudes (an entry point computed a caller's term, or
+    // a device loop stopped while the last iteration's T remap ran)
     int tph_prec = CCD_PREC_FP64;
     int tas_prec = CCD_PREC_FP64;
+    // chi / psi may not be those of the current amplitudes: the psi blocks of
+    // the reduction that stopped a device loop may have returned at entry
+    bool onebody_stale = false;
     // device tables
     DChan *d_chans = nullptr;
     DMod *d_mods = nullptr;
@@ -325,14 +329,27 @@ int launch_tremap(ccd_ctx *c, const double *t, hipStream_t st, bool force = fals
 
 int launch_antisym(ccd_ctx *c, const double *t);
 
-// Rebuild T^ph and the antisymmetrised rows from the current amplitudes when
-// their type is not the one the next iterations read (after a change of
-// precision) or they hold a caller's term (kStale).
+int launch_reduce(ccd_ctx *c, hipStream_t st, bool bookkeep = true);
+template <bool UPDATE>
+int launch_update(ccd_ctx *c, const double *t_old, double *t_new, double mixing, hipStream_t st);
+
+// Rebuild what the next iteration reads besides the amplitudes when it is
+// not derived from them in the type the iterations read: the antisymmetrised
+// rows (after a change of precision), chi and psi (after a device loop
+// stopped), T^ph (after either, or after an entry point held a caller's term).
+// chi and psi come from the W sums of the current amplitudes, the values the
+// update wrote for them.
 int sync_state(ccd_ctx *c) {
-    if (c->tas_prec != c->iter_prec)
-        if (int rc = launch_antisym(c, c->d_t[c->cur])) return rc;
+    int rc;
+    double *t = c->d_t[c->cur];
+    if (c->tas_prec != c->iter_prec && (rc = launch_antisym(c, t))) return rc;
+    if (c->onebody_stale) {
+        if ((rc = launch_update<false>(c, t, nullptr, 0.0, c->stream)) || (rc = launch_reduce(c, c->stream, false)))
+            return rc;
+        c->onebody_stale = false;
+    }
     if (c->tph_prec == c->iter_prec) return CCD_OK;
-    return launch_tremap(c, c->d_t[c->cur], c->stream, true);
+    return launch_tremap(c, t, c->stream, true);
 }
 
 // phhp -> pphh: the ring output R to G, in the contractions' arithmetic type
@@ -466,10 +483,12 @@ int launch_update(ccd_ctx *c, const double *t_old, double *t_new, double mixing,
     return check_launch("k_update");
 }
 
-int launch_reduce(ccd_ctx *c, hipStream_t st) {
+// bookkeep = false: chi and psi only, whatever the loop state (no energy, no
+// loop step)
+int launch_reduce(ccd_ctx *c, hipStream_t st, bool bookkeep) {
     hipLaunchKernelGGL(k_reduce, dim3((unsigned)c->nsp), dim3(kThreads), 0, st, c->d_red_ptr, c->nh, c->d_red,
                        c->d_onebody, c->d_ticket, c->d_scalar, c->d_ctl, c->d_elog, c->nprob, c->d_hlo,
-                       c->d_pctl);
+                       c->d_pctl, bookkeep ? 1 : 0);
     return check_launch("k_reduce");
 }
 
@@ -1470,6 +1489,20 @@ int arm_free_run(ccd_ctx *c) {
     return rc ? rc : sync_state(c);   // T^ph in the type the next iterations read
 }
 
+// A device loop stopped: the T remap and the psi blocks of the reduction of
+// its last iteration ran beside the reduction block that set the stop flag,
+// and those that started after it returned at entry.  The amplitudes and
+// their antisymmetrised rows (written by the update before) are whole;
+// T^ph, chi and psi are rebuilt from them before the next iteration.
+void mark_stopped(ccd_ctx *c) {
+    c->tph_prec = kStale;
+    c->onebody_stale = true;
+}
+
+// The loop state of a free run, with no rebuild (the end of a solve: the
+// next entry point that iterates calls arm_free_run).
+int end_solve(ccd_ctx *c) { return arm_loop(c, -1.0, INT_MAX, true); }
+
 // Queue iterations in batches without waiting for each energy; after each
 // batch read the loop state.  Iterations queued past the stopping point do
 // nothing.  Returns with the loop state in h_ctl and the current amplitudes
@@ -1489,6 +1522,7 @@ int run_loop(ccd_ctx *c, double mixing, int batch) {
     c->cur = (cur0 + (c->h_ctl->count - count0)) & 1;
     if ((rc = join_aux(c))) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
+    mark_stopped(c);
     return CCD_OK;
 }
 
@@ -1513,6 +1547,7 @@ int zero_state(ccd_ctx *c) {
     HIP_TRY(hipMemsetAsync(c->d_tas, 0, sizeof(double) * (size_t)std::max<int64_t>(c->n_tp, 1), c->stream));
     c->tas_prec = c->iter_prec;
     HIP_TRY(hipMemsetAsync(c->d_onebody, 0, sizeof(double) * c->nsp, c->stream));
+    c->onebody_stale = false;
     c->cur = 0;
     return arm_free_run(c);
 }
@@ -1520,13 +1555,14 @@ int zero_state(ccd_ctx *c) {
 // Load host amplitudes as the current T and derive T^ph, the antisymmetrised
 // packed rows, chi and psi from them.
 int set_state(ccd_ctx *c, const double *t2) {
-    if (int rc0 = arm_free_run(c)) return rc0;
+    if (int rc0 = arm_loop(c, -1.0, INT_MAX, true)) return rc0;   // a free run; everything derived is rebuilt here
     double *t = c->d_t[c->cur];
     int rc;
     if ((rc = t2_upload(c, t2, t, c->d_g))) return rc;
     if ((rc = launch_tremap(c, t, c->stream)) || (rc = launch_antisym(c, t)) ||
         (rc = launch_update<false>(c, t, nullptr, 0.0, c->stream)) || (rc = launch_reduce(c, c->stream)))
         return rc;
+    c->onebody_stale = false;
     return CCD_OK;
 }
 
@@ -1668,7 +1704,7 @@ int ccd_batch_solve(ccd_ctx *c, double tol, int max_iters, double mixing, double
         if (n_iter) n_iter[q] = st.count;
         if (stop) stop[q] = st.reason;
     }
-    return arm_free_run(c);
+    return end_solve(c);
 }
 
 int ccd_batch_solve_mixed(ccd_ctx *c, double tol, double switch_tol, int max_iters, double mixing,
@@ -1696,10 +1732,8 @@ int ccd_batch_solve_mixed(ccd_ctx *c, double tol, double switch_tol, int max_ite
         if (!rc) rc = run_loop(c, mixing, kSolveBatch);
         if (!rc) rc = read_pctl(c);
     }
-    c->iter_prec = saved;
-    const int rs = sync_state(c);
+    c->iter_prec = saved;   // the derived state follows at the next iteration (sync_state)
     if (rc) return rc;
-    if (rs) return rs;
     for (int q = 0; q < c->nprob; q++) {
         const LoopCtl st = c->nprob == 1 ? *c->h_ctl : c->h_pctl[q];
         if ((rc = copy_log(c, energies ? energies + (size_t)q * cap : nullptr, cap, st.count, q))) return rc;
@@ -1707,7 +1741,7 @@ int ccd_batch_solve_mixed(ccd_ctx *c, double tol, double switch_tol, int max_ite
         if (n_fp32) n_fp32[q] = phase1[q].count;
         if (stop) stop[q] = st.reason;
     }
-    return arm_free_run(c);
+    return end_solve(c);
 }
 
 int ccd_batch_iterate_n(ccd_ctx *c, int n, double mixing, double *energies) {
@@ -1720,6 +1754,7 @@ int ccd_batch_iterate_n(ccd_ctx *c, int n, double mixing, double *energies) {
     HIP_TRY(hipMemcpyAsync(c->h_ctl, c->d_ctl, sizeof(LoopCtl), hipMemcpyDeviceToHost, c->stream));
     if ((rc = read_pctl(c))) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->h_ctl->done) mark_stopped(c);   // every solve has stopped (NaN)
     if (c->nprob == 1 && c->h_ctl->count != n) {   // a NaN energy stops even a free run
         c->cur = (c->cur + c->h_ctl->count - n) & 1;
         return ccd::fail(CCD_E_STATE, "iteration %d produced a NaN energy", c->h_ctl->count);
@@ -1802,6 +1837,7 @@ int ccd_iterate(ccd_ctx *c, double mixing, double *corr) {
     if (rc || (rc = fork_aux(c)) || (rc = run_iteration(c, mixing, nullptr)) || (rc = join_aux(c))) return rc;
     double e = 0.0;
     if ((rc = read_scalar(c, &e))) return rc;
+    if (std::isnan(e)) mark_stopped(c);   // the loop state stopped this iteration
     if (corr) *corr = e;
     return CCD_OK;
 }
@@ -1818,7 +1854,7 @@ int ccd_solve(ccd_ctx *c, double tol, int max_iters, double mixing, double *ener
     if ((rc = copy_log(c, energies, cap, st.count))) return rc;
     if (n_iter) *n_iter = st.count;
     if (stop) *stop = st.reason;
-    return arm_free_run(c);
+    return end_solve(c);
 }
 
 int ccd_set_precision(ccd_ctx *c, int precision) {
@@ -1856,16 +1892,14 @@ int ccd_solve_mixed(ccd_ctx *c, double tol, double switch_tol, int max_iters, do
         if (!rc) rc = sync_state(c);
         if (!rc) rc = run_loop(c, mixing, kSolveBatch);
     }
-    c->iter_prec = saved;
-    const int rs = sync_state(c);
+    c->iter_prec = saved;   // the derived state follows at the next iteration (sync_state)
     if (rc) return rc;
-    if (rs) return rs;
     const LoopCtl st = *c->h_ctl;
     if ((rc = copy_log(c, energies, cap, st.count))) return rc;
     if (n_iter) *n_iter = st.count;
     if (n_fp32) *n_fp32 = n32;
     if (stop) *stop = st.reason;
-    return arm_free_run(c);
+    return end_solve(c);
 }
 
 int ccd_iterate_n(ccd_ctx *c, int n, double mixing, double *energies) {
@@ -1880,6 +1914,7 @@ int ccd_iterate_n(ccd_ctx *c, int n, double mixing, double *energies) {
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (c->h_ctl->count != n) {   // a NaN energy stops even a free run
         c->cur = (c->cur + c->h_ctl->count - n) & 1;
+        mark_stopped(c);
         return ccd::fail(CCD_E_STATE, "iteration %d produced a NaN energy", c->h_ctl->count);
     }
     return copy_log(c, energies, n, n);
@@ -1904,8 +1939,8 @@ int ccd_set_t2(ccd_ctx *c, const double *t2) {
 
 int ccd_ladder(ccd_ctx *c, const double *t, double *d) {
     ccd::clear_error();
-    if (!c || !t || !d) return ccd::fail(CCD_E_INVALID, "ccd_ladder: null argument");    if (int rc0 = arm_free_run(c)) return rc0;
-
+    if (!c || !t || !d) return ccd::fail(CCD_E_INVALID, "ccd_ladder: null argument");
+    if (int rc0 = arm_free_run(c)) return rc0;
     double *dt = c->d_t[1 - c->cur];  // scratch: the non-current T buffer
     int rc;
     if ((rc = t2_upload(c, t, dt, c->d_g))) return rc;
@@ -1931,8 +1966,8 @@ int ccd_ladder(ccd_ctx *c, const double *t, double *d) {
 
 int ccd_ring(ccd_ctx *c, const double *tph, double *r) {
     ccd::clear_error();
-    if (!c || !tph || !r) return ccd::fail(CCD_E_INVALID, "ccd_ring: null argument");    if (int rc0 = arm_free_run(c)) return rc0;
-
+    if (!c || !tph || !r) return ccd::fail(CCD_E_INVALID, "ccd_ring: null argument");
+    if (int rc0 = arm_free_run(c)) return rc0;
     if (c->iter_prec == CCD_PREC_FP32) {   // T^ph in FP32, as the FP32 iterations hold it
         HIP_TRY(hipMemcpyAsync(c->d_g, tph, sizeof(double) * c->nt2, hipMemcpyHostToDevice, c->stream));
         if (c->nt2 > 0) {
@@ -1961,8 +1996,8 @@ int ccd_ring(ccd_ctx *c, const double *tph, double *r) {
 
 int ccd_remap_ph(ccd_ctx *c, const double *t, double *tph) {
     ccd::clear_error();
-    if (!c || !t || !tph) return ccd::fail(CCD_E_INVALID, "ccd_remap_ph: null argument");    if (int rc0 = arm_free_run(c)) return rc0;
-
+    if (!c || !t || !tph) return ccd::fail(CCD_E_INVALID, "ccd_remap_ph: null argument");
+    if (int rc0 = arm_free_run(c)) return rc0;
     double *dt = c->d_t[1 - c->cur];
     int rc = t2_upload(c, t, dt, c->d_g);
     if (!rc) rc = launch_transpose(c, true, dt, c->d_tph, c->stream);
@@ -1995,6 +2030,7 @@ int ccd_step(ccd_ctx *c, const double *t_old, double *t_new, double mixing, doub
     if ((rc = read_scalar(c, &e))) return rc;
     if ((rc = t2_download(c, c->d_t[c->cur], t_new, c->d_g))) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
+    if (std::isnan(e)) mark_stopped(c);
     if (corr) *corr = e;
     return CCD_OK;
 }

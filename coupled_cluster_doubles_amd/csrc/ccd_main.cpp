@@ -37,29 +37,29 @@ enum {
 };
 
 argp_option options[] = {
-    {"config", 'c', "FILE", 0, "use FILE to get default configuration", 0},
-    {"saveMemory", 'm', "INT", 0, "memory saving mode (0 for off, 1 for on)", 0},
-    {"time", 'T', "FILE", 0, "record timing info for load balancer in FILE", 0},
-    {"timeOverwrite", 'W', 0, 0, "overwrite existing timings", 0},
-    {"model", 'M', "FILE", 0, "use model info in FILE for load balancer", 0},
-    {"verbose", 'v', "INT", 0, "verbosity level", 0},
+    {"config", 'c', "FILE", 0, "read option defaults from FILE (one option per line)", 0},
+    {"saveMemory", 'm', "INT", 0, "1: rebuild V^pppp on the device every iteration instead of storing it; 0: store it", 0},
+    {"time", 'T', "FILE", 0, "accepted for compatibility; one GPU needs no load-balancer timings", 0},
+    {"timeOverwrite", 'W', 0, 0, "accepted for compatibility; no effect", 0},
+    {"model", 'M', "FILE", 0, "accepted for compatibility; no load-balancer model is read", 0},
+    {"verbose", 'v', "INT", 0, "how much to print (0 = the report lines only)", 0},
     {"precision", k_precision, "fp64|fp32", 0, "arithmetic of the contractions (default fp64)", 0},
-    {0, 0, 0, 0, "Basis Choices:", 0},
-    {"pairing", k_pairing, 0, 0, "pairing basis", 0},
-    {"infMatter", k_infMatter, 0, 0, "infinite matter basis", 0},
-    {"qDots", k_qDots, 0, 0, "quantum dots basis", 0},
-    {"electronGas", k_electronGas, 0, 0, "electron gas basis", 0},
-    {0, 0, 0, 0, "Basis-Specific options:", 0},
-    {"density", 'd', "FLOAT", 0, "particle density in fm^-3", 0},
-    {"g", 'g', "FLOAT", 0, "pairing interaction strength parameter", 0},
-    {"nParticleShells", 'p', "INT", 0, "how many shells of particles", 0},
-    {"nParticles", 'n', "INT", 0, "Number of particles", 0},
-    {"nPairStates", 'l', "INT", 0, "Number of degenerate levels", 0},
-    {"r_s", 'r', "FLOAT", 0, "Wigner-Seitz Radius", 0},
-    {"nShells", 's', "INT", 0, "how many non-empty energy shells (Fermi spheres)", 0},
-    {"tolerance", 't', "FLOAT", 0, "tolerance of CCD energy for iterative solver", 0},
-    {"tzMax", 'z', "INT", 0, "number of species, 1 for neutrons, 2 for protons and neutrons", 0},
-    {"xi", 'x', "FLOAT", 0, "single particle level energy", 0},
+    {0, 0, 0, 0, "Which basis:", 0},
+    {"pairing", k_pairing, 0, 0, "doubly degenerate equally spaced levels with constant pairing", 0},
+    {"infMatter", k_infMatter, 0, 0, "nuclear matter in a periodic box, Minnesota interaction", 0},
+    {"qDots", k_qDots, 0, 0, "quantum dots (not available)", 0},
+    {"electronGas", k_electronGas, 0, 0, "homogeneous electron gas in a periodic box, Coulomb interaction", 0},
+    {0, 0, 0, 0, "Basis parameters:", 0},
+    {"density", 'd', "FLOAT", 0, "nuclear matter: density in fm^-3", 0},
+    {"g", 'g', "FLOAT", 0, "pairing model: strength of the pairing term", 0},
+    {"nParticleShells", 'p', "INT", 0, "number of filled momentum shells (sets the particle count)", 0},
+    {"nParticles", 'n', "INT", 0, "pairing model: particle count", 0},
+    {"nPairStates", 'l', "INT", 0, "pairing model: number of levels", 0},
+    {"r_s", 'r', "FLOAT", 0, "electron gas: Wigner-Seitz radius in Bohr", 0},
+    {"nShells", 's', "INT", 0, "number of momentum shells in the basis", 0},
+    {"tolerance", 't', "FLOAT", 0, "stop once the correlation energy changes by at most this much", 0},
+    {"tzMax", 'z', "INT", 0, "nuclear matter: 1 = neutrons only, 2 = neutrons and protons", 0},
+    {"xi", 'x', "FLOAT", 0, "pairing model: spacing of the levels", 0},
     {0, 0, 0, 0, 0, 0},
 };
 
@@ -80,7 +80,7 @@ error_t parse_opt(int key, char *arg, argp_state *state) {
         case 'z': o->tzMax = atoi(arg); break;
         case 'x': o->xi = atof(arg); break;
         case 'm': o->saveMemory = atoi(arg); break;
-        case 'T': case 'W': case 'M': break;  // parsed but unused by the reference
+        case 'T': case 'W': case 'M': break;  // load-balancer options: nothing to balance on one GPU
         case 'v': o->verbose = atoi(arg); break;
         case k_precision:
             if (!strcmp(arg, "fp64")) o->precision = CCD_PREC_FP64;

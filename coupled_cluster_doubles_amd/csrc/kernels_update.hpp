@@ -45,7 +45,7 @@ struct UpdArgs {
 template <bool UPDATE, typename OT>
 __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
     const DTile tl = a.tiles[blockIdx.x];
-    if (a.ctl->done) return;
+    if (UPDATE && a.ctl->done) return;   // the W sums alone run whatever the loop state
     const OT *gg = static_cast<const OT *>(a.g);
     __shared__ double w1s[kThreads], w2s[kThreads], cs[kThreads];
     const DChan ch = a.chans[tl.chan];
@@ -241,14 +241,17 @@ __global__ __launch_bounds__(kThreads) void k_transpose(const int32_t *__restric
 // more than max_iters iterations have run.  A single solve has pctl == ctl;
 // a batch counts its iterations in ctl and stops once every solve has.  The
 // last block resets the ticket counter for the next launch.  Without holes
-// (an empty problem) block 0 does that part.
+// (an empty problem) block 0 does that part.  bookkeep = 0: chi and psi only
+// (rebuilt from the current amplitudes), whatever the loop state, no energy
+// and no loop step.
 __global__ __launch_bounds__(kThreads) void k_reduce(const int32_t *__restrict__ ptr, int nh,
                                                      const double *__restrict__ parts,
                                                      double *__restrict__ out, int *__restrict__ ticket,
                                                      double *__restrict__ energy, LoopCtl *ctl,
                                                      double *__restrict__ log, int nprob,
-                                                     const int32_t *__restrict__ hlo, LoopCtl *pctl) {
-    if (ctl->done) return;
+                                                     const int32_t *__restrict__ hlo, LoopCtl *pctl,
+                                                     int bookkeep) {
+    if (bookkeep && ctl->done) return;
     __shared__ double red[kThreads / 64];
     __shared__ int last, pdone;
     const int k = blockIdx.x;
@@ -266,6 +269,10 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const int32_t *__restrict__
     }
     __syncthreads();
     if (!last) return;
+    if (!bookkeep) {
+        if (threadIdx.x == 0) *ticket = 0;
+        return;
+    }
     __threadfence();
     for (int p = 0; p < nprob; p++) {
         if (threadIdx.x == 0) pdone = pctl[p].done;

@@ -43,8 +43,8 @@ def test_heg66_solve_matches_oracle(require_gpu, golden, model):
     e_g, stop_g, n_g = ctx.solve(golden["tol"])
     e_o = np.array(golden["energies"])
     assert stop_g == golden["stop"] == "converged"
-    assert abs(n_g - len(e_o)) <= 1
-    k = min(n_g, len(e_o))
+    assert n_g == len(e_o)
+    k = n_g
     assert np.max(np.abs(e_g[:k] - e_o[:k]) / np.abs(e_o[:k])) < E_REL
     assert e_g[0] == pytest.approx(0.5 * golden["mbpt2"], rel=1e-12)
     t = ctx.get_t2()

@@ -247,3 +247,26 @@ def test_heg66_golden_file_is_consistent():
     o = Oracle.electron_gas(*g["args"], load=False)
     assert o.nsp == g["nsp"] and o.nh == g["nh"] and o.nt2 == g["nt2"]
     assert o.eref() == g["eref"]
+
+
+def test_heg54_golden_file_is_what_the_oracle_computes():
+    """The committed N=54 oracle solve (the headline workload): protocol
+    invariants, and its MBPT2 and first two iterations recomputed here by the
+    oracle (OpenMP sums over channels: equal to 1e-13 relative)."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "heg54_oracle.json")) as f:
+        g = json.load(f)
+    e = np.array(g["energies"])
+    assert g["nsp"] == 1850 and g["nh"] == 54 and g["nt2"] == 2925948
+    assert g["stop"] == "converged" and len(e) == 33
+    assert e[0] == pytest.approx(0.5 * g["mbpt2"], rel=1e-13)
+    assert abs(e[-1] - e[-2]) <= g["tol"] < np.min(np.abs(np.diff(e[:-1])))
+    o = Oracle.electron_gas(*g["args"])
+    assert o.nsp == g["nsp"] and o.nh == g["nh"] and o.nt2 == g["nt2"]
+    assert o.eref() == g["eref"]
+    assert _rel(o.mbpt2(), g["mbpt2"]) < 1e-13
+    t = np.zeros(o.nt2)
+    for k in range(2):
+        t, ek = o.step(t)
+        assert _rel(ek, e[k]) < 1e-13
