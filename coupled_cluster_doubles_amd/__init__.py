@@ -153,6 +153,13 @@ class Context:
         check(lib().ccd_mbpt2(self.h, C.byref(e)), "ccd_mbpt2")
         return e.value
 
+    def mbpt2_noalloc(self, d_corr: int, stream: int = 0):
+        """ccd_mbpt2_noalloc: MBPT2 into the device double at address d_corr,
+        queued on `stream` (a hipStream_t handle; 0 = the context's stream),
+        with no allocation and no synchronisation."""
+        check(lib().ccd_mbpt2_noalloc(self.h, C.c_void_p(d_corr), C.c_void_p(stream or None)),
+              "ccd_mbpt2_noalloc")
+
     def reset(self):
         check(lib().ccd_reset(self.h), "ccd_reset")
 

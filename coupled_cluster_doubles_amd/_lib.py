@@ -41,6 +41,7 @@ SIGNATURES = [
     ("ccd_ctx_info", C.c_int, [vp, i64p, i64p]),
     ("ccd_set_precision", C.c_int, [vp, C.c_int]),
     ("ccd_mbpt2", C.c_int, [vp, dp]),
+    ("ccd_mbpt2_noalloc", C.c_int, [vp, vp, vp]),
     ("ccd_reset", C.c_int, [vp]),
     ("ccd_iterate", C.c_int, [vp, C.c_double, dp]),
     ("ccd_iterate_n", C.c_int, [vp, C.c_int, C.c_double, ip]),

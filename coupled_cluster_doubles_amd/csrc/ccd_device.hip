@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <limits>
 #include <climits>
+#include <mutex>
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
@@ -67,6 +68,42 @@ extern "C" const char *ccd_last_error(void) { return ccd::g_err.c_str(); }
             return ccd::fail(CCD_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
                              __FILE__, __LINE__);                                           \
     } while (0)
+
+// ------------------------------------------------- scratch of ccd_mbpt2_noalloc
+//
+// ccd_mbpt2_noalloc keeps its per-tile partial sums in this file-scope array
+// instead of context memory, so that the call allocates nothing.  Each GPU
+// holds its own copy for the life of the process; in host code `scratch` is
+// only a stand-in for it, so kernels get the device address that
+// hipGetSymbolAddress returns for the device, looked up once per device when
+// a context is created there.  Calls on one GPU share the array: calls that
+// overlap on one GPU race.
+constexpr int kNoAllocPartials = 1 << 16;   // 512 KB per GPU
+static __device__ __align__(256) double scratch[kNoAllocPartials];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kMaxDevices] = {};
+
+// With `device` current: its address of `scratch` (once per device).
+static int init_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices)
+        return ccd::fail(CCD_E_INVALID, "device %d is beyond the %d devices this library supports", device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return CCD_OK;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return ccd::fail(CCD_E_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", device,
+                         hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<double *>(p);
+    return CCD_OK;
+}
+
+static double *scratch_ptr(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
 
 // ------------------------------------------------------------------ context
 
@@ -213,7 +250,7 @@ int check_launch(const char *what) {
     return CCD_OK;
 }
 
-UpdArgs upd_args(ccd_ctx *c, const double *t_old, double *t_new, double mixing) {
+UpdArgs upd_args(const ccd_ctx *c, const double *t_old, double *t_new, double mixing) {
     UpdArgs a{};
     a.tiles = c->d_utiles; a.chans = c->d_chans; a.ppl = c->d_ppl; a.hhl = c->d_hhl;
     a.prow = c->d_prow; a.prow_sw = c->d_prow_sw; a.hh_swap = c->d_hh_swap;
@@ -1604,6 +1641,10 @@ int create_ctx(const ccd_problem *prob, const BatchInfo &bi, int precision, ccd_
             free_ctx(c);
             return ccd::fail(CCD_E_HIP, "could not create HIP events");
         }
+    if ((rc = init_scratch(c->device))) {
+        free_ctx(c);
+        return rc;
+    }
     if ((rc = allow_ring_lds_all())) {
         free_ctx(c);
         return rc;
@@ -1818,6 +1859,26 @@ int ccd_mbpt2(ccd_ctx *c, double *corr) {
                        0.25, c->d_scalar);
     if ((rc = check_launch("k_sum_partials"))) return rc;
     return read_scalar(c, corr);
+}
+
+int ccd_mbpt2_noalloc(const ccd_ctx *c, double *d_corr, void *stream) {
+    ccd::clear_error();
+    if (!c || !d_corr) return ccd::fail(CCD_E_INVALID, "ccd_mbpt2_noalloc: null argument");
+    if (c->nprob > 1) return batch_only(c, "ccd_mbpt2_noalloc");
+    const int n = (int)c->utiles.size();
+    if (n > kNoAllocPartials)
+        return ccd::fail(CCD_E_INVALID, "ccd_mbpt2_noalloc: the problem has %d update tiles, the scratch holds %d", n,
+                         kNoAllocPartials);
+    double *ptr = scratch_ptr(c->device);
+    if (!ptr) return ccd::fail(CCD_E_STATE, "ccd_mbpt2_noalloc: no scratch address for device %d", c->device);
+    const hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    if (n > 0) {   // every partial is written before it is read: nothing to clear
+        UpdArgs a = upd_args(c, nullptr, nullptr, 0.0);
+        hipLaunchKernelGGL(k_mbpt2, dim3((unsigned)n), dim3(kThreads), 0, st, a, ptr);
+        if (int rc = check_launch("k_mbpt2")) return rc;
+    }
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kThreads), 0, st, ptr, n, 0.25, d_corr);
+    return check_launch("k_sum_partials");
 }
 
 int ccd_reset(ccd_ctx *c) {

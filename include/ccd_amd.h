@@ -134,6 +134,21 @@ int ccd_set_precision(ccd_ctx *ctx, int precision);
  * (replaces MBPT2Corr, src/MBPT2Corr.cpp:6-38). */
 int ccd_mbpt2(ccd_ctx *ctx, double *corr);
 
+/* ccd_mbpt2 that allocates nothing and does not synchronise.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory or
+ * wait on the host (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc).  The MBPT2 energy of the context's problem is
+ * written to d_corr, one double in device memory, by kernels queued on
+ * `stream` (a hipStream_t of the context's device; NULL = the context's own
+ * stream); it is there once that stream has reached them.  The per-tile
+ * partial sums live in a fixed per-GPU scratch array of 65,536 doubles
+ * (512 KB), shared by every call on that GPU: calls that overlap on one GPU
+ * race.  Fails with CCD_E_INVALID for problems of more than 65,536 update
+ * tiles (the electron gas at N=66, n^2 <= 52, has 15,858) and CCD_E_STATE on
+ * a batch.  Same kernels and summation order as ccd_mbpt2: the same value
+ * bit for bit. */
+int ccd_mbpt2_noalloc(const ccd_ctx *ctx, double *d_corr, void *stream);
+
 /* T2 = 0 and the one-body intermediates = 0 (src/main.cpp:497-503). */
 int ccd_reset(ccd_ctx *ctx);
 
