@@ -10,15 +10,22 @@ with the amplitudes and interaction resident in HBM.
 
 --precision mixed (default): the steps are FP32-contraction iterations (the
 bulk of a mixed solve); afterwards a full mixed solve (FP32 iterations while
-|dE| > --switch-tol, then FP64 sweeps to |dE| <= 1e-10) and a full FP64 solve
-are timed from T2 = 0 and their energies compared per particle; the mixed
-solve is also timed with the switch at 1e-6 and with no FP64 sweeps
-(switch_sweep).  fp64 / fp32: steps in that arithmetic only.
+|dE| > --switch-tol, then FP64 sweeps to |dE| <= 1e-10) and full FP64 solves
+(the same V mode, and the stored-V "plain" FP64 path) are timed from T2 = 0;
+their energies are compared per particle with each other and, where the
+oracle's solve of the workload is committed (tests/golden), with the
+oracle's.  The mixed solve is also timed with the switch at 1e-6 and with no
+FP64 sweeps (switch_sweep).  fp64 / fp32: steps in that arithmetic only.
 
 --mode free (the default for the electron gas) is the reference's
 matrix-free mode (-m 1): V^pppp is rebuilt inside the pppp ladder every
 iteration instead of being read from HBM; the iterates are bit-identical to
 --mode stored (the default for the pairing model and the sweep).
+
+Per-kernel figures come from ccd_time_kernels (HIP events around each launch
+group, the groups in order on the context's stream) and the algorithmic work
+of each group (ccd_kernel_work); `roofline` is the group with the largest
+single-stream time.
 
 --config sweep64: the seeded sweep of 64 independent solves (electron gas
 N=14/38/54 and pairing models, coupled_cluster_doubles_amd/sweep.py) as one
@@ -39,10 +46,10 @@ sys.path.insert(0, ROOT)
 
 CONFIGS = {
     # reference CLI: --electronGas -r 1.0 -s 32 -p 4 -t 1e-10
-    "heg54": dict(kind="heg", args=(1.0, 32, 4),
+    "heg54": dict(kind="heg", args=(1.0, 32, 4), golden="heg54_oracle.json",
                   name="electron gas N=54, r_s=1.0, n^2<=36 (1850 s.p. states), CCD iteration"),
     # reference CLI: --electronGas -r 0.5 -s 46 -p 5
-    "heg66": dict(kind="heg", args=(0.5, 46, 5),
+    "heg66": dict(kind="heg", args=(0.5, 46, 5), golden="heg66_oracle.json",
                   name="electron gas N=66, r_s=0.5, n^2<=52 (3150 s.p. states), CCD iteration"),
     # reference CLI: --electronGas -r 1.0 -s 19 -p 2 -t 1e-10
     "heg14": dict(kind="heg", args=(1.0, 19, 2),
@@ -54,9 +61,22 @@ CONFIGS = {
                     name="64 seeded solves in one batch (electron gas N=14/38/54, r_s in [0.5,5]; "
                          "pairing 8-16 levels, g in [0.2,0.8]), one iteration of every solve"),
 }
+# MI355X published peaks
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
+N_CU = 256
+CLOCK_GHZ = 2.4            # peak engine clock; one wave64 VALU instruction per CU per clock
+
+LADDER_GROUPS = ("ladder_tx", "ladder_v1", "ladder_v2", "ladder_v34", "ladder_v58")
+KERNEL_OF_GROUP = {
+    "ring": "k_ring_staged / k_ring", "remap_r": "k_transpose (R to pphh)",
+    "ladder_x": "k_ladder_x", "ladder_tx": "k_ladder_tx",
+    "ladder_v1": "k_ladder_v<{ct},1,{src}>", "ladder_v2": "k_ladder_v<{ct},2,{src}>",
+    "ladder_v34": "k_ladder_v2<{ct},3,4,{src}>", "ladder_v58": "k_ladder_v2<{ct},5..8,{src}>",
+    "update": "k_update<true,{ct}>", "remap_t": "k_transpose<double,{ct}> (T to T^ph)",
+    "reduce": "k_reduce",
+}
 
 
 def make_model(cfg):
@@ -66,29 +86,29 @@ def make_model(cfg):
     return Model.pairing(*cfg["args"])
 
 
-def make_context(cfg, prec, free, models=None):
+def make_context(cfg, prec, free, models):
     from coupled_cluster_doubles_amd import Batch, Context
     if cfg["kind"] == "sweep":
         return Batch(models, prec, matrix_free=free)
     return Context(models[0], prec, matrix_free=free)
 
 
-def packed_ladder_flops(models):
-    """MFMA-side flops of one pass of the packed pppp ladder: per direct
-    channel 2 npk^2 hh (V t over pairs c<d) + 2 npk hh^2 (T X), npk = pairs a<b."""
-    f = 0.0
-    for model in models:
-        d = model.dims()
-        pairs = model.pairs("pp")
-        o = 0
-        for pp, hh in zip(d["pp_dim"], d["hh_dim"]):
-            blk = pairs[o:o + pp]
-            o += pp
-            if pp == 0 or hh == 0:
-                continue
-            npk = int(np.count_nonzero(blk[:, 0] < blk[:, 1]))
-            f += 2.0 * npk * npk * hh + 2.0 * npk * hh * hh
-    return f
+def cpu_threads():
+    """The host threads the CPU baseline uses: the process's CPU share
+    (OMP_NUM_THREADS where the host sets it), else every visible CPU."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return int(env) if env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(kind, args, iters, threads, matrix_free=False):
@@ -125,6 +145,42 @@ def cpu_baseline_sweep(iters, threads):
     return 1.0 / sec
 
 
+REPS = 5   # back-to-back launches of each group per timed iteration
+
+
+def kernel_report(ctx, iters, ct, src):
+    """Single-stream time per launch, algorithmic work and roofline fraction
+    of every launch group of one iteration in the context's current
+    arithmetic; and the sum of the groups' times."""
+    t = ctx.time_kernels(iters, REPS)
+    w = ctx.kernel_work()
+    peak_tf = FP64_PEAK_TFLOPS if ct == "double" else FP32_PEAK_TFLOPS
+    out = {}
+    for g, ms in t.items():
+        if g == "iteration":
+            continue
+        sec = ms * 1e-3
+        e = {"kernel": KERNEL_OF_GROUP[g].format(ct=ct, src=src), "ms": ms,
+             "flops": w[g]["flops"], "bytes": w[g]["bytes"]}
+        if sec > 0:
+            e["tflops"] = w[g]["flops"] / sec / 1e12
+            e["gbs"] = w[g]["bytes"] / sec / 1e9
+            e["frac_mfma"] = e["tflops"] / peak_tf
+            e["frac_hbm"] = e["gbs"] / HBM_PEAK_GBS
+        if w[g]["elements"] > 0:
+            e["generated_elements"] = w[g]["elements"]
+            e["elements_per_s"] = w[g]["elements"] / sec if sec > 0 else None
+        out[g] = e
+    return out, sum(v["ms"] for v in out.values())
+
+
+def load_json(path):
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +194,7 @@ def main():
                     help="stored: V^pppp read from HBM; free: rebuilt in the ladder (reference -m 1); "
                          "default free for the electron gas, stored otherwise")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-solves", action="store_true", help="skip the full solves (profiling runs)")
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (a kernel of known traffic, "
@@ -161,9 +218,11 @@ def main():
 
     cfg = CONFIGS[args.config]
     prec = FP64 if args.precision == "fp64" else FP32
+    ct = "double" if prec == FP64 else "float"
     if args.mode is None:
         args.mode = "free" if cfg["kind"] == "heg" else "stored"
     free = args.mode == "free"
+    src = "LV_GAS" if (free and cfg["kind"] == "heg") else "LV_GENERIC" if free else "LV_STORED"
     sweep = cfg["kind"] == "sweep"
     if sweep:
         from coupled_cluster_doubles_amd.sweep import sweep_models
@@ -173,8 +232,10 @@ def main():
     t_setup = time.perf_counter()
     ctx = make_context(cfg, prec, free, models)
     setup_s = time.perf_counter() - t_setup
+    ctx.set_schedule("auto")      # time the iteration schedules on this problem, keep the fastest
     mbpt2 = ctx.mbpt2()
     work = ctx.work()
+    sched, sched_us = ctx.schedule()
 
     # the steps are queued back to back (no host round trip between them);
     # the call returns after the last one has finished on the device
@@ -186,69 +247,142 @@ def main():
     ms_step = 1e3 * dt / args.steps
     iters_per_s = args.steps / dt
 
-    # per-phase device time from HIP events on the context's stream
-    phases = ctx.time_phases(max(3, min(args.steps, 10)))
-    ladder_ms = float(phases[2])
-    lb = work["ladder_bytes"]
-    gemm_tflops = work["flops"] / (ms_step * 1e-3) / 1e12
-    if args.mode == "stored":
-        achieved = lb / (ladder_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "kernel": "k_ladder_v", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes": lb}
-    else:
-        lf = packed_ladder_flops(models)
-        peak = FP64_PEAK_TFLOPS if prec == FP64 else FP32_PEAK_TFLOPS
-        achieved = lf / (ladder_ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "kernel": "k_ladder_v", "achieved": achieved, "peak": peak,
-                "unit": "TFLOP/s", "frac": achieved / peak, "algorithmic_flops": lf,
-                "algorithmic_bytes": lb}
+    # per launch group, single stream, in the steps' arithmetic
+    n_timed = max(3, min(args.steps, 10))
+    kern, iter_ms_serial = kernel_report(ctx, n_timed, ct, src)
+    kern64 = None
+    if prec == FP32:     # the same context's FP64 iterations
+        ctx.set_precision(FP64)
+        kern64, iter64_ms_serial = kernel_report(ctx, n_timed, "double", src)
+        ctx.set_precision(FP32)
 
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles",
-                         f"ladder_d_traffic_{args.config}_{args.precision}_{args.mode}.json")
-    if os.path.exists(tfile):
-        with open(tfile) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+    # executed flops of an iteration: the products on packed pairs
+    executed_flops = sum(v["flops"] for v in kern.values())
+    out_eff = {"gemm_tflops_effective": work["flops"] / (ms_step * 1e-3) / 1e12,
+               "gemm_tflops_effective_note": "the reference's ordered-pair flops (gemm_flops_per_iter) "
+                                             "over the step time; the device runs the pppp ladder on "
+                                             "pairs a<b, a quarter of those flops",
+               "gemm_tflops_executed": executed_flops / (ms_step * 1e-3) / 1e12,
+               "gemm_flops_per_iter": work["flops"], "executed_flops_per_iter": executed_flops}
+
+    # roofline of the dominant launch group, measured on the stream it runs on
+    dom = max(kern, key=lambda g: kern[g]["ms"])
+    prof = load_json(os.path.join(ROOT, "profiles", f"{dom}_{args.config}_{args.precision}_{args.mode}.json"))
+
+    def roof_of(g, rep, peak_tf):
+        r = rep[g]
+        if g.startswith("ladder_v") and free:
+            d = {"bound": "mfma", "kernel": r["kernel"], "achieved": r["tflops"], "peak": peak_tf,
+                 "unit": "TFLOP/s", "frac": r["tflops"] / peak_tf, "algorithmic_flops": r["flops"],
+                 "algorithmic_bytes": r["bytes"], "launch_ms": r["ms"]}
+        elif g in ("ring", "ladder_x", "ladder_tx") or g.startswith("ladder_v"):
+            d = {"bound": "mfma" if g == "ring" else "hbm", "kernel": r["kernel"], "launch_ms": r["ms"],
+                 "algorithmic_flops": r["flops"], "algorithmic_bytes": r["bytes"]}
+            if d["bound"] == "mfma":
+                d.update(achieved=r["tflops"], peak=peak_tf, unit="TFLOP/s", frac=r["tflops"] / peak_tf)
+            else:
+                d.update(achieved=r["gbs"], peak=HBM_PEAK_GBS, unit="GB/s", frac=r["gbs"] / HBM_PEAK_GBS)
+        else:
+            d = {"bound": "hbm", "kernel": r["kernel"], "achieved": r["gbs"], "peak": HBM_PEAK_GBS,
+                 "unit": "GB/s", "frac": r["gbs"] / HBM_PEAK_GBS, "algorithmic_bytes": r["bytes"],
+                 "launch_ms": r["ms"]}
+        return d
+
+    peak = FP64_PEAK_TFLOPS if prec == FP64 else FP32_PEAK_TFLOPS
+    roof = roof_of(dom, kern, peak)
+    roof["traffic"] = prof.get("bytes_per_launch") if prof else None
+    roof["traffic_source"] = (f"profiles/{dom}_{args.config}_{args.precision}_{args.mode}.json "
+                              "(rocprofv3 FETCH_SIZE / WRITE_SIZE, calibrated)") if prof else None
+    if prof and prof.get("valu_insts_per_element") and kern[dom].get("generated_elements"):
+        # VALU-issue roof of the generated ladder: every generated element costs
+        # ipe VALU lane-instructions; a CU issues one wave64 instruction per clock
+        ipe = prof["valu_insts_per_element"]
+        peak_el = N_CU * CLOCK_GHZ * 1e9 * 64 / ipe
+        ach = kern[dom]["elements_per_s"]
+        roof["valu_roof"] = {"unit": "elements/s", "achieved": ach, "peak": peak_el, "frac": ach / peak_el,
+                             "valu_insts_per_element": ipe, "insts_source": prof.get("valu_source")}
+
+    ladder_ms = sum(kern[g]["ms"] for g in LADDER_GROUPS)
+    ladder_flops = sum(kern[g]["flops"] for g in LADDER_GROUPS)
+    phase = {"groups": list(LADDER_GROUPS), "ms": ladder_ms, "algorithmic_flops": ladder_flops,
+             "achieved": ladder_flops / (ladder_ms * 1e-3) / 1e12, "peak": peak, "unit": "TFLOP/s"}
+    phase["frac"] = phase["achieved"] / peak
+    fp64 = None
+    if kern64 is not None:
+        r64 = roof_of(dom, kern64, FP64_PEAK_TFLOPS)
+        l64 = sum(kern64[g]["ms"] for g in LADDER_GROUPS)
+        f64 = sum(kern64[g]["flops"] for g in LADDER_GROUPS)
+        fp64 = {"dominant": r64, "ladder_phase": {"ms": l64, "achieved": f64 / (l64 * 1e-3) / 1e12,
+                                                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                                 "frac": f64 / (l64 * 1e-3) / 1e12 / FP64_PEAK_TFLOPS},
+                "iteration_ms_single_stream": iter64_ms_serial}
+    hbm = {g: {"kernel": kern[g]["kernel"], "ms": kern[g]["ms"], "algorithmic_GBs": kern[g].get("gbs"),
+               "frac": kern[g].get("frac_hbm")} for g in ("remap_t", "remap_r", "update")}
+    if not sweep:
+        # the remaps' index tables: a source and a destination address (int32)
+        # and a permutation entry (uint16) per element
+        idx = 10.0 * ctx.n_t2
+        for g in ("remap_t", "remap_r"):
+            if kern[g]["ms"] > 0:
+                hbm[g]["kernel_GBs_with_index_tables"] = (kern[g]["bytes"] + idx) / (kern[g]["ms"] * 1e-3) / 1e9
 
     mixed = None
-    if args.precision == "mixed" and not sweep:
-        # full solves from T2 = 0, timed on the host clock around the call
+    gold = load_json(os.path.join(ROOT, "tests", "golden", cfg["golden"])) if cfg.get("golden") else None
+    if args.precision == "mixed" and not sweep and not args.no_solves:
+        # full solves from T2 = 0, each run once untimed, then timed on the
+        # host clock around the call
         model = models[0]
         n_part = model.n_holes
-        t0 = time.perf_counter()
-        e_m, stop_m, n_m, n32 = ctx.solve_mixed(1e-10, args.switch_tol)
-        t_m = time.perf_counter() - t0
+
+        def timed(f):
+            f()
+            t0 = time.perf_counter()
+            r = f()
+            return r, time.perf_counter() - t0
+
+        (e_m, stop_m, n_m, n32), t_m = timed(lambda: ctx.solve_mixed(1e-10, args.switch_tol))
         ctx64 = Context(model, FP64, matrix_free=free)
-        ctx64.solve(1e-10)                       # warm
-        t0 = time.perf_counter()
-        e_d, stop_d, n_d = ctx64.solve(1e-10)
-        t_d = time.perf_counter() - t0
+        ctx64.set_schedule("auto")
+        (e_d, stop_d, n_d), t_d = timed(lambda: ctx64.solve(1e-10))
+        ctx64.close()
+        ctx_st = Context(model, FP64, matrix_free=False)     # the plain FP64 GPU path: V^pppp stored
+        ctx_st.set_schedule("auto")
+        (e_s, stop_s, n_s), t_s = timed(lambda: ctx_st.solve(1e-10))
+        ctx_st.close()
         mixed = {"tol": 1e-10, "switch_tol": args.switch_tol,
                  "mixed_iters": n_m, "mixed_fp32_iters": n32, "mixed_stop": stop_m,
                  "mixed_solve_s": t_m, "mixed_energy": float(e_m[-1]),
                  "fp64_iters": n_d, "fp64_stop": stop_d, "fp64_solve_s": t_d,
-                 "fp64_energy": float(e_d[-1]),
+                 "fp64_energy": float(e_d[-1]), "fp64_mode": args.mode,
+                 "fp64_stored_iters": n_s, "fp64_stored_stop": stop_s, "fp64_stored_solve_s": t_s,
                  "abs_dE_per_particle": abs(float(e_m[-1]) - float(e_d[-1])) / n_part,
-                 "speedup_vs_fp64_solve": t_d / t_m}
+                 "speedup_vs_fp64_solve": t_d / t_m, "speedup_vs_fp64_stored_solve": t_s / t_m}
+        if gold:
+            e_o = gold["energies"][-1]
+            mixed["oracle_energy"] = e_o
+            mixed["oracle_iters"] = len(gold["energies"])
+            mixed["abs_dE_per_particle_vs_oracle"] = abs(float(e_m[-1]) - e_o) / n_part
+            mixed["fp64_abs_dE_per_particle_vs_oracle"] = abs(float(e_d[-1]) - e_o) / n_part
+            mixed["fp64_stored_abs_dE_per_particle_vs_oracle"] = abs(float(e_s[-1]) - e_o) / n_part
         # the same solve with other switch points (1e-10 = tol: FP32 iterations only)
         sw_runs = []
         for sw in (1e-6, 1e-8, 1e-10):
-            ctx.solve_mixed(1e-10, sw)                 # warm
-            t0 = time.perf_counter()
-            e_s, stop_s, n_s, n32_s = ctx.solve_mixed(1e-10, sw)
-            t_s = time.perf_counter() - t0
-            sw_runs.append({"switch_tol": sw, "iters": n_s, "fp32_iters": n32_s, "stop": stop_s,
-                          "solve_s": t_s, "speedup_vs_fp64_solve": t_d / t_s,
-                          "abs_dE_per_particle": abs(float(e_s[-1]) - float(e_d[-1])) / n_part})
+            (e_w, stop_w, n_w, n32_w), t_w = timed(lambda: ctx.solve_mixed(1e-10, sw))
+            r = {"switch_tol": sw, "iters": n_w, "fp32_iters": n32_w, "stop": stop_w,
+                 "solve_s": t_w, "speedup_vs_fp64_solve": t_d / t_w,
+                 "abs_dE_per_particle": abs(float(e_w[-1]) - float(e_d[-1])) / n_part}
+            if gold:
+                r["abs_dE_per_particle_vs_oracle"] = abs(float(e_w[-1]) - gold["energies"][-1]) / n_part
+            sw_runs.append(r)
         mixed["switch_sweep"] = sw_runs
-        ctx64.close()
-    elif args.precision == "mixed":
+    elif args.precision == "mixed" and not args.no_solves:
         # every solve of the sweep from T2 = 0, mixed and FP64, one batch each
         t0 = time.perf_counter()
         rm = ctx.solve_mixed(1e-10, args.switch_tol)
         t_m = time.perf_counter() - t0
         ctx.close()
         ctx64 = make_context(cfg, FP64, free, models)
+        ctx64.set_schedule("auto")
         t0 = time.perf_counter()
         rd = ctx64.solve(1e-10)
         t_d = time.perf_counter() - t0
@@ -264,7 +398,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(os.cpu_count() or 1, 16)
+        threads = cpu_threads()
         # V^pppp in the reference's layout is 54 GB at N=66: the oracle then
         # rebuilds it per channel (the reference's -m 1), else it reads it
         cpu_free = args.config == "heg66"
@@ -275,9 +409,12 @@ def main():
         else:
             v, _ = cpu_baseline(cfg["kind"], cfg["args"], args.cpu_iters, threads, cpu_free)
             sample = (f"{args.cpu_iters} FP64 iterations of the same workload by the C oracle "
-                      f"(OpenMP over channels, naive dgemm, V^pppp "
+                      f"(OpenMP over channels, V^pppp "
                       f"{'rebuilt per channel' if cpu_free else 'stored'}), after one untimed iteration")
-        cpu = {"value": v, "unit": "iter/s", "cores": threads, "kind": "port", "sample": sample}
+        cpu = {"value": v, "unit": "iter/s", "cores": threads, "kind": "port", "sample": sample,
+               "cpu_model": cpu_model(), "host_cpus_visible": os.cpu_count(),
+               "threads_rule": "OMP_NUM_THREADS (the process's CPU share) where set, else every visible CPU",
+               "blas": "none: the oracle's own row-major triple-loop dgemm (oracle/ccd_oracle.c)"}
 
     out = {
         "metric": "CCD iter/s + channel-GEMM TFLOP/s, HEG N=54; mixed vs FP64 at |dE|/N<=1e-7",
@@ -295,14 +432,17 @@ def main():
         "data": "synthetic: the reference's closed-form basis and Coulomb interaction, T2 from zero",
         "config": {"workload": cfg["name"], "precision": args.precision, "mode": args.mode,
                    "n_t2": ctx.n_t2, "device_bytes": ctx.device_bytes},
-        "gemm_tflops": gemm_tflops,
-        "gemm_flops_per_iter": work["flops"],
+        **out_eff,
         "mbpt2": mbpt2.tolist() if sweep else mbpt2,
         "corr_after_steps": e,
-        "phase_ms": {"ring": phases[0], "ladder_x": phases[1], "ladder_d": phases[2],
-                     "update": phases[3], "reductions": phases[4], "iteration": phases[5]},
+        "iteration_ms_single_stream": iter_ms_serial,
+        "kernels": kern,
         "setup_s": setup_s,
-        "roofline": dict(roof, traffic=traffic),
+        "schedule": {"chosen": sched, "tuner_us_per_iter": sched_us},
+        "roofline": roof,
+        "roofline_ladder_phase": phase,
+        "fp64": fp64,
+        "hbm": hbm,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

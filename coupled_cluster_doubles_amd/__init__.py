@@ -19,6 +19,11 @@ FP64 = 0
 FP32 = 1
 MATRIX_FREE = 0x100   # OR into the precision: V^pppp rebuilt inside the ladder (reference -m 1)
 STOP_REASONS = {0: "converged", 1: "max_iterations", 2: "nan"}
+# iteration schedules (CCD_SCHED_*): how the same kernels are launched
+SCHEDULES = {"auto": 0, "streams": 1, "serial": 2, "serial_graph": 3}
+# launch groups of an iteration, in CCD_KT_* order (include/ccd_amd.h)
+KERNEL_GROUPS = ("ring", "remap_r", "ladder_x", "ladder_tx", "ladder_v1", "ladder_v2", "ladder_v34",
+                 "ladder_v58", "update", "remap_t", "reduce")
 
 __all__ = ["Model", "Context", "Batch", "CCDError", "FP64", "FP32", "MATRIX_FREE", "STOP_REASONS"]
 
@@ -191,6 +196,18 @@ class Context:
                                     C.byref(n), C.byref(n32), C.byref(stop)), "ccd_solve_mixed")
         return en[:min(n.value, cap)].copy(), STOP_REASONS[stop.value], n.value, n32.value
 
+    def set_schedule(self, sched: str):
+        """'auto' (time the others on this problem and keep the fastest;
+        resets T2 to zero), 'streams', 'serial' or 'serial_graph'."""
+        check(lib().ccd_set_schedule(self.h, SCHEDULES[sched]), "ccd_set_schedule")
+
+    def schedule(self):
+        """(schedule in use, {schedule: tuner's microseconds per iteration})."""
+        k, us = C.c_int(), np.zeros(3)
+        check(lib().ccd_get_schedule(self.h, C.byref(k), _ptr(us)), "ccd_get_schedule")
+        names = {v: n for n, v in SCHEDULES.items()}
+        return names[k.value], {names[q + 1]: float(us[q]) for q in range(3)}
+
     def set_precision(self, precision: int):
         """Arithmetic of the contractions of the following iterations (FP64 or FP32)."""
         check(lib().ccd_set_precision(self.h, precision), "ccd_set_precision")
@@ -239,6 +256,26 @@ class Context:
         ms = np.zeros(6)
         check(lib().ccd_time_phases(self.h, iters, mixing, _ptr(ms)), "ccd_time_phases")
         return ms
+
+    def time_kernels(self, iters: int, reps: int = 1, mixing: float = 0.5) -> dict:
+        """Average device time (ms) of one launch of each launch group of an
+        iteration, the groups run in order on one stream, each `reps` times
+        back to back (ccd_time_kernels)."""
+        ms = np.zeros(len(KERNEL_GROUPS) + 1)
+        check(lib().ccd_time_kernels(self.h, iters, reps, mixing, _ptr(ms), ms.size), "ccd_time_kernels")
+        out = dict(zip(KERNEL_GROUPS, ms[:-1].tolist()))
+        out["iteration"] = float(ms[-1])
+        return out
+
+    def kernel_work(self) -> dict:
+        """Algorithmic flops, bytes and generated V elements of each launch
+        group of one iteration, in the arithmetic of the next iterations
+        (ccd_kernel_work)."""
+        n = len(KERNEL_GROUPS)
+        f, b, e = np.zeros(n), np.zeros(n), np.zeros(n)
+        check(lib().ccd_kernel_work(self.h, _ptr(f), _ptr(b), _ptr(e), n), "ccd_kernel_work")
+        return {g: {"flops": float(f[k]), "bytes": float(b[k]), "elements": float(e[k])}
+                for k, g in enumerate(KERNEL_GROUPS)}
 
     def work(self):
         f, fp, b = C.c_double(), C.c_double(), C.c_double()

@@ -40,6 +40,8 @@ SIGNATURES = [
     ("ccd_ctx_destroy", C.c_int, [vp]),
     ("ccd_ctx_info", C.c_int, [vp, i64p, i64p]),
     ("ccd_set_precision", C.c_int, [vp, C.c_int]),
+    ("ccd_set_schedule", C.c_int, [vp, C.c_int]),
+    ("ccd_get_schedule", C.c_int, [vp, C.POINTER(C.c_int), ip]),
     ("ccd_mbpt2", C.c_int, [vp, dp]),
     ("ccd_mbpt2_noalloc", C.c_int, [vp, vp, vp]),
     ("ccd_reset", C.c_int, [vp]),
@@ -57,7 +59,9 @@ SIGNATURES = [
     ("ccd_onebody", C.c_int, [vp, ip, ip, ip]),
     ("ccd_step", C.c_int, [vp, ip, ip, C.c_double, dp]),
     ("ccd_time_phases", C.c_int, [vp, C.c_int, C.c_double, ip]),
+    ("ccd_time_kernels", C.c_int, [vp, C.c_int, C.c_int, C.c_double, ip, C.c_int]),
     ("ccd_work", C.c_int, [vp, dp, dp, dp]),
+    ("ccd_kernel_work", C.c_int, [vp, ip, ip, ip, C.c_int]),
     ("ccd_batch_create", C.c_int, [C.POINTER(vp), C.c_int, C.c_int, C.POINTER(vp)]),
     ("ccd_batch_info", C.c_int, [vp, C.POINTER(C.c_int), ip]),
     ("ccd_batch_mbpt2", C.c_int, [vp, ip]),

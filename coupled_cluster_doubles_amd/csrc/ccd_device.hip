@@ -174,6 +174,17 @@ struct ccd_ctx {
     LoopCtl *d_ctl = nullptr;    // device loop state
     int *d_ticket = nullptr;     // last-block counter of k_reduce (zero between launches)
     LoopCtl *h_ctl = nullptr;    // pinned staging for it
+    LoopCtl *h_ctl2 = nullptr;   // pinned: the loop state after the batch in flight (run_loop)
+    hipEvent_t ctl_ev[2] = {};   // the copies of the loop state into h_ctl / h_ctl2 have landed
+    // schedule of the untimed iterations (CCD_SCHED_*, never AUTO once tuned)
+    // and the tuner's measurements, microseconds per iteration per schedule
+    int sched = CCD_SCHED_STREAMS;
+    double sched_us[3] = {0, 0, 0};
+    // kSolveBatch iterations captured as a graph, by the parity of cur at its
+    // start; the capture holds the kernels of one precision and mixing
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    int g_prec[2] = {-1, -1}, g_sched[2] = {-1, -1};
+    double g_mixing[2] = {0, 0};
     double *d_elog = nullptr;    // energy of every iteration of a device-run loop (kElogCap)
     double *h_scalar = nullptr;  // pinned
     int32_t *d_qn = nullptr;
@@ -232,8 +243,13 @@ void free_ctx(ccd_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (hipStream_t s : c->aux)
         if (s) (void)hipStreamSynchronize(s);
+    for (hipGraphExec_t g : c->gexec)
+        if (g) (void)hipGraphExecDestroy(g);
     for (hipEvent_t e : c->sync_ev)
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->ctl_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->h_ctl2) (void)hipHostFree(c->h_ctl2);
     for (hipStream_t s : c->aux)
         if (s) (void)hipStreamDestroy(s);
     for (void *p : c->allocs) (void)hipFree(p);
@@ -546,10 +562,13 @@ int join_aux(ccd_ctx *c) {
     return CCD_OK;
 }
 
-// One iteration.  Timed (ev = 6 events): every phase in order on the main
-// stream, the events bracketing ring + R remap | X | pppp ladder | update |
-// T remap + reductions.  Untimed: the independent phases run concurrently
-// on four streams, ordered by events --
+// One iteration.  Timed (ev = CCD_KT_COUNT + 1 events): every launch group in
+// order on the main stream, event k before group k (CCD_KT_* order: ring, R
+// remap, X, 1/2 T X, the pppp ladder classes 1, 2, 3-4 and 5-8, update, T
+// remap, reductions) and one after the last.  Untimed, one stream
+// (CCD_SCHED_SERIAL*): the same launches in dependency order.  Untimed, four
+// streams (CCD_SCHED_STREAMS): the independent phases run concurrently,
+// ordered by events --
 //   main: pppp ladder (1 and 2 tiles)        | update
 //   aux1: ring, R remap                      | after the update: T remap (which
 //                                            |   the next iteration's ring follows)
@@ -557,25 +576,44 @@ int join_aux(ccd_ctx *c) {
 //   aux3: pppp ladder (3+ tiles)             | after the update: reductions +
 //                                            |   energy (which the next update waits for)
 // (the pppp ladder does not need X; aux2 starts after the previous update).
-// Untimed iterations must run between fork_aux and join_aux; the loop state
-// is final only after a join.
-int run_iteration(ccd_ctx *c, double mixing, hipEvent_t *ev) {
+// Untimed four-stream iterations run between fork_aux and join_aux
+// (begin_run, end_run); the loop state is final only after the join.
+int run_iteration(ccd_ctx *c, double mixing, hipEvent_t *ev, int reps = 1) {
     const double *t_old = c->d_t[c->cur];
     double *t_new = c->d_t[1 - c->cur];
     int rc;
     if (ev) {
+        // each group launched `reps` times back to back (every group but the
+        // reductions rewrites the same outputs from the same inputs; repeated
+        // reductions count extra iterations of the free run's loop state)
         const hipStream_t st = c->stream;
-        HIP_TRY(hipEventRecord(ev[0], st));
-        if ((rc = launch_ring(c, st)) || (rc = launch_remap_r(c, st))) return rc;
-        HIP_TRY(hipEventRecord(ev[1], st));
-        if ((rc = launch_ladder_x(c, st))) return rc;
-        HIP_TRY(hipEventRecord(ev[2], st));
-        if ((rc = launch_ladder_tx(c, st)) || (rc = launch_ladder_v(c, 1, NTMAX, st))) return rc;
-        HIP_TRY(hipEventRecord(ev[3], st));
-        if ((rc = launch_update<true>(c, t_old, t_new, mixing, st))) return rc;
-        HIP_TRY(hipEventRecord(ev[4], st));
-        if ((rc = launch_tremap(c, t_new, st)) || (rc = launch_reduce(c, st))) return rc;
-        HIP_TRY(hipEventRecord(ev[5], st));
+        int k = 0;
+        auto group = [&](auto &&launch) -> int {
+            HIP_TRY(hipEventRecord(ev[k++], st));
+            for (int r = 0; r < reps; r++)
+                if (int rg = launch()) return rg;
+            return CCD_OK;
+        };
+        if ((rc = group([&] { return launch_ring(c, st); })) || (rc = group([&] { return launch_remap_r(c, st); })) ||
+            (rc = group([&] { return launch_ladder_x(c, st); })) ||
+            (rc = group([&] { return launch_ladder_tx(c, st); })) ||
+            (rc = group([&] { return launch_ladder_v(c, 1, 1, st); })) ||
+            (rc = group([&] { return launch_ladder_v(c, 2, 2, st); })) ||
+            (rc = group([&] { return launch_ladder_v(c, 3, 4, st); })) ||
+            (rc = group([&] { return launch_ladder_v(c, 5, NTMAX, st); })) ||
+            (rc = group([&] { return launch_update<true>(c, t_old, t_new, mixing, st); })) ||
+            (rc = group([&] { return launch_tremap(c, t_new, st); })) ||
+            (rc = group([&] { return launch_reduce(c, st); })))
+            return rc;
+        HIP_TRY(hipEventRecord(ev[k++], st));
+        static_assert(CCD_KT_COUNT == 11, "one event per launch group and one at the end");
+    } else if (c->sched == CCD_SCHED_SERIAL || c->sched == CCD_SCHED_SERIAL_GRAPH) {
+        const hipStream_t st = c->stream;
+        if ((rc = launch_ladder_x(c, st)) || (rc = launch_ladder_tx(c, st)) || (rc = launch_ring(c, st)) ||
+            (rc = launch_remap_r(c, st)) || (rc = launch_ladder_v(c, 1, NTMAX, st)) ||
+            (rc = launch_update<true>(c, t_old, t_new, mixing, st)) || (rc = launch_tremap(c, t_new, st)) ||
+            (rc = launch_reduce(c, st)))
+            return rc;
     } else {
         const hipStream_t s0 = c->stream, s1 = c->aux[0], s2 = c->aux[1], s3 = c->aux[2];
         hipEvent_t *e = c->sync_ev;
@@ -601,6 +639,82 @@ int run_iteration(ccd_ctx *c, double mixing, hipEvent_t *ev) {
     }
     c->cur = 1 - c->cur;
     return CCD_OK;
+}
+
+// kSolveBatch untimed one-stream iterations captured as one graph, starting
+// at the current parity of cur (the capture runs nothing: cur and the
+// host-side bookkeeping of the launches end where they started, kSolveBatch
+// being even).  Only kernel launches are captured: no event, memset or copy.
+int capture_batch(ccd_ctx *c, double mixing, hipGraph_t *g) {
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = CCD_OK;
+    for (int k = 0; k < kSolveBatch && !rc; k++) rc = run_iteration(c, mixing, nullptr);
+    const hipError_t ee = hipStreamEndCapture(c->stream, g);   // ends the capture whatever happened
+    if (!rc && ee != hipSuccess) rc = ccd::fail(CCD_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ee));
+    if (rc && *g) {
+        (void)hipGraphDestroy(*g);
+        *g = nullptr;
+    }
+    return rc;
+}
+
+int drop_graphs(ccd_ctx *c) {
+    for (int k = 0; k < 2; k++) {
+        if (c->gexec[k]) HIP_TRY(hipGraphExecDestroy(c->gexec[k]));
+        c->gexec[k] = nullptr;
+        c->g_prec[k] = c->g_sched[k] = -1;
+    }
+    return CCD_OK;
+}
+
+// A run of untimed iterations starts with begin_run (four streams: the aux
+// streams start after everything queued on the main stream) and ends with
+// end_run (the main stream continues after everything queued on them).
+// Inside a run, iterations carry their own cross-stream ordering.
+int begin_run(ccd_ctx *c) { return c->sched == CCD_SCHED_STREAMS ? fork_aux(c) : CCD_OK; }
+int end_run(ccd_ctx *c) { return c->sched == CCD_SCHED_STREAMS ? join_aux(c) : CCD_OK; }
+
+// The stream whose work writes the loop state last in an iteration (the
+// reductions'): a copy of the state queued there follows the iteration.
+hipStream_t ctl_stream(const ccd_ctx *c) { return c->sched == CCD_SCHED_STREAMS ? c->aux[2] : c->stream; }
+
+// n untimed iterations inside a run: a graph replay for a whole batch under
+// the graph schedule, else launched one by one.
+int run_batch(ccd_ctx *c, double mixing, int n) {
+    if (n <= 0) return CCD_OK;
+    if (c->sched == CCD_SCHED_SERIAL_GRAPH && n == kSolveBatch) {
+        const int k = c->cur;
+        if (!c->gexec[k] || c->g_prec[k] != c->iter_prec || c->g_sched[k] != c->sched || c->g_mixing[k] != mixing) {
+            if (c->gexec[k]) HIP_TRY(hipGraphExecDestroy(c->gexec[k]));
+            c->gexec[k] = nullptr;
+            hipGraph_t g = nullptr;
+            if (int rc = capture_batch(c, mixing, &g)) return rc;
+            const hipError_t e = hipGraphInstantiate(&c->gexec[k], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                c->gexec[k] = nullptr;
+                return ccd::fail(CCD_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+            }
+            c->g_prec[k] = c->iter_prec;
+            c->g_sched[k] = c->sched;
+            c->g_mixing[k] = mixing;
+        }
+        HIP_TRY(hipGraphLaunch(c->gexec[k], c->stream));
+        c->tph_prec = c->iter_prec;   // what the replayed T remaps and updates write
+        c->tas_prec = c->iter_prec;
+        return CCD_OK;
+    }
+    for (int k = 0; k < n; k++)
+        if (int rc = run_iteration(c, mixing, nullptr)) return rc;
+    return CCD_OK;
+}
+
+// n untimed iterations as one run.
+int run_n(ccd_ctx *c, double mixing, int n) {
+    int rc = begin_run(c);
+    for (int k = 0; k < n && !rc; k += kSolveBatch) rc = run_batch(c, mixing, std::min(kSolveBatch, n - k));
+    if (!rc) rc = end_run(c);
+    return rc;
 }
 
 // Caller amplitudes (the reference's pphh order) to an internal-order device
@@ -1348,6 +1462,9 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
         (rc = dalloc(c, &c->d_elog, (size_t)kElogCap * nprob)) || (rc = upload(c, &c->d_hlo, bi.hlo)))
         return rc;
     HIP_TRY(hipHostMalloc((void **)&c->h_ctl, sizeof(LoopCtl), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&c->h_ctl2, sizeof(LoopCtl), hipHostMallocDefault));
+    *c->h_ctl = LoopCtl{};
+    *c->h_ctl2 = LoopCtl{};
     if (nprob > 1) {
         if ((rc = dalloc(c, &c->d_pctl, (size_t)nprob))) return rc;
         HIP_TRY(hipHostMalloc((void **)&c->h_pctl, sizeof(LoopCtl) * nprob, hipHostMallocDefault));
@@ -1540,25 +1657,35 @@ void mark_stopped(ccd_ctx *c) {
 // next entry point that iterates calls arm_free_run).
 int end_solve(ccd_ctx *c) { return arm_loop(c, -1.0, INT_MAX, true); }
 
-// Queue iterations in batches without waiting for each energy; after each
-// batch read the loop state.  Iterations queued past the stopping point do
-// nothing.  Returns with the loop state in h_ctl and the current amplitudes
-// set to those of the last real iteration.
+// Queue iterations in batches without waiting for each energy, one batch
+// ahead: the loop state after batch k is read while batch k+1 runs.
+// Iterations queued past the stopping point do nothing.  Returns with the
+// loop state in h_ctl and the current amplitudes set to those of the last
+// real iteration.
 int run_loop(ccd_ctx *c, double mixing, int batch) {
     const int cur0 = c->cur, count0 = c->h_ctl->count;
-    int rc = fork_aux(c);
-    if (rc) return rc;
-    for (;;) {
-        for (int k = 0; k < batch; k++)
-            if ((rc = run_iteration(c, mixing, nullptr))) return rc;
-        if ((rc = join_aux(c))) return rc;
-        HIP_TRY(hipMemcpyAsync(c->h_ctl, c->d_ctl, sizeof(LoopCtl), hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        if (c->h_ctl->done) break;
+    LoopCtl *slot[2] = {c->h_ctl, c->h_ctl2};
+    const hipStream_t cs = ctl_stream(c);
+    int rc;
+    auto queue = [&](int k) -> int {
+        if (int rq = run_batch(c, mixing, batch)) return rq;
+        HIP_TRY(hipMemcpyAsync(slot[k], c->d_ctl, sizeof(LoopCtl), hipMemcpyDeviceToHost, cs));
+        HIP_TRY(hipEventRecord(c->ctl_ev[k], cs));
+        return CCD_OK;
+    };
+    if ((rc = begin_run(c)) || (rc = queue(0))) return rc;
+    int k = 0;
+    for (;; k ^= 1) {
+        if ((rc = queue(k ^ 1))) return rc;
+        HIP_TRY(hipEventSynchronize(c->ctl_ev[k]));
+        if (slot[k]->done) break;
     }
-    c->cur = (cur0 + (c->h_ctl->count - count0)) & 1;
-    if ((rc = join_aux(c))) return rc;
+    // the batch queued after the stop has run (returning at entry) and left
+    // the same state in the other slot
+    if ((rc = end_run(c))) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
+    if (slot[k] != c->h_ctl) *c->h_ctl = *slot[k];
+    c->cur = (cur0 + (c->h_ctl->count - count0)) & 1;
     mark_stopped(c);
     return CCD_OK;
 }
@@ -1641,6 +1768,11 @@ int create_ctx(const ccd_problem *prob, const BatchInfo &bi, int precision, ccd_
             free_ctx(c);
             return ccd::fail(CCD_E_HIP, "could not create HIP events");
         }
+    for (hipEvent_t &e : c->ctl_ev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            free_ctx(c);
+            return ccd::fail(CCD_E_HIP, "could not create HIP events");
+        }
     if ((rc = init_scratch(c->device))) {
         free_ctx(c);
         return rc;
@@ -1665,6 +1797,49 @@ int create_ctx(const ccd_problem *prob, const BatchInfo &bi, int precision, ccd_
     }
     *out = c;
     return CCD_OK;
+}
+
+// The schedule tuner: a free run of a few batches under each schedule (the
+// first batch untimed: it captures the graph), HIP events on the main
+// stream around the timed batches; the fastest is kept, and the amplitudes
+// go back to T2 = 0.  Every schedule gives the same results bit for bit
+// (the same kernels in a dependency-respecting order, every reduction in a
+// fixed order), so the choice is about time only.
+int tune_schedule(ccd_ctx *c) {
+    static const int cand[3] = {CCD_SCHED_STREAMS, CCD_SCHED_SERIAL, CCD_SCHED_SERIAL_GRAPH};
+    constexpr int kTimed = 2;   // batches
+    hipEvent_t a = nullptr, b = nullptr;
+    HIP_TRY(hipEventCreate(&a));
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return ccd::fail(CCD_E_HIP, "could not create HIP events");
+    }
+    int rc = CCD_OK, pick = CCD_SCHED_STREAMS;
+    double best = 0.0;
+    for (int q = 0; q < 3 && !rc; q++) {
+        c->sched = cand[q];
+        rc = arm_free_run(c);
+        if (!rc) rc = run_n(c, 0.5, kSolveBatch);
+        hipError_t e = rc ? hipSuccess : hipEventRecord(a, c->stream);
+        if (!rc && e == hipSuccess) rc = run_n(c, 0.5, kTimed * kSolveBatch);
+        if (!rc && e == hipSuccess) e = hipEventRecord(b, c->stream);
+        if (!rc && e == hipSuccess) e = hipEventSynchronize(b);
+        float ms = 0.f;
+        if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+        if (!rc && e != hipSuccess) rc = ccd::fail(CCD_E_HIP, "schedule timing failed: %s", hipGetErrorString(e));
+        if (rc) break;
+        c->sched_us[q] = 1e3 * ms / (kTimed * kSolveBatch);
+        if (q == 0 || c->sched_us[q] < best) {
+            best = c->sched_us[q];
+            pick = cand[q];
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    c->sched = pick;
+    if (!rc && pick != CCD_SCHED_SERIAL_GRAPH) rc = drop_graphs(c);
+    if (!rc) rc = zero_state(c);
+    return rc;
 }
 
 int batch_only(const ccd_ctx *c, const char *what) {
@@ -1789,9 +1964,7 @@ int ccd_batch_iterate_n(ccd_ctx *c, int n, double mixing, double *energies) {
     ccd::clear_error();
     if (!c || n < 0) return ccd::fail(CCD_E_INVALID, "ccd_batch_iterate_n: bad argument");
     int rc = arm_free_run(c);
-    if (!rc) rc = fork_aux(c);
-    for (int k = 0; k < n && !rc; k++) rc = run_iteration(c, mixing, nullptr);
-    if (rc || (rc = join_aux(c))) return rc;
+    if (rc || (rc = run_n(c, mixing, n))) return rc;
     HIP_TRY(hipMemcpyAsync(c->h_ctl, c->d_ctl, sizeof(LoopCtl), hipMemcpyDeviceToHost, c->stream));
     if ((rc = read_pctl(c))) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1839,6 +2012,44 @@ int ccd_work(const ccd_ctx *ctx, double *flops, double *flops_pppp, double *ladd
             b += (ctx->matrix_free ? 0.0 : vb * ch.npk * ch.npk) + 2.0 * vb * ch.npk * ch.hh +
                  8.0 * (double)ch.hh * ch.hh;
         *ladder_bytes = b;
+    }
+    return CCD_OK;
+}
+
+int ccd_kernel_work(const ccd_ctx *ctx, double *flops, double *bytes, double *elements, int n) {
+    if (!ctx || n < 0) return ccd::fail(CCD_E_INVALID, "ccd_kernel_work: bad argument");
+    double f[CCD_KT_COUNT] = {}, b[CCD_KT_COUNT] = {}, e[CCD_KT_COUNT] = {};
+    const double cs = ctx->iter_prec == CCD_PREC_FP64 ? 8.0 : 4.0;   // contraction operands and outputs
+    const double nt = (double)ctx->nt2;
+    for (const DMod &m : ctx->mods) {
+        const double ph = m.ph, hp = m.hp;
+        f[CCD_KT_RING] += 2.0 * hp * hp * ph + 2.0 * ph * hp * hp;
+        b[CCD_KT_RING] += cs * (hp * ph + hp * hp + 2.0 * ph * hp);   // V^A, V^B, T^ph in, R out
+    }
+    b[CCD_KT_REMAP_R] = 2.0 * cs * nt;
+    b[CCD_KT_REMAP_T] = (8.0 + cs) * nt;   // FP64 amplitudes in, T^ph in the contractions' type out
+    for (const DChan &ch : ctx->chans) {
+        const double npk = ch.npk, hh = ch.hh, ppk = (double)ch.npk * ch.npk;
+        f[CCD_KT_X] += 2.0 * hh * hh * npk;
+        b[CCD_KT_X] += (8.0 + cs) * npk * hh + 16.0 * hh * hh;      // V^hhpp rows, T rows, V^hhhh in, X out
+        f[CCD_KT_TX] += 2.0 * npk * hh * hh;
+        b[CCD_KT_TX] += 2.0 * cs * npk * hh + 8.0 * hh * hh;        // T rows, X in, D part out
+        const int nt16 = (ch.hh + 15) / 16;
+        const int g = nt16 == 1 ? CCD_KT_LADDER1 : nt16 == 2 ? CCD_KT_LADDER2 : nt16 <= 4 ? CCD_KT_LADDER34
+                                                                                          : CCD_KT_LADDER58;
+        f[g] += 2.0 * ppk * hh;
+        b[g] += (ctx->matrix_free ? 0.0 : cs * ppk) + 2.0 * cs * npk * hh;   // V^pppp (stored), T rows in, D out
+        if (ctx->matrix_free) e[g] += ppk;
+        // the update: D read once per packed row (one array in the algorithm)
+        b[CCD_KT_UPDATE] += cs * npk * hh + cs * npk * hh;               // D in, T rows out
+    }
+    // per element: <ij||ab>, T_old, the ring term (contraction type) in, T_new out
+    b[CCD_KT_UPDATE] += (8.0 + 8.0 + cs + 8.0) * nt;
+    b[CCD_KT_REDUCE] = 8.0 * (double)ctx->n_red + 8.0 * ctx->nsp;
+    for (int k = 0; k < n && k < CCD_KT_COUNT; k++) {
+        if (flops) flops[k] = f[k];
+        if (bytes) bytes[k] = b[k];
+        if (elements) elements[k] = e[k];
     }
     return CCD_OK;
 }
@@ -1895,7 +2106,7 @@ int ccd_iterate(ccd_ctx *c, double mixing, double *corr) {
     if (!c) return ccd::fail(CCD_E_INVALID, "null context");
     if (c->nprob > 1) return batch_only(c, "ccd_iterate");
     int rc = arm_free_run(c);
-    if (rc || (rc = fork_aux(c)) || (rc = run_iteration(c, mixing, nullptr)) || (rc = join_aux(c))) return rc;
+    if (rc || (rc = run_n(c, mixing, 1))) return rc;
     double e = 0.0;
     if ((rc = read_scalar(c, &e))) return rc;
     if (std::isnan(e)) mark_stopped(c);   // the loop state stopped this iteration
@@ -1927,6 +2138,34 @@ int ccd_set_precision(ccd_ctx *c, int precision) {
         return ccd::fail(CCD_E_STATE, "FP32 iterations need a context created with CCD_PREC_FP32");
     c->iter_prec = precision;
     return sync_state(c);
+}
+
+int ccd_set_schedule(ccd_ctx *c, int sched) {
+    ccd::clear_error();
+    if (!c) return ccd::fail(CCD_E_INVALID, "null context");
+    if (sched < CCD_SCHED_AUTO || sched > CCD_SCHED_SERIAL_GRAPH)
+        return ccd::fail(CCD_E_INVALID, "unknown schedule %d", sched);
+    if (sched == CCD_SCHED_AUTO) {
+        const int saved = c->iter_prec;
+        int rc = tune_schedule(c);
+        c->iter_prec = saved;
+        if (rc) return rc;
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        return CCD_OK;
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (int rc = drop_graphs(c)) return rc;
+    c->sched = sched;
+    return CCD_OK;
+}
+
+int ccd_get_schedule(const ccd_ctx *c, int *sched, double *us_per_iter) {
+    ccd::clear_error();
+    if (!c) return ccd::fail(CCD_E_INVALID, "null context");
+    if (sched) *sched = c->sched;
+    if (us_per_iter)
+        for (int q = 0; q < 3; q++) us_per_iter[q] = c->sched_us[q];
+    return CCD_OK;
 }
 
 int ccd_solve_mixed(ccd_ctx *c, double tol, double switch_tol, int max_iters, double mixing, double *energies,
@@ -1968,9 +2207,7 @@ int ccd_iterate_n(ccd_ctx *c, int n, double mixing, double *energies) {
     if (!c || n < 0) return ccd::fail(CCD_E_INVALID, "ccd_iterate_n: bad argument");
     if (c->nprob > 1) return batch_only(c, "ccd_iterate_n");
     int rc = arm_free_run(c);
-    if (!rc) rc = fork_aux(c);
-    for (int k = 0; k < n && !rc; k++) rc = run_iteration(c, mixing, nullptr);
-    if (rc || (rc = join_aux(c))) return rc;
+    if (rc || (rc = run_n(c, mixing, n))) return rc;
     HIP_TRY(hipMemcpyAsync(c->h_ctl, c->d_ctl, sizeof(LoopCtl), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (c->h_ctl->count != n) {   // a NaN energy stops even a free run
@@ -2086,7 +2323,7 @@ int ccd_step(ccd_ctx *c, const double *t_old, double *t_new, double mixing, doub
     if (c->nprob > 1) return batch_only(c, "ccd_step");
     int rc = set_state(c, t_old);
     if (rc) return rc;
-    if ((rc = fork_aux(c)) || (rc = run_iteration(c, mixing, nullptr)) || (rc = join_aux(c))) return rc;
+    if ((rc = run_n(c, mixing, 1))) return rc;
     double e = 0.0;
     if ((rc = read_scalar(c, &e))) return rc;
     if ((rc = t2_download(c, c->d_t[c->cur], t_new, c->d_g))) return rc;
@@ -2096,35 +2333,67 @@ int ccd_step(ccd_ctx *c, const double *t_old, double *t_new, double mixing, doub
     return CCD_OK;
 }
 
-int ccd_time_phases(ccd_ctx *c, int iters, double mixing, double *ms) {
-    ccd::clear_error();
-    if (!c || !ms || iters <= 0) return ccd::fail(CCD_E_INVALID, "ccd_time_phases: bad argument");
-    int rc0 = arm_free_run(c);
-    if (rc0) return rc0;
-    hipEvent_t ev[6];
-    for (int k = 0; k < 6; k++) HIP_TRY(hipEventCreate(&ev[k]));
-    double acc[6] = {0, 0, 0, 0, 0, 0};
+}  // extern "C"
+
+namespace {
+
+// Average device time of each launch group over `iters` iterations run in
+// order on the main stream, each group launched `reps` times back to back
+// (acc[CCD_KT_COUNT] = the whole iteration, acc[k] per launch of group k).
+int time_groups(ccd_ctx *c, int iters, double mixing, int reps, double (&acc)[CCD_KT_COUNT + 1]) {
+    if (int rc0 = arm_free_run(c)) return rc0;
+    hipEvent_t ev[CCD_KT_COUNT + 1];
+    for (int k = 0; k <= CCD_KT_COUNT; k++) HIP_TRY(hipEventCreate(&ev[k]));
+    for (double &a : acc) a = 0.0;
     int rc = CCD_OK;
     for (int it = 0; it < iters && !rc; it++) {
-        rc = run_iteration(c, mixing, ev);
+        rc = run_iteration(c, mixing, ev, reps);
         if (rc) break;
-        hipError_t e = hipEventSynchronize(ev[5]);
+        hipError_t e = hipEventSynchronize(ev[CCD_KT_COUNT]);
         if (e != hipSuccess) {
             rc = ccd::fail(CCD_E_HIP, "iteration failed: %s", hipGetErrorString(e));
             break;
         }
-        for (int k = 0; k < 5; k++) {
+        for (int k = 0; k < CCD_KT_COUNT; k++) {
             float f = 0.f;
             (void)hipEventElapsedTime(&f, ev[k], ev[k + 1]);
-            acc[k] += f;
+            acc[k] += f / reps;
         }
         float f = 0.f;
-        (void)hipEventElapsedTime(&f, ev[0], ev[5]);
-        acc[5] += f;
+        (void)hipEventElapsedTime(&f, ev[0], ev[CCD_KT_COUNT]);
+        acc[CCD_KT_COUNT] += f;
     }
-    for (int k = 0; k < 6; k++) (void)hipEventDestroy(ev[k]);
+    for (int k = 0; k <= CCD_KT_COUNT; k++) (void)hipEventDestroy(ev[k]);
     if (rc) return rc;
-    for (int k = 0; k < 6; k++) ms[k] = acc[k] / iters;
+    for (double &a : acc) a /= iters;
+    return CCD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ccd_time_phases(ccd_ctx *c, int iters, double mixing, double *ms) {
+    ccd::clear_error();
+    if (!c || !ms || iters <= 0) return ccd::fail(CCD_E_INVALID, "ccd_time_phases: bad argument");
+    double g[CCD_KT_COUNT + 1];
+    if (int rc = time_groups(c, iters, mixing, 1, g)) return rc;
+    ms[0] = g[CCD_KT_RING] + g[CCD_KT_REMAP_R];
+    ms[1] = g[CCD_KT_X];
+    ms[2] = g[CCD_KT_TX] + g[CCD_KT_LADDER1] + g[CCD_KT_LADDER2] + g[CCD_KT_LADDER34] + g[CCD_KT_LADDER58];
+    ms[3] = g[CCD_KT_UPDATE];
+    ms[4] = g[CCD_KT_REMAP_T] + g[CCD_KT_REDUCE];
+    ms[5] = g[CCD_KT_COUNT];
+    return CCD_OK;
+}
+
+int ccd_time_kernels(ccd_ctx *c, int iters, int reps, double mixing, double *ms, int n) {
+    ccd::clear_error();
+    if (!c || !ms || iters <= 0 || reps <= 0 || n < 0)
+        return ccd::fail(CCD_E_INVALID, "ccd_time_kernels: bad argument");
+    double g[CCD_KT_COUNT + 1];
+    if (int rc = time_groups(c, iters, mixing, reps, g)) return rc;
+    for (int k = 0; k < n && k <= CCD_KT_COUNT; k++) ms[k] = g[k];
     return CCD_OK;
 }
 

@@ -162,6 +162,8 @@ int main(int argc, char **argv) {
     // -m 1 regenerates V^pppp inside the ladder
     const int mode = o.precision | (o.saveMemory ? CCD_MATRIX_FREE : 0);
     if ((rc = ccd_ctx_create(prob, mode, &ctx))) return die("ccd_ctx_create", rc);
+    // the fastest launch schedule for this problem (same results under any)
+    if ((rc = ccd_set_schedule(ctx, CCD_SCHED_AUTO))) return die("ccd_set_schedule", rc);
     int64_t dbytes = 0;
     ccd_ctx_info(ctx, nullptr, &dbytes);
     printf("rank %d device arrays require %g GB of memory\n", 0, dbytes / 1.e9);

@@ -255,8 +255,24 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const int32_t *__restrict__
     __shared__ double red[kThreads / 64];
     __shared__ int last, pdone;
     const int k = blockIdx.x;
+    // thread t adds parts[lo + t], parts[lo + t + 256], ... in order; the
+    // loads of 8 strides are issued before their adds (a slot past the list
+    // reads element lo and is not added), so a thread waits for one memory
+    // round trip per 8 elements
+    constexpr int U = 8;
+    const int lo = ptr[k], hi = ptr[k + 1];
     double s = 0.0;
-    for (int q = ptr[k] + threadIdx.x; q < ptr[k + 1]; q += kThreads) s += parts[q];
+    for (int q0 = lo + threadIdx.x; q0 < hi; q0 += kThreads * U) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int q = q0 + u * kThreads;
+            v[u] = parts[q < hi ? q : lo];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (q0 + u * kThreads < hi) s += v[u];
+    }
     const double b = block_sum(s, red);
     if (threadIdx.x == 0) {
         out[k] = b;

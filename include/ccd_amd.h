@@ -130,6 +130,27 @@ int ccd_ctx_info(const ccd_ctx *ctx, int64_t *n_t2, int64_t *device_bytes);
  * only. */
 int ccd_set_precision(ccd_ctx *ctx, int precision);
 
+/* How the iterations are launched (ccd_set_schedule).  No counterpart in the
+ * reference.  Every schedule runs the same kernels in an order that keeps
+ * their dependencies, and every reduction is in a fixed order, so all give
+ * the same results bit for bit. */
+#define CCD_SCHED_AUTO 0           /* the fastest of the three below, timed on the
+                                      context's own problem */
+#define CCD_SCHED_STREAMS 1        /* independent phases on four streams, launched
+                                      iteration by iteration (the initial one) */
+#define CCD_SCHED_SERIAL 2         /* every launch in order on one stream */
+#define CCD_SCHED_SERIAL_GRAPH 3   /* the one-stream iterations captured as a HIP
+                                      graph of 4 iterations and replayed */
+
+/* Choose the schedule of the following iterations.  CCD_SCHED_AUTO times the
+ * three on this problem (a few free iterations) and resets T2 to zero; the
+ * others keep the amplitudes. */
+int ccd_set_schedule(ccd_ctx *ctx, int sched);
+/* The schedule in use (never CCD_SCHED_AUTO) and, if us_per_iter is not NULL,
+ * the tuner's last measurements (0 before any): microseconds per iteration
+ * of schedules 1..3 in us_per_iter[0..2]. */
+int ccd_get_schedule(const ccd_ctx *ctx, int *sched, double *us_per_iter);
+
 /* MBPT2 correlation energy, 1/4 sum v^2/(e_i+e_j-e_a-e_b)
  * (replaces MBPT2Corr, src/MBPT2Corr.cpp:6-38). */
 int ccd_mbpt2(ccd_ctx *ctx, double *corr);
@@ -215,6 +236,32 @@ int ccd_step(ccd_ctx *ctx, const double *t_old, double *t_new, double mixing,
  * ms[5] whole iteration.  ms must hold 6 doubles. */
 int ccd_time_phases(ccd_ctx *ctx, int iters, double mixing, double *ms);
 
+/* Launch groups of one iteration, in the order they run in the timed
+ * iterations of ccd_time_phases / ccd_time_kernels (all on the context's
+ * stream). */
+#define CCD_KT_RING 0       /* calcT2Diff_phhp: X = V^B + 1/2 V^A T^ph, R = T^ph X */
+#define CCD_KT_REMAP_R 1    /* R to the pphh layout (the 4 gathers of main.cpp:636-639) */
+#define CCD_KT_X 2          /* X = V^hhhh + 1/2 V^hhpp T (calc.cpp:171-187) */
+#define CCD_KT_TX 3         /* 1/2 T X into the ladder output */
+#define CCD_KT_LADDER1 4    /* 1/2 V^pppp T, channels with hh <= 16 */
+#define CCD_KT_LADDER2 5    /*   17 <= hh <= 32 */
+#define CCD_KT_LADDER34 6   /*   33 <= hh <= 64 */
+#define CCD_KT_LADDER58 7   /*   65 <= hh <= 128 */
+#define CCD_KT_UPDATE 8     /* calcT2_pphh: assembly, denominators, mixing, W sums */
+#define CCD_KT_REMAP_T 9    /* T to T^ph (mapToDistributed, Chain.hpp:757-802) */
+#define CCD_KT_REDUCE 10    /* chi, psi, the energy and the stop rule */
+#define CCD_KT_COUNT 11
+
+/* Like ccd_time_phases, per launch group: `iters` iterations with the groups
+ * in order on the context's stream, each group launched `reps` times back to
+ * back (the groups rewrite the same outputs from the same inputs, so a
+ * repetition changes no result; the reductions also count the extra runs in
+ * the loop state of the free run), HIP events around each group.  ms[k]
+ * (k < n): the average time of one launch of group k in milliseconds;
+ * ms[CCD_KT_COUNT] (if n > CCD_KT_COUNT): the average time of a whole
+ * timed iteration.  No counterpart in the reference. */
+int ccd_time_kernels(ccd_ctx *ctx, int iters, int reps, double mixing, double *ms, int n);
+
 /* Algorithmic work of one iteration in the reference's ordered-pair algebra:
  * flops F = sum_c (2hh^2 pp + 2pp hh^2 + 2pp^2 hh) + sum_m (2hp^2 ph + 2ph hp^2)
  * and the pppp-ladder part of it; bytes one pass of the packed pppp ladder
@@ -224,6 +271,18 @@ int ccd_time_phases(ccd_ctx *ctx, int iters, double mixing, double *ms);
  * FP64). */
 int ccd_work(const ccd_ctx *ctx, double *flops, double *flops_pppp,
              double *bytes_vpppp);
+
+/* Algorithmic work of each launch group (CCD_KT_* order, k < n) of one
+ * iteration in the arithmetic of the next iterations: flops[k] the flops of
+ * its products on the packed pairs a<b the device uses (the reference's
+ * ordered pairs do 4x the pppp ladder's flops), bytes[k] the bytes its
+ * operands and results take when each is read or written once (contraction
+ * operands and outputs in the iteration's type, amplitudes, V^hhpp and X in
+ * FP64; no index tables), elements[k] the V^pppp elements the matrix-free
+ * ladder generates (0 otherwise).  Any pointer may be NULL.  No counterpart
+ * in the reference. */
+int ccd_kernel_work(const ccd_ctx *ctx, double *flops, double *bytes,
+                    double *elements, int n);
 
 /* ---------------------------------------------------------------- batches
  *
