@@ -77,6 +77,8 @@ KERNEL_OF_GROUP = {
     "update": "k_update<true,{ct}>", "remap_t": "k_transpose<double,{ct}> (T to T^ph)",
     "reduce": "k_reduce",
 }
+# FP32 iterations run class 1 of the pppp ladder on the one-row-per-lane kernel
+KERNEL_OF_GROUP_F32 = {"ladder_v1": "k_ladder_w<1,{src}>"}
 
 
 def make_model(cfg):
@@ -160,7 +162,8 @@ def kernel_report(ctx, iters, ct, src):
         if g == "iteration":
             continue
         sec = ms * 1e-3
-        e = {"kernel": KERNEL_OF_GROUP[g].format(ct=ct, src=src), "ms": ms,
+        lab = (KERNEL_OF_GROUP_F32 if ct == "float" else {}).get(g, KERNEL_OF_GROUP[g])
+        e = {"kernel": lab.format(ct=ct, src=src), "ms": ms,
              "flops": w[g]["flops"], "bytes": w[g]["bytes"]}
         if sec > 0:
             e["tflops"] = w[g]["flops"] / sec / 1e12

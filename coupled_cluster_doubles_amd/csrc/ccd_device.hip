@@ -189,7 +189,7 @@ struct ccd_ctx {
     double *h_scalar = nullptr;  // pinned
     int32_t *d_qn = nullptr;
     double *d_table = nullptr;
-    int2 *d_rec1 = nullptr, *d_rec2 = nullptr;  // generated-V pair records
+    int2 *d_rec1 = nullptr, *d_rec2 = nullptr, *d_rec1f = nullptr, *d_rec2f = nullptr, *d_rec1w = nullptr;  // generated-V pair records
     RingItem *d_ritems = nullptr;
     RingGroup *d_rgroups = nullptr;
     float *d_vp32 = nullptr, *d_va32 = nullptr, *d_vb32 = nullptr;  // FP32 copies (mixed precision)
@@ -328,11 +328,19 @@ int allow_ring_lds(size_t bytes) {
     return CCD_OK;
 }
 
+template <int NTW, int SRC>
+int allow_wave_lds() {
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_ladder_w<NTW, SRC>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)WaveLadder<NTW>::lds_bytes(SRC == LV_GAS ? kGasTableMax : 0)));
+    return CCD_OK;
+}
+
 int allow_ring_lds_all() {
     int rc;
     if ((rc = allow_ring_lds<double, 1, 4>(RingStage<double, 4>::BYTES)) ||
         (rc = allow_ring_lds<double, 5, 5>(RingStage<double, 5>::BYTES)) ||
-        (rc = allow_ring_lds<float, 5, 5>(RingStage<float, 5>::BYTES)))
+        (rc = allow_ring_lds<float, 5, 5>(RingStage<float, 5>::BYTES)) || (rc = allow_wave_lds<1, LV_GAS>()))
         return rc;
     return CCD_OK;
 }
@@ -449,11 +457,32 @@ template <typename CT, int NT, int SRC>
 void launch_ladder_v_nt(ccd_ctx *c, const CT *vp, hipStream_t st) {
     const int grid = c->lb_grid[NT];
     if (grid <= 0) return;
-    const size_t lds = SRC == LV_GAS ? sizeof(double) * (size_t)c->lgen.ntab : 0;
+    const size_t lds = SRC == LV_GAS ? sizeof(CT) * (size_t)c->lgen.ntab : 0;   // the table in CT
     hipLaunchKernelGGL((k_ladder_v<CT, NT, SRC>), dim3((unsigned)grid), dim3(kThreads), lds, st,
                        c->d_ltiles + c->lt_off[NT - 1], c->d_lbptr + c->lb_base[NT], c->d_chans, vp, c->lgen,
                        reinterpret_cast<const CT *>(c->d_tas), c->d_zero, reinterpret_cast<CT *>(c->d_dp),
                        c->d_ctl);
+}
+
+// FP32 classes up to kWaveClassMax with one packed row per lane: the same
+// tiles and block runs as k_ladder_v.  Measured on the N=54 electron gas
+// (matrix-free, FP32): class 1 37.3 us against 42.3 us with k_ladder_v;
+// class 2 25.4 us against 19.6 us, so class 2 stays on k_ladder_v.
+constexpr int kWaveClassMax = 1;
+template <int NT, int SRC>
+void launch_ladder_w(ccd_ctx *c, const float *vp, hipStream_t st) {
+    const int grid = c->lb_grid[NT];
+    if (grid <= 0) return;
+    const size_t lds = WaveLadder<NT>::lds_bytes(SRC == LV_GAS ? c->lgen.ntab : 0);
+    hipLaunchKernelGGL((k_ladder_w<NT, SRC>), dim3((unsigned)grid), dim3(kThreads), lds, st,
+                       c->d_ltiles + c->lt_off[NT - 1], c->d_lbptr + c->lb_base[NT], c->d_chans, vp, c->lgen,
+                       reinterpret_cast<const float *>(c->d_tas), reinterpret_cast<float *>(c->d_dp), c->d_ctl);
+}
+
+template <int NT, int SRC, typename CT>
+void launch_ladder_class(ccd_ctx *c, const CT *vp, hipStream_t st) {
+    if constexpr (std::is_same<CT, float>::value && NT <= kWaveClassMax) launch_ladder_w<NT, SRC>(c, vp, st);
+    else launch_ladder_v_nt<CT, NT, SRC>(c, vp, st);
 }
 
 // Classes NT1 and NT2 in one launch (either may be empty)
@@ -461,7 +490,7 @@ template <typename CT, int NT1, int NT2, int SRC>
 void launch_ladder_v_pair(ccd_ctx *c, const CT *vp, hipStream_t st) {
     const int g1 = c->lb_grid[NT1], g2 = c->lb_grid[NT2];
     if (g1 + g2 <= 0) return;
-    const size_t lds = SRC == LV_GAS ? sizeof(double) * (size_t)c->lgen.ntab : 0;
+    const size_t lds = SRC == LV_GAS ? sizeof(CT) * (size_t)c->lgen.ntab : 0;
     hipLaunchKernelGGL((k_ladder_v2<CT, NT1, NT2, SRC>), dim3((unsigned)(g1 + g2)), dim3(kThreads), lds, st,
                        c->d_ltiles + c->lt_off[NT1 - 1], c->d_lbptr + c->lb_base[NT1], g1,
                        c->d_ltiles + c->lt_off[NT2 - 1], c->d_lbptr + c->lb_base[NT2], c->d_chans, vp, c->lgen,
@@ -473,8 +502,8 @@ void launch_ladder_v_pair(ccd_ctx *c, const CT *vp, hipStream_t st) {
 // launched on their own, the wider classes (few long blocks) in pairs
 template <typename CT, int SRC>
 void launch_ladder_v_src(ccd_ctx *c, const CT *vp, int lo, int hi, hipStream_t st) {
-    if (lo <= 1 && 1 <= hi) launch_ladder_v_nt<CT, 1, SRC>(c, vp, st);
-    if (lo <= 2 && 2 <= hi) launch_ladder_v_nt<CT, 2, SRC>(c, vp, st);
+    if (lo <= 1 && 1 <= hi) launch_ladder_class<1, SRC>(c, vp, st);
+    if (lo <= 2 && 2 <= hi) launch_ladder_class<2, SRC>(c, vp, st);
     static_assert(NTMAX == 8, "pairs (3,4), (5,6), (7,8)");
     if (lo <= 3 && 4 <= hi) launch_ladder_v_pair<CT, 3, 4, SRC>(c, vp, st);
     else if (lo <= 3 && 3 <= hi) launch_ladder_v_nt<CT, 3, SRC>(c, vp, st);
@@ -904,8 +933,11 @@ int merge_problems(const ccd_problem *const *probs, int n, Merged &m) {
 // otherwise the ladder calls the generic generator per element.
 int build_generated_ladder(ccd_ctx *c, const ccd_problem *p, const std::vector<int2> &ppl,
                            const std::vector<int32_t> &prow) {
-    const size_t nrec = std::max<size_t>(prow.size(), 1);
-    std::vector<int2> rec1(nrec, int2{0, 0}), rec2(nrec, int2{0, 0});
+    // kRecPad zero entries past the last pair: the wave-tile ladder's last
+    // chunk of a channel reads records beyond it (their amplitude rows are 0)
+    const size_t nrec = prow.size() + kRecPad;
+    std::vector<int2> rec1(nrec, int2{0, 0}), rec2(nrec, int2{0, 0}), rec1f(nrec, int2{0, 0}),
+        rec2f(nrec, int2{0, 0}), rec1w(nrec, int2{0, 0});
     bool gas = p->gen_kind == CCD_GEN_ELECTRON_GAS;
     const int m = p->gen_nmax, D = 2 * m + 1, w = 4 * m + 1;
     if (gas && (int64_t)D * D * D + 1 > kGasTableMax) gas = false;
@@ -948,29 +980,30 @@ int build_generated_ladder(ccd_ctx *c, const ccd_problem *p, const std::vector<i
                     tab[((size_t)x * D + y) * D + z] = v;
                 }
     }
-    // 16-bit fields: 8 D^2 (2m) + 8 D (2m) + 8 (2m) < zoff = 8 D^3 <= 32768
-    const uint32_t zoff = (uint32_t)(8 * D * D * D);
-    auto scaled = [&](int s) {
-        const int32_t *a = qn + 5 * (size_t)s;
-        const uint32_t x = 8u * D * D * (a[0] + m), y = 8u * D * (a[1] + m), z = 8u * (a[2] + m);
-        const uint32_t sp = a[3] == s_hi && s_hi != s_lo ? zoff : 0u;
-        return int2{(int)(x | y << 16), (int)(z | sp << 16)};
-    };
+    // Records: the momentum of a state pre-scaled into 16-bit fields,
+    // esize D^2 (n_x + m) | esize D (n_y + m) << 16 and esize (n_z + m), so that
+    // two v_sad_u16 of two records give the byte offset of T(|dn|) in a table
+    // of esize-byte entries (at most 8 D^3 <= 32768 bytes).
     // Channels whose pairs all have one particle of each spin (total spin 0):
     // there <ab||cd> = s(ab) s(cd) T(k_up(ab) - k_up(cd)), with k_up the
     // momentum of the pair's up-spin particle and s = +1 when the first
     // particle is the up one, -1 otherwise (exactly one of the two terms
-    // survives, and T is even).  Their records: rec1 = k_up, scaled as above
-    // without a spin field; rec2 = (sign bit of s as a float/double sign
-    // mask, 0).
-    auto scaled_up = [&](int s) {
+    // survives, and T is even).  Their records: rec1 = k_up (rec1f for the
+    // FP32 table); rec2 = (sign bit of s as a float/double sign mask, 0);
+    // rec1w = rec1f with s in the high half of its second word, 0 for +1 and
+    // the byte size of the FP32 table for -1 (the table [T, -T] of k_ladder_w:
+    // the two fields' distance selects the copy of sign s(ab) s(cd)).
+    static_assert(4 * kGasTableMax < 65536, "the sign field holds the FP32 table's byte size");
+    const uint32_t sbytes = 4u * (uint32_t)(D * D * D + 1);   // bytes of the FP32 table
+    auto scaled_up = [&](int s, uint32_t esize = 8u) {   // esize: bytes per table entry
         const int32_t *a = qn + 5 * (size_t)s;
-        const uint32_t x = 8u * D * D * (a[0] + m), y = 8u * D * (a[1] + m), z = 8u * (a[2] + m);
+        const uint32_t x = esize * D * D * (a[0] + m), y = esize * D * (a[1] + m), z = esize * (a[2] + m);
         return int2{(int)(x | y << 16), (int)z};
     };
     // Channels whose pairs have two particles of the same spin (or a basis
-    // of one spin): both terms survive for every element, no spin mask, so
-    // the records carry no spin field either (spin0 = 2).
+    // of one spin): both terms survive for every element (spin0 = 2); rec1,
+    // rec2 = the momenta of the two states (rec1f, rec2f for the FP32 table).
+    // A gas channel with pairs is one of the two kinds.
     for (DChan &ch : c->chans) {
         ch.spin0 = 0;
         if (gas && ch.npk + ch.ndiag > 0) {
@@ -982,13 +1015,15 @@ int build_generated_ladder(ccd_ctx *c, const ccd_problem *p, const std::vector<i
             if (ch.spin0 == 1) {
                 const bool first_up = qn[5 * ab.x + 3] == s_hi;
                 rec1[ch.pr_off + u] = scaled_up(first_up ? ab.x : ab.y);
+                rec1f[ch.pr_off + u] = scaled_up(first_up ? ab.x : ab.y, 4u);   // the FP32 table
+                rec1w[ch.pr_off + u] = rec1f[ch.pr_off + u];
+                rec1w[ch.pr_off + u].y |= (int)((first_up ? 0u : sbytes) << 16);
                 rec2[ch.pr_off + u] = int2{first_up ? 0 : (int)0x80000000u, 0};
             } else if (ch.spin0 == 2) {
                 rec1[ch.pr_off + u] = scaled_up(ab.x);
                 rec2[ch.pr_off + u] = scaled_up(ab.y);
-            } else if (gas) {
-                rec1[ch.pr_off + u] = scaled(ab.x);
-                rec2[ch.pr_off + u] = scaled(ab.y);
+                rec1f[ch.pr_off + u] = scaled_up(ab.x, 4u);
+                rec2f[ch.pr_off + u] = scaled_up(ab.y, 4u);
             } else {
                 rec1[ch.pr_off + u] = int2{ab.x, ab.y};
             }
@@ -997,14 +1032,19 @@ int build_generated_ladder(ccd_ctx *c, const ccd_problem *p, const std::vector<i
     int rc;
     if (!c->chans.empty())   // the spin0 flags
         HIP_TRY(hipMemcpy(c->d_chans, c->chans.data(), sizeof(DChan) * c->chans.size(), hipMemcpyHostToDevice));
-    if ((rc = upload(c, &c->d_rec1, rec1)) || (rc = upload(c, &c->d_rec2, rec2))) return rc;
+    if ((rc = upload(c, &c->d_rec1, rec1)) || (rc = upload(c, &c->d_rec2, rec2)) ||
+        (rc = upload(c, &c->d_rec1f, rec1f)) || (rc = upload(c, &c->d_rec2f, rec2f)) ||
+        (rc = upload(c, &c->d_rec1w, rec1w)))
+        return rc;
     c->lgen.rec1 = c->d_rec1;
     c->lgen.rec2 = c->d_rec2;
+    c->lgen.rec1f = c->d_rec1f;
+    c->lgen.rec2f = c->d_rec2f;
+    c->lgen.rec1w = c->d_rec1w;
     if (gas) {
         if ((rc = upload(c, &c->d_gtab, tab))) return rc;
         c->lgen.tab = c->d_gtab;
         c->lgen.ntab = (int)tab.size();
-        c->lgen.zoff = zoff;
         c->lsrc = LV_GAS;
     } else {
         c->lsrc = LV_GENERIC;
@@ -1339,8 +1379,38 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
         int ncu = 0;
         HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
         ncu = std::max(1, ncu);
-        c->gen_blocks = ncu * 4;
-        std::vector<DTile> cls;
+        c->gen_blocks = ncu * 3;   // (3 per CU: class 1 37.3 us at N=54 against 43.6 us with 4)
+        // tiles [lo, lo + n) of a class over G blocks (block b on CU b mod ncu):
+        // each block's run of tiles written back to `out`, the run ends to `ptr`
+        auto assign = [&](int lo, int n, int G, const std::vector<int64_t> &cost, int per_cu,
+                          std::vector<DTile> &out, std::vector<int32_t> &ptr) {
+            auto cu_max = [&](const std::vector<std::vector<int>> &own) {
+                std::vector<int64_t> cu(ncu, 0);
+                for (int b = 0; b < G; b++)
+                    for (int i : own[b]) cu[(b / per_cu) % ncu] += cost[i];
+                return *std::max_element(cu.begin(), cu.end());
+            };
+            std::vector<std::vector<int>> rr(G), lpt(G);
+            for (int i = 0; i < n; i++) rr[i % G].push_back(i);
+            std::vector<int> idx(n);
+            for (int i = 0; i < n; i++) idx[i] = i;
+            std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+            std::vector<int64_t> load(G, 0);
+            for (int i : idx) {   // least loaded slot, lowest index on ties
+                int b = 0;
+                for (int q = 1; q < G; q++)
+                    if (load[q] < load[b]) b = q;
+                load[b] += cost[i];
+                lpt[b].push_back(i);
+            }
+            const auto &own = 10 * cu_max(lpt) <= 9 * cu_max(rr) ? lpt : rr;
+            std::vector<DTile> cls;
+            for (int b = 0; b < G; b++) {
+                for (int i : own[b]) cls.push_back(c->ltiles[lo + i]);
+                ptr.push_back((int32_t)cls.size());
+            }
+            std::copy(cls.begin(), cls.end(), out.begin() + lo);
+        };
         for (int nt = 1; nt <= NTMAX; nt++) {
             const int lo = c->lt_off[nt - 1], n = c->lt_off[nt] - lo;
             const int G = c->matrix_free ? std::min(n, c->gen_blocks) : n;
@@ -1356,32 +1426,7 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
                 const int klen = std::min(c->chans[t.chan].npk, (t.aux + 1) * kseg) - t.aux * kseg;
                 cost[i] = (klen + kch - 1) / kch + 1;
             }
-            auto cu_max = [&](const std::vector<std::vector<int>> &own) {
-                std::vector<int64_t> cu(ncu, 0);
-                for (int b = 0; b < G; b++)
-                    for (int i : own[b]) cu[b % ncu] += cost[i];
-                return *std::max_element(cu.begin(), cu.end());
-            };
-            std::vector<std::vector<int>> rr(G), lpt(G);
-            for (int i = 0; i < n; i++) rr[i % G].push_back(i);
-            std::vector<int> idx(n);
-            for (int i = 0; i < n; i++) idx[i] = i;
-            std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-            std::vector<int64_t> load(G, 0);
-            for (int i : idx) {   // least loaded block, lowest index on ties
-                int b = 0;
-                for (int q = 1; q < G; q++)
-                    if (load[q] < load[b]) b = q;
-                load[b] += cost[i];
-                lpt[b].push_back(i);
-            }
-            const auto &own = 10 * cu_max(lpt) <= 9 * cu_max(rr) ? lpt : rr;
-            cls.clear();
-            for (int b = 0; b < G; b++) {
-                for (int i : own[b]) cls.push_back(c->ltiles[lo + i]);
-                lbptr.push_back((int32_t)cls.size());
-            }
-            std::copy(cls.begin(), cls.end(), c->ltiles.begin() + lo);
+            assign(lo, n, G, cost, 1, c->ltiles, lbptr);
         }
     }
     for (int wide = 0; wide < 2; wide++) {   // channels with hh <= 80 first
@@ -1505,6 +1550,11 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
         } else if (!c->rtiles.empty()) {
             hipLaunchKernelGGL(k_gen_vp, dim3((unsigned)c->rtiles.size()), dim3(kThreads), 0, c->stream,
                                c->d_rtiles, c->d_chans, c->d_ppl, c->d_prow, G, c->d_vp);
+            if (c->precision == CCD_PREC_FP32) {   // the element the generated FP32 ladders build
+                if ((rc = dalloc(c, &c->d_vp32, (size_t)std::max<int64_t>(c->n_vp, 1)))) return rc;
+                hipLaunchKernelGGL(k_gen_vp32, dim3((unsigned)c->rtiles.size()), dim3(kThreads), 0, c->stream,
+                                   c->d_rtiles, c->d_chans, c->d_ppl, c->d_prow, G, c->d_vp32);
+            }
         }
         if (!c->chans.empty())
             hipLaunchKernelGGL(k_gen_direct_small, dim3((unsigned)c->chans.size()), dim3(kThreads), 0,
@@ -1575,8 +1625,9 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
             hipLaunchKernelGGL(k_to_f32, dim3(g), dim3(kThreads), 0, c->stream, src, *dst, n);
             return check_launch("k_to_f32");
         };
+        // V^pppp: generated contexts built theirs above (or generate it in the ladder)
         if ((rc = to_f32(c->d_va, &c->d_va32, c->nt2)) || (rc = to_f32(c->d_vb, &c->d_vb32, c->n_b)) ||
-            (rc = to_f32(c->d_vp, &c->d_vp32, c->matrix_free ? 0 : c->n_vp)))
+            (!c->d_vp32 && (rc = to_f32(c->d_vp, &c->d_vp32, c->matrix_free ? 0 : c->n_vp))))
             return rc;
     }
     HIP_TRY(hipStreamSynchronize(c->stream));

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "ccd_amd.h"
 
@@ -167,7 +168,13 @@ __device__ __attribute__((noinline)) double minnesota_direct(const int32_t *qi, 
 // the host model for the pairing model (src/PairingSPBasis.cpp:127-141) and
 // the electron gas (src/ElectronGasSPBasis.cpp:419-471 with the direct term
 // tabulated); infinite matter up to the device exp (src/InfMatterSPBasis.cpp:355-359).
-__device__ inline double gen_tbme(const Gen &G, int p, int q, int r, int s) {
+// F32 = true: the element of the FP32 iterations -- for the electron gas its
+// two direct terms rounded to FP32 and then subtracted, (float) d1 - (float) d2
+// (one of them is 0 unless both pairs have one spin, so a total-spin-0 element
+// is the FP64 one rounded); for the other generators the FP64 element rounded.
+template <bool F32 = false>
+__device__ inline auto gen_tbme_t(const Gen &G, int p, int q, int r, int s) {
+    typedef typename std::conditional<F32, float, double>::type R;
     const int32_t *a = G.qn + 5 * p, *b = G.qn + 5 * q, *c = G.qn + 5 * r, *d = G.qn + 5 * s;
     int kind = G.kind, nmax = G.nmax;
     double g = G.g;
@@ -176,25 +183,28 @@ __device__ inline double gen_tbme(const Gen &G, int p, int q, int r, int s) {
         const GenPart P = G.part[a[4]];
         kind = P.kind; nmax = P.nmax; g = P.g; table = G.table + P.tab_off;
     }
-    if (kind == CCD_GEN_INF_MATTER) return minnesota_direct(a, b, c, d, g, table) - minnesota_direct(a, b, d, c, g, table);
+    if (kind == CCD_GEN_INF_MATTER)
+        return (R)(minnesota_direct(a, b, c, d, g, table) - minnesota_direct(a, b, d, c, g, table));
     if (kind == CCD_GEN_PAIRING) {
         double v = 0.0;
         if (a[0] == b[0] && c[0] == d[0] && a[1] == -b[1] && -c[1] == d[1]) {
             v = -0.5 * g;
             if (a[1] == -c[1]) v = -v;
         }
-        return v;
+        return (R)v;
     }
     const bool cons = a[0] + b[0] == c[0] + d[0] && a[1] + b[1] == c[1] + d[1] && a[2] + b[2] == c[2] + d[2];
-    if (!cons) return 0.0;
+    if (!cons) return (R)0;
     const int m2 = 2 * nmax, w = 4 * nmax + 1;
     double d1 = 0.0, d2 = 0.0;
     if (a[3] == c[3] && b[3] == d[3])
         d1 = table[((a[0] - c[0] + m2) * w + (a[1] - c[1] + m2)) * w + (a[2] - c[2] + m2)];
     if (a[3] == d[3] && b[3] == c[3])
         d2 = table[((a[0] - d[0] + m2) * w + (a[1] - d[1] + m2)) * w + (a[2] - d[2] + m2)];
-    return d1 - d2;
+    return (R)d1 - (R)d2;
 }
+__device__ inline double gen_tbme(const Gen &G, int p, int q, int r, int s) { return gen_tbme_t<false>(G, p, q, r, s); }
+__device__ inline float gen_tbme_f32(const Gen &G, int p, int q, int r, int s) { return gen_tbme_t<true>(G, p, q, r, s); }
 
 // v_mfma_f64_16x16x4f64: A is 16x4 (lane l: row l%16, k-slot l/16), B is 4x16
 // (lane l: k-slot l/16, column l%16), C is 16x16 with register q of lane l

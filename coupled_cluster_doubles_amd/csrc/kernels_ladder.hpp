@@ -83,30 +83,28 @@ enum LadderSource {
 // Electron gas: inside a channel ka+kb = kc+kd and sa+sb = sc+sd, so
 //   <ab||cd> = [sa==sc] T(ka-kc) - [sb==sc] T(kb-kc),
 // with T the direct term, even in each component of the momentum transfer.
-// rec1 holds for the first state of a pair, rec2 for the second, the
-// momentum components and the spin pre-scaled into four 16-bit fields,
-// (x | y << 16, z | s << 16), so that two v_sad_u16 give
-//   sum of |rec(a) - rec(c)| over the fields = byte offset of T(|ka-kc|),
-// and a spin mismatch adds at least the offset of the table's final zero.
+// A channel with pairs has either one particle of each spin in every pair
+// (total spin 0: one term survives, <ab||cd> = s(ab) s(cd) T(k_up(ab) -
+// k_up(cd)), s = +-1 by which particle is the up one) or two of one spin
+// (both terms, no spin test).  The records hold momenta pre-scaled into
+// 16-bit fields, (x | y << 16, z), so that two v_sad_u16 give
+//   sum of |rec(a) - rec(c)| over the fields = byte offset of T(|ka-kc|)
+// in the table (8-byte entries: rec1, rec2; 4-byte: rec1f, rec2f).
+// Total spin 0: rec1 = k_up, rec2 = (sign mask of s, 0).  Same spin: rec1 and
+// rec2 = the momenta of the first and the second state.
 // Generic: rec1 = (first state, second state).
 struct LadderGen {
     const int2 *rec1;
     const int2 *rec2;
+    const int2 *rec1f;   // gas: rec1 scaled for the FP32 table (4-byte entries)
+    const int2 *rec2f;   // gas, same-spin channels: rec2 scaled for the FP32 table
+    const int2 *rec1w;   // gas, total-spin-0 channels: rec1f with the sign field of the table [T, -T]
     const double *tab;   // gas: T over (|dx|,|dy|,|dz|), row-major, then one 0
-    uint32_t zoff;       // gas: byte offset of the final 0
     int ntab;            // gas: entries including the final 0
     Gen G;               // generic
 };
 
 typedef __attribute__((address_space(3))) const double lds_double;
-
-// LDS address of T(|ka-kc|) (or of the final 0): base + the field distances,
-// clamped to the final 0.
-__device__ __forceinline__ uint32_t gas_addr(int2 r, int2 c, uint32_t base, uint32_t zaddr) {
-    return min(__builtin_amdgcn_sad_u16((uint32_t)r.x, (uint32_t)c.x,
-                                        __builtin_amdgcn_sad_u16((uint32_t)r.y, (uint32_t)c.y, base)),
-               zaddr);
-}
 
 // LDS address of T(|k_up(ab) - k_up(cd)|) for the records of a total-spin-0
 // channel (no spin field: the address stays inside the table).
@@ -123,6 +121,11 @@ __device__ __forceinline__ double flip_sign(double v, uint32_t m) {
 
 __device__ __forceinline__ double lds_load(uint32_t addr) {
     return *reinterpret_cast<lds_double *>(static_cast<uintptr_t>(addr));
+}
+
+typedef __attribute__((address_space(3))) const float lds_float;
+__device__ __forceinline__ float lds_loadf(uint32_t addr) {
+    return *reinterpret_cast<lds_float *>(static_cast<uintptr_t>(addr));
 }
 
 // Value of lane S of this lane's 16-lane row (DPP row_newbcast).
@@ -207,7 +210,9 @@ __device__ __forceinline__ void ladder_run(const DTile *__restrict__ tiles, int 
     const int tid = threadIdx.x & (kThreads - 1);
     const int wave = tid >> 6, lane = tid & 63, m = lane & 15, kq = lane >> 4;
     const uint32_t tbase = (uint32_t)(uintptr_t)(lds_double *)gtab;  // LDS byte address
-    const uint32_t zaddr = tbase + lg.zoff;
+    // FP32 iterations read the FP32 table through the records scaled for it
+    constexpr bool F32 = sizeof(CT) == 4;
+    const int2 *grec1 = F32 ? lg.rec1f : lg.rec1, *grec2 = F32 ? lg.rec2f : lg.rec2;
 
     for (int ti = b0; ti < b1; ti++) {
         const DTile tl = tiles[ti];
@@ -230,8 +235,8 @@ __device__ __forceinline__ void ladder_run(const DTile *__restrict__ tiles, int 
             int2 ra = {0, 0}, rb = {0, 0};
             if (GEN) {
                 const int2 *z2 = reinterpret_cast<const int2 *>(zero);
-                ra = *(rok ? lg.rec1 + ch.pr_off + u : z2);
-                if (SRC == LV_GAS) rb = *(rok ? lg.rec2 + ch.pr_off + u : z2);
+                ra = *(rok ? (SRC == LV_GAS ? grec1 : lg.rec1) + ch.pr_off + u : z2);
+                if (SRC == LV_GAS && SAME) rb = *(rok ? grec2 + ch.pr_off + u : z2);
             }
 
             // Every load is unconditional: an element outside the block reads the
@@ -254,7 +259,8 @@ __device__ __forceinline__ void ladder_run(const DTile *__restrict__ tiles, int 
             };
             // Buffer loads of the records (generated V) and of the B tiles: rows
             // at or past kend lie outside their buffers and read 0.
-            const BufRsrc rsrc = GEN ? make_rsrc(lg.rec1 + ch.pr_off, (uint32_t)kend * 8u) : make_rsrc(zero, 0);
+            const BufRsrc rsrc =
+                GEN ? make_rsrc((SRC == LV_GAS ? grec1 : lg.rec1) + ch.pr_off, (uint32_t)kend * 8u) : make_rsrc(zero, 0);
             const uint32_t roff = m < G ? (uint32_t)(kbeg + kslot<CT>(kq, m)) * 8u : kOOB;
             auto load_r = [&](int v, int2 &rr) {  // generated V: record of k-slot (kq, m) of V chunk v
                 rr = buf_i2(rsrc, roff + (uint32_t)(v * KCH) * 8u);
@@ -263,12 +269,16 @@ __device__ __forceinline__ void ladder_run(const DTile *__restrict__ tiles, int 
                 static_for<0, G>([&](auto S) {
                     constexpr int s = decltype(S)::value;
                     const int2 r = {row_bcast<s>(rr.x), row_bcast<s>(rr.y)};
-                    if (SRC == LV_GAS && ONE) {
+                    if (F32 && SRC == LV_GAS && ONE) {
+                        a[s] = lds_loadf(gas_addr1(ra, r, tbase));
+                    } else if (F32 && SRC == LV_GAS) {
+                        a[s] = lds_loadf(gas_addr1(ra, r, tbase)) - lds_loadf(gas_addr1(rb, r, tbase));
+                    } else if (SRC == LV_GAS && ONE) {
                         a[s] = (CT)lds_load(gas_addr1(ra, r, tbase));
-                    } else if (SRC == LV_GAS && SAME) {
-                        a[s] = (CT)(lds_load(gas_addr1(ra, r, tbase)) - lds_load(gas_addr1(rb, r, tbase)));
                     } else if (SRC == LV_GAS) {
-                        a[s] = (CT)(lds_load(gas_addr(ra, r, tbase, zaddr)) - lds_load(gas_addr(rb, r, tbase, zaddr)));
+                        a[s] = (CT)(lds_load(gas_addr1(ra, r, tbase)) - lds_load(gas_addr1(rb, r, tbase)));
+                    } else if (F32) {
+                        a[s] = gen_tbme_f32(lg.G, ra.x, ra.y, r.x, r.y);
                     } else {
                         a[s] = (CT)gen_tbme(lg.G, ra.x, ra.y, r.x, r.y);
                     }
@@ -371,18 +381,21 @@ __device__ __forceinline__ void ladder_run(const DTile *__restrict__ tiles, int 
                 }
             }
         };
+        // a gas channel with pairs is total-spin-0 or same-spin (the host sets spin0 so)
         if (SRC == LV_GAS && ch.spin0 == 1) tile(std::integral_constant<int, 1>{});
-        else if (SRC == LV_GAS && ch.spin0 == 2) tile(std::integral_constant<int, 2>{});
+        else if (SRC == LV_GAS) tile(std::integral_constant<int, 2>{});
         else tile(std::integral_constant<int, 0>{});
     }
 }
 
-// The gas table into LDS, once per block
-template <int SRC>
+// The gas table into LDS, once per block, in the arithmetic type (FP32: every
+// entry rounded, the values of the FP32 element)
+template <typename CT, int SRC>
 __device__ __forceinline__ void stage_gas_table(const LadderGen &lg) {
     extern __shared__ double gtab[];
     if (SRC == LV_GAS) {
-        for (int i = threadIdx.x & (kThreads - 1); i < lg.ntab; i += kThreads) gtab[i] = lg.tab[i];
+        CT *t = reinterpret_cast<CT *>(gtab);
+        for (int i = threadIdx.x & (kThreads - 1); i < lg.ntab; i += kThreads) t[i] = (CT)lg.tab[i];
         __syncthreads();
     }
 }
@@ -398,7 +411,7 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__
                                                        CT *__restrict__ dp,
                                                        const LoopCtl *__restrict__ ctl) {
     if (ctl->done) return;
-    stage_gas_table<SRC>(lg);
+    stage_gas_table<CT, SRC>(lg);
     ladder_run<CT, NT, SRC>(tiles, bptr[blockIdx.x], bptr[blockIdx.x + 1], chans, vp, lg, tas, zero, dp);
 }
 
@@ -416,12 +429,284 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v2(const DTile *__restrict_
                                                         CT *__restrict__ dp,
                                                         const LoopCtl *__restrict__ ctl) {
     if (ctl->done) return;
-    stage_gas_table<SRC>(lg);
+    stage_gas_table<CT, SRC>(lg);
     const int b = blockIdx.x;
     if (b < g1)
         ladder_run<CT, NT1, SRC>(tiles1, bptr1[b], bptr1[b + 1], chans, vp, lg, tas, zero, dp);
     else
         ladder_run<CT, NT2, SRC>(tiles2, bptr2[b - g1], bptr2[b - g1 + 1], chans, vp, lg, tas, zero, dp);
+}
+
+// ---- FP32 pppp ladder, channels with hh <= 16 NTW: one packed row per lane
+//
+// v_mfma_f32_4x4x1f32 runs 16 blocks of 4x4 outer products with k = 1: lane
+// l, register r of the result holds A[lane 4(l/4) + r] * B[lane l] (+ C).
+// With A_l = V[u0 + l][k] (one packed row per lane) and
+// B_l = tas[k][4g + l%4] every block sees the same k, so a wave's 64 rows
+// step through k together:
+//   D[u0 + 4(l/4) + r][4g + l%4] += V[u0 + 4(l/4) + r][k] tas[k][4g + l%4],
+// one MFMA per k and column group g of 4 columns, ceil(hh/4) groups (the
+// channel's hh sets the MFMA work).  The k-slot records of a generated V are
+// the same for the whole wave: scalar loads bring a chunk's records into
+// SGPRs a chunk ahead, and a lane builds the element of its own row from
+// them: two v_sad_u16 and one FP32 table read for a total-spin-0 channel,
+// four, two reads and an FP32 subtraction for a same-spin one (the element of
+// gen_tbme_f32), gen_tbme_f32 for other generators, a load for a stored V.
+// Total-spin-0 channels read a table [T, -T]: the sign s(ab) s(cd) of the
+// element is the distance of the records' sign fields (0 or the byte size
+// of T), so the element comes signed from one read.
+//
+// A block runs the same tiles as k_ladder_v (64 rows x one k segment); its
+// four waves take the tile's k chunks in turn (wave w: chunks w, w + 4, ...).
+// Each wave stages its chunks' B rows (transposed, column-major: a lane reads
+// 4 consecutive k of its column in one 16-byte read) in a double-buffered
+// region of its own, loaded two chunks ahead; the V elements of its next
+// chunk are built while the MFMAs of this one run.  Within a wave LDS
+// accesses complete in order, so no barrier is needed inside the k loop.  At
+// the end of a tile the four partial tiles are added in wave order through
+// LDS.  The stored and generated modes give the same sums bit for bit (a
+// sign flip is exact and commutes with every rounding).
+constexpr int WKC = 16;        // k per chunk
+constexpr int kRecPad = WKC;   // zero record entries past the last pair
+
+template <int NTW>
+struct WaveLadder {
+    static constexpr int W = 16 * NTW;         // padded columns of the B tile
+    static constexpr int NG = 4 * NTW;         // column groups of 4
+    static constexpr int BSTR = WKC + 4;       // floats per transposed column (bank spread of the 16-byte reads)
+    static constexpr int BUF = W * BSTR;       // floats per buffer
+    static constexpr int PER = WKC * W / 64;   // B elements a lane stages per chunk
+    // accumulator sets (k parity): two when few column groups would chain
+    // dependent MFMAs back to back
+    static constexpr int NACC = NTW == 1 ? 2 : 1;
+    static constexpr int WREG = 2 * BUF;                // floats of a wave's region
+    static constexpr int RED = 3 * NG * 4 * 64;         // floats of the partial tiles of waves 1-3
+    static constexpr int REGION = 4 * WREG > RED ? 4 * WREG : RED;
+    // bytes of dynamic LDS: the FP32 gas table [T, -T], then the wave
+    // regions, which the partial tiles reuse at the end of a tile
+    static __host__ __device__ constexpr size_t tab_bytes(int ntab) { return ((size_t)8 * ntab + 15) / 16 * 16; }
+    static __host__ __device__ constexpr size_t lds_bytes(int ntab) {
+        return tab_bytes(ntab) + (size_t)REGION * sizeof(float);
+    }
+};
+
+typedef __attribute__((address_space(4))) const int const_int;
+
+template <int NTW, int SRC, int MODE>
+__device__ __forceinline__ void wave_tile(const DTile &tl, const DChan &ch, const float *__restrict__ vp,
+                                          const LadderGen &lg, const float *__restrict__ tas,
+                                          float *__restrict__ dp, float *__restrict__ region, uint32_t tb32,
+                                          int wave, int lane) {
+    typedef WaveLadder<NTW> S;
+    constexpr bool ONE = MODE == 1, SAME = MODE == 2, GEN = SRC != LV_STORED;
+    const int H = ch.hh, npk = ch.npk, seg = tl.aux;
+    const int kbeg = seg * LadderShape<NTW>::KSEG, kend = min(npk, kbeg + LadderShape<NTW>::KSEG);
+    const int ng = (H + 3) >> 2;
+    const int nch = (kend - kbeg + WKC - 1) / WKC;
+    float *bw = region + wave * S::WREG;
+    // rows past the tile repeat its first row: finite values, never stored
+    const int64_t u = tl.row0 + (lane < tl.nrows ? lane : 0);
+    // k-slot records (uniform: scalar loads) and this lane's row records
+    const int2 *rk = SRC == LV_GAS ? (ONE ? lg.rec1w : lg.rec1f) : lg.rec1;
+    int2 ra = {0, 0}, rb = {0, 0};
+    if (GEN) ra = rk[ch.pr_off + u];
+    if (SRC == LV_GAS && SAME) rb = lg.rec2f[ch.pr_off + u];
+    // the channel's records as constant memory (scalar loads for uniform addresses)
+    const_int *er = (const_int *)(uintptr_t)(rk + ch.pr_off);
+    const float *av = vp + ch.off_vp + u * ch.ldv;   // stored V: this lane's row (pad entries are 0)
+
+    // staging of chunk c: B element idx = lane + 64 q is (k row idx / W,
+    // column idx % W), rows at or past kend and columns past hh read 0
+    const BufRsrc bsrc = make_rsrc(tas + ch.off_tp, (uint32_t)kend * (uint32_t)H * 4u);
+    // (W is a power of two: kk = idx / W and col = idx % W are shifts and masks)
+    static_assert((S::W & (S::W - 1)) == 0, "W a power of two");
+    const int kk0 = lane / S::W, col = lane & (S::W - 1);   // element q: row kk0 + q 64 / W, column col
+    const uint32_t boff0 = col < H ? (uint32_t)(kk0 * H + col) * 4u : kOOB;
+    float breg[S::PER];
+    auto load = [&](int c) {   // chunk c (its k rows may lie past kend: 0)
+        const int k0 = kbeg + c * WKC;
+#pragma unroll
+        for (int q = 0; q < S::PER; q++)
+            breg[q] = buf_load<float>(bsrc, boff0 + (uint32_t)(k0 + q * (64 / S::W)) * (uint32_t)H * 4u);
+    };
+    auto store = [&](int buf) {
+        float *b = bw + buf * S::BUF + col * S::BSTR + kk0;
+#pragma unroll
+        for (int q = 0; q < S::PER; q++) b[q * (64 / S::W)] = breg[q];
+    };
+    // the WKC records of chunk c; past the last chunk those of row kend - 1
+    // on (the kRecPad zero entries keep them inside the list), never used
+    auto recs = [&](int c, int2 (&e)[WKC]) {
+        if (GEN) {
+            const_int *p = er + 2 * min(kbeg + c * WKC, kend - 1);
+#pragma unroll
+            for (int j = 0; j < WKC; j++) e[j] = int2{p[2 * j], p[2 * j + 1]};
+        }
+    };
+    flt4 acc[S::NACC][S::NG];
+#pragma unroll
+    for (int g = 0; g < S::NG; g++)
+#pragma unroll
+        for (int h = 0; h < S::NACC; h++) acc[h][g] = flt4{0.f, 0.f, 0.f, 0.f};
+
+    // V element of a row from a k-slot record e; same-spin channels return
+    // the two terms (x1 - x2 is the element)
+    auto term = [&](int2 e, float &a, float &x1, float &x2) {
+        if (SRC == LV_GAS && ONE) a = lds_loadf(gas_addr1(ra, e, tb32));
+        else if (SRC == LV_GAS) {
+            x1 = lds_loadf(gas_addr1(ra, e, tb32));
+            x2 = lds_loadf(gas_addr1(rb, e, tb32));
+        } else a = gen_tbme_f32(lg.G, ra.x, ra.y, e.x, e.y);
+    };
+    auto mfma_quad = [&](int buf, int q, const float (&a)[WKC]) {
+        const float *bb = bw + buf * S::BUF + (lane & 3) * S::BSTR + 4 * q;   // column 4 g + l % 4
+#pragma unroll
+        for (int g = 0; g < S::NG; g++) {
+            if (g < ng) {   // uniform
+                const flt4 b = *reinterpret_cast<const flt4 *>(bb + g * 4 * S::BSTR);
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    acc[j % S::NACC][g] =
+                        __builtin_amdgcn_mfma_f32_4x4x1f32(a[4 * q + j], b[j], acc[j % S::NACC][g], 0, 0, 0);
+            }
+        }
+    };
+    // the whole V chunk c (prologue)
+    auto gen = [&](int c, const int2 (&e)[WKC], float (&a)[WKC]) {
+        const int k0 = kbeg + c * WKC;
+#pragma unroll
+        for (int q = 0; q < WKC / 4; q++) {
+            float x1[4], x2[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if (SRC == LV_STORED) a[4 * q + j] = av[k0 + 4 * q + j];
+                else term(e[4 * q + j], a[4 * q + j], x1[j], x2[j]);
+            }
+            if (SRC == LV_GAS && SAME)
+#pragma unroll
+                for (int j = 0; j < 4; j++) a[4 * q + j] = x1[j] - x2[j];
+        }
+    };
+
+    if (wave < nch) {   // uniform: this wave has chunks wave, wave + 4, ...
+        float a0[WKC], a1[WKC];
+        int2 e0[WKC], e1[WKC];
+        recs(wave, e0);
+        load(wave);
+        store(0);
+        load(wave + 4);
+        recs(wave + 4, e1);
+        gen(wave, e0, a0);
+        // step: A of chunk c in acur, B of chunk c in buffer buf, the records
+        // of chunk c + 4 in enext; the next chunk (c + 4) goes to the other
+        // buffer and its A to anext while this one's MFMAs run, and the
+        // records of chunk c + 8 are loaded into eload.  The order is kept by
+        // scheduling barriers: per quarter of this chunk its MFMAs and the
+        // table reads of a quarter of the next chunk, so that the LDS latency
+        // sits behind MFMAs instead of in front of them.
+        auto step = [&](int c, int buf, float (&acur)[WKC], float (&anext)[WKC], const int2 (&enext)[WKC],
+                        int2 (&eload)[WKC]) {
+            store(buf ^ 1);
+            load(c + 8);
+            const bool more = c + 4 < nch;   // uniform
+            const int k4 = kbeg + (c + 4) * WKC;
+#pragma unroll
+            for (int q = 0; q < WKC / 4; q++) {
+                float x1[4], x2[4];
+                if (more) {
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        if (SRC == LV_STORED) anext[4 * q + j] = av[k4 + 4 * q + j];
+                        else term(enext[4 * q + j], anext[4 * q + j], x1[j], x2[j]);
+                    }
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                mfma_quad(buf, q, acur);
+                __builtin_amdgcn_sched_barrier(0);
+                if (more && SRC == LV_GAS && SAME)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) anext[4 * q + j] = x1[j] - x2[j];
+            }
+            recs(c + 8, eload);
+        };
+        for (int c = wave; c < nch; c += 8) {
+            step(c, 0, a0, a1, e1, e0);
+            if (c + 4 >= nch) break;
+            step(c + 4, 1, a1, a0, e0, e1);
+        }
+    }
+    // the four partial tiles, added in wave order: waves 1-3 park theirs
+    // (register r of group g at [w - 1][g][r][lane]), wave 0 adds and stores
+    __syncthreads();   // every wave is out of its k loop: the regions are free
+    float *red = region;
+    if (wave > 0) {
+#pragma unroll
+        for (int g = 0; g < S::NG; g++)
+            if (g < ng)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const float v = S::NACC == 2 ? acc[0][g][r] + acc[S::NACC - 1][g][r] : acc[0][g][r];
+                    red[(((wave - 1) * S::NG + g) * 4 + r) * 64 + lane] = v;
+                }
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int row = 4 * (lane >> 2) + r;
+            const bool ok = row < tl.nrows;
+            const int64_t o = ch.off_dp + (int64_t)seg * npk * H + (int64_t)(tl.row0 + (ok ? row : 0)) * H;
+#pragma unroll
+            for (int g = 0; g < S::NG; g++) {
+                const int col = 4 * g + (lane & 3);
+                if (g < ng) {
+                    float v = S::NACC == 2 ? acc[0][g][r] + acc[S::NACC - 1][g][r] : acc[0][g][r];
+#pragma unroll
+                    for (int w = 1; w < 4; w++) v += red[(((w - 1) * S::NG + g) * 4 + r) * 64 + lane];
+                    if (ok && col < H) dp[o + col] = v;
+                }
+            }
+        }
+    }
+    __syncthreads();   // the regions are reused by the next tile
+}
+
+// Block b runs the tiles bptr[b] .. bptr[b+1]-1 (the tiles and block runs of
+// k_ladder_v for the same class).
+template <int NTW, int SRC>
+__global__ __launch_bounds__(kThreads) void k_ladder_w(const DTile *__restrict__ tiles,
+                                                       const int32_t *__restrict__ bptr,
+                                                       const DChan *__restrict__ chans,
+                                                       const float *__restrict__ vp, LadderGen lg,
+                                                       const float *__restrict__ tas, float *__restrict__ dp,
+                                                       const LoopCtl *__restrict__ ctl) {
+    if (ctl->done) return;
+    typedef WaveLadder<NTW> S;
+    extern __shared__ double wl_lds[];
+    const int tid = threadIdx.x & (kThreads - 1);
+    const int ntab = SRC == LV_GAS ? lg.ntab : 0;
+    float *t32 = reinterpret_cast<float *>(wl_lds);
+    if (SRC == LV_GAS)
+        for (int i = tid; i < ntab; i += kThreads) {
+            const float v = (float)lg.tab[i];
+            t32[i] = v;
+            t32[ntab + i] = -v;
+        }
+    __syncthreads();
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    float *region = reinterpret_cast<float *>(reinterpret_cast<unsigned char *>(wl_lds) + S::tab_bytes(ntab));
+    const uint32_t tb32 = (uint32_t)(uintptr_t)(lds_float *)t32;
+    const int t1 = bptr[blockIdx.x + 1];
+    for (int ti = bptr[blockIdx.x]; ti < t1; ti++) {
+        const DTile tl = tiles[ti];
+        const DChan ch = chans[tl.chan];
+        if (SRC == LV_GAS && ch.spin0 == 1)
+            wave_tile<NTW, SRC, 1>(tl, ch, vp, lg, tas, dp, region, tb32, wave, lane);
+        else if (SRC == LV_GAS)
+            wave_tile<NTW, SRC, 2>(tl, ch, vp, lg, tas, dp, region, tb32, wave, lane);
+        else
+            wave_tile<NTW, SRC, 0>(tl, ch, vp, lg, tas, dp, region, tb32, wave, lane);
+    }
 }
 
 // 1/2 tas X, the hhhh quadratic term of D (calc.cpp:171-187 with X from

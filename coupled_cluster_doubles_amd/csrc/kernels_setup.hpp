@@ -28,6 +28,29 @@ __global__ __launch_bounds__(kThreads) void k_gen_vp(const DTile *__restrict__ t
     }
 }
 
+// The FP32 packed V^pppp of the FP32 iterations with a stored V: the element
+// the generated FP32 ladders build (gen_tbme_f32), so the two modes agree.
+__global__ __launch_bounds__(kThreads) void k_gen_vp32(const DTile *__restrict__ tiles,
+                                                       const DChan *__restrict__ chans,
+                                                       const int2 *__restrict__ ppl,
+                                                       const int32_t *__restrict__ prow, Gen G,
+                                                       float *__restrict__ vp) {
+    const DTile tl = tiles[blockIdx.x];
+    const DChan ch = chans[tl.chan];
+    const int ldv = ch.ldv, npk = ch.npk;
+    const int64_t n = (int64_t)tl.nrows * ldv;
+    for (int64_t l = threadIdx.x; l < n; l += kThreads) {
+        const int u = tl.row0 + (int)(l / ldv), w = (int)(l % ldv);
+        float v = 0.f;
+        if (w < npk) {
+            const int2 ab = ppl[ch.pp_off + prow[ch.pr_off + u]];
+            const int2 cd = ppl[ch.pp_off + prow[ch.pr_off + w]];
+            v = gen_tbme_f32(G, ab.x, ab.y, cd.x, cd.y);
+        }
+        vp[ch.off_vp + (int64_t)u * ldv + w] = v;
+    }
+}
+
 // V^hhpp transposed into the pphh layout, [(a,b),(i,j)] = <ij||ab>, and V^hhhh.
 __global__ __launch_bounds__(kThreads) void k_gen_direct_small(const DChan *__restrict__ chans,
                                                                const int2 *__restrict__ ppl,
