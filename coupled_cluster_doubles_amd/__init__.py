@@ -13,7 +13,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import CCDError, Problem, check, lib
+from ._lib import CCDError, LadderPlan, Problem, check, lib
 
 FP64 = 0
 FP32 = 1
@@ -21,11 +21,18 @@ MATRIX_FREE = 0x100   # OR into the precision: V^pppp rebuilt inside the ladder 
 STOP_REASONS = {0: "converged", 1: "max_iterations", 2: "nan"}
 # iteration schedules (CCD_SCHED_*): how the same kernels are launched
 SCHEDULES = {"auto": 0, "streams": 1, "serial": 2, "serial_graph": 3}
+# pppp-ladder plan vocabulary (CCD_LK_*, CCD_LA_*, CCD_LP_*, CCD_TUNE_*)
+LADDER_KINDS = {"default": 0, "mfma16": 1, "bf16x3": 2}
+LADDER_ASSIGN = {"model": 0, "channel": 1, "longest": 2}
+LADDER_PARTITIONS = {"split": 0, "one": 1}
+TUNE_KINDS = 1
+TUNE_BF16X3 = 2
 # launch groups of an iteration, in CCD_KT_* order (include/ccd_amd.h)
 KERNEL_GROUPS = ("ring", "remap_r", "ladder_x", "ladder_tx", "ladder_v1", "ladder_v2", "ladder_v34",
                  "ladder_v58", "update", "remap_t", "reduce")
 
-__all__ = ["Model", "Context", "Batch", "CCDError", "FP64", "FP32", "MATRIX_FREE", "STOP_REASONS"]
+__all__ = ["Model", "Context", "Batch", "CCDError", "FP64", "FP32", "MATRIX_FREE", "STOP_REASONS",
+           "TUNE_KINDS", "TUNE_BF16X3"]
 
 
 def _ptr(a: np.ndarray):
@@ -207,6 +214,44 @@ class Context:
         check(lib().ccd_get_schedule(self.h, C.byref(k), _ptr(us)), "ccd_get_schedule")
         names = {v: n for n, v in SCHEDULES.items()}
         return names[k.value], {names[q + 1]: float(us[q]) for q in range(3)}
+
+    def ladder_plan(self) -> dict:
+        """The pppp-ladder plan: per size class 1..8 (16(n-1) < hh <= 16n) the
+        kernel kind for FP64 / FP32 iterations, blocks per compute unit (0: one
+        per tile) and tile assignment; the launch partition; the tuner's
+        microseconds per class launch and per iteration of each partition."""
+        p = LadderPlan()
+        check(lib().ccd_get_ladder_plan(self.h, C.byref(p)), "ccd_get_ladder_plan")
+        kinds = {v: k for k, v in LADDER_KINDS.items()}
+        assign = {v: k for k, v in LADDER_ASSIGN.items()}
+        parts = {v: k for k, v in LADDER_PARTITIONS.items()}
+        return {
+            "kind_fp64": [kinds[p.kind[0][n]] for n in range(8)],
+            "kind_fp32": [kinds[p.kind[1][n]] for n in range(8)],
+            "blocks_per_cu": [int(p.blocks_per_cu[n]) for n in range(8)],
+            "assign": [assign[p.assign[n]] for n in range(8)],
+            "partition": parts[p.partition],
+            "us_fp64": [float(p.us[0][n]) for n in range(8)],
+            "us_fp32": [float(p.us[1][n]) for n in range(8)],
+            "us_partition": {"split": float(p.us_partition[0]), "one": float(p.us_partition[1])},
+        }
+
+    def set_ladder_plan(self, plan: dict):
+        """Set the plan from a dict shaped like ladder_plan() (the us fields are ignored)."""
+        p = LadderPlan()
+        for n in range(8):
+            p.kind[0][n] = LADDER_KINDS[plan["kind_fp64"][n]]
+            p.kind[1][n] = LADDER_KINDS[plan["kind_fp32"][n]]
+            p.blocks_per_cu[n] = plan["blocks_per_cu"][n]
+            p.assign[n] = LADDER_ASSIGN[plan["assign"][n]]
+        p.partition = LADDER_PARTITIONS[plan["partition"]]
+        check(lib().ccd_set_ladder_plan(self.h, C.byref(p)), "ccd_set_ladder_plan")
+
+    def tune_ladder(self, flags: int = 0) -> dict:
+        """Tune the ladder plan for the current arithmetic (TUNE_KINDS,
+        TUNE_BF16X3 widen the kernels tried); resets T2 to zero."""
+        check(lib().ccd_tune_ladder(self.h, flags), "ccd_tune_ladder")
+        return self.ladder_plan()
 
     def set_precision(self, precision: int):
         """Arithmetic of the contractions of the following iterations (FP64 or FP32)."""

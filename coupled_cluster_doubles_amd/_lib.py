@@ -31,6 +31,16 @@ class Problem(C.Structure):
     ]
 
 
+class LadderPlan(C.Structure):
+    """ccd_ladder_plan: per size class of the pppp ladder, the kernel kind (per
+    arithmetic), persistent blocks per compute unit and tile assignment, and
+    the launch partition; the tuner's timings."""
+    _fields_ = [
+        ("kind", (C.c_int32 * 8) * 2), ("blocks_per_cu", C.c_int32 * 8), ("assign", C.c_int32 * 8),
+        ("partition", C.c_int32), ("us", (C.c_double * 8) * 2), ("us_partition", C.c_double * 2),
+    ]
+
+
 # (name, restype, argtypes) of every exported function in include/ccd_amd.h
 vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.c_void_p
 i32p, i64p = C.POINTER(C.c_int32), C.POINTER(C.c_int64)
@@ -42,6 +52,9 @@ SIGNATURES = [
     ("ccd_set_precision", C.c_int, [vp, C.c_int]),
     ("ccd_set_schedule", C.c_int, [vp, C.c_int]),
     ("ccd_get_schedule", C.c_int, [vp, C.POINTER(C.c_int), ip]),
+    ("ccd_get_ladder_plan", C.c_int, [vp, C.POINTER(LadderPlan)]),
+    ("ccd_set_ladder_plan", C.c_int, [vp, C.POINTER(LadderPlan)]),
+    ("ccd_tune_ladder", C.c_int, [vp, C.c_int]),
     ("ccd_mbpt2", C.c_int, [vp, dp]),
     ("ccd_mbpt2_noalloc", C.c_int, [vp, vp, vp]),
     ("ccd_reset", C.c_int, [vp]),

@@ -25,6 +25,8 @@
 #include <limits>
 #include <climits>
 #include <mutex>
+#include <queue>
+#include <utility>
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
@@ -39,6 +41,7 @@
 #include "device_common.hpp"
 #include "internal.hpp"
 #include "kernels_ladder.hpp"
+#include "ladder_launch.hpp"
 #include "kernels_ring.hpp"
 #include "kernels_setup.hpp"
 #include "kernels_update.hpp"
@@ -114,7 +117,8 @@ struct ccd_ctx {
     int precision = CCD_PREC_FP64;
     bool matrix_free = false;    // V^pppp generated inside the ladder, never stored
     int lsrc = LV_STORED;        // LadderSource of the pppp ladder
-    int gen_blocks = 1024;       // grid of the generated-V ladder launches
+    int ncu = 1;                 // compute units of the device
+    ccd_ladder_plan plan{};      // launch plan of the pppp ladder
     LadderGen lgen{};
     hipStream_t stream = nullptr;         // main stream: every entry point is ordered on it
     hipStream_t aux[3] = {nullptr, nullptr, nullptr};  // concurrent phases of an iteration
@@ -126,9 +130,12 @@ struct ccd_ctx {
     std::vector<DMod> mods;
     std::vector<DTile> rtiles, ltiles, utiles, xtiles, ttiles;  // packed-row, ladder, update, X, T X tiles
     int lt_off[NTMAX + 1] = {0};  // ladder tiles grouped by 16-column tile count: [lt_off[n-1], lt_off[n])
+    std::vector<DTile> ltiles0;   // the same tiles, channel order inside each class
     // blocks of the ladder launch of class n: lb_grid[n] blocks, block b runs the
     // class's tiles lbptr[lb_base[n] + b] .. lbptr[lb_base[n] + b + 1] - 1
+    // (lb_base[n] reserves one entry per tile of the class and one more)
     int lb_base[NTMAX + 1] = {0}, lb_grid[NTMAX + 1] = {0};
+    std::vector<int32_t> lbptr;
     int32_t *d_lbptr = nullptr;
     int64_t n_cp = 0;            // column-partial slots
     std::vector<RingItem> ritems;  // ring work items, largest channels first
@@ -328,19 +335,11 @@ int allow_ring_lds(size_t bytes) {
     return CCD_OK;
 }
 
-template <int NTW, int SRC>
-int allow_wave_lds() {
-    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_ladder_w<NTW, SRC>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)WaveLadder<NTW>::lds_bytes(SRC == LV_GAS ? kGasTableMax : 0)));
-    return CCD_OK;
-}
-
 int allow_ring_lds_all() {
     int rc;
     if ((rc = allow_ring_lds<double, 1, 4>(RingStage<double, 4>::BYTES)) ||
         (rc = allow_ring_lds<double, 5, 5>(RingStage<double, 5>::BYTES)) ||
-        (rc = allow_ring_lds<float, 5, 5>(RingStage<float, 5>::BYTES)) || (rc = allow_wave_lds<1, LV_GAS>()))
+        (rc = allow_ring_lds<float, 5, 5>(RingStage<float, 5>::BYTES)))
         return rc;
     return CCD_OK;
 }
@@ -451,76 +450,69 @@ int launch_ladder_x(ccd_ctx *c, hipStream_t st) {
     return check_launch("k_ladder_x");
 }
 
-// Stored V: one block per tile.  Generated V: at most gen_blocks blocks, each
-// running its own run of tiles (lbptr), the gas table in dynamic LDS.
-template <typename CT, int NT, int SRC>
-void launch_ladder_v_nt(ccd_ctx *c, const CT *vp, hipStream_t st) {
-    const int grid = c->lb_grid[NT];
-    if (grid <= 0) return;
-    const size_t lds = SRC == LV_GAS ? sizeof(CT) * (size_t)c->lgen.ntab : 0;   // the table in CT
-    hipLaunchKernelGGL((k_ladder_v<CT, NT, SRC>), dim3((unsigned)grid), dim3(kThreads), lds, st,
-                       c->d_ltiles + c->lt_off[NT - 1], c->d_lbptr + c->lb_base[NT], c->d_chans, vp, c->lgen,
-                       reinterpret_cast<const CT *>(c->d_tas), c->d_zero, reinterpret_cast<CT *>(c->d_dp),
-                       c->d_ctl);
+// Bytes of the B-tile LDS of class n under kernel kind `kind`.
+template <typename CT>
+size_t class_bytes(int n, int kind) {
+    switch (n) {
+        case 1: return ladder_class_bytes<CT, 1>(kind);
+        case 2: return ladder_class_bytes<CT, 2>(kind);
+        case 3: return ladder_class_bytes<CT, 3>(kind);
+        case 4: return ladder_class_bytes<CT, 4>(kind);
+        case 5: return ladder_class_bytes<CT, 5>(kind);
+        case 6: return ladder_class_bytes<CT, 6>(kind);
+        case 7: return ladder_class_bytes<CT, 7>(kind);
+        default: return ladder_class_bytes<CT, 8>(kind);
+    }
 }
 
-// FP32 classes up to kWaveClassMax with one packed row per lane: the same
-// tiles and block runs as k_ladder_v.  Measured on the N=54 electron gas
-// (matrix-free, FP32): class 1 37.3 us against 42.3 us with k_ladder_v;
-// class 2 25.4 us against 19.6 us, so class 2 stays on k_ladder_v.
-constexpr int kWaveClassMax = 1;
-template <int NT, int SRC>
-void launch_ladder_w(ccd_ctx *c, const float *vp, hipStream_t st) {
-    const int grid = c->lb_grid[NT];
-    if (grid <= 0) return;
-    const size_t lds = WaveLadder<NT>::lds_bytes(SRC == LV_GAS ? c->lgen.ntab : 0);
-    hipLaunchKernelGGL((k_ladder_w<NT, SRC>), dim3((unsigned)grid), dim3(kThreads), lds, st,
-                       c->d_ltiles + c->lt_off[NT - 1], c->d_lbptr + c->lb_base[NT], c->d_chans, vp, c->lgen,
-                       reinterpret_cast<const float *>(c->d_tas), reinterpret_cast<float *>(c->d_dp), c->d_ctl);
+// The grouped ladder launch g (classes [LO, HI] = [1, 1], [2, 2], [3, 4],
+// [5, 8] or [1, 8] for g = 0..4), running the classes of [lo, hi] among them
+// (the others get no blocks): each class its lb_grid blocks and the plan's
+// kernel kind; dynamic LDS = the gas table and the B tiles of the largest
+// class launched.
+template <typename CT>
+void launch_grp(ccd_ctx *c, const CT *vp, int g, int lo, int hi, hipStream_t st) {
+    static const int glo[5] = {1, 2, 3, 5, 1}, ghi[5] = {1, 2, 4, 8, 8};
+    const int LO = glo[g], HI = ghi[g];
+    LadderLaunch L{};
+    const int prec = sizeof(CT) == 4 ? CCD_PREC_FP32 : CCD_PREC_FP64;
+    int nb = 0;
+    size_t need = 0;
+    for (int n = LO; n <= HI; n++) {
+        L.tile0[n] = c->lt_off[n - 1];
+        L.ptr0[n] = c->lb_base[n];
+        L.kind[n] = c->plan.kind[prec][n - 1];
+        L.blk0[n] = nb;
+        if (n >= lo && n <= hi && c->lb_grid[n] > 0) {
+            nb += c->lb_grid[n];
+            need = std::max(need, class_bytes<CT>(n, L.kind[n]));
+        }
+    }
+    L.blk0[HI + 1] = nb;
+    if (nb == 0) return;
+    const size_t lds = ladder_tab_bytes(c->lsrc == LV_GAS ? c->lgen.ntab : 0) + need;
+    const LadderOps<CT> o{c->d_ltiles, c->d_lbptr, c->d_chans, vp, c->lgen, reinterpret_cast<const CT *>(c->d_tas),
+                          c->d_zero, reinterpret_cast<CT *>(c->d_dp), c->d_ctl};
+    launch_ladder_grp(c->lsrc, g, (unsigned)nb, lds, st, L, o);
 }
 
-template <int NT, int SRC, typename CT>
-void launch_ladder_class(ccd_ctx *c, const CT *vp, hipStream_t st) {
-    if constexpr (std::is_same<CT, float>::value && NT <= kWaveClassMax) launch_ladder_w<NT, SRC>(c, vp, st);
-    else launch_ladder_v_nt<CT, NT, SRC>(c, vp, st);
-}
-
-// Classes NT1 and NT2 in one launch (either may be empty)
-template <typename CT, int NT1, int NT2, int SRC>
-void launch_ladder_v_pair(ccd_ctx *c, const CT *vp, hipStream_t st) {
-    const int g1 = c->lb_grid[NT1], g2 = c->lb_grid[NT2];
-    if (g1 + g2 <= 0) return;
-    const size_t lds = SRC == LV_GAS ? sizeof(CT) * (size_t)c->lgen.ntab : 0;
-    hipLaunchKernelGGL((k_ladder_v2<CT, NT1, NT2, SRC>), dim3((unsigned)(g1 + g2)), dim3(kThreads), lds, st,
-                       c->d_ltiles + c->lt_off[NT1 - 1], c->d_lbptr + c->lb_base[NT1], g1,
-                       c->d_ltiles + c->lt_off[NT2 - 1], c->d_lbptr + c->lb_base[NT2], c->d_chans, vp, c->lgen,
-                       reinterpret_cast<const CT *>(c->d_tas), c->d_zero, reinterpret_cast<CT *>(c->d_dp),
-                       c->d_ctl);
-}
-
-// NT classes [lo, hi] of the pppp ladder: classes 1 and 2 (many blocks each)
-// launched on their own, the wider classes (few long blocks) in pairs
-template <typename CT, int SRC>
-void launch_ladder_v_src(ccd_ctx *c, const CT *vp, int lo, int hi, hipStream_t st) {
-    if (lo <= 1 && 1 <= hi) launch_ladder_class<1, SRC>(c, vp, st);
-    if (lo <= 2 && 2 <= hi) launch_ladder_class<2, SRC>(c, vp, st);
-    static_assert(NTMAX == 8, "pairs (3,4), (5,6), (7,8)");
-    if (lo <= 3 && 4 <= hi) launch_ladder_v_pair<CT, 3, 4, SRC>(c, vp, st);
-    else if (lo <= 3 && 3 <= hi) launch_ladder_v_nt<CT, 3, SRC>(c, vp, st);
-    else if (lo <= 4 && 4 <= hi) launch_ladder_v_nt<CT, 4, SRC>(c, vp, st);
-    if (lo <= 5 && 6 <= hi) launch_ladder_v_pair<CT, 5, 6, SRC>(c, vp, st);
-    else if (lo <= 5 && 5 <= hi) launch_ladder_v_nt<CT, 5, SRC>(c, vp, st);
-    else if (lo <= 6 && 6 <= hi) launch_ladder_v_nt<CT, 6, SRC>(c, vp, st);
-    if (lo <= 7 && 8 <= hi) launch_ladder_v_pair<CT, 7, 8, SRC>(c, vp, st);
-    else if (lo <= 7 && 7 <= hi) launch_ladder_v_nt<CT, 7, SRC>(c, vp, st);
-    else if (lo <= 8 && 8 <= hi) launch_ladder_v_nt<CT, 8, SRC>(c, vp, st);
-}
-
+// Classes [lo, hi] of the pppp ladder in the plan's partition: launches of
+// classes 1, 2, 3-4 and 5-8, or one grouped launch.
 template <typename CT>
 void launch_ladder_v_ct(ccd_ctx *c, const CT *vp, int lo, int hi, hipStream_t st) {
-    if (c->lsrc == LV_GAS) launch_ladder_v_src<CT, LV_GAS>(c, vp, lo, hi, st);
-    else if (c->lsrc == LV_GENERIC) launch_ladder_v_src<CT, LV_GENERIC>(c, vp, lo, hi, st);
-    else launch_ladder_v_src<CT, LV_STORED>(c, vp, lo, hi, st);
+    if (c->plan.partition == CCD_LP_ONE) {
+        launch_grp<CT>(c, vp, 4, lo, hi, st);
+        return;
+    }
+    for (int g = 0; g < 4; g++) launch_grp<CT>(c, vp, g, lo, hi, st);
+}
+
+int allow_ladder_lds_all() {
+    int e;
+    if ((e = allow_ladder_lds_f32_gas()) || (e = allow_ladder_lds_f32_other()) || (e = allow_ladder_lds_f64_gas()) ||
+        (e = allow_ladder_lds_f64_other()))
+        return ccd::fail(CCD_E_HIP, "raising the ladder's LDS limit failed: %s", hipGetErrorString((hipError_t)e));
+    return CCD_OK;
 }
 
 // 1/2 T X into the last D_p slab of every channel
@@ -653,8 +645,11 @@ int run_iteration(ccd_ctx *c, double mixing, hipEvent_t *ev, int reps = 1) {
         HIP_TRY(hipEventRecord(e[2], s2));
         if ((rc = launch_ring(c, s1)) || (rc = launch_remap_r(c, s1))) return rc;
         HIP_TRY(hipEventRecord(e[0], s1));
-        if ((rc = launch_ladder_v(c, 1, 2, s0))) return rc;
-        if ((rc = launch_ladder_v(c, 3, NTMAX, s3))) return rc;
+        if (c->plan.partition == CCD_LP_ONE) {   // the whole ladder in one launch on the main stream
+            if ((rc = launch_ladder_v(c, 1, NTMAX, s0))) return rc;
+        } else {
+            if ((rc = launch_ladder_v(c, 1, 2, s0)) || (rc = launch_ladder_v(c, 3, NTMAX, s3))) return rc;
+        }
         HIP_TRY(hipEventRecord(e[3], s3));
         HIP_TRY(hipStreamWaitEvent(s0, e[0], 0));
         HIP_TRY(hipStreamWaitEvent(s0, e[2], 0));
@@ -1052,6 +1047,96 @@ int build_generated_ladder(ccd_ctx *c, const ccd_problem *p, const std::vector<i
     return CCD_OK;
 }
 
+// The plan a context starts with: the default kernels, 3 persistent blocks
+// per compute unit for a generated V (each stages the generator table once;
+// class 1 at N=54 37.3 us against 43.6 us with 4) or one block per tile for
+// a stored one, the CU-load model's assignment, launches per class group.
+void default_plan(ccd_ctx *c) {
+    ccd_ladder_plan &p = c->plan;
+    p = ccd_ladder_plan{};
+    for (int n = 0; n < NTMAX; n++) {
+        p.kind[CCD_PREC_FP64][n] = p.kind[CCD_PREC_FP32][n] = CCD_LK_DEFAULT;
+        p.blocks_per_cu[n] = c->matrix_free ? 3 : 0;
+        p.assign[n] = CCD_LA_MODEL;
+    }
+    p.partition = CCD_LP_SPLIT;
+}
+
+// Block runs of class nt's ladder tiles under the plan (host tables ltiles,
+// lbptr, lb_grid).  The grid is one block per tile or blocks_per_cu blocks
+// per compute unit, block b running a contiguous run of the class's tiles.
+// Two assignments are costed with a model of the CU loads (block b on CU
+// b mod ncu, cost of a tile = its k chunks + 1): round robin in channel
+// order (block b takes tiles b, b+G, ...), and longest tile first to the
+// least loaded block.  CCD_LA_MODEL takes the second when its most loaded CU
+// carries at least 10% less (the large electron gases) and keeps the channel
+// order otherwise (it measured faster where the two are close).  Tiles write
+// disjoint outputs: the assignment changes no result.
+void plan_class_tiles(ccd_ctx *c, int nt) {
+    const int lo = c->lt_off[nt - 1], n = c->lt_off[nt] - lo, ncu = c->ncu;
+    const int bpc = c->plan.blocks_per_cu[nt - 1];
+    const int G = bpc > 0 ? std::min(n, bpc * ncu) : n;
+    c->lb_grid[nt] = G;
+    int32_t *ptr = c->lbptr.data() + c->lb_base[nt];
+    ptr[0] = 0;
+    if (G <= 0) return;
+    const int kch = nt <= 5 ? LadderShape<1>::KCH : LadderShape<8>::KCH;
+    const int kseg = nt <= 2 ? LadderShape<1>::KSEG : LadderShape<3>::KSEG;
+    std::vector<int64_t> cost(n);
+    for (int i = 0; i < n; i++) {   // k chunks of the tile, plus one for its prologue
+        const DTile &t = c->ltiles0[lo + i];
+        const int klen = std::min(c->chans[t.chan].npk, (t.aux + 1) * kseg) - t.aux * kseg;
+        cost[i] = (klen + kch - 1) / kch + 1;
+    }
+    auto cu_max = [&](const std::vector<std::vector<int>> &own) {
+        std::vector<int64_t> cu(ncu, 0);
+        for (int b = 0; b < G; b++)
+            for (int i : own[b]) cu[b % ncu] += cost[i];
+        return *std::max_element(cu.begin(), cu.end());
+    };
+    std::vector<std::vector<int>> rr(G), lpt(G);
+    for (int i = 0; i < n; i++) rr[i % G].push_back(i);
+    const int mode = c->plan.assign[nt - 1];
+    if (mode != CCD_LA_CHANNEL) {
+        std::vector<int> idx(n);
+        for (int i = 0; i < n; i++) idx[i] = i;
+        std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+        // least loaded block, lowest index on ties
+        std::priority_queue<std::pair<int64_t, int>, std::vector<std::pair<int64_t, int>>,
+                            std::greater<std::pair<int64_t, int>>>
+            heap;
+        for (int b = 0; b < G; b++) heap.push({0, b});
+        for (int i : idx) {
+            auto [load, b] = heap.top();
+            heap.pop();
+            lpt[b].push_back(i);
+            heap.push({load + cost[i], b});
+        }
+    }
+    const bool use_lpt = mode == CCD_LA_LONGEST || (mode == CCD_LA_MODEL && 10 * cu_max(lpt) <= 9 * cu_max(rr));
+    const auto &own = use_lpt ? lpt : rr;
+    int k = 0;
+    for (int b = 0; b < G; b++) {
+        for (int i : own[b]) c->ltiles[lo + k++] = c->ltiles0[lo + i];
+        ptr[b + 1] = k;
+    }
+}
+
+void plan_ladder_tiles(ccd_ctx *c) {
+    for (int nt = 1; nt <= NTMAX; nt++) plan_class_tiles(c, nt);
+}
+
+// The host tables of plan_ladder_tiles to the device (after a change of plan).
+int upload_ladder_tiles(ccd_ctx *c) {
+    if (!c->ltiles.empty())
+        HIP_TRY(hipMemcpyAsync(c->d_ltiles, c->ltiles.data(), sizeof(DTile) * c->ltiles.size(),
+                               hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_lbptr, c->lbptr.data(), sizeof(int32_t) * c->lbptr.size(), hipMemcpyHostToDevice,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CCD_OK;
+}
+
 // Host tables: channel descriptors, swaps, packed rows, tiles, transposition
 // plans and reduction lists.  Every index a kernel dereferences is produced
 // and range-checked here.
@@ -1364,71 +1449,21 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
         }
         c->lt_off[nt] = (int)c->ltiles.size();
     }
-    // Block assignment.  Stored V: one block per tile.  Generated V: a
-    // persistent grid of gen_blocks blocks (each stages the generator table
-    // once), block b running a contiguous run of the class's tiles.  Two
-    // assignments are costed per class with a model of the CU loads (block b
-    // on CU b mod ncu, cost of a tile = its k chunks + 1): round robin in
-    // channel order (block b takes tiles b, b+G, ...), and longest tile first
-    // to the least loaded block.  The second is taken when its most loaded
-    // CU carries at least 10% less (the large electron gases); otherwise the
-    // channel order is kept (it measured faster where the two are close).
-    // Tiles write disjoint outputs: the assignment changes no result.
-    std::vector<int32_t> lbptr;
+    // Block runs of the ladder tiles: per class the plan's block count and
+    // assignment (plan_ladder_tiles), uploaded with the other tables below.
+    HIP_TRY(hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+    c->ncu = std::max(1, c->ncu);
+    c->ltiles0 = c->ltiles;
     {
-        int ncu = 0;
-        HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
-        ncu = std::max(1, ncu);
-        c->gen_blocks = ncu * 3;   // (3 per CU: class 1 37.3 us at N=54 against 43.6 us with 4)
-        // tiles [lo, lo + n) of a class over G blocks (block b on CU b mod ncu):
-        // each block's run of tiles written back to `out`, the run ends to `ptr`
-        auto assign = [&](int lo, int n, int G, const std::vector<int64_t> &cost, int per_cu,
-                          std::vector<DTile> &out, std::vector<int32_t> &ptr) {
-            auto cu_max = [&](const std::vector<std::vector<int>> &own) {
-                std::vector<int64_t> cu(ncu, 0);
-                for (int b = 0; b < G; b++)
-                    for (int i : own[b]) cu[(b / per_cu) % ncu] += cost[i];
-                return *std::max_element(cu.begin(), cu.end());
-            };
-            std::vector<std::vector<int>> rr(G), lpt(G);
-            for (int i = 0; i < n; i++) rr[i % G].push_back(i);
-            std::vector<int> idx(n);
-            for (int i = 0; i < n; i++) idx[i] = i;
-            std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-            std::vector<int64_t> load(G, 0);
-            for (int i : idx) {   // least loaded slot, lowest index on ties
-                int b = 0;
-                for (int q = 1; q < G; q++)
-                    if (load[q] < load[b]) b = q;
-                load[b] += cost[i];
-                lpt[b].push_back(i);
-            }
-            const auto &own = 10 * cu_max(lpt) <= 9 * cu_max(rr) ? lpt : rr;
-            std::vector<DTile> cls;
-            for (int b = 0; b < G; b++) {
-                for (int i : own[b]) cls.push_back(c->ltiles[lo + i]);
-                ptr.push_back((int32_t)cls.size());
-            }
-            std::copy(cls.begin(), cls.end(), out.begin() + lo);
-        };
+        int base = 0;
         for (int nt = 1; nt <= NTMAX; nt++) {
-            const int lo = c->lt_off[nt - 1], n = c->lt_off[nt] - lo;
-            const int G = c->matrix_free ? std::min(n, c->gen_blocks) : n;
-            c->lb_base[nt] = (int)lbptr.size();
-            c->lb_grid[nt] = G;
-            lbptr.push_back(0);
-            if (G <= 0) continue;
-            const int kch = nt <= 5 ? LadderShape<1>::KCH : LadderShape<8>::KCH;
-            const int kseg = nt <= 2 ? LadderShape<1>::KSEG : LadderShape<3>::KSEG;
-            std::vector<int64_t> cost(n);
-            for (int i = 0; i < n; i++) {   // k chunks of the tile, plus one for its prologue
-                const DTile &t = c->ltiles[lo + i];
-                const int klen = std::min(c->chans[t.chan].npk, (t.aux + 1) * kseg) - t.aux * kseg;
-                cost[i] = (klen + kch - 1) / kch + 1;
-            }
-            assign(lo, n, G, cost, 1, c->ltiles, lbptr);
+            c->lb_base[nt] = base;
+            base += c->lt_off[nt] - c->lt_off[nt - 1] + 1;
         }
+        c->lbptr.assign((size_t)base, 0);
     }
+    default_plan(c);
+    plan_ladder_tiles(c);
     for (int wide = 0; wide < 2; wide++) {   // channels with hh <= 80 first
         for (int k = 0; k < (int)c->chans.size(); k++)
             if ((c->chans[k].hh > 80) == (wide == 1))
@@ -1479,7 +1514,7 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
         (rc = upload(c, &c->d_rtiles, c->rtiles)) || (rc = upload(c, &c->d_ltiles, c->ltiles)) ||
         (rc = upload(c, &c->d_utiles, c->utiles)) ||
         (rc = upload(c, &c->d_xtiles, c->xtiles)) || (rc = upload(c, &c->d_ttiles, c->ttiles)) || (rc = upload(c, &c->d_ritems, c->ritems)) ||
-        (rc = upload(c, &c->d_rgroups, c->rgroups)) || (rc = upload(c, &c->d_lbptr, lbptr)) ||
+        (rc = upload(c, &c->d_rgroups, c->rgroups)) || (rc = upload(c, &c->d_lbptr, c->lbptr)) ||
         (rc = upload(c, &c->d_t2_in, t2_in)) || (rc = upload(c, &c->d_t2_out, t2_out)) ||
         (rc = upload(c, &c->d_pp_old, pp_old)) ||
         (rc = upload(c, &c->d_ppl, ppl)) ||
@@ -1828,7 +1863,7 @@ int create_ctx(const ccd_problem *prob, const BatchInfo &bi, int precision, ccd_
         free_ctx(c);
         return rc;
     }
-    if ((rc = allow_ring_lds_all())) {
+    if ((rc = allow_ring_lds_all()) || (rc = allow_ladder_lds_all())) {
         free_ctx(c);
         return rc;
     }
@@ -1889,6 +1924,127 @@ int tune_schedule(ccd_ctx *c) {
     (void)hipEventDestroy(b);
     c->sched = pick;
     if (!rc && pick != CCD_SCHED_SERIAL_GRAPH) rc = drop_graphs(c);
+    if (!rc) rc = zero_state(c);
+    return rc;
+}
+
+// Device time of one launch of ladder class nt (microseconds), averaged over
+// `reps` launches back to back on the main stream after one untimed launch.
+int time_class(ccd_ctx *c, int nt, int reps, double *us) {
+    hipEvent_t a = nullptr, b = nullptr;
+    HIP_TRY(hipEventCreate(&a));
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return ccd::fail(CCD_E_HIP, "could not create HIP events");
+    }
+    int rc = launch_ladder_v(c, nt, nt, c->stream);
+    hipError_t e = rc ? hipSuccess : hipEventRecord(a, c->stream);
+    for (int r = 0; r < reps && !rc && e == hipSuccess; r++) rc = launch_ladder_v(c, nt, nt, c->stream);
+    if (!rc && e == hipSuccess) e = hipEventRecord(b, c->stream);
+    if (!rc && e == hipSuccess) e = hipEventSynchronize(b);
+    float ms = 0.f;
+    if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    if (!rc && e != hipSuccess) rc = ccd::fail(CCD_E_HIP, "ladder timing failed: %s", hipGetErrorString(e));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (!rc) *us = 1e3 * ms / reps;
+    return rc;
+}
+
+// Microseconds per iteration of the context's schedule (a free run: one
+// untimed batch, then kTimed timed ones).
+int time_iterations(ccd_ctx *c, double *us) {
+    constexpr int kTimed = 2;
+    hipEvent_t a = nullptr, b = nullptr;
+    HIP_TRY(hipEventCreate(&a));
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return ccd::fail(CCD_E_HIP, "could not create HIP events");
+    }
+    int rc = arm_free_run(c);
+    if (!rc) rc = run_n(c, 0.5, kSolveBatch);
+    hipError_t e = rc ? hipSuccess : hipEventRecord(a, c->stream);
+    if (!rc && e == hipSuccess) rc = run_n(c, 0.5, kTimed * kSolveBatch);
+    if (!rc && e == hipSuccess) e = hipEventRecord(b, c->stream);
+    if (!rc && e == hipSuccess) e = hipEventSynchronize(b);
+    float ms = 0.f;
+    if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    if (!rc && e != hipSuccess) rc = ccd::fail(CCD_E_HIP, "iteration timing failed: %s", hipGetErrorString(e));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (!rc) *us = 1e3 * ms / (kTimed * kSolveBatch);
+    return rc;
+}
+
+int plan_valid(const ccd_ladder_plan &p) {
+    for (int n = 0; n < NTMAX; n++) {
+        for (int q = 0; q < 2; q++)
+            if (p.kind[q][n] < CCD_LK_DEFAULT || p.kind[q][n] > CCD_LK_BF16X3)
+                return ccd::fail(CCD_E_INVALID, "ladder plan: unknown kernel kind %d (class %d)", p.kind[q][n], n + 1);
+        if (p.kind[CCD_PREC_FP64][n] == CCD_LK_BF16X3)
+            return ccd::fail(CCD_E_INVALID, "ladder plan: BF16x3 products are for FP32 iterations (class %d)", n + 1);
+        if (p.blocks_per_cu[n] < 0 || p.blocks_per_cu[n] > 16)
+            return ccd::fail(CCD_E_INVALID, "ladder plan: %d blocks per compute unit (class %d)", p.blocks_per_cu[n],
+                             n + 1);
+        if (p.assign[n] < CCD_LA_MODEL || p.assign[n] > CCD_LA_LONGEST)
+            return ccd::fail(CCD_E_INVALID, "ladder plan: unknown assignment %d (class %d)", p.assign[n], n + 1);
+    }
+    if (p.partition != CCD_LP_SPLIT && p.partition != CCD_LP_ONE)
+        return ccd::fail(CCD_E_INVALID, "ladder plan: unknown partition %d", p.partition);
+    return CCD_OK;
+}
+
+// The ladder tuner (ccd_tune_ladder): per class with tiles, a coordinate
+// search -- kernel kind, then blocks per compute unit, then assignment --
+// each candidate timed over repeated launches of the class alone; then the
+// partition over whole iterations.  The amplitudes of two free iterations
+// give the kernels realistic operands; T2 goes back to zero at the end.
+int tune_ladder(ccd_ctx *c, int flags) {
+    constexpr int kReps = 5;
+    const int prec = c->iter_prec;
+    ccd_ladder_plan &p = c->plan;
+    int rc = drop_graphs(c);
+    if (!rc) rc = arm_free_run(c);
+    if (!rc) rc = run_n(c, 0.5, 2);
+    if (!rc) HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int nt = 1; nt <= NTMAX && !rc; nt++) {
+        p.us[prec][nt - 1] = 0.0;
+        if (c->lt_off[nt] == c->lt_off[nt - 1]) continue;
+        double best = 0.0;
+        auto consider = [&](int32_t &field, const std::vector<int> &vals) -> int {
+            int pick = field;
+            for (size_t q = 0; q < vals.size(); q++) {
+                field = vals[q];
+                plan_class_tiles(c, nt);
+                double us = 0.0;
+                int r = upload_ladder_tiles(c);
+                if (!r) r = time_class(c, nt, kReps, &us);
+                if (r) return r;
+                if (best == 0.0 || us < best) {
+                    best = us;
+                    pick = vals[q];
+                }
+            }
+            field = pick;
+            plan_class_tiles(c, nt);
+            return upload_ladder_tiles(c);
+        };
+        std::vector<int> kinds{CCD_LK_DEFAULT};
+        if (prec == CCD_PREC_FP32 && nt == 1 && (flags & CCD_TUNE_KINDS)) kinds.push_back(CCD_LK_MFMA16);
+        if (prec == CCD_PREC_FP32 && (flags & CCD_TUNE_BF16X3)) kinds.push_back(CCD_LK_BF16X3);
+        std::vector<int> bpcs{1, 2, 3, 4, 6};
+        if (!c->matrix_free) bpcs.insert(bpcs.begin(), 0);
+        if ((rc = consider(p.kind[prec][nt - 1], kinds)) || (rc = consider(p.blocks_per_cu[nt - 1], bpcs)) ||
+            (rc = consider(p.assign[nt - 1], {CCD_LA_CHANNEL, CCD_LA_LONGEST})))
+            break;
+        p.us[prec][nt - 1] = best;
+    }
+    for (int q = 0; q < 2 && !rc; q++) {
+        p.partition = q == 0 ? CCD_LP_SPLIT : CCD_LP_ONE;
+        if (!(rc = drop_graphs(c))) rc = time_iterations(c, &p.us_partition[q]);
+    }
+    if (!rc) p.partition = p.us_partition[1] < p.us_partition[0] ? CCD_LP_ONE : CCD_LP_SPLIT;
+    if (!rc) rc = drop_graphs(c);
     if (!rc) rc = zero_state(c);
     return rc;
 }
@@ -2207,6 +2363,41 @@ int ccd_set_schedule(ccd_ctx *c, int sched) {
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (int rc = drop_graphs(c)) return rc;
     c->sched = sched;
+    return CCD_OK;
+}
+
+int ccd_get_ladder_plan(const ccd_ctx *c, ccd_ladder_plan *plan) {
+    ccd::clear_error();
+    if (!c || !plan) return ccd::fail(CCD_E_INVALID, "ccd_get_ladder_plan: null argument");
+    *plan = c->plan;
+    return CCD_OK;
+}
+
+int ccd_set_ladder_plan(ccd_ctx *c, const ccd_ladder_plan *plan) {
+    ccd::clear_error();
+    if (!c || !plan) return ccd::fail(CCD_E_INVALID, "ccd_set_ladder_plan: null argument");
+    if (int rc = plan_valid(*plan)) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    ccd_ladder_plan p = *plan;
+    for (int q = 0; q < 2; q++)   // the measurements stay those of the tuner
+        for (int n = 0; n < NTMAX; n++) p.us[q][n] = c->plan.us[q][n];
+    p.us_partition[0] = c->plan.us_partition[0];
+    p.us_partition[1] = c->plan.us_partition[1];
+    c->plan = p;
+    plan_ladder_tiles(c);
+    int rc = upload_ladder_tiles(c);
+    return rc ? rc : drop_graphs(c);
+}
+
+int ccd_tune_ladder(ccd_ctx *c, int flags) {
+    ccd::clear_error();
+    if (!c) return ccd::fail(CCD_E_INVALID, "null context");
+    if (flags & ~(CCD_TUNE_KINDS | CCD_TUNE_BF16X3)) return ccd::fail(CCD_E_INVALID, "unknown tuner flags %d", flags);
+    const int saved = c->iter_prec;
+    int rc = tune_ladder(c, flags);
+    c->iter_prec = saved;
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return CCD_OK;
 }
 

@@ -184,17 +184,62 @@ struct LadderShape {
     static constexpr int KSEG = NT <= 2 ? 1024 : 256;
 };
 
-// Tiles b0 .. b1-1 of one class (NT) by one block; the gas table is already in LDS.
-template <typename CT, int NT, int SRC>
+// The kernels of the pppp ladder a class can run on (the plan's kernel kind).
+enum LadderKind {
+    LK_DEFAULT = 0,   // FP32 iterations: the one-row-per-lane kernel for class 1, the
+                      // 16x16x4 one for the others; FP64: the 16x16x4 one
+    LK_MFMA16 = 1,    // v_mfma_{f32,f64}_16x16x4 (ladder_run) in every class
+    LK_BF16X3 = 2,    // FP32 iterations: BF16x3 split products, v_mfma_f32_16x16x32_bf16
+};
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// B-tile LDS of ladder_run: two buffers of the chunk's KCH x 16 NT
+// amplitudes.  Split products hold them as BF16 high and low parts, column
+// by column: column c of buffer b, part h (0 high, 1 low) holds the 32
+// k-slots of the chunk in MFMA order (lane group q, element e at 8q + e) at
+// BF16 offset ((2 b + h) W + c) CSTR, 80-byte columns (16-byte reads of
+// neighbouring columns fall on different banks).
+template <typename CT, int NT, bool SPLIT>
+struct LadderTile {
+    static constexpr int KCH = SPLIT ? 32 : LadderShape<NT>::KCH;
+    static constexpr int W = NT * 16;
+    // row stride (elements) of the unsplit tile: an MFMA step reads rows
+    // kslot(kq, s) for the four lane groups kq; for FP32 the two groups of a
+    // half-wave then sit 16 banks apart (4 STR = 16 mod 32), so they do not
+    // conflict
+    static constexpr int STR = sizeof(CT) == 8 ? W + 1 : W + 4;
+    static constexpr int CSTR = 40;
+    static constexpr size_t BYTES = SPLIT ? (size_t)2 * 2 * W * CSTR * 2 : (size_t)2 * KCH * STR * sizeof(CT);
+};
+
+// Bytes of the gas table at the start of the ladder's dynamic LDS: FP64
+// iterations hold T in FP64, FP32 ones [T, -T] in FP32 (the one-row-per-lane
+// kernel reads both halves, the others the first); 16-byte aligned.
+__host__ __device__ constexpr size_t ladder_tab_bytes(int ntab) { return ((size_t)8 * ntab + 15) / 16 * 16; }
+
+// Tiles b0 .. b1-1 of one class (NT) by one block; the gas table is already
+// in LDS at byte address tbase, the B tiles go to lds_b
+// (LadderTile<CT, NT, SPLIT>::BYTES).
+//
+// SPLIT (CT = float): BF16x3 split products.  Every FP32 operand x is split
+// into x_h = bf16(x) and x_l = bf16(x - x_h) (round to nearest even), and
+// a b is taken as a_l b_h + a_h b_l + a_h b_h, three
+// v_mfma_f32_16x16x32_bf16 per 32-k chunk and 16 columns with FP32
+// accumulation (the a_l b_l term, below 2^-16 |a b|, is dropped), against
+// eight v_mfma_f32_16x16x4f32.  Lane (q, m) of the A operand holds k-slots
+// (q, 0..7) of row m, the values the FP32 path uses in its eight MFMA steps;
+// the B tile is split once as it goes to LDS.
+template <typename CT, int NT, int SRC, bool SPLIT>
 __device__ __forceinline__ void ladder_run(const DTile *__restrict__ tiles, int b0, int b1,
                                            const DChan *__restrict__ chans, const CT *__restrict__ vp,
                                            const LadderGen &lg, const CT *__restrict__ tas,
-                                           const double *__restrict__ zero, CT *__restrict__ dp) {
-    constexpr int KCH = LadderShape<NT>::KCH, KSEG = LadderShape<NT>::KSEG, G = KCH / 4;
-    // LDS row stride of the B tile: an MFMA step reads rows kslot(kq, s) for
-    // the four lane groups kq; for FP32 the two groups of a half-wave then
-    // sit 16 banks apart (4 STR = 16 mod 32), so they do not conflict
-    constexpr int W = NT * 16, STR = sizeof(CT) == 8 ? W + 1 : W + 4;
+                                           const double *__restrict__ zero, CT *__restrict__ dp,
+                                           unsigned char *lds_b, uint32_t tbase) {
+    typedef LadderTile<CT, NT, SPLIT> LT;
+    static_assert(!SPLIT || sizeof(CT) == 4, "split products from FP32 operands");
+    constexpr int KCH = LT::KCH, KSEG = LadderShape<NT>::KSEG, G = KCH / 4;
+    constexpr int W = LT::W, STR = LT::STR, CSTR = LT::CSTR;
     constexpr int BPT = (KCH * W + kThreads - 1) / kThreads;
     constexpr bool GEN = SRC != LV_STORED;
     constexpr int VW = 16 / sizeof(CT);   // elements per 16-byte load of stored V
@@ -202,17 +247,17 @@ __device__ __forceinline__ void ladder_run(const DTile *__restrict__ tiles, int 
     typedef CT vec __attribute__((ext_vector_type(VW)));
     static_assert(G <= 16, "one k-slot record per lane of a 16-lane row");
     static_assert(KSEG % KCH == 0 && G % VW == 0, "k segments of whole chunks, chunks of whole 16-byte loads");
-    __shared__ CT Bs[2][KCH][STR];
-    extern __shared__ double gtab[];
+    static_assert(!SPLIT || (G == 8 && BPT == 2 * NT), "split: 32-k chunks, one MFMA k-group per lane group");
+    CT(*Bs)[KCH][STR] = reinterpret_cast<CT(*)[KCH][STR]>(lds_b);
+    __bf16 *Bsp = reinterpret_cast<__bf16 *>(lds_b);
     // tid < kThreads (the launch bound) as a value the compiler can see, so that
     // the B-tile slot bounds (KCH * W is a multiple of kThreads) fold away and
     // the tile loads stay ahead of their stores, outside any branch
     const int tid = threadIdx.x & (kThreads - 1);
     const int wave = tid >> 6, lane = tid & 63, m = lane & 15, kq = lane >> 4;
-    const uint32_t tbase = (uint32_t)(uintptr_t)(lds_double *)gtab;  // LDS byte address
     // FP32 iterations read the FP32 table through the records scaled for it
     constexpr bool F32 = sizeof(CT) == 4;
-    const int2 *grec1 = F32 ? lg.rec1f : lg.rec1, *grec2 = F32 ? lg.rec2f : lg.rec2;
+    const int2 *grec1_ = F32 ? lg.rec1f : lg.rec1, *grec2 = F32 ? lg.rec2f : lg.rec2;
 
     for (int ti = b0; ti < b1; ti++) {
         const DTile tl = tiles[ti];
@@ -225,6 +270,15 @@ __device__ __forceinline__ void ladder_run(const DTile *__restrict__ tiles, int 
         auto tile = [&](auto mode_) {
             constexpr int MODE = decltype(mode_)::value;
             constexpr bool ONE = MODE == 1, SAME = MODE == 2;
+            // Total-spin-0 signs: the unsplit path folds s(cd) into the B rows
+            // and s(ab) into the output rows (exact: the FP32 MFMA rounds
+            // symmetrically).  The BF16 MFMA's sums are not symmetric under a
+            // sign flip of all products, so the split path builds the signed
+            // element itself from the table [T, -T] through the records with
+            // the sign field (rec1w), like the one-row-per-lane kernel: stored
+            // and generated V then feed the same products.
+            constexpr bool FOLD = ONE && !SPLIT;
+            const int2 *grec1 = SPLIT && ONE ? lg.rec1w : grec1_;
             const int H = ch.hh, npk = ch.npk, ldv = ch.ldv, seg = tl.aux;
             const int kbeg = seg * KSEG, kend = min(npk, kbeg + KSEG);
             const int nv = (kend - kbeg + KCH - 1) / KCH;
@@ -285,29 +339,64 @@ __device__ __forceinline__ void ladder_run(const DTile *__restrict__ tiles, int 
                 });
             };
             // B tile: k rows k0.. of the channel's npk x H amplitude rows
-            // (ONE: with the sign mask of row k, applied when the tile is stored)
+            // (ONE: with the sign mask of row k, applied when the tile is
+            // stored).  Unsplit: element q of a thread is slot idx = tid + 256 q
+            // of the KCH x W tile.  Split: elements 2q and 2q+1 are rows 2p and
+            // 2p + 1 of column c for the pair slot tid + 256 q = p W + c, so a
+            // thread stores the adjacent k-slots of both rows as one 32-bit word
+            // per part.
             const BufRsrc bsrc = make_rsrc(tas + ch.off_tp, (uint32_t)kend * (uint32_t)H * (uint32_t)sizeof(CT));
-            const BufRsrc ssrc = ONE ? make_rsrc(lg.rec2 + ch.pr_off, (uint32_t)kend * 8u) : make_rsrc(zero, 0);
+            const BufRsrc ssrc = FOLD ? make_rsrc(lg.rec2 + ch.pr_off, (uint32_t)kend * 8u) : make_rsrc(zero, 0);
             uint32_t boff[BPT], soff[BPT];
 #pragma unroll
             for (int q = 0; q < BPT; q++) {
-                const int idx = tid + kThreads * q, kk = idx / W, col = idx - kk * W;
-                boff[q] = (idx < KCH * W && col < H) ? (uint32_t)(kk * H + col) * (uint32_t)sizeof(CT) : kOOB;
-                soff[q] = idx < KCH * W ? (uint32_t)kk * 8u : kOOB;
+                int kk, col;
+                bool ok;
+                if (SPLIT) {
+                    const int idx = tid + kThreads * (q >> 1);
+                    kk = 2 * (idx / W) + (q & 1);
+                    col = idx - (idx / W) * W;
+                    ok = true;
+                } else {
+                    const int idx = tid + kThreads * q;
+                    kk = idx / W;
+                    col = idx - kk * W;
+                    ok = idx < KCH * W;
+                }
+                boff[q] = (ok && col < H) ? (uint32_t)(kk * H + col) * (uint32_t)sizeof(CT) : kOOB;
+                soff[q] = ok ? (uint32_t)kk * 8u : kOOB;
             }
             auto load_b = [&](int k0, CT (&b)[BPT], uint32_t (&sm)[BPT]) {
                 const uint32_t ob = (uint32_t)k0 * (uint32_t)H * (uint32_t)sizeof(CT), os = (uint32_t)k0 * 8u;
 #pragma unroll
                 for (int q = 0; q < BPT; q++) {
                     b[q] = buf_load<CT>(bsrc, boff[q] + ob);
-                    if (ONE) sm[q] = buf_u32(ssrc, soff[q] + os);
+                    if (FOLD) sm[q] = buf_u32(ssrc, soff[q] + os);
                 }
             };
             auto store_b = [&](int buf, const CT (&b)[BPT], const uint32_t (&sm)[BPT]) {
+                if constexpr (SPLIT) {
 #pragma unroll
-                for (int q = 0; q < BPT; q++) {
-                    const int idx = tid + kThreads * q, kk = idx / W, col = idx - kk * W;
-                    if (idx < KCH * W) Bs[buf][kk][col] = ONE ? flip_sign(b[q], sm[q]) : b[q];
+                    for (int q = 0; q < BPT / 2; q++) {
+                        const int idx = tid + kThreads * q, p = idx / W, col = idx - p * W;
+                        // k-slot of row 2p: lane group ((2p) % 16) / 4, element
+                        // 4 ((2p) / 16) + (2p) % 4 (row 2p + 1 is the next element)
+                        const int k = 2 * p, pos = ((k & 15) >> 2) * 8 + 4 * (k >> 4) + (k & 3);
+                        const float x0 = b[2 * q], x1 = b[2 * q + 1];
+                        const __bf16 h0 = (__bf16)x0, h1 = (__bf16)x1;
+                        const __bf16 l0 = (__bf16)(x0 - (float)h0), l1 = (__bf16)(x1 - (float)h1);
+                        __bf16 *ph = Bsp + ((2 * buf) * W + col) * CSTR + pos;
+                        __bf16 *pl = Bsp + ((2 * buf + 1) * W + col) * CSTR + pos;
+                        typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+                        *reinterpret_cast<bf2 *>(ph) = bf2{h0, h1};
+                        *reinterpret_cast<bf2 *>(pl) = bf2{l0, l1};
+                    }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < BPT; q++) {
+                        const int idx = tid + kThreads * q, kk = idx / W, col = idx - kk * W;
+                        if (idx < KCH * W) Bs[buf][kk][col] = FOLD ? flip_sign(b[q], sm[q]) : b[q];
+                    }
                 }
             };
 
@@ -315,11 +404,30 @@ __device__ __forceinline__ void ladder_run(const DTile *__restrict__ tiles, int 
 #pragma unroll
             for (int n = 0; n < NT; n++) acc[n] = Acc{};
             auto mfma_chunk = [&](int buf, const CT (&a)[G]) {
+                if constexpr (SPLIT) {
+                    bf16x8 ah, al;
 #pragma unroll
-                for (int s = 0; s < G; s++) {
+                    for (int s = 0; s < G; s++) {
+                        const __bf16 h = (__bf16)a[s];
+                        ah[s] = h;
+                        al[s] = (__bf16)(a[s] - (float)h);
+                    }
 #pragma unroll
-                    for (int n = 0; n < NT; n++)
-                        acc[n] = mfma16(a[s], Bs[buf][kslot<CT>(kq, s)][n * 16 + m], acc[n]);
+                    for (int n = 0; n < NT; n++) {
+                        const __bf16 *col = Bsp + ((2 * buf) * W + n * 16 + m) * CSTR + 8 * kq;
+                        const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(col);
+                        const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(col + W * CSTR);
+                        acc[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[n], 0, 0, 0);
+                        acc[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[n], 0, 0, 0);
+                        acc[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[n], 0, 0, 0);
+                    }
+                } else {
+#pragma unroll
+                    for (int s = 0; s < G; s++) {
+#pragma unroll
+                        for (int n = 0; n < NT; n++)
+                            acc[n] = mfma16(a[s], Bs[buf][kslot<CT>(kq, s)][n * 16 + m], acc[n]);
+                    }
                 }
             };
             CT b0[BPT], b1[BPT];
@@ -374,7 +482,7 @@ __device__ __forceinline__ void ladder_run(const DTile *__restrict__ tiles, int 
                     const int row = wave * 16 + Mma<CT>::row(q, lane), col = n * 16 + m;
                     if (row < tl.nrows && col < H) {
                         CT val = acc[n][q];
-                        if (ONE)
+                        if (FOLD)
                             val = flip_sign(val, *reinterpret_cast<const uint32_t *>(lg.rec2 + ch.pr_off + tl.row0 + row));
                         dp[ch.off_dp + (int64_t)seg * npk * H + (int64_t)(tl.row0 + row) * H + col] = val;
                     }
@@ -386,55 +494,6 @@ __device__ __forceinline__ void ladder_run(const DTile *__restrict__ tiles, int 
         else if (SRC == LV_GAS) tile(std::integral_constant<int, 2>{});
         else tile(std::integral_constant<int, 0>{});
     }
-}
-
-// The gas table into LDS, once per block, in the arithmetic type (FP32: every
-// entry rounded, the values of the FP32 element)
-template <typename CT, int SRC>
-__device__ __forceinline__ void stage_gas_table(const LadderGen &lg) {
-    extern __shared__ double gtab[];
-    if (SRC == LV_GAS) {
-        CT *t = reinterpret_cast<CT *>(gtab);
-        for (int i = threadIdx.x & (kThreads - 1); i < lg.ntab; i += kThreads) t[i] = (CT)lg.tab[i];
-        __syncthreads();
-    }
-}
-
-// One class: block b runs the class's tiles bptr[b] .. bptr[b+1]-1.
-template <typename CT, int NT, int SRC>
-__global__ __launch_bounds__(kThreads) void k_ladder_v(const DTile *__restrict__ tiles,
-                                                       const int32_t *__restrict__ bptr,
-                                                       const DChan *__restrict__ chans,
-                                                       const CT *__restrict__ vp, LadderGen lg,
-                                                       const CT *__restrict__ tas,
-                                                       const double *__restrict__ zero,
-                                                       CT *__restrict__ dp,
-                                                       const LoopCtl *__restrict__ ctl) {
-    if (ctl->done) return;
-    stage_gas_table<CT, SRC>(lg);
-    ladder_run<CT, NT, SRC>(tiles, bptr[blockIdx.x], bptr[blockIdx.x + 1], chans, vp, lg, tas, zero, dp);
-}
-
-// Two classes in one launch, so that the few long blocks of both run side by
-// side: blocks 0 .. g1-1 run class NT1 (tiles1, bptr1), the others class NT2.
-template <typename CT, int NT1, int NT2, int SRC>
-__global__ __launch_bounds__(kThreads) void k_ladder_v2(const DTile *__restrict__ tiles1,
-                                                        const int32_t *__restrict__ bptr1, int g1,
-                                                        const DTile *__restrict__ tiles2,
-                                                        const int32_t *__restrict__ bptr2,
-                                                        const DChan *__restrict__ chans,
-                                                        const CT *__restrict__ vp, LadderGen lg,
-                                                        const CT *__restrict__ tas,
-                                                        const double *__restrict__ zero,
-                                                        CT *__restrict__ dp,
-                                                        const LoopCtl *__restrict__ ctl) {
-    if (ctl->done) return;
-    stage_gas_table<CT, SRC>(lg);
-    const int b = blockIdx.x;
-    if (b < g1)
-        ladder_run<CT, NT1, SRC>(tiles1, bptr1[b], bptr1[b + 1], chans, vp, lg, tas, zero, dp);
-    else
-        ladder_run<CT, NT2, SRC>(tiles2, bptr2[b - g1], bptr2[b - g1 + 1], chans, vp, lg, tas, zero, dp);
 }
 
 // ---- FP32 pppp ladder, channels with hh <= 16 NTW: one packed row per lane
@@ -456,7 +515,7 @@ __global__ __launch_bounds__(kThreads) void k_ladder_v2(const DTile *__restrict_
 // element is the distance of the records' sign fields (0 or the byte size
 // of T), so the element comes signed from one read.
 //
-// A block runs the same tiles as k_ladder_v (64 rows x one k segment); its
+// A block runs the same tiles as ladder_run (64 rows x one k segment); its
 // four waves take the tile's k chunks in turn (wave w: chunks w, w + 4, ...).
 // Each wave stages its chunks' B rows (transposed, column-major: a lane reads
 // 4 consecutive k of its column in one 16-byte read) in a double-buffered
@@ -481,13 +540,9 @@ struct WaveLadder {
     static constexpr int NACC = NTW == 1 ? 2 : 1;
     static constexpr int WREG = 2 * BUF;                // floats of a wave's region
     static constexpr int RED = 3 * NG * 4 * 64;         // floats of the partial tiles of waves 1-3
-    static constexpr int REGION = 4 * WREG > RED ? 4 * WREG : RED;
-    // bytes of dynamic LDS: the FP32 gas table [T, -T], then the wave
+    // floats of the block's region after the gas table: the four wave
     // regions, which the partial tiles reuse at the end of a tile
-    static __host__ __device__ constexpr size_t tab_bytes(int ntab) { return ((size_t)8 * ntab + 15) / 16 * 16; }
-    static __host__ __device__ constexpr size_t lds_bytes(int ntab) {
-        return tab_bytes(ntab) + (size_t)REGION * sizeof(float);
-    }
+    static constexpr int REGION = 4 * WREG > RED ? 4 * WREG : RED;
 };
 
 typedef __attribute__((address_space(4))) const int const_int;
@@ -671,42 +726,99 @@ __device__ __forceinline__ void wave_tile(const DTile &tl, const DChan &ch, cons
     __syncthreads();   // the regions are reused by the next tile
 }
 
-// Block b runs the tiles bptr[b] .. bptr[b+1]-1 (the tiles and block runs of
-// k_ladder_v for the same class).
-template <int NTW, int SRC>
-__global__ __launch_bounds__(kThreads) void k_ladder_w(const DTile *__restrict__ tiles,
-                                                       const int32_t *__restrict__ bptr,
-                                                       const DChan *__restrict__ chans,
-                                                       const float *__restrict__ vp, LadderGen lg,
-                                                       const float *__restrict__ tas, float *__restrict__ dp,
-                                                       const LoopCtl *__restrict__ ctl) {
+// ---- the grouped pppp-ladder launch
+//
+// One persistent launch covers the classes LO..HI (channels with hh in
+// (16 (n-1), 16 n] for class n): the plan gives each class a run of the
+// launch's blocks, its kernel kind and its tile runs.  Block b belongs to
+// class n for L.blk0[n] <= b < L.blk0[n+1] and runs the class's tiles
+// bptr[L.ptr0[n] + r] .. bptr[L.ptr0[n] + r + 1] - 1 (r = b - L.blk0[n],
+// tiles counted from L.tile0[n]).  The gas table is staged into LDS once per
+// block; the B tiles of the block's class follow it in the same dynamic LDS,
+// sized on the host for the largest class and kind of the launch.
+struct LadderLaunch {
+    int32_t tile0[NTMAX + 1];
+    int32_t ptr0[NTMAX + 1];
+    int32_t blk0[NTMAX + 2];
+    int32_t kind[NTMAX + 1];
+};
+
+// Bytes of the B-tile LDS of class NT under kernel kind `kind` (CT: the
+// iterations' arithmetic type).
+template <typename CT, int NT>
+__host__ __device__ constexpr size_t ladder_class_bytes(int kind) {
+    if (sizeof(CT) == 4 && kind == LK_BF16X3) return LadderTile<float, NT, true>::BYTES;
+    if (sizeof(CT) == 4 && NT == 1 && kind == LK_DEFAULT) return (size_t)WaveLadder<1>::REGION * sizeof(float);
+    return LadderTile<CT, NT, false>::BYTES;
+}
+
+template <typename CT, int NT, int SRC>
+__device__ __forceinline__ void ladder_class(int kind, const DTile *__restrict__ tiles, int b0, int b1,
+                                             const DChan *__restrict__ chans, const CT *__restrict__ vp,
+                                             const LadderGen &lg, const CT *__restrict__ tas,
+                                             const double *__restrict__ zero, CT *__restrict__ dp,
+                                             unsigned char *lds_b, uint32_t tbase) {
+    if constexpr (std::is_same<CT, float>::value) {
+        if (kind == LK_BF16X3) {
+            ladder_run<float, NT, SRC, true>(tiles, b0, b1, chans, vp, lg, tas, zero, dp, lds_b, tbase);
+            return;
+        }
+        if constexpr (NT == 1) {
+            if (kind == LK_DEFAULT) {
+                const int tid = threadIdx.x & (kThreads - 1);
+                const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+                for (int ti = b0; ti < b1; ti++) {
+                    const DTile tl = tiles[ti];
+                    const DChan ch = chans[tl.chan];
+                    float *region = reinterpret_cast<float *>(lds_b);
+                    if (SRC == LV_GAS && ch.spin0 == 1)
+                        wave_tile<1, SRC, 1>(tl, ch, vp, lg, tas, dp, region, tbase, wave, lane);
+                    else if (SRC == LV_GAS)
+                        wave_tile<1, SRC, 2>(tl, ch, vp, lg, tas, dp, region, tbase, wave, lane);
+                    else
+                        wave_tile<1, SRC, 0>(tl, ch, vp, lg, tas, dp, region, tbase, wave, lane);
+                }
+                return;
+            }
+        }
+    }
+    ladder_run<CT, NT, SRC, false>(tiles, b0, b1, chans, vp, lg, tas, zero, dp, lds_b, tbase);
+}
+
+template <typename CT, int SRC, int LO, int HI>
+__global__ __launch_bounds__(kThreads) void k_ladder_grp(LadderLaunch L, const DTile *__restrict__ tiles,
+                                                         const int32_t *__restrict__ bptr,
+                                                         const DChan *__restrict__ chans,
+                                                         const CT *__restrict__ vp, LadderGen lg,
+                                                         const CT *__restrict__ tas,
+                                                         const double *__restrict__ zero, CT *__restrict__ dp,
+                                                         const LoopCtl *__restrict__ ctl) {
     if (ctl->done) return;
-    typedef WaveLadder<NTW> S;
-    extern __shared__ double wl_lds[];
+    extern __shared__ double lg_lds[];
     const int tid = threadIdx.x & (kThreads - 1);
     const int ntab = SRC == LV_GAS ? lg.ntab : 0;
-    float *t32 = reinterpret_cast<float *>(wl_lds);
-    if (SRC == LV_GAS)
+    if (SRC == LV_GAS) {   // FP64: T; FP32: [T, -T], every entry rounded
+        CT *t = reinterpret_cast<CT *>(lg_lds);
         for (int i = tid; i < ntab; i += kThreads) {
-            const float v = (float)lg.tab[i];
-            t32[i] = v;
-            t32[ntab + i] = -v;
+            const CT v = (CT)lg.tab[i];
+            t[i] = v;
+            if (sizeof(CT) == 4) t[ntab + i] = -v;
         }
-    __syncthreads();
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
-    float *region = reinterpret_cast<float *>(reinterpret_cast<unsigned char *>(wl_lds) + S::tab_bytes(ntab));
-    const uint32_t tb32 = (uint32_t)(uintptr_t)(lds_float *)t32;
-    const int t1 = bptr[blockIdx.x + 1];
-    for (int ti = bptr[blockIdx.x]; ti < t1; ti++) {
-        const DTile tl = tiles[ti];
-        const DChan ch = chans[tl.chan];
-        if (SRC == LV_GAS && ch.spin0 == 1)
-            wave_tile<NTW, SRC, 1>(tl, ch, vp, lg, tas, dp, region, tb32, wave, lane);
-        else if (SRC == LV_GAS)
-            wave_tile<NTW, SRC, 2>(tl, ch, vp, lg, tas, dp, region, tb32, wave, lane);
-        else
-            wave_tile<NTW, SRC, 0>(tl, ch, vp, lg, tas, dp, region, tb32, wave, lane);
+        __syncthreads();
     }
+    const uint32_t tbase = (uint32_t)(uintptr_t)(lds_double *)lg_lds;   // LDS byte address
+    unsigned char *lds_b = reinterpret_cast<unsigned char *>(lg_lds) + ladder_tab_bytes(ntab);
+    const int b = blockIdx.x;
+    int nt = LO;
+    while (nt < HI && b >= L.blk0[nt + 1]) nt++;   // uniform
+    const int r = b - L.blk0[nt];
+    const DTile *tl = tiles + L.tile0[nt];
+    const int b0 = bptr[L.ptr0[nt] + r], b1 = bptr[L.ptr0[nt] + r + 1];
+    const int kind = L.kind[nt];
+    static_for<LO, HI + 1>([&](auto N) {
+        constexpr int n = decltype(N)::value;
+        if (nt == n) ladder_class<CT, n, SRC>(kind, tl, b0, b1, chans, vp, lg, tas, zero, dp, lds_b, tbase);
+    });
 }
 
 // 1/2 tas X, the hhhh quadratic term of D (calc.cpp:171-187 with X from

@@ -1,0 +1,8 @@
+// Grouped pppp-ladder instances: FP32 iterations, generated electron gas.
+#include "ladder_inst.hpp"
+
+void launch_ladder_grp_f32_gas(int g, unsigned grid, size_t lds, hipStream_t st, const LadderLaunch &L,
+                               const LadderOps<float> &o) {
+    ladder_inst::launch_src<float, LV_GAS>(g, grid, lds, st, L, o);
+}
+int allow_ladder_lds_f32_gas() { return (int)ladder_inst::allow_src<float, LV_GAS>(); }

@@ -151,6 +151,61 @@ int ccd_set_schedule(ccd_ctx *ctx, int sched);
  * of schedules 1..3 in us_per_iter[0..2]. */
 int ccd_get_schedule(const ccd_ctx *ctx, int *sched, double *us_per_iter);
 
+/* The launch plan of the pppp ladder (the ladder of calcChannelT2Diff_pphh,
+ * src/calc.cpp:188-201; its counterpart in the reference is the static
+ * channel distribution of src/balance.cpp:91-144 and src/main.cpp:94-115).
+ * No counterpart in the reference.  Channels are bucketed by size into 8
+ * classes: class n holds the channels with 16 (n-1) < hh <= 16 n.  Per
+ * class the plan fixes the kernel (kind, per arithmetic of the iterations),
+ * how many persistent blocks per compute unit run its tiles, and how its
+ * tiles are dealt to them; `partition` fixes which classes share one grouped
+ * launch.  The block count, the assignment and the partition change no
+ * result (tiles write disjoint outputs); the kernel kind does (another
+ * summation order, or split products). */
+#define CCD_LADDER_CLASSES 8
+#define CCD_LK_DEFAULT 0     /* FP64: v_mfma_f64_16x16x4f64; FP32: the one-row-per-lane
+                                v_mfma_f32_4x4x1f32 kernel for class 1, v_mfma_f32_16x16x4f32
+                                for the others */
+#define CCD_LK_MFMA16 1      /* v_mfma_{f64,f32}_16x16x4 in every class */
+#define CCD_LK_BF16X3 2      /* FP32 iterations only: BF16x3 split products (x = x_hi + x_lo
+                                in bfloat16, a b ~ a_lo b_hi + a_hi b_lo + a_hi b_hi on
+                                v_mfma_f32_16x16x32_bf16, FP32 accumulation; about 2^-16
+                                relative per product against 2^-24 for FP32) */
+#define CCD_LA_MODEL 0       /* tile assignment: the one a model of the CU loads prefers */
+#define CCD_LA_CHANNEL 1     /*   round robin in channel order */
+#define CCD_LA_LONGEST 2     /*   longest tile first to the least loaded block */
+#define CCD_LP_SPLIT 0       /* launches: classes 1, 2, 3-4 and 5-8 (in the four-stream
+                                schedule classes 1-2 and 3-8 run on two streams) */
+#define CCD_LP_ONE 1         /* one grouped launch of every class */
+typedef struct ccd_ladder_plan {
+    int32_t kind[2][CCD_LADDER_CLASSES];   /* [CCD_PREC_FP64 / CCD_PREC_FP32][class - 1]: CCD_LK_* */
+    int32_t blocks_per_cu[CCD_LADDER_CLASSES];  /* 0: one block per tile; k > 0: k blocks
+                                                   per compute unit, each running a run of tiles */
+    int32_t assign[CCD_LADDER_CLASSES];    /* CCD_LA_* */
+    int32_t partition;                     /* CCD_LP_* */
+    double us[2][CCD_LADDER_CLASSES];      /* set by ccd_tune_ladder: microseconds per launch of
+                                              each class as chosen (0: not measured) */
+    double us_partition[2];                /* ccd_tune_ladder: microseconds per iteration of
+                                              CCD_LP_SPLIT and CCD_LP_ONE (0: not measured) */
+} ccd_ladder_plan;
+/* The plan in use.  A context starts with CCD_LK_DEFAULT everywhere, 3
+ * blocks per compute unit (generated V) or one block per tile (stored V),
+ * CCD_LA_MODEL and CCD_LP_SPLIT. */
+int ccd_get_ladder_plan(const ccd_ctx *ctx, ccd_ladder_plan *plan);
+/* Use `plan` from the next iteration on (its us fields are ignored).  Fails
+ * with CCD_E_INVALID on an unknown kind, assignment or partition, a
+ * CCD_LK_BF16X3 entry for FP64 iterations, or a block count outside 0..16. */
+int ccd_set_ladder_plan(ccd_ctx *ctx, const ccd_ladder_plan *plan);
+/* Tune the plan on this problem, for the arithmetic of the following
+ * iterations: per class, the kernel kinds that `flags` allow, then the
+ * blocks per compute unit (1, 2, 3, 4, 6; stored V also one block per tile),
+ * then the assignment, each timed with HIP events over repeated launches of
+ * that class alone; then the partition, timed over whole iterations in the
+ * context's schedule.  Resets T2 to zero. */
+#define CCD_TUNE_KINDS 1     /* also try CCD_LK_MFMA16 where it differs from the default */
+#define CCD_TUNE_BF16X3 2    /* also try CCD_LK_BF16X3 (FP32 iterations) */
+int ccd_tune_ladder(ccd_ctx *ctx, int flags);
+
 /* MBPT2 correlation energy, 1/4 sum v^2/(e_i+e_j-e_a-e_b)
  * (replaces MBPT2Corr, src/MBPT2Corr.cpp:6-38). */
 int ccd_mbpt2(ccd_ctx *ctx, double *corr);
