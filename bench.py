@@ -22,6 +22,12 @@ matrix-free mode (-m 1): V^pppp is rebuilt inside the pppp ladder every
 iteration instead of being read from HBM; the iterates are bit-identical to
 --mode stored (the default for the pairing model and the sweep).
 
+Before timing, every context tunes its iteration schedule and its pppp-ladder
+plan on the workload (ccd_set_schedule(AUTO), ccd_tune_ladder): FP32
+iterations may then run some ladder classes on BF16x3 split products where
+those are faster (--no-ladder-tune keeps the default plan); the plan and an
+A/B of the kernel kinds per class are reported.
+
 Per-kernel figures come from ccd_time_kernels (HIP events around each launch
 group, the groups in order on the context's stream) and the algorithmic work
 of each group (ccd_kernel_work); `roofline` is the group with the largest
@@ -77,8 +83,26 @@ KERNEL_OF_GROUP = {
     "update": "k_update<true,{ct}>", "remap_t": "k_transpose<double,{ct}> (T to T^ph)",
     "reduce": "k_reduce",
 }
-# FP32 iterations run class 1 of the pppp ladder on the one-row-per-lane kernel
-KERNEL_OF_GROUP_F32 = {"ladder_v1": "k_ladder_w<1,{src}>"}
+# the size classes of the pppp ladder each launch group times
+CLASSES_OF_GROUP = {"ladder_v1": (1,), "ladder_v2": (2,), "ladder_v34": (3, 4), "ladder_v58": (5, 6, 7, 8)}
+KIND_LABEL = {"default": "{dflt}", "mfma16": "16x16x4 MFMA", "bf16x3": "BF16x3 split products, 16x16x32 BF16 MFMA"}
+BF16_PEAK_TFLOPS = 2500.0   # dense BF16 MFMA; a BF16x3 product costs three
+
+
+def ladder_label(g, ct, src, plan):
+    """k_ladder_grp instance and kernel kind of a ladder launch group."""
+    classes = CLASSES_OF_GROUP[g]
+    kinds = plan["kind_fp32" if ct == "float" else "kind_fp64"]
+    parts = []
+    for n in classes:
+        dflt = "one-row-per-lane 4x4x1 MFMA" if (ct == "float" and n == 1) else "16x16x4 MFMA"
+        parts.append(f"class {n}: " + KIND_LABEL[kinds[n - 1]].format(dflt=dflt))
+    return f"k_ladder_grp<{ct},{src}> (" + "; ".join(parts) + ")"
+
+
+def group_kind(g, ct, plan):
+    kinds = {plan["kind_fp32" if ct == "float" else "kind_fp64"][n - 1] for n in CLASSES_OF_GROUP[g]}
+    return kinds.pop() if len(kinds) == 1 else "mixed"
 
 
 def make_model(cfg):
@@ -156,15 +180,17 @@ def kernel_report(ctx, iters, ct, src):
     arithmetic; and the sum of the groups' times."""
     t = ctx.time_kernels(iters, REPS)
     w = ctx.kernel_work()
+    plan = ctx.ladder_plan()
     peak_tf = FP64_PEAK_TFLOPS if ct == "double" else FP32_PEAK_TFLOPS
     out = {}
     for g, ms in t.items():
         if g == "iteration":
             continue
         sec = ms * 1e-3
-        lab = (KERNEL_OF_GROUP_F32 if ct == "float" else {}).get(g, KERNEL_OF_GROUP[g])
-        e = {"kernel": lab.format(ct=ct, src=src), "ms": ms,
-             "flops": w[g]["flops"], "bytes": w[g]["bytes"]}
+        lab = ladder_label(g, ct, src, plan) if g in CLASSES_OF_GROUP else KERNEL_OF_GROUP[g].format(ct=ct, src=src)
+        e = {"kernel": lab, "ms": ms, "flops": w[g]["flops"], "bytes": w[g]["bytes"]}
+        if g in CLASSES_OF_GROUP:
+            e["kind"] = group_kind(g, ct, plan)
         if sec > 0:
             e["tflops"] = w[g]["flops"] / sec / 1e12
             e["gbs"] = w[g]["bytes"] / sec / 1e9
@@ -199,6 +225,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solves", action="store_true", help="skip the full solves (profiling runs)")
     ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--no-ladder-tune", action="store_true",
+                    help="keep the default pppp-ladder plan instead of tuning it on the workload "
+                         "(FP32 iterations may then use BF16x3 split products where they are faster)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (a kernel of known traffic, "
                          "to calibrate hardware byte counters)")
@@ -208,7 +237,15 @@ def main():
               "MI355X (multi-GPU channel sharding is a design note, not built).", file=sys.stderr)
         sys.exit(2)
 
-    from coupled_cluster_doubles_amd import FP32, FP64, Context
+    from coupled_cluster_doubles_amd import FP32, FP64, TUNE_BF16X3, TUNE_KINDS, Context
+
+    def tune(c, p):
+        """The schedule and the ladder plan tuned on the context's problem (the
+        ladder in the arithmetic p; FP32 may pick other kernel kinds)."""
+        c.set_schedule("auto")
+        if not args.no_ladder_tune:
+            c.tune_ladder(TUNE_KINDS | TUNE_BF16X3 if p == FP32 else 0)
+            c.set_schedule("auto")
 
     if args.copy_probe_mib > 0:
         import torch
@@ -235,7 +272,10 @@ def main():
     t_setup = time.perf_counter()
     ctx = make_context(cfg, prec, free, models)
     setup_s = time.perf_counter() - t_setup
-    ctx.set_schedule("auto")      # time the iteration schedules on this problem, keep the fastest
+    t_tune = time.perf_counter()
+    tune(ctx, prec)               # time the schedules and ladder plans on this problem, keep the fastest
+    tune_s = time.perf_counter() - t_tune
+    plan = ctx.ladder_plan()
     mbpt2 = ctx.mbpt2()
     work = ctx.work()
     sched, sched_us = ctx.schedule()
@@ -258,6 +298,17 @@ def main():
         ctx.set_precision(FP64)
         kern64, iter64_ms_serial = kernel_report(ctx, n_timed, "double", src)
         ctx.set_precision(FP32)
+    # A/B of the FP32 ladder kernel kinds: every class on one kind, the other
+    # plan fields as tuned, single stream, microseconds per launch of each group
+    kind_ab = None
+    if prec == FP32:
+        kind_ab = {}
+        for kind in ("default", "mfma16", "bf16x3"):
+            q = dict(plan, kind_fp32=[kind] * 8)
+            ctx.set_ladder_plan(q)
+            tk = ctx.time_kernels(n_timed, REPS)
+            kind_ab[kind] = {g: tk[g] * 1e3 for g in CLASSES_OF_GROUP}
+        ctx.set_ladder_plan(plan)
 
     # executed flops of an iteration: the products on packed pairs
     executed_flops = sum(v["flops"] for v in kern.values())
@@ -275,9 +326,13 @@ def main():
     def roof_of(g, rep, peak_tf):
         r = rep[g]
         if g.startswith("ladder_v") and free:
+            if r.get("kind") == "bf16x3":   # three BF16 products per FP32 product
+                peak_tf = BF16_PEAK_TFLOPS / 3
             d = {"bound": "mfma", "kernel": r["kernel"], "achieved": r["tflops"], "peak": peak_tf,
                  "unit": "TFLOP/s", "frac": r["tflops"] / peak_tf, "algorithmic_flops": r["flops"],
                  "algorithmic_bytes": r["bytes"], "launch_ms": r["ms"]}
+            if r.get("kind") == "bf16x3":
+                d["peak_note"] = "dense BF16 MFMA peak / 3 (each product is three BF16 products)"
         elif g in ("ring", "ladder_x", "ladder_tx") or g.startswith("ladder_v"):
             d = {"bound": "mfma" if g == "ring" else "hbm", "kernel": r["kernel"], "launch_ms": r["ms"],
                  "algorithmic_flops": r["flops"], "algorithmic_bytes": r["bytes"]}
@@ -345,11 +400,11 @@ def main():
 
         (e_m, stop_m, n_m, n32), t_m = timed(lambda: ctx.solve_mixed(1e-10, args.switch_tol))
         ctx64 = Context(model, FP64, matrix_free=free)
-        ctx64.set_schedule("auto")
+        tune(ctx64, FP64)
         (e_d, stop_d, n_d), t_d = timed(lambda: ctx64.solve(1e-10))
         ctx64.close()
         ctx_st = Context(model, FP64, matrix_free=False)     # the plain FP64 GPU path: V^pppp stored
-        ctx_st.set_schedule("auto")
+        tune(ctx_st, FP64)
         (e_s, stop_s, n_s), t_s = timed(lambda: ctx_st.solve(1e-10))
         ctx_st.close()
         mixed = {"tol": 1e-10, "switch_tol": args.switch_tol,
@@ -385,7 +440,7 @@ def main():
         t_m = time.perf_counter() - t0
         ctx.close()
         ctx64 = make_context(cfg, FP64, free, models)
-        ctx64.set_schedule("auto")
+        tune(ctx64, FP64)
         t0 = time.perf_counter()
         rd = ctx64.solve(1e-10)
         t_d = time.perf_counter() - t0
@@ -430,7 +485,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == FP64 else "f32+f64",
+        "dtype": "f64" if prec == FP64 else ("f32+bf16x3+f64" if "bf16x3" in plan["kind_fp32"] else "f32+f64"),
         "mixed_vs_fp64": mixed,
         "data": "synthetic: the reference's closed-form basis and Coulomb interaction, T2 from zero",
         "config": {"workload": cfg["name"], "precision": args.precision, "mode": args.mode,
@@ -442,6 +497,9 @@ def main():
         "kernels": kern,
         "setup_s": setup_s,
         "schedule": {"chosen": sched, "tuner_us_per_iter": sched_us},
+        "ladder_plan": plan,
+        "ladder_kind_ab_us": kind_ab,
+        "tune_s": tune_s,
         "roofline": roof,
         "roofline_ladder_phase": phase,
         "fp64": fp64,

@@ -1954,7 +1954,7 @@ int time_class(ccd_ctx *c, int nt, int reps, double *us) {
 // Microseconds per iteration of the context's schedule (a free run: one
 // untimed batch, then kTimed timed ones).
 int time_iterations(ccd_ctx *c, double *us) {
-    constexpr int kTimed = 2;
+    constexpr int kTimed = 4;
     hipEvent_t a = nullptr, b = nullptr;
     HIP_TRY(hipEventCreate(&a));
     if (hipEventCreate(&b) != hipSuccess) {
@@ -1994,21 +1994,30 @@ int plan_valid(const ccd_ladder_plan &p) {
     return CCD_OK;
 }
 
-// The ladder tuner (ccd_tune_ladder): per class with tiles, a coordinate
+// The ladder tuner (ccd_tune_ladder).  Per class with tiles, a coordinate
 // search -- kernel kind, then blocks per compute unit, then assignment --
-// each candidate timed over repeated launches of the class alone; then the
-// partition over whole iterations.  The amplitudes of two free iterations
-// give the kernels realistic operands; T2 goes back to zero at the end.
+// each candidate timed over repeated launches of the class alone.  Classes
+// timed alone do not show how they share the GPU with each other and with
+// the ring and the remaps in the four-stream schedule, so the final choice
+// is made on whole iterations in the context's schedule, among: the plan
+// the tuner started from, the per-class result, the starting plan with each
+// other allowed kind in every class, each under both partitions.  The
+// amplitudes of two free iterations give the kernels realistic operands;
+// T2 goes back to zero at the end.
 int tune_ladder(ccd_ctx *c, int flags) {
     constexpr int kReps = 5;
     const int prec = c->iter_prec;
+    const ccd_ladder_plan start = c->plan;
     ccd_ladder_plan &p = c->plan;
+    std::vector<int> kinds{CCD_LK_DEFAULT};
+    if (prec == CCD_PREC_FP32 && (flags & CCD_TUNE_KINDS)) kinds.push_back(CCD_LK_MFMA16);
+    if (prec == CCD_PREC_FP32 && (flags & CCD_TUNE_BF16X3)) kinds.push_back(CCD_LK_BF16X3);
     int rc = drop_graphs(c);
     if (!rc) rc = arm_free_run(c);
     if (!rc) rc = run_n(c, 0.5, 2);
     if (!rc) HIP_TRY(hipStreamSynchronize(c->stream));
+    double us_class[NTMAX] = {};
     for (int nt = 1; nt <= NTMAX && !rc; nt++) {
-        p.us[prec][nt - 1] = 0.0;
         if (c->lt_off[nt] == c->lt_off[nt - 1]) continue;
         double best = 0.0;
         auto consider = [&](int32_t &field, const std::vector<int> &vals) -> int {
@@ -2029,21 +2038,53 @@ int tune_ladder(ccd_ctx *c, int flags) {
             plan_class_tiles(c, nt);
             return upload_ladder_tiles(c);
         };
-        std::vector<int> kinds{CCD_LK_DEFAULT};
-        if (prec == CCD_PREC_FP32 && nt == 1 && (flags & CCD_TUNE_KINDS)) kinds.push_back(CCD_LK_MFMA16);
-        if (prec == CCD_PREC_FP32 && (flags & CCD_TUNE_BF16X3)) kinds.push_back(CCD_LK_BF16X3);
+        std::vector<int> ck{CCD_LK_DEFAULT};   // class 1's 16x16x4 kernel differs from its default
+        for (int k : kinds)
+            if (k == CCD_LK_BF16X3 || (k == CCD_LK_MFMA16 && nt == 1)) ck.push_back(k);
         std::vector<int> bpcs{1, 2, 3, 4, 6};
         if (!c->matrix_free) bpcs.insert(bpcs.begin(), 0);
-        if ((rc = consider(p.kind[prec][nt - 1], kinds)) || (rc = consider(p.blocks_per_cu[nt - 1], bpcs)) ||
+        if ((rc = consider(p.kind[prec][nt - 1], ck)) || (rc = consider(p.blocks_per_cu[nt - 1], bpcs)) ||
             (rc = consider(p.assign[nt - 1], {CCD_LA_CHANNEL, CCD_LA_LONGEST})))
             break;
-        p.us[prec][nt - 1] = best;
+        us_class[nt - 1] = best;
     }
-    for (int q = 0; q < 2 && !rc; q++) {
-        p.partition = q == 0 ? CCD_LP_SPLIT : CCD_LP_ONE;
-        if (!(rc = drop_graphs(c))) rc = time_iterations(c, &p.us_partition[q]);
+    // whole plans, whole iterations
+    std::vector<ccd_ladder_plan> cands{start, p};
+    for (int k : kinds) {
+        if (k == CCD_LK_DEFAULT) continue;
+        ccd_ladder_plan q = start;
+        for (int n = 0; n < NTMAX; n++) q.kind[prec][n] = k;
+        cands.push_back(q);
     }
-    if (!rc) p.partition = p.us_partition[1] < p.us_partition[0] ? CCD_LP_ONE : CCD_LP_SPLIT;
+    double best = 0.0;
+    ccd_ladder_plan pick = start;
+    for (size_t q = 0; q < cands.size() && !rc; q++)
+        for (int part = CCD_LP_SPLIT; part <= CCD_LP_ONE && !rc; part++) {
+            p = cands[q];
+            p.partition = part;
+            plan_ladder_tiles(c);
+            double us = 0.0;
+            if ((rc = upload_ladder_tiles(c)) || (rc = drop_graphs(c)) || (rc = time_iterations(c, &us))) break;
+            cands[q].us_partition[part] = us;
+            if (best == 0.0 || us < best) {
+                best = us;
+                pick = cands[q];
+                pick.partition = part;
+            }
+        }
+    if (!rc) {
+        p = pick;
+        for (int n = 0; n < NTMAX; n++) p.us[prec][n] = us_class[n];
+        for (size_t q = 0; q < cands.size(); q++)   // the partitions' timings of the plan chosen
+            if (std::memcmp(cands[q].kind, pick.kind, sizeof pick.kind) == 0 &&
+                std::memcmp(cands[q].blocks_per_cu, pick.blocks_per_cu, sizeof pick.blocks_per_cu) == 0 &&
+                std::memcmp(cands[q].assign, pick.assign, sizeof pick.assign) == 0) {
+                p.us_partition[0] = cands[q].us_partition[0];
+                p.us_partition[1] = cands[q].us_partition[1];
+            }
+        plan_ladder_tiles(c);
+        rc = upload_ladder_tiles(c);
+    }
     if (!rc) rc = drop_graphs(c);
     if (!rc) rc = zero_state(c);
     return rc;

@@ -197,11 +197,13 @@ int ccd_get_ladder_plan(const ccd_ctx *ctx, ccd_ladder_plan *plan);
  * CCD_LK_BF16X3 entry for FP64 iterations, or a block count outside 0..16. */
 int ccd_set_ladder_plan(ccd_ctx *ctx, const ccd_ladder_plan *plan);
 /* Tune the plan on this problem, for the arithmetic of the following
- * iterations: per class, the kernel kinds that `flags` allow, then the
+ * iterations.  Per class: the kernel kinds that `flags` allow, then the
  * blocks per compute unit (1, 2, 3, 4, 6; stored V also one block per tile),
  * then the assignment, each timed with HIP events over repeated launches of
- * that class alone; then the partition, timed over whole iterations in the
- * context's schedule.  Resets T2 to zero. */
+ * that class alone (us).  Then whole iterations in the context's schedule
+ * decide among the plan in use, that per-class result and the plan in use
+ * with each allowed kind in every class, each under both partitions
+ * (us_partition: the chosen plan's).  Resets T2 to zero. */
 #define CCD_TUNE_KINDS 1     /* also try CCD_LK_MFMA16 where it differs from the default */
 #define CCD_TUNE_BF16X3 2    /* also try CCD_LK_BF16X3 (FP32 iterations) */
 int ccd_tune_ladder(ccd_ctx *ctx, int flags);
