@@ -241,10 +241,15 @@ def main():
 
     def tune(c, p):
         """The schedule and the ladder plan tuned on the context's problem (the
-        ladder in the arithmetic p; FP32 may pick other kernel kinds)."""
+        ladder in the arithmetic p; FP32 may pick other kernel kinds).  The
+        sweep does not offer BF16x3 products to the tuner: its slowest
+        near-divergent solves contract so slowly that the BF16x3 rounding
+        (about 2^-16 per product) can hold |dE| above the switch tolerance
+        in the FP32 phase."""
         c.set_schedule("auto")
         if not args.no_ladder_tune:
-            c.tune_ladder(TUNE_KINDS | TUNE_BF16X3 if p == FP32 else 0)
+            flags = 0 if p == FP64 else TUNE_KINDS if sweep else TUNE_KINDS | TUNE_BF16X3
+            c.tune_ladder(flags)
             c.set_schedule("auto")
 
     if args.copy_probe_mib > 0:
@@ -445,7 +450,11 @@ def main():
         rd = ctx64.solve(1e-10)
         t_d = time.perf_counter() - t0
         conv = [q for q in range(len(models)) if rd[q][1] == "converged"]
+        cfgs = __import__("coupled_cluster_doubles_amd.sweep", fromlist=["sweep_configs"]).sweep_configs()
         mixed = {"tol": 1e-10, "switch_tol": args.switch_tol,
+                 "fp64_iters_per_solve": [r[2] for r in rd],
+                 "not_converged": [{"solve": q, "config": list(cfgs[q]), "fp64": rd[q][1], "iters": rd[q][2]}
+                                   for q in range(len(models)) if rd[q][1] != "converged"],
                  "fp64_stops": {k: sum(r[1] == k for r in rd) for k in ("converged", "max_iterations", "nan")},
                  "mixed_converged_where_fp64_did": all(rm[q][1] == "converged" for q in conv),
                  "max_abs_dE_per_particle": max(abs(float(rm[q][0][-1]) - float(rd[q][0][-1]))

@@ -39,7 +39,8 @@ def _run(cmd):
 # HIP translation units: the context and the small kernels, and the grouped
 # pppp-ladder instances split by arithmetic type and V source (the largest
 # part of the build), compiled side by side.
-HIP_UNITS = ["ccd_device", "ladder_f32_gas", "ladder_f32_other", "ladder_f64_gas", "ladder_f64_other"]
+HIP_UNITS = ["ccd_device", "ladder_f32_gas", "ladder_f32_other", "ladder_f64_gas", "ladder_f64_other",
+             "ladder_f32_gasb", "ladder_f64_gasb"]
 
 
 def _run_all(cmds):

@@ -209,6 +209,8 @@ struct ccd_ctx {
     std::vector<int64_t> t2_lo;  // its amplitudes in the pphh layout: [t2_lo[p], t2_lo[p+1])
     std::vector<int32_t> ulo;    // its update tiles: [ulo[p], ulo[p+1])
     int32_t *d_hlo = nullptr;
+    int32_t *d_pprob = nullptr;  // solve of each particle (transposition blocks)
+    int2 *d_ptab = nullptr;      // per-solve gas tables of a matrix-free batch (LadderGen::ptab)
     LoopCtl *d_pctl = nullptr;   // loop state per solve (== d_ctl for a single solve)
     LoopCtl *h_pctl = nullptr;   // pinned staging for it (batches only)
     GenPart *d_gpart = nullptr;
@@ -273,6 +275,11 @@ int check_launch(const char *what) {
     return CCD_OK;
 }
 
+// The per-solve loop states the contraction kernels of a batch read to skip
+// stopped solves (nullptr for one problem: its kernels return at entry once
+// its loop has stopped).
+const LoopCtl *skip_ctl(const ccd_ctx *c) { return c->nprob > 1 ? c->d_pctl : nullptr; }
+
 UpdArgs upd_args(const ccd_ctx *c, const double *t_old, double *t_new, double mixing) {
     UpdArgs a{};
     a.tiles = c->d_utiles; a.chans = c->d_chans; a.ppl = c->d_ppl; a.hhl = c->d_hhl;
@@ -293,7 +300,8 @@ void launch_ring_ct(ccd_ctx *c, const CT *va, const CT *vb, int lo, int hi, hipS
     if (n <= 0) return;
     const dim3 grid((unsigned)((n + kThreads / 64 - 1) / (kThreads / 64)));
     hipLaunchKernelGGL((k_ring<CT, WIDE>), grid, dim3(kThreads), 0, st, c->d_ritems + lo, n, c->d_mods,
-                       reinterpret_cast<const CT *>(c->d_tph), va, vb, c->d_zero, reinterpret_cast<CT *>(c->d_r), c->d_ctl);
+                       reinterpret_cast<const CT *>(c->d_tph), va, vb, c->d_zero, reinterpret_cast<CT *>(c->d_r), c->d_ctl,
+                       skip_ctl(c));
 }
 
 template <typename CT>
@@ -317,13 +325,13 @@ void launch_ring_staged(ccd_ctx *c, const CT *va, const CT *vb, hipStream_t st) 
         const size_t lds = RingStage<CT, 5>::BYTES;
         hipLaunchKernelGGL((k_ring_staged<CT, 5, 5>), dim3((unsigned)n5), dim3(kThreads), lds, st, c->d_rgroups,
                            c->d_mods, reinterpret_cast<const CT *>(c->d_tph), va, vb, c->d_zero,
-                           reinterpret_cast<CT *>(c->d_r), c->d_ctl);
+                           reinterpret_cast<CT *>(c->d_r), c->d_ctl, skip_ctl(c));
     }
     if (n > n5) {
         const size_t lds = c->rg_lds[sizeof(CT) == 8 ? 0 : 1];
         hipLaunchKernelGGL((k_ring_staged<CT, 1, 4>), dim3((unsigned)(n - n5)), dim3(kThreads), lds, st,
                            c->d_rgroups + n5, c->d_mods, reinterpret_cast<const CT *>(c->d_tph), va, vb, c->d_zero,
-                           reinterpret_cast<CT *>(c->d_r), c->d_ctl);
+                           reinterpret_cast<CT *>(c->d_r), c->d_ctl, skip_ctl(c));
     }
 }
 
@@ -361,10 +369,10 @@ int launch_transpose(ccd_ctx *c, bool to_ph, const double *src, double *dst, hip
     if (c->np == 0 || c->nt2 == 0) return CCD_OK;
     if (to_ph)
         hipLaunchKernelGGL(k_transpose<double>, dim3((unsigned)c->np), dim3(kThreads), c->tr_lds, st, c->d_tr_base,
-                           c->d_addr_pp, c->d_addr_ph, c->d_perm_ph, src, dst, c->d_ctl);
+                           c->d_addr_pp, c->d_addr_ph, c->d_perm_ph, src, dst, c->d_ctl, nullptr, nullptr);
     else
         hipLaunchKernelGGL(k_transpose<double>, dim3((unsigned)c->np), dim3(kThreads), c->tr_lds, st, c->d_tr_base,
-                           c->d_addr_ph, c->d_addr_pp, c->d_perm_pp, src, dst, c->d_ctl);
+                           c->d_addr_ph, c->d_addr_pp, c->d_perm_pp, src, dst, c->d_ctl, nullptr, nullptr);
     return check_launch("k_transpose");
 }
 
@@ -377,13 +385,14 @@ int launch_tremap(ccd_ctx *c, const double *t, hipStream_t st, bool force = fals
     c->tph_prec = c->iter_prec;
     if (c->np == 0 || c->nt2 == 0) return CCD_OK;
     const LoopCtl *ctl = force ? reinterpret_cast<const LoopCtl *>(c->d_zero) : c->d_ctl;
+    const LoopCtl *pctl = force ? nullptr : skip_ctl(c);
     if (c->iter_prec == CCD_PREC_FP32)
         hipLaunchKernelGGL((k_transpose<double, float>), dim3((unsigned)c->np), dim3(kThreads), c->tr_lds, st,
                            c->d_tr_base, c->d_addr_pp, c->d_addr_ph, c->d_perm_ph, t,
-                           reinterpret_cast<float *>(c->d_tph), ctl);
+                           reinterpret_cast<float *>(c->d_tph), ctl, c->d_pprob, pctl);
     else
         hipLaunchKernelGGL(k_transpose<double>, dim3((unsigned)c->np), dim3(kThreads), c->tr_lds, st, c->d_tr_base,
-                           c->d_addr_pp, c->d_addr_ph, c->d_perm_ph, t, c->d_tph, ctl);
+                           c->d_addr_pp, c->d_addr_ph, c->d_perm_ph, t, c->d_tph, ctl, c->d_pprob, pctl);
     return check_launch("k_transpose");
 }
 
@@ -418,10 +427,11 @@ int launch_remap_r(ccd_ctx *c, hipStream_t st) {
     if (c->iter_prec == CCD_PREC_FP32)
         hipLaunchKernelGGL(k_transpose<float>, dim3((unsigned)c->np), dim3(kThreads), c->tr_lds, st, c->d_tr_base,
                            c->d_addr_ph, c->d_addr_pp, c->d_perm_pp, reinterpret_cast<const float *>(c->d_r),
-                           reinterpret_cast<float *>(c->d_g), c->d_ctl);
+                           reinterpret_cast<float *>(c->d_g), c->d_ctl, c->d_pprob, skip_ctl(c));
     else
         hipLaunchKernelGGL(k_transpose<double>, dim3((unsigned)c->np), dim3(kThreads), c->tr_lds, st, c->d_tr_base,
-                           c->d_addr_ph, c->d_addr_pp, c->d_perm_pp, c->d_r, c->d_g, c->d_ctl);
+                           c->d_addr_ph, c->d_addr_pp, c->d_perm_pp, c->d_r, c->d_g, c->d_ctl, c->d_pprob,
+                           skip_ctl(c));
     return check_launch("k_transpose");
 }
 
@@ -443,10 +453,10 @@ int launch_ladder_x(ccd_ctx *c, hipStream_t st) {
     if (c->iter_prec == CCD_PREC_FP32)
         hipLaunchKernelGGL(k_ladder_x<float>, dim3((unsigned)c->xtiles.size()), dim3(kThreads), 0, st, c->d_xtiles,
                            c->d_chans, c->d_vtp, reinterpret_cast<const float *>(c->d_tas), c->d_vh, c->d_zero,
-                           c->d_x, c->d_ctl);
+                           c->d_x, c->d_ctl, skip_ctl(c));
     else
         hipLaunchKernelGGL(k_ladder_x<double>, dim3((unsigned)c->xtiles.size()), dim3(kThreads), 0, st, c->d_xtiles,
-                           c->d_chans, c->d_vtp, c->d_tas, c->d_vh, c->d_zero, c->d_x, c->d_ctl);
+                           c->d_chans, c->d_vtp, c->d_tas, c->d_vh, c->d_zero, c->d_x, c->d_ctl, skip_ctl(c));
     return check_launch("k_ladder_x");
 }
 
@@ -490,7 +500,7 @@ void launch_grp(ccd_ctx *c, const CT *vp, int g, int lo, int hi, hipStream_t st)
     }
     L.blk0[HI + 1] = nb;
     if (nb == 0) return;
-    const size_t lds = ladder_tab_bytes(c->lsrc == LV_GAS ? c->lgen.ntab : 0) + need;
+    const size_t lds = ladder_tab_bytes(gas_source(c->lsrc) ? c->lgen.ntab : 0) + need;
     const LadderOps<CT> o{c->d_ltiles, c->d_lbptr, c->d_chans, vp, c->lgen, reinterpret_cast<const CT *>(c->d_tas),
                           c->d_zero, reinterpret_cast<CT *>(c->d_dp), c->d_ctl};
     launch_ladder_grp(c->lsrc, g, (unsigned)nb, lds, st, L, o);
@@ -510,7 +520,7 @@ void launch_ladder_v_ct(ccd_ctx *c, const CT *vp, int lo, int hi, hipStream_t st
 int allow_ladder_lds_all() {
     int e;
     if ((e = allow_ladder_lds_f32_gas()) || (e = allow_ladder_lds_f32_other()) || (e = allow_ladder_lds_f64_gas()) ||
-        (e = allow_ladder_lds_f64_other()))
+        (e = allow_ladder_lds_f64_other()) || (e = allow_ladder_lds_f32_gasb()) || (e = allow_ladder_lds_f64_gasb()))
         return ccd::fail(CCD_E_HIP, "raising the ladder's LDS limit failed: %s", hipGetErrorString((hipError_t)e));
     return CCD_OK;
 }
@@ -523,7 +533,7 @@ void launch_ladder_tx_ct(ccd_ctx *c, int lo, int hi, hipStream_t st) {
     const dim3 grid((unsigned)((n + kThreads / 64 - 1) / (kThreads / 64)));
     hipLaunchKernelGGL((k_ladder_tx<CT, WIDE>), grid, dim3(kThreads), 0, st, c->d_ttiles + lo, n, c->d_chans,
                        reinterpret_cast<const CT *>(c->d_tas), c->d_x, c->d_zero, reinterpret_cast<CT *>(c->d_dp),
-                       c->d_ctl);
+                       c->d_ctl, skip_ctl(c));
 }
 
 int launch_ladder_tx(ccd_ctx *c, hipStream_t st) {
@@ -923,62 +933,94 @@ int merge_problems(const ccd_problem *const *probs, int n, Merged &m) {
 }
 
 // Pair records of the generated-V ladder (LadderGen).  The electron-gas form
-// needs, and checks here: two spin values, conserved total momentum and spin
-// in every direct channel, and a direct-term table even in each component;
-// otherwise the ladder calls the generic generator per element.
-int build_generated_ladder(ccd_ctx *c, const ccd_problem *p, const std::vector<int2> &ppl,
+// needs, and checks here per solve: two spin values, conserved total momentum
+// and spin in every direct channel, and a direct-term table even in each
+// component that fits the LDS table (kGasTableMax entries); the channels of
+// a solve that fails (and of every non-gas solve of a batch) use the generic
+// generator per element.  A batch gets one folded table per gas solve, back
+// to back, with (offset, entries) per solve in ptab; the ladder stages the
+// table of the solve it works on.
+int build_generated_ladder(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi, const std::vector<int2> &ppl,
                            const std::vector<int32_t> &prow) {
     // kRecPad zero entries past the last pair: the wave-tile ladder's last
     // chunk of a channel reads records beyond it (their amplitude rows are 0)
     const size_t nrec = prow.size() + kRecPad;
     std::vector<int2> rec1(nrec, int2{0, 0}), rec2(nrec, int2{0, 0}), rec1f(nrec, int2{0, 0}),
         rec2f(nrec, int2{0, 0}), rec1w(nrec, int2{0, 0});
-    bool gas = p->gen_kind == CCD_GEN_ELECTRON_GAS;
-    const int m = p->gen_nmax, D = 2 * m + 1, w = 4 * m + 1;
-    if (gas && (int64_t)D * D * D + 1 > kGasTableMax) gas = false;
+    const int nprob = bi.nprob;
     const int32_t *qn = p->qn;
-    int s_lo = 0, s_hi = 0;
-    if (gas) {
-        s_lo = s_hi = qn[3];
-        for (int s = 0; s < c->nsp; s++) {
-            s_lo = std::min(s_lo, qn[5 * s + 3]);
-            s_hi = std::max(s_hi, qn[5 * s + 3]);
-        }
-        for (int s = 0; s < c->nsp && gas; s++)
-            if (qn[5 * s + 3] != s_lo && qn[5 * s + 3] != s_hi) gas = false;
-    }
-    if (gas)
-        for (const DChan &ch : c->chans) {
-            const int2 f = ppl[ch.pp_off];
-            int tot[4];
-            for (int k = 0; k < 4; k++) tot[k] = qn[5 * f.x + k] + qn[5 * f.y + k];
-            for (int q = 1; q < ch.pp && gas; q++) {
-                const int2 e = ppl[ch.pp_off + q];
-                for (int k = 0; k < 4; k++)
-                    if (qn[5 * e.x + k] + qn[5 * e.y + k] != tot[k]) gas = false;
+    // per solve: its generator kind, nmax and full table, the spin values
+    // of its states, whether it takes the gas form
+    std::vector<char> gas(nprob, 0);
+    std::vector<int> mq(nprob, 0), s_lo(nprob, 0), s_hi(nprob, 0);
+    std::vector<const double *> tq(nprob, nullptr);
+    for (int q = 0; q < nprob; q++) {
+        const int kind = nprob == 1 ? p->gen_kind : bi.parts[q].kind;
+        mq[q] = nprob == 1 ? p->gen_nmax : bi.parts[q].nmax;
+        tq[q] = p->gen_table + (nprob == 1 ? 0 : bi.parts[q].tab_off);
+        const int D = 2 * mq[q] + 1;
+        gas[q] = kind == CCD_GEN_ELECTRON_GAS && (int64_t)D * D * D + 1 <= kGasTableMax;
+        if (!gas[q]) continue;
+        bool first = true;
+        auto spins = [&](int lo, int hi) {
+            for (int st = lo; st < hi; st++) {
+                const int sz = qn[5 * st + 3];
+                if (first) s_lo[q] = s_hi[q] = sz;
+                first = false;
+                s_lo[q] = std::min(s_lo[q], sz);
+                s_hi[q] = std::max(s_hi[q], sz);
             }
-            if (!gas) break;
+        };
+        spins(bi.hlo[q], bi.hlo[q + 1]);
+        spins(bi.plo[q], bi.plo[q + 1]);
+        auto two_spins = [&](int lo, int hi) {
+            for (int st = lo; st < hi; st++)
+                if (qn[5 * st + 3] != s_lo[q] && qn[5 * st + 3] != s_hi[q]) return false;
+            return true;
+        };
+        gas[q] = two_spins(bi.hlo[q], bi.hlo[q + 1]) && two_spins(bi.plo[q], bi.plo[q + 1]);
+    }
+    for (const DChan &ch : c->chans) {   // conserved momentum and spin in every channel
+        if (!gas[ch.prob]) continue;
+        const int2 f = ppl[ch.pp_off];
+        int tot[4];
+        for (int k = 0; k < 4; k++) tot[k] = qn[5 * f.x + k] + qn[5 * f.y + k];
+        for (int u = 1; u < ch.pp && gas[ch.prob]; u++) {
+            const int2 e = ppl[ch.pp_off + u];
+            for (int k = 0; k < 4; k++)
+                if (qn[5 * e.x + k] + qn[5 * e.y + k] != tot[k]) gas[ch.prob] = 0;
         }
+    }
+    // the folded tables T(|dx|, |dy|, |dz|) of the gas solves, back to back
     std::vector<double> tab;
-    if (gas) {
-        tab.assign((size_t)D * D * D + 1, 0.0);
-        const double *T = p->gen_table;
+    std::vector<int2> ptab(nprob, int2{0, 0});
+    int ntab_max = 0;
+    for (int q = 0; q < nprob; q++) {
+        if (!gas[q]) continue;
+        const int m = mq[q], D = 2 * m + 1, w = 4 * m + 1;
+        const double *T = tq[q];
         auto at = [&](int dx, int dy, int dz) {
             return T[((size_t)(dx + 2 * m) * w + (dy + 2 * m)) * w + (dz + 2 * m)];
         };
-        for (int x = 0; x < D && gas; x++)
-            for (int y = 0; y < D && gas; y++)
-                for (int z = 0; z < D && gas; z++) {
+        std::vector<double> f((size_t)D * D * D + 1, 0.0);
+        for (int x = 0; x < D && gas[q]; x++)
+            for (int y = 0; y < D && gas[q]; y++)
+                for (int z = 0; z < D && gas[q]; z++) {
                     const double v = at(x, y, z);
                     for (int sg = 1; sg < 8; sg++)
-                        if (at(sg & 1 ? -x : x, sg & 2 ? -y : y, sg & 4 ? -z : z) != v) gas = false;
-                    tab[((size_t)x * D + y) * D + z] = v;
+                        if (at(sg & 1 ? -x : x, sg & 2 ? -y : y, sg & 4 ? -z : z) != v) gas[q] = 0;
+                    f[((size_t)x * D + y) * D + z] = v;
                 }
+        if (!gas[q]) continue;
+        ptab[q] = int2{(int)tab.size(), (int)f.size()};
+        ntab_max = std::max(ntab_max, (int)f.size());
+        tab.insert(tab.end(), f.begin(), f.end());
     }
     // Records: the momentum of a state pre-scaled into 16-bit fields,
     // esize D^2 (n_x + m) | esize D (n_y + m) << 16 and esize (n_z + m), so that
     // two v_sad_u16 of two records give the byte offset of T(|dn|) in a table
-    // of esize-byte entries (at most 8 D^3 <= 32768 bytes).
+    // of esize-byte entries (at most 8 D^3 <= 32768 bytes; D and m of the
+    // pair's solve).
     // Channels whose pairs all have one particle of each spin (total spin 0):
     // there <ab||cd> = s(ab) s(cd) T(k_up(ab) - k_up(cd)), with k_up the
     // momentum of the pair's up-spin particle and s = +1 when the first
@@ -986,11 +1028,11 @@ int build_generated_ladder(ccd_ctx *c, const ccd_problem *p, const std::vector<i
     // survives, and T is even).  Their records: rec1 = k_up (rec1f for the
     // FP32 table); rec2 = (sign bit of s as a float/double sign mask, 0);
     // rec1w = rec1f with s in the high half of its second word, 0 for +1 and
-    // the byte size of the FP32 table for -1 (the table [T, -T] of k_ladder_w:
-    // the two fields' distance selects the copy of sign s(ab) s(cd)).
+    // the byte size of the solve's FP32 table for -1 (the FP32 tables are
+    // [T, -T]: the two fields' distance selects the copy of sign s(ab) s(cd)).
     static_assert(4 * kGasTableMax < 65536, "the sign field holds the FP32 table's byte size");
-    const uint32_t sbytes = 4u * (uint32_t)(D * D * D + 1);   // bytes of the FP32 table
-    auto scaled_up = [&](int s, uint32_t esize = 8u) {   // esize: bytes per table entry
+    auto scaled_up = [&](int q, int s, uint32_t esize) {   // esize: bytes per table entry
+        const int m = mq[q], D = 2 * m + 1;
         const int32_t *a = qn + 5 * (size_t)s;
         const uint32_t x = esize * D * D * (a[0] + m), y = esize * D * (a[1] + m), z = esize * (a[2] + m);
         return int2{(int)(x | y << 16), (int)z};
@@ -1000,25 +1042,27 @@ int build_generated_ladder(ccd_ctx *c, const ccd_problem *p, const std::vector<i
     // rec2 = the momenta of the two states (rec1f, rec2f for the FP32 table).
     // A gas channel with pairs is one of the two kinds.
     for (DChan &ch : c->chans) {
+        const int q = ch.prob;
         ch.spin0 = 0;
-        if (gas && ch.npk + ch.ndiag > 0) {
+        if (gas[q] && ch.npk + ch.ndiag > 0) {
             const int2 f = ppl[ch.pp_off + prow[ch.pr_off]];
             ch.spin0 = qn[5 * f.x + 3] != qn[5 * f.y + 3] && ch.ndiag == 0 ? 1 : 2;
         }
+        const uint32_t sbytes = gas[q] ? 4u * (uint32_t)ptab[q].y : 0u;   // bytes of the solve's FP32 table
         for (int u = 0; u < ch.npk + ch.ndiag; u++) {
             const int2 ab = ppl[ch.pp_off + prow[ch.pr_off + u]];
             if (ch.spin0 == 1) {
-                const bool first_up = qn[5 * ab.x + 3] == s_hi;
-                rec1[ch.pr_off + u] = scaled_up(first_up ? ab.x : ab.y);
-                rec1f[ch.pr_off + u] = scaled_up(first_up ? ab.x : ab.y, 4u);   // the FP32 table
+                const bool first_up = qn[5 * ab.x + 3] == s_hi[q];
+                rec1[ch.pr_off + u] = scaled_up(q, first_up ? ab.x : ab.y, 8u);
+                rec1f[ch.pr_off + u] = scaled_up(q, first_up ? ab.x : ab.y, 4u);   // the FP32 table
                 rec1w[ch.pr_off + u] = rec1f[ch.pr_off + u];
                 rec1w[ch.pr_off + u].y |= (int)((first_up ? 0u : sbytes) << 16);
                 rec2[ch.pr_off + u] = int2{first_up ? 0 : (int)0x80000000u, 0};
             } else if (ch.spin0 == 2) {
-                rec1[ch.pr_off + u] = scaled_up(ab.x);
-                rec2[ch.pr_off + u] = scaled_up(ab.y);
-                rec1f[ch.pr_off + u] = scaled_up(ab.x, 4u);
-                rec2f[ch.pr_off + u] = scaled_up(ab.y, 4u);
+                rec1[ch.pr_off + u] = scaled_up(q, ab.x, 8u);
+                rec2[ch.pr_off + u] = scaled_up(q, ab.y, 8u);
+                rec1f[ch.pr_off + u] = scaled_up(q, ab.x, 4u);
+                rec2f[ch.pr_off + u] = scaled_up(q, ab.y, 4u);
             } else {
                 rec1[ch.pr_off + u] = int2{ab.x, ab.y};
             }
@@ -1036,11 +1080,17 @@ int build_generated_ladder(ccd_ctx *c, const ccd_problem *p, const std::vector<i
     c->lgen.rec1f = c->d_rec1f;
     c->lgen.rec2f = c->d_rec2f;
     c->lgen.rec1w = c->d_rec1w;
-    if (gas) {
+    c->lgen.pctl = nprob > 1 ? c->d_pctl : nullptr;
+    c->lgen.ptab = nullptr;
+    if (!tab.empty()) {
         if ((rc = upload(c, &c->d_gtab, tab))) return rc;
         c->lgen.tab = c->d_gtab;
-        c->lgen.ntab = (int)tab.size();
-        c->lsrc = LV_GAS;
+        c->lgen.ntab = ntab_max;
+        if (nprob > 1) {
+            if ((rc = upload(c, &c->d_ptab, ptab))) return rc;
+            c->lgen.ptab = c->d_ptab;
+        }
+        c->lsrc = nprob > 1 ? LV_GAS_BATCH : LV_GAS;
     } else {
         c->lsrc = LV_GENERIC;
     }
@@ -1096,7 +1146,25 @@ void plan_class_tiles(ccd_ctx *c, int nt) {
     };
     std::vector<std::vector<int>> rr(G), lpt(G);
     for (int i = 0; i < n; i++) rr[i % G].push_back(i);
-    const int mode = c->plan.assign[nt - 1];
+    int mode = c->plan.assign[nt - 1];
+    // a matrix-free batch: tiles of one solve share its gas table, which a
+    // block stages when it moves on to another solve
+    if (mode == CCD_LA_MODEL && c->nprob > 1 && c->matrix_free) mode = CCD_LA_RUNS;
+    if (mode == CCD_LA_RUNS) {   // contiguous runs of about equal cost, in channel order
+        int64_t total = 0, acc = 0;
+        for (int i = 0; i < n; i++) total += cost[i];
+        std::vector<std::vector<int>> runs(G);
+        for (int i = 0; i < n; i++) {
+            runs[(size_t)std::min<int64_t>(G - 1, acc * G / std::max<int64_t>(total, 1))].push_back(i);
+            acc += cost[i];
+        }
+        int k = 0;
+        for (int b = 0; b < G; b++) {
+            for (int i : runs[b]) c->ltiles[lo + k++] = c->ltiles0[lo + i];
+            ptr[b + 1] = k;
+        }
+        return;
+    }
     if (mode != CCD_LA_CHANNEL) {
         std::vector<int> idx(n);
         for (int i = 0; i < n; i++) idx[i] = i;
@@ -1321,7 +1389,9 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
             if (PH > NTMAX * 16 || HP > NTMAX * 16)
                 return ccd::fail(CCD_E_INVALID, "mod channel %d has ph=%d hp=%d > %d", k, PH, HP, NTMAX * 16);
             for (int jt = 0; jt < (HP + 15) / 16; jt++) c->ritems.push_back(RingItem{(int32_t)c->mods.size(), jt});
-            c->mods.push_back(DMod{off_ph, off_b, PH, HP, (int32_t)iph, (int32_t)ihp});
+            int32_t mprob = 0;   // channels (direct and mod alike) of solve q: [chan_lo[q], chan_lo[q+1])
+            while (mprob + 1 < nprob && k >= bi.chan_lo[mprob + 1]) mprob++;
+            c->mods.push_back(DMod{off_ph, off_b, PH, HP, (int32_t)iph, (int32_t)ihp, mprob, 0});
             off_b += (int64_t)HP * HP;
             c->flops += 2.0 * HP * HP * PH + 2.0 * PH * HP * HP;
         }
@@ -1539,7 +1609,8 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
         (rc = dalloc(c, &c->d_partials, std::max(c->utiles.size(), (size_t)1))) ||
         (rc = dalloc(c, &c->d_scalar, (size_t)std::max(4, nprob))) ||
         (rc = dalloc(c, &c->d_zero, (size_t)kZeroDoubles)) || (rc = dalloc(c, &c->d_ctl, 1)) || (rc = dalloc(c, &c->d_ticket, 1)) ||
-        (rc = dalloc(c, &c->d_elog, (size_t)kElogCap * nprob)) || (rc = upload(c, &c->d_hlo, bi.hlo)))
+        (rc = dalloc(c, &c->d_elog, (size_t)kElogCap * nprob)) || (rc = upload(c, &c->d_hlo, bi.hlo)) ||
+        (rc = upload(c, &c->d_pprob, std::vector<int32_t>(sprob.begin() + nh, sprob.end()))))
         return rc;
     HIP_TRY(hipHostMalloc((void **)&c->h_ctl, sizeof(LoopCtl), hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void **)&c->h_ctl2, sizeof(LoopCtl), hipHostMallocDefault));
@@ -1547,6 +1618,7 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
     *c->h_ctl2 = LoopCtl{};
     if (nprob > 1) {
         if ((rc = dalloc(c, &c->d_pctl, (size_t)nprob))) return rc;
+        c->lgen.pctl = c->d_pctl;   // the ladder skips stopped solves
         HIP_TRY(hipHostMalloc((void **)&c->h_pctl, sizeof(LoopCtl) * nprob, hipHostMallocDefault));
     } else {
         c->d_pctl = c->d_ctl;
@@ -1580,7 +1652,7 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
             G.table = c->d_table;
         }
         if (c->matrix_free) {
-            if ((rc = build_generated_ladder(c, p, ppl, prow))) return rc;
+            if ((rc = build_generated_ladder(c, p, bi, ppl, prow))) return rc;
             c->lgen.G = G;
         } else if (!c->rtiles.empty()) {
             hipLaunchKernelGGL(k_gen_vp, dim3((unsigned)c->rtiles.size()), dim3(kThreads), 0, c->stream,
@@ -1986,7 +2058,7 @@ int plan_valid(const ccd_ladder_plan &p) {
         if (p.blocks_per_cu[n] < 0 || p.blocks_per_cu[n] > 16)
             return ccd::fail(CCD_E_INVALID, "ladder plan: %d blocks per compute unit (class %d)", p.blocks_per_cu[n],
                              n + 1);
-        if (p.assign[n] < CCD_LA_MODEL || p.assign[n] > CCD_LA_LONGEST)
+        if (p.assign[n] < CCD_LA_MODEL || p.assign[n] > CCD_LA_RUNS)
             return ccd::fail(CCD_E_INVALID, "ladder plan: unknown assignment %d (class %d)", p.assign[n], n + 1);
     }
     if (p.partition != CCD_LP_SPLIT && p.partition != CCD_LP_ONE)
@@ -2044,7 +2116,7 @@ int tune_ladder(ccd_ctx *c, int flags) {
         std::vector<int> bpcs{1, 2, 3, 4, 6};
         if (!c->matrix_free) bpcs.insert(bpcs.begin(), 0);
         if ((rc = consider(p.kind[prec][nt - 1], ck)) || (rc = consider(p.blocks_per_cu[nt - 1], bpcs)) ||
-            (rc = consider(p.assign[nt - 1], {CCD_LA_CHANNEL, CCD_LA_LONGEST})))
+            (rc = consider(p.assign[nt - 1], {CCD_LA_CHANNEL, CCD_LA_LONGEST, CCD_LA_RUNS})))
             break;
         us_class[nt - 1] = best;
     }

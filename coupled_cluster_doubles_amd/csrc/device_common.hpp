@@ -67,6 +67,8 @@ struct DMod {           // active mod channel (ph > 0 and hp > 0)
     int64_t off_b;      // offset of its hp x hp block of V^B
     int32_t ph, hp;
     int32_t ph_off, hp_off;
+    int32_t prob;       // solve the channel belongs to (0 outside a batch)
+    int32_t pad;
 };
 
 struct DTile {          // a run of rows (or row pairs) of one active direct channel
@@ -92,8 +94,31 @@ struct LoopCtl {
     double prev;     // energy of the previous iteration (100.0 at the start)
     double tol;      // continue while |dE| > tol
     int cap;         // entries of the energy log
-    int pad;
+    int idle;        // a solve of a batch: iterations of the batch since it stopped
 };
+
+// Whether solve `prob` of a batch has stopped (pctl: the per-solve loop
+// states; nullptr outside a batch, or where nothing may be skipped).  The
+// contraction kernels skip the work of a stopped solve: the update keeps its
+// amplitudes and never reads their output.  The flag may be set while a
+// kernel runs (the reductions of the previous iteration run on another
+// stream), so one thread reads it and shares it through LDS: the answer is
+// the same for the whole block.  Every thread of the block must call it.
+__device__ __forceinline__ bool block_solve_stopped(const LoopCtl *pctl, int prob) {
+    if (!pctl) return false;
+    __shared__ int stop_flag;
+    __syncthreads();   // the previous call's readers are done with stop_flag
+    if (threadIdx.x == 0) stop_flag = __hip_atomic_load(&pctl[prob].done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    return stop_flag != 0;
+}
+
+// The same for kernels whose waves are independent: one answer per wave.
+__device__ __forceinline__ bool wave_solve_stopped(const LoopCtl *pctl, int prob) {
+    if (!pctl) return false;
+    return __builtin_amdgcn_readfirstlane(
+               __hip_atomic_load(&pctl[prob].done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0;
+}
 
 // A batch of independent solves merged into one problem: every s.p. state
 // carries its solve's index in qn[5 s + 4], and the solve's generator

@@ -30,12 +30,14 @@ __global__ __launch_bounds__(kThreads) void k_ladder_x(const DTile *__restrict__
                                                        const double *__restrict__ vh,
                                                        const double *__restrict__ zero,
                                                        double *__restrict__ x,
-                                                       const LoopCtl *__restrict__ ctl) {
+                                                       const LoopCtl *__restrict__ ctl,
+                                                       const LoopCtl *__restrict__ pctl) {
     if (ctl->done) return;
     __shared__ double part[kThreads / 64][64][4];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
     const DTile tl = tiles[blockIdx.x];
     const DChan ch = chans[tl.chan];
+    if (block_solve_stopped(pctl, ch.prob)) return;   // a stopped solve of a batch
     const int H = ch.hh, npk = ch.npk, kl0 = tl.row0, ij0 = tl.aux;
     const bool aok = kl0 + m < H, bok = ij0 + m < H;
     const TT *zt = reinterpret_cast<const TT *>(zero);
@@ -75,7 +77,11 @@ enum LadderSource {
     LV_STORED = 0,    // packed blocks in HBM (the stored-interaction mode)
     LV_GAS = 1,       // electron gas, generated per element from a table in LDS
     LV_GENERIC = 2,   // any generator, per element through gen_tbme
+    LV_GAS_BATCH = 3, // a batch: the electron-gas form with each solve's table in
+                      // LDS (staged when a block moves on to another solve), the
+                      // generic generator for the channels of other solves
 };
+constexpr bool gas_source(int src) { return src == LV_GAS || src == LV_GAS_BATCH; }
 
 // Per-element generation of packed V^pppp (the matrix-free mode, the
 // calcFree* twins of src/calc.cpp, which rebuild V per channel with calcTBME).
@@ -100,9 +106,51 @@ struct LadderGen {
     const int2 *rec2f;   // gas, same-spin channels: rec2 scaled for the FP32 table
     const int2 *rec1w;   // gas, total-spin-0 channels: rec1f with the sign field of the table [T, -T]
     const double *tab;   // gas: T over (|dx|,|dy|,|dz|), row-major, then one 0
-    int ntab;            // gas: entries including the final 0
+    int ntab;            // gas: entries including the final 0 (a batch: the largest solve's)
     Gen G;               // generic
+    // a batch: per solve, (offset in tab, entries) of its gas table ((0, 0)
+    // for a solve whose channels use the generic generator); nullptr for one
+    // problem, whose table is tab[0 .. ntab)
+    const int2 *ptab;
+    const LoopCtl *pctl;   // a batch: per-solve loop states (stopped solves are skipped)
 };
+
+// The gas table into LDS in the arithmetic type: FP64 iterations T, FP32
+// ones [T, -T] with every entry rounded (n entries each).
+template <typename CT>
+__device__ __forceinline__ void stage_gas_table(const double *tab, int n, void *lds) {
+    CT *t = reinterpret_cast<CT *>(lds);
+    for (int i = threadIdx.x & (kThreads - 1); i < n; i += kThreads) {
+        const CT v = (CT)tab[i];
+        t[i] = v;
+        if (sizeof(CT) == 4) t[n + i] = -v;
+    }
+}
+
+// Start of a ladder tile in a batch: false when the tile's solve has
+// stopped; with per-solve gas tables, the table of the tile's solve goes to
+// LDS when the block moves on to another solve (the block's previous tile
+// ended with a barrier after its last table read).  prob / stopped cache
+// the last solve seen.  Block-uniform; every thread calls it.
+template <typename CT, int SRC>
+__device__ __forceinline__ bool ladder_tile_begin(const LadderGen &lg, const DChan &ch, int &prob, bool &stopped,
+                                                  void *tab_lds) {
+    if constexpr (SRC == LV_GAS) return true;   // one problem, its table staged once
+    if (!lg.pctl && SRC != LV_GAS_BATCH) return true;
+    if (ch.prob != prob) {
+        prob = ch.prob;
+        stopped = block_solve_stopped(lg.pctl, prob);
+        if (SRC == LV_GAS_BATCH && !stopped) {
+            const int2 pt = lg.ptab[prob];
+            if (pt.y > 0) {
+                __syncthreads();
+                stage_gas_table<CT>(lg.tab + pt.x, pt.y, tab_lds);
+                __syncthreads();
+            }
+        }
+    }
+    return !stopped;
+}
 
 typedef __attribute__((address_space(3))) const double lds_double;
 
@@ -235,7 +283,7 @@ __device__ __forceinline__ void ladder_run(const DTile *__restrict__ tiles, int 
                                            const DChan *__restrict__ chans, const CT *__restrict__ vp,
                                            const LadderGen &lg, const CT *__restrict__ tas,
                                            const double *__restrict__ zero, CT *__restrict__ dp,
-                                           unsigned char *lds_b, uint32_t tbase) {
+                                           unsigned char *lds_b, uint32_t tbase, void *tab_lds) {
     typedef LadderTile<CT, NT, SPLIT> LT;
     static_assert(!SPLIT || sizeof(CT) == 4, "split products from FP32 operands");
     constexpr int KCH = LT::KCH, KSEG = LadderShape<NT>::KSEG, G = KCH / 4;
@@ -259,9 +307,12 @@ __device__ __forceinline__ void ladder_run(const DTile *__restrict__ tiles, int 
     constexpr bool F32 = sizeof(CT) == 4;
     const int2 *grec1_ = F32 ? lg.rec1f : lg.rec1, *grec2 = F32 ? lg.rec2f : lg.rec2;
 
+    int prob = -1;
+    bool stopped = false;
     for (int ti = b0; ti < b1; ti++) {
         const DTile tl = tiles[ti];
         const DChan ch = chans[tl.chan];
+        if (!ladder_tile_begin<CT, SRC>(lg, ch, prob, stopped, tab_lds)) continue;
         // MODE 1 (ONE): a total-spin-0 channel of the generated electron gas,
         // V from one table term (records of k_up; the signs s(cd) applied to
         // the B rows, s(ab) to the output rows, so every product equals the
@@ -278,6 +329,7 @@ __device__ __forceinline__ void ladder_run(const DTile *__restrict__ tiles, int 
             // the sign field (rec1w), like the one-row-per-lane kernel: stored
             // and generated V then feed the same products.
             constexpr bool FOLD = ONE && !SPLIT;
+            constexpr bool GAS = gas_source(SRC) && MODE != 0;   // else the generic generator
             const int2 *grec1 = SPLIT && ONE ? lg.rec1w : grec1_;
             const int H = ch.hh, npk = ch.npk, ldv = ch.ldv, seg = tl.aux;
             const int kbeg = seg * KSEG, kend = min(npk, kbeg + KSEG);
@@ -289,8 +341,8 @@ __device__ __forceinline__ void ladder_run(const DTile *__restrict__ tiles, int 
             int2 ra = {0, 0}, rb = {0, 0};
             if (GEN) {
                 const int2 *z2 = reinterpret_cast<const int2 *>(zero);
-                ra = *(rok ? (SRC == LV_GAS ? grec1 : lg.rec1) + ch.pr_off + u : z2);
-                if (SRC == LV_GAS && SAME) rb = *(rok ? grec2 + ch.pr_off + u : z2);
+                ra = *(rok ? (GAS ? grec1 : lg.rec1) + ch.pr_off + u : z2);
+                if (GAS && SAME) rb = *(rok ? grec2 + ch.pr_off + u : z2);
             }
 
             // Every load is unconditional: an element outside the block reads the
@@ -314,7 +366,7 @@ __device__ __forceinline__ void ladder_run(const DTile *__restrict__ tiles, int 
             // Buffer loads of the records (generated V) and of the B tiles: rows
             // at or past kend lie outside their buffers and read 0.
             const BufRsrc rsrc =
-                GEN ? make_rsrc((SRC == LV_GAS ? grec1 : lg.rec1) + ch.pr_off, (uint32_t)kend * 8u) : make_rsrc(zero, 0);
+                GEN ? make_rsrc((GAS ? grec1 : lg.rec1) + ch.pr_off, (uint32_t)kend * 8u) : make_rsrc(zero, 0);
             const uint32_t roff = m < G ? (uint32_t)(kbeg + kslot<CT>(kq, m)) * 8u : kOOB;
             auto load_r = [&](int v, int2 &rr) {  // generated V: record of k-slot (kq, m) of V chunk v
                 rr = buf_i2(rsrc, roff + (uint32_t)(v * KCH) * 8u);
@@ -323,13 +375,13 @@ __device__ __forceinline__ void ladder_run(const DTile *__restrict__ tiles, int 
                 static_for<0, G>([&](auto S) {
                     constexpr int s = decltype(S)::value;
                     const int2 r = {row_bcast<s>(rr.x), row_bcast<s>(rr.y)};
-                    if (F32 && SRC == LV_GAS && ONE) {
+                    if (F32 && GAS && ONE) {
                         a[s] = lds_loadf(gas_addr1(ra, r, tbase));
-                    } else if (F32 && SRC == LV_GAS) {
+                    } else if (F32 && GAS) {
                         a[s] = lds_loadf(gas_addr1(ra, r, tbase)) - lds_loadf(gas_addr1(rb, r, tbase));
-                    } else if (SRC == LV_GAS && ONE) {
+                    } else if (GAS && ONE) {
                         a[s] = (CT)lds_load(gas_addr1(ra, r, tbase));
-                    } else if (SRC == LV_GAS) {
+                    } else if (GAS) {
                         a[s] = (CT)(lds_load(gas_addr1(ra, r, tbase)) - lds_load(gas_addr1(rb, r, tbase)));
                     } else if (F32) {
                         a[s] = gen_tbme_f32(lg.G, ra.x, ra.y, r.x, r.y);
@@ -489,9 +541,10 @@ __device__ __forceinline__ void ladder_run(const DTile *__restrict__ tiles, int 
                 }
             }
         };
-        // a gas channel with pairs is total-spin-0 or same-spin (the host sets spin0 so)
-        if (SRC == LV_GAS && ch.spin0 == 1) tile(std::integral_constant<int, 1>{});
-        else if (SRC == LV_GAS) tile(std::integral_constant<int, 2>{});
+        // a gas channel with pairs is total-spin-0 or same-spin (the host sets
+        // spin0 so); in a batch spin0 = 0: the generic generator (other solves)
+        if (gas_source(SRC) && ch.spin0 == 1) tile(std::integral_constant<int, 1>{});
+        else if (SRC == LV_GAS || (SRC == LV_GAS_BATCH && ch.spin0 == 2)) tile(std::integral_constant<int, 2>{});
         else tile(std::integral_constant<int, 0>{});
     }
 }
@@ -554,6 +607,7 @@ __device__ __forceinline__ void wave_tile(const DTile &tl, const DChan &ch, cons
                                           int wave, int lane) {
     typedef WaveLadder<NTW> S;
     constexpr bool ONE = MODE == 1, SAME = MODE == 2, GEN = SRC != LV_STORED;
+    constexpr bool GAS = gas_source(SRC) && MODE != 0;   // else the generic generator
     const int H = ch.hh, npk = ch.npk, seg = tl.aux;
     const int kbeg = seg * LadderShape<NTW>::KSEG, kend = min(npk, kbeg + LadderShape<NTW>::KSEG);
     const int ng = (H + 3) >> 2;
@@ -562,10 +616,10 @@ __device__ __forceinline__ void wave_tile(const DTile &tl, const DChan &ch, cons
     // rows past the tile repeat its first row: finite values, never stored
     const int64_t u = tl.row0 + (lane < tl.nrows ? lane : 0);
     // k-slot records (uniform: scalar loads) and this lane's row records
-    const int2 *rk = SRC == LV_GAS ? (ONE ? lg.rec1w : lg.rec1f) : lg.rec1;
+    const int2 *rk = GAS ? (ONE ? lg.rec1w : lg.rec1f) : lg.rec1;
     int2 ra = {0, 0}, rb = {0, 0};
     if (GEN) ra = rk[ch.pr_off + u];
-    if (SRC == LV_GAS && SAME) rb = lg.rec2f[ch.pr_off + u];
+    if (GAS && SAME) rb = lg.rec2f[ch.pr_off + u];
     // the channel's records as constant memory (scalar loads for uniform addresses)
     const_int *er = (const_int *)(uintptr_t)(rk + ch.pr_off);
     const float *av = vp + ch.off_vp + u * ch.ldv;   // stored V: this lane's row (pad entries are 0)
@@ -607,8 +661,8 @@ __device__ __forceinline__ void wave_tile(const DTile &tl, const DChan &ch, cons
     // V element of a row from a k-slot record e; same-spin channels return
     // the two terms (x1 - x2 is the element)
     auto term = [&](int2 e, float &a, float &x1, float &x2) {
-        if (SRC == LV_GAS && ONE) a = lds_loadf(gas_addr1(ra, e, tb32));
-        else if (SRC == LV_GAS) {
+        if (GAS && ONE) a = lds_loadf(gas_addr1(ra, e, tb32));
+        else if (GAS) {
             x1 = lds_loadf(gas_addr1(ra, e, tb32));
             x2 = lds_loadf(gas_addr1(rb, e, tb32));
         } else a = gen_tbme_f32(lg.G, ra.x, ra.y, e.x, e.y);
@@ -637,7 +691,7 @@ __device__ __forceinline__ void wave_tile(const DTile &tl, const DChan &ch, cons
                 if (SRC == LV_STORED) a[4 * q + j] = av[k0 + 4 * q + j];
                 else term(e[4 * q + j], a[4 * q + j], x1[j], x2[j]);
             }
-            if (SRC == LV_GAS && SAME)
+            if (GAS && SAME)
 #pragma unroll
                 for (int j = 0; j < 4; j++) a[4 * q + j] = x1[j] - x2[j];
         }
@@ -678,7 +732,7 @@ __device__ __forceinline__ void wave_tile(const DTile &tl, const DChan &ch, cons
                 __builtin_amdgcn_sched_barrier(0);
                 mfma_quad(buf, q, acur);
                 __builtin_amdgcn_sched_barrier(0);
-                if (more && SRC == LV_GAS && SAME)
+                if (more && GAS && SAME)
 #pragma unroll
                     for (int j = 0; j < 4; j++) anext[4 * q + j] = x1[j] - x2[j];
             }
@@ -757,23 +811,26 @@ __device__ __forceinline__ void ladder_class(int kind, const DTile *__restrict__
                                              const DChan *__restrict__ chans, const CT *__restrict__ vp,
                                              const LadderGen &lg, const CT *__restrict__ tas,
                                              const double *__restrict__ zero, CT *__restrict__ dp,
-                                             unsigned char *lds_b, uint32_t tbase) {
+                                             unsigned char *lds_b, uint32_t tbase, void *tab_lds) {
     if constexpr (std::is_same<CT, float>::value) {
         if (kind == LK_BF16X3) {
-            ladder_run<float, NT, SRC, true>(tiles, b0, b1, chans, vp, lg, tas, zero, dp, lds_b, tbase);
+            ladder_run<float, NT, SRC, true>(tiles, b0, b1, chans, vp, lg, tas, zero, dp, lds_b, tbase, tab_lds);
             return;
         }
         if constexpr (NT == 1) {
             if (kind == LK_DEFAULT) {
                 const int tid = threadIdx.x & (kThreads - 1);
                 const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+                int prob = -1;
+                bool stopped = false;
                 for (int ti = b0; ti < b1; ti++) {
                     const DTile tl = tiles[ti];
                     const DChan ch = chans[tl.chan];
+                    if (!ladder_tile_begin<float, SRC>(lg, ch, prob, stopped, tab_lds)) continue;
                     float *region = reinterpret_cast<float *>(lds_b);
-                    if (SRC == LV_GAS && ch.spin0 == 1)
+                    if (gas_source(SRC) && ch.spin0 == 1)
                         wave_tile<1, SRC, 1>(tl, ch, vp, lg, tas, dp, region, tbase, wave, lane);
-                    else if (SRC == LV_GAS)
+                    else if (SRC == LV_GAS || (SRC == LV_GAS_BATCH && ch.spin0 == 2))
                         wave_tile<1, SRC, 2>(tl, ch, vp, lg, tas, dp, region, tbase, wave, lane);
                     else
                         wave_tile<1, SRC, 0>(tl, ch, vp, lg, tas, dp, region, tbase, wave, lane);
@@ -782,7 +839,7 @@ __device__ __forceinline__ void ladder_class(int kind, const DTile *__restrict__
             }
         }
     }
-    ladder_run<CT, NT, SRC, false>(tiles, b0, b1, chans, vp, lg, tas, zero, dp, lds_b, tbase);
+    ladder_run<CT, NT, SRC, false>(tiles, b0, b1, chans, vp, lg, tas, zero, dp, lds_b, tbase, tab_lds);
 }
 
 template <typename CT, int SRC, int LO, int HI>
@@ -795,15 +852,9 @@ __global__ __launch_bounds__(kThreads) void k_ladder_grp(LadderLaunch L, const D
                                                          const LoopCtl *__restrict__ ctl) {
     if (ctl->done) return;
     extern __shared__ double lg_lds[];
-    const int tid = threadIdx.x & (kThreads - 1);
-    const int ntab = SRC == LV_GAS ? lg.ntab : 0;
-    if (SRC == LV_GAS) {   // FP64: T; FP32: [T, -T], every entry rounded
-        CT *t = reinterpret_cast<CT *>(lg_lds);
-        for (int i = tid; i < ntab; i += kThreads) {
-            const CT v = (CT)lg.tab[i];
-            t[i] = v;
-            if (sizeof(CT) == 4) t[ntab + i] = -v;
-        }
+    const int ntab = gas_source(SRC) ? lg.ntab : 0;
+    if (SRC == LV_GAS) {   // one problem: its table, once (a batch stages each solve's as it goes)
+        stage_gas_table<CT>(lg.tab, ntab, lg_lds);
         __syncthreads();
     }
     const uint32_t tbase = (uint32_t)(uintptr_t)(lds_double *)lg_lds;   // LDS byte address
@@ -817,7 +868,7 @@ __global__ __launch_bounds__(kThreads) void k_ladder_grp(LadderLaunch L, const D
     const int kind = L.kind[nt];
     static_for<LO, HI + 1>([&](auto N) {
         constexpr int n = decltype(N)::value;
-        if (nt == n) ladder_class<CT, n, SRC>(kind, tl, b0, b1, chans, vp, lg, tas, zero, dp, lds_b, tbase);
+        if (nt == n) ladder_class<CT, n, SRC>(kind, tl, b0, b1, chans, vp, lg, tas, zero, dp, lds_b, tbase, lg_lds);
     });
 }
 
@@ -881,12 +932,14 @@ __global__ __launch_bounds__(kThreads) void k_ladder_tx(const DTile *__restrict_
                                                         const CT *__restrict__ tas,
                                                         const double *__restrict__ x,
                                                         const double *__restrict__ zero, CT *__restrict__ dp,
-                                                        const LoopCtl *__restrict__ ctl) {
+                                                        const LoopCtl *__restrict__ ctl,
+                                                        const LoopCtl *__restrict__ pctl) {
     if (ctl->done) return;
     const int it = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
     if (it >= ntiles) return;
     const DTile tl = tiles[it];
     const DChan ch = chans[tl.chan];
+    if (wave_solve_stopped(pctl, ch.prob)) return;   // a stopped solve of a batch
     const int nt = (ch.hh + 15) / 16;
     if (WIDE) {
         switch (nt) {

@@ -271,10 +271,18 @@ __global__ __launch_bounds__(kThreads) void k_ring_staged(const RingGroup *__res
                                                           const CT *__restrict__ va, const CT *__restrict__ vb,
                                                           const double *__restrict__ zero,
                                                           CT *__restrict__ r,
-                                                          const LoopCtl *__restrict__ ctl) {
+                                                          const LoopCtl *__restrict__ ctl,
+                                                          const LoopCtl *__restrict__ pctl) {
     if (ctl->done) return;
     extern __shared__ __align__(16) unsigned char ring_lds[];
     const RingGroup grp = groups[blockIdx.x];
+    if (pctl) {   // a batch: skip a group whose channels all belong to stopped solves
+        bool all = true;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (grp.mod[k] >= 0 && !block_solve_stopped(pctl, mods[grp.mod[k]].prob)) all = false;
+        if (all) return;
+    }
     const DMod m0 = mods[grp.mod[0]];
     const int nt = (max(m0.ph, m0.hp) + 15) / 16;
     static_for<LO, HI + 1>([&](auto N) {
@@ -294,12 +302,14 @@ __global__ __launch_bounds__(kThreads) void k_ring(const RingItem *__restrict__ 
                                                    const CT *__restrict__ va, const CT *__restrict__ vb,
                                                    const double *__restrict__ zero,
                                                    CT *__restrict__ r,
-                                                   const LoopCtl *__restrict__ ctl) {
+                                                   const LoopCtl *__restrict__ ctl,
+                                                   const LoopCtl *__restrict__ pctl) {
     if (ctl->done) return;
     const int it = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
     if (it >= nitems) return;
     const RingItem w = items[it];
     const DMod md = mods[w.mod];
+    if (wave_solve_stopped(pctl, md.prob)) return;   // a stopped solve of a batch
     const int nt = (max(md.ph, md.hp) + 15) / 16;
     if (WIDE) {
         switch (nt) {

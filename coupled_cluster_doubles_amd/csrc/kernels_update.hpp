@@ -40,7 +40,7 @@ struct UpdArgs {
 // UPDATE = true: one Jacobi step for both rows of each pair; UPDATE = false:
 // only the W sums of the current amplitudes (to derive chi, psi, E from them).
 // A solve of a batch that has stopped keeps its amplitudes: its new T is the
-// old one.  OT: type of the ladder and ring outputs (float after FP32
+// old one (written while the other buffer may still hold older amplitudes).  OT: type of the ladder and ring outputs (float after FP32
 // contractions, which they hold exactly); the update itself runs in FP64.
 template <bool UPDATE, typename OT>
 __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
@@ -50,6 +50,12 @@ __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
     __shared__ double w1s[kThreads], w2s[kThreads], cs[kThreads];
     const DChan ch = a.chans[tl.chan];
     const bool frozen = UPDATE && a.pctl[ch.prob].done;
+    // A solve of a batch that stopped two or more iterations ago: both
+    // amplitude buffers and the packed rows already hold its final
+    // amplitudes, and nothing reads its W sums until it resumes (which
+    // rebuilds chi and psi), so its tiles have nothing to do.  (The reductions
+    // that update the flags have finished before the update starts.)
+    if (frozen && a.pctl[ch.prob].idle >= 2) return;
     const int H = ch.hh, n = tl.nrows * H;
     const int l = threadIdx.x & (kThreads - 1);
     // places of this slot's sums in the reduction order, loaded with the
@@ -176,7 +182,9 @@ __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
 // reference's mapToDistributed(ai_jb); with src = phhp it gathers R into the
 // pphh layout.  Elements are converted from S to E on the way (T^ph for FP32
 // contractions is the FP32 rounding of T, which is all the FP32 ring reads).
-// The LDS block holds the largest particle block plus one spare slot.
+// The LDS block holds the largest particle block plus one spare slot.  In a
+// batch (pctl, pprob: the solve of each particle) the blocks of stopped
+// solves return at entry.
 template <typename S, typename E = S>
 __global__ __launch_bounds__(kThreads) void k_transpose(const int32_t *__restrict__ base,
                                                         const int32_t *__restrict__ src_addr,
@@ -184,11 +192,14 @@ __global__ __launch_bounds__(kThreads) void k_transpose(const int32_t *__restric
                                                         const uint16_t *__restrict__ perm,
                                                         const S *__restrict__ src,
                                                         E *__restrict__ dst,
-                                                        const LoopCtl *__restrict__ ctl) {
+                                                        const LoopCtl *__restrict__ ctl,
+                                                        const int32_t *__restrict__ pprob,
+                                                        const LoopCtl *__restrict__ pctl) {
     if (ctl->done) return;
     extern __shared__ double tr_lds[];
     E *buf = reinterpret_cast<E *>(tr_lds);
     const int a = blockIdx.x;
+    if (pctl && block_solve_stopped(pctl, pprob[a])) return;   // particle a's solve (a batch) has stopped
     const int b0 = base[a], n = base[a + 1] - b0;
     // rounds of U slots per thread: every address load of a round is issued,
     // then every data load, so a thread waits for two memory round trips per
@@ -295,7 +306,10 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const int32_t *__restrict__
         __syncthreads();
         const int skip = pdone;
         __syncthreads();
-        if (skip) continue;
+        if (skip) {
+            if (threadIdx.x == 0) pctl[p].idle += 1;
+            continue;
+        }
         double c = 0.0;
         for (int i = hlo[p] + threadIdx.x; i < hlo[p + 1]; i += kThreads)
             c += __hip_atomic_load(out + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -312,9 +326,11 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const int32_t *__restrict__
             if (!(fabs(diff) > pc->tol)) {
                 pc->reason = isnan(diff) ? CCD_STOP_NAN : CCD_STOP_CONVERGED;
                 pc->done = 1;
+                pc->idle = 0;
             } else if (it + 1 > pc->max_iters) {
                 pc->reason = CCD_STOP_MAX_ITERS;
                 pc->done = 1;
+                pc->idle = 0;
             }
         }
     }

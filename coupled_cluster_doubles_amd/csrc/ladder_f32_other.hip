@@ -3,10 +3,13 @@
 
 void launch_ladder_grp_f32_gas(int g, unsigned grid, size_t lds, hipStream_t st, const LadderLaunch &L,
                                const LadderOps<float> &o);
+void launch_ladder_grp_f32_gasb(int g, unsigned grid, size_t lds, hipStream_t st, const LadderLaunch &L,
+                                const LadderOps<float> &o);
 
 void launch_ladder_grp(int src, int g, unsigned grid, size_t lds, hipStream_t st, const LadderLaunch &L,
                        const LadderOps<float> &o) {
     if (src == LV_GAS) launch_ladder_grp_f32_gas(g, grid, lds, st, L, o);
+    else if (src == LV_GAS_BATCH) launch_ladder_grp_f32_gasb(g, grid, lds, st, L, o);
     else if (src == LV_GENERIC) ladder_inst::launch_src<float, LV_GENERIC>(g, grid, lds, st, L, o);
     else ladder_inst::launch_src<float, LV_STORED>(g, grid, lds, st, L, o);
 }

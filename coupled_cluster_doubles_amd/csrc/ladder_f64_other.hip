@@ -3,10 +3,13 @@
 
 void launch_ladder_grp_f64_gas(int g, unsigned grid, size_t lds, hipStream_t st, const LadderLaunch &L,
                                const LadderOps<double> &o);
+void launch_ladder_grp_f64_gasb(int g, unsigned grid, size_t lds, hipStream_t st, const LadderLaunch &L,
+                                const LadderOps<double> &o);
 
 void launch_ladder_grp(int src, int g, unsigned grid, size_t lds, hipStream_t st, const LadderLaunch &L,
                        const LadderOps<double> &o) {
     if (src == LV_GAS) launch_ladder_grp_f64_gas(g, grid, lds, st, L, o);
+    else if (src == LV_GAS_BATCH) launch_ladder_grp_f64_gasb(g, grid, lds, st, L, o);
     else if (src == LV_GENERIC) ladder_inst::launch_src<double, LV_GENERIC>(g, grid, lds, st, L, o);
     else ladder_inst::launch_src<double, LV_STORED>(g, grid, lds, st, L, o);
 }

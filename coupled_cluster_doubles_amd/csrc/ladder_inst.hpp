@@ -43,7 +43,7 @@ size_t range_bytes(int lo, int hi) {
 
 template <typename CT, int SRC, int LO, int HI>
 hipError_t allow() {
-    const size_t lds = ladder_tab_bytes(SRC == LV_GAS ? kGasTableMax : 0) + range_bytes<CT>(LO, HI);
+    const size_t lds = ladder_tab_bytes(gas_source(SRC) ? kGasTableMax : 0) + range_bytes<CT>(LO, HI);
     return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_ladder_grp<CT, SRC, LO, HI>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }

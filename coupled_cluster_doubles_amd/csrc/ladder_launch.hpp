@@ -31,3 +31,5 @@ int allow_ladder_lds_f32_gas();
 int allow_ladder_lds_f32_other();
 int allow_ladder_lds_f64_gas();
 int allow_ladder_lds_f64_other();
+int allow_ladder_lds_f32_gasb();
+int allow_ladder_lds_f64_gasb();

@@ -171,9 +171,11 @@ int ccd_get_schedule(const ccd_ctx *ctx, int *sched, double *us_per_iter);
                                 in bfloat16, a b ~ a_lo b_hi + a_hi b_lo + a_hi b_hi on
                                 v_mfma_f32_16x16x32_bf16, FP32 accumulation; about 2^-16
                                 relative per product against 2^-24 for FP32) */
-#define CCD_LA_MODEL 0       /* tile assignment: the one a model of the CU loads prefers */
+#define CCD_LA_MODEL 0       /* tile assignment: the one a model of the CU loads prefers
+                                (a matrix-free batch: CCD_LA_RUNS) */
 #define CCD_LA_CHANNEL 1     /*   round robin in channel order */
 #define CCD_LA_LONGEST 2     /*   longest tile first to the least loaded block */
+#define CCD_LA_RUNS 3        /*   contiguous runs of about equal cost, in channel order */
 #define CCD_LP_SPLIT 0       /* launches: classes 1, 2, 3-4 and 5-8 (in the four-stream
                                 schedule classes 1-2 and 3-8 run on two streams) */
 #define CCD_LP_ONE 1         /* one grouped launch of every class */
