@@ -228,6 +228,10 @@ def main():
     ap.add_argument("--no-ladder-tune", action="store_true",
                     help="keep the default pppp-ladder plan instead of tuning it on the workload "
                          "(FP32 iterations may then use BF16x3 split products where they are faster)")
+    ap.add_argument("--plan-json", default=None,
+                    help="use this pppp-ladder plan (a JSON object shaped like the line's ladder_plan) "
+                         "instead of tuning one (profiling runs: the same kernels as the tuned run)")
+    ap.add_argument("--no-kind-ab", action="store_true", help="skip the A/B of the ladder kernel kinds")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (a kernel of known traffic, "
                          "to calibrate hardware byte counters)")
@@ -278,7 +282,12 @@ def main():
     ctx = make_context(cfg, prec, free, models)
     setup_s = time.perf_counter() - t_setup
     t_tune = time.perf_counter()
-    tune(ctx, prec)               # time the schedules and ladder plans on this problem, keep the fastest
+    if args.plan_json:
+        with open(args.plan_json) as f:
+            ctx.set_ladder_plan(json.load(f))
+        ctx.set_schedule("auto")
+    else:
+        tune(ctx, prec)           # time the schedules and ladder plans on this problem, keep the fastest
     tune_s = time.perf_counter() - t_tune
     plan = ctx.ladder_plan()
     mbpt2 = ctx.mbpt2()
@@ -306,7 +315,7 @@ def main():
     # A/B of the FP32 ladder kernel kinds: every class on one kind, the other
     # plan fields as tuned, single stream, microseconds per launch of each group
     kind_ab = None
-    if prec == FP32:
+    if prec == FP32 and not args.no_kind_ab:
         kind_ab = {}
         for kind in ("default", "mfma16", "bf16x3"):
             q = dict(plan, kind_fp32=[kind] * 8)

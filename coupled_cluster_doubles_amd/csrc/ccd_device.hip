@@ -160,7 +160,7 @@ struct ccd_ctx {
     DMod *d_mods = nullptr;
     DTile *d_rtiles = nullptr, *d_ltiles = nullptr, *d_utiles = nullptr, *d_xtiles = nullptr, *d_ttiles = nullptr;
     int2 *d_ppl = nullptr, *d_hhl = nullptr, *d_phl = nullptr, *d_hpl = nullptr;
-    int32_t *d_hh_swap = nullptr, *d_prow = nullptr, *d_prow_sw = nullptr;
+    int32_t *d_hh_old = nullptr, *d_prow = nullptr, *d_prow_sw = nullptr;   // d_hh_old: caller column of each internal hh column
     int32_t *d_red_ptr = nullptr, *d_cpos = nullptr, *d_rpos = nullptr;
     int64_t n_red = 0;
     int32_t *d_tr_base = nullptr, *d_addr_pp = nullptr, *d_addr_ph = nullptr;
@@ -283,7 +283,7 @@ const LoopCtl *skip_ctl(const ccd_ctx *c) { return c->nprob > 1 ? c->d_pctl : nu
 UpdArgs upd_args(const ccd_ctx *c, const double *t_old, double *t_new, double mixing) {
     UpdArgs a{};
     a.tiles = c->d_utiles; a.chans = c->d_chans; a.ppl = c->d_ppl; a.hhl = c->d_hhl;
-    a.prow = c->d_prow; a.prow_sw = c->d_prow_sw; a.hh_swap = c->d_hh_swap;
+    a.prow = c->d_prow; a.prow_sw = c->d_prow_sw;
     a.eps = c->d_eps; a.chi = c->d_onebody; a.psi = c->d_onebody + c->nh;
     a.vt = c->d_vt; a.dp = c->d_dp; a.g = c->d_g; a.t_old = t_old; a.t_new = t_new;
     a.tas = c->d_tas; a.red = c->d_red; a.cpos = c->d_cpos; a.rpos = c->d_rpos; a.mixing = mixing; a.nh = c->nh; a.ctl = c->d_ctl;
@@ -1245,16 +1245,18 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
     for (int64_t k = 0; k < nhpp; k++) hpl[k] = make_int2(p->hp_pairs[2 * k], p->hp_pairs[2 * k + 1]);
 
     // ---- direct channels: swaps, packed rows, row pairs
-    std::vector<int32_t> pp_swap(npp, -1), hh_swap(nhh, -1);
+    std::vector<int32_t> pp_swap(npp, -1), hh_old(nhh);
     std::vector<int32_t> prow, prow_sw;
     int64_t off_t = 0, off_vp = 0, off_tp = 0, off_dp = 0, off_hhhh = 0, ipp = 0, ihh = 0;
     c->flops = 0; c->flops_pppp = 0;
     c->t2_lo.assign(nprob + 1, 0);
     std::unordered_map<int64_t, int32_t> pos;
-    std::vector<int32_t> order, t2_in, t2_out, pp_old(npp);
+    std::vector<int32_t> order, hord, t2_in, t2_out, pp_old(npp);
     std::vector<int2> seg;
     for (int64_t k = 0, o = 0; k < p->n_chan; k++)
         for (int q = 0; q < p->pp_dim[k]; q++) pp_old[o++] = q;
+    for (int64_t k = 0, o = 0; k < p->n_chan; k++)
+        for (int q = 0; q < p->hh_dim[k]; q++) hh_old[o++] = q;
     int prob = 0;
     for (int k = 0; k < p->n_chan; k++) {
         while (prob + 1 < nprob && k >= bi.chan_lo[prob + 1]) c->t2_lo[++prob] = off_t;
@@ -1279,6 +1281,29 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
             for (int q = 0; q < P; q++)
                 if (ppl[ipp + q].x == ppl[ipp + q].y) order.push_back(q);
             if ((int)order.size() != P) return ccd::fail(CCD_E_INVALID, "pp channel %d lacks a swapped pair", k);
+            // internal column order, likewise: each hole pair (i,j) with i<j
+            // directly followed by (j,i), then the pairs (i,i), so that the
+            // update finds the partner column (j,i) of column IJ at IJ ^ 1
+            pos.clear();
+            for (int q = 0; q < H; q++) pos[(int64_t)hhl[ihh + q].x * nsp + hhl[ihh + q].y] = q;
+            if ((int)pos.size() != H) return ccd::fail(CCD_E_INVALID, "hh channel %d lists a pair twice", k);
+            hord.clear();
+            for (int q = 0; q < H; q++)
+                if (hhl[ihh + q].x < hhl[ihh + q].y) {
+                    auto it = pos.find((int64_t)hhl[ihh + q].y * nsp + hhl[ihh + q].x);
+                    if (it == pos.end()) return ccd::fail(CCD_E_INVALID, "hh channel %d lacks a swapped pair", k);
+                    hord.push_back(q);
+                    hord.push_back(it->second);
+                }
+            const int hh2 = (int)hord.size();
+            for (int q = 0; q < H; q++)
+                if (hhl[ihh + q].x == hhl[ihh + q].y) hord.push_back(q);
+            if ((int)hord.size() != H) return ccd::fail(CCD_E_INVALID, "hh channel %d lacks a swapped pair", k);
+            seg.assign(hhl.begin() + ihh, hhl.begin() + ihh + H);
+            for (int q = 0; q < H; q++) {
+                hhl[ihh + q] = seg[hord[q]];
+                hh_old[ihh + q] = hord[q];
+            }
             seg.assign(ppl.begin() + ipp, ppl.begin() + ipp + P);
             t2_in.resize((size_t)(off_t + (int64_t)P * H));
             t2_out.resize((size_t)(off_t + (int64_t)P * H));
@@ -1286,8 +1311,8 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
                 ppl[ipp + q] = seg[order[q]];
                 pp_old[ipp + q] = order[q];
                 for (int ij = 0; ij < H; ij++) {
-                    t2_in[off_t + (int64_t)q * H + ij] = (int32_t)(off_t + (int64_t)order[q] * H + ij);
-                    t2_out[off_t + (int64_t)order[q] * H + ij] = (int32_t)(off_t + (int64_t)q * H + ij);
+                    t2_in[off_t + (int64_t)q * H + ij] = (int32_t)(off_t + (int64_t)order[q] * H + hord[ij]);
+                    t2_out[off_t + (int64_t)order[q] * H + hord[ij]] = (int32_t)(off_t + (int64_t)q * H + ij);
                 }
             }
             pos.clear();
@@ -1299,11 +1324,10 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
             }
             pos.clear();
             for (int q = 0; q < H; q++) pos[(int64_t)hhl[ihh + q].x * nsp + hhl[ihh + q].y] = q;
-            if ((int)pos.size() != H) return ccd::fail(CCD_E_INVALID, "hh channel %d lists a pair twice", k);
-            for (int q = 0; q < H; q++) {
+            for (int q = 0; q < H; q++) {   // the update computes the partner column from IJ
                 auto it = pos.find((int64_t)hhl[ihh + q].y * nsp + hhl[ihh + q].x);
-                if (it == pos.end()) return ccd::fail(CCD_E_INVALID, "hh channel %d lacks a swapped pair", k);
-                hh_swap[ihh + q] = it->second;
+                if (it == pos.end() || it->second != (q < hh2 ? (q ^ 1) : q))
+                    return ccd::fail(CCD_E_INVALID, "hh channel %d: unexpected internal column order", k);
             }
             // pair list: packed rows (a<b) first, then rows (a,a)
             const int pr_off = (int)prow.size();
@@ -1329,7 +1353,7 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
             const int ldv = (npk + 15) / 16 * 16;
             const int nseg = ladder_segments(npk, (H + 15) / 16);
             DChan ch{off_t, off_vp, off_tp, off_hhhh, P, H, (int32_t)ipp, (int32_t)ihh, npk, ldv, pr_off, ndiag,
-                     off_dp, nseg, prob, 0, 0};
+                     off_dp, nseg, prob, 0, hh2};
             c->chans.push_back(ch);
             off_vp += (int64_t)npk * ldv;
             off_tp += (int64_t)npk * H;
@@ -1589,7 +1613,7 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
         (rc = upload(c, &c->d_pp_old, pp_old)) ||
         (rc = upload(c, &c->d_ppl, ppl)) ||
         (rc = upload(c, &c->d_hhl, hhl)) || (rc = upload(c, &c->d_phl, phl)) || (rc = upload(c, &c->d_hpl, hpl)) ||
-        (rc = upload(c, &c->d_hh_swap, hh_swap)) || (rc = upload(c, &c->d_prow, prow)) ||
+        (rc = upload(c, &c->d_hh_old, hh_old)) || (rc = upload(c, &c->d_prow, prow)) ||
         (rc = upload(c, &c->d_prow_sw, prow_sw)) || (rc = upload(c, &c->d_red_ptr, red_ptr)) ||
         (rc = upload(c, &c->d_cpos, cpos)) || (rc = upload(c, &c->d_rpos, rpos)) || (rc = upload(c, &c->d_tr_base, tr_base)) ||
         (rc = upload(c, &c->d_addr_pp, addr_pp)) || (rc = upload(c, &c->d_addr_ph, addr_ph)) ||
@@ -1701,14 +1725,19 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
                 for (int w = 0; w < ch.npk; w++)
                     blkv[(size_t)u * ch.ldv + w] = V[(size_t)old[prow[ch.pr_off + u]] * P + old[prow[ch.pr_off + w]]];
             HIP_TRY(hipMemcpy(c->d_vp + ch.off_vp, blkv.data(), sizeof(double) * blkv.size(), hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(c->d_vh + ch.off_hhhh, p->v_hhhh + o_hhhh[k], sizeof(double) * ch.hh * (size_t)ch.hh,
-                              hipMemcpyHostToDevice));
+            // V^hhhh in the internal column order on both sides
+            const int H = ch.hh;
+            const int32_t *hold = hh_old.data() + ch.hh_off;
+            std::vector<double> vh((size_t)H * H);
+            for (int r = 0; r < H; r++)
+                for (int q = 0; q < H; q++) vh[(size_t)r * H + q] = p->v_hhhh[o_hhhh[k] + (int64_t)hold[r] * H + hold[q]];
+            HIP_TRY(hipMemcpy(c->d_vh + ch.off_hhhh, vh.data(), sizeof(double) * vh.size(), hipMemcpyHostToDevice));
         }
         // V^hhpp has the pphh element count and offsets; stage through d_g, then transpose
         HIP_TRY(hipMemcpy(c->d_g, p->v_hhpp, sizeof(double) * nt, hipMemcpyHostToDevice));
         if (!c->chans.empty())
             hipLaunchKernelGGL(k_transpose_hhpp, dim3((unsigned)c->chans.size()), dim3(kThreads), 0, c->stream,
-                               c->d_chans, c->d_pp_old, c->d_g, c->d_vt);
+                               c->d_chans, c->d_pp_old, c->d_hh_old, c->d_g, c->d_vt);
         if ((rc = check_launch("k_transpose_hhpp"))) return rc;
         HIP_TRY(hipMemcpy(c->d_va, p->v_modA, sizeof(double) * nt, hipMemcpyHostToDevice));
         int mi = 0;

@@ -58,8 +58,9 @@ struct DChan {          // active direct channel (pp > 0 and hh > 0)
     int32_t prob;       // solve the channel belongs to (0 outside a batch)
     int32_t spin0;      // generated electron-gas V: 1 when every pair has one particle of each
                         // spin (one table term per element), 2 when both have the same spin
-                        // (two terms, no spin mask), see k_ladder_v
-    int32_t pad;
+                        // (two terms, no spin mask), see the pppp ladder
+    int32_t hh2;        // columns of the hole pairs i != j: (i,j) at 2v, (j,i) at 2v + 1
+                        // (the pairs (i,i) follow), so column IJ < hh2 has its partner at IJ ^ 1
 };
 
 struct DMod {           // active mod channel (ph > 0 and hp > 0)

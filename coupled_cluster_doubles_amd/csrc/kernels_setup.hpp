@@ -92,16 +92,18 @@ __global__ __launch_bounds__(kThreads) void k_gen_mod(const DMod *__restrict__ m
     }
 }
 
-// Caller V^hhpp (hh x pp) -> pphh layout.
+// Caller V^hhpp (hh x pp) -> pphh layout (internal rows and columns).
 __global__ __launch_bounds__(kThreads) void k_transpose_hhpp(const DChan *__restrict__ chans,
                                                              const int32_t *__restrict__ pp_old,
+                                                             const int32_t *__restrict__ hh_old,
                                                              const double *__restrict__ vhhpp,
                                                              double *__restrict__ vt) {
     const DChan ch = chans[blockIdx.x];
     const int P = ch.pp, H = ch.hh;
     for (int l = threadIdx.x; l < P * H; l += kThreads) {
-        const int AB = l / H, IJ = l - AB * H;   // internal row AB is the caller's row pp_old[AB]
-        vt[ch.off_t + l] = vhhpp[ch.off_t + (int64_t)IJ * P + pp_old[ch.pp_off + AB]];
+        // internal row AB is the caller's row pp_old[AB], column IJ its column hh_old[IJ]
+        const int AB = l / H, IJ = l - AB * H;
+        vt[ch.off_t + l] = vhhpp[ch.off_t + (int64_t)hh_old[ch.hh_off + IJ] * P + pp_old[ch.pp_off + AB]];
     }
 }
 

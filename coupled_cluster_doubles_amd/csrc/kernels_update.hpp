@@ -19,7 +19,6 @@ struct UpdArgs {
     const DChan *chans;
     const int2 *ppl, *hhl;
     const int32_t *prow, *prow_sw;   // row of (a,b) and of (b,a) per pair
-    const int32_t *hh_swap;          // index of (j,i) within the channel
     const double *eps, *chi, *psi;   // chi [nh], psi [np]
     const double *vt;                // <ij||ab> in the pphh layout
     const void *dp;                  // ladder D on packed rows: one slab per k segment, then 1/2 T X
@@ -67,7 +66,7 @@ __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
         // stores alias nothing the loads read, but the compiler cannot know
         // that)
         const int pr = tl.row0 + l / H, IJ = l % H;
-        const int JI = a.hh_swap[ch.hh_off + IJ];
+        const int JI = IJ < ch.hh2 ? (IJ ^ 1) : IJ;   // column (j,i) of column (i,j)
         // rows of the pair in the internal order: packed pair u at 2u (a,b) and
         // 2u+1 (b,a), the pairs (a,a) after them (the prow / prow_sw lists)
         const bool diag = pr >= ch.npk;
