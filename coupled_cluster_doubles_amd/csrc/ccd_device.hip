@@ -129,6 +129,7 @@ struct ccd_ctx {
     std::vector<DChan> chans;
     std::vector<DMod> mods;
     std::vector<DTile> rtiles, ltiles, utiles, xtiles, ttiles;  // packed-row, ladder, update, X, T X tiles
+    std::vector<DTile> xftiles;  // X tiles of channels with more than one k segment (k_fold_x)
     int lt_off[NTMAX + 1] = {0};  // ladder tiles grouped by 16-column tile count: [lt_off[n-1], lt_off[n])
     std::vector<DTile> ltiles0;   // the same tiles, channel order inside each class
     // blocks of the ladder launch of class n: lb_grid[n] blocks, block b runs the
@@ -159,6 +160,7 @@ struct ccd_ctx {
     DChan *d_chans = nullptr;
     DMod *d_mods = nullptr;
     DTile *d_rtiles = nullptr, *d_ltiles = nullptr, *d_utiles = nullptr, *d_xtiles = nullptr, *d_ttiles = nullptr;
+    DTile *d_xftiles = nullptr;
     int2 *d_ppl = nullptr, *d_hhl = nullptr, *d_phl = nullptr, *d_hpl = nullptr;
     int32_t *d_hh_old = nullptr, *d_prow = nullptr, *d_prow_sw = nullptr;   // d_hh_old: caller column of each internal hh column
     int32_t *d_red_ptr = nullptr, *d_cpos = nullptr, *d_rpos = nullptr;
@@ -173,6 +175,7 @@ struct ccd_ctx {
     double *d_t[2] = {nullptr, nullptr};
     int cur = 0;
     double *d_dp = nullptr, *d_tph = nullptr, *d_r = nullptr, *d_g = nullptr, *d_x = nullptr;
+    double *d_xp = nullptr;      // k-segment partial blocks of X (channels of more than one segment)
     double *d_tas = nullptr, *d_red = nullptr;  // d_red: W partial sums in reduction-list order
     double *d_vt = nullptr, *d_vtp = nullptr, *d_vh = nullptr, *d_vp = nullptr, *d_va = nullptr,
            *d_vb = nullptr;
@@ -201,7 +204,7 @@ struct ccd_ctx {
     RingGroup *d_rgroups = nullptr;
     float *d_vp32 = nullptr, *d_va32 = nullptr, *d_vb32 = nullptr;  // FP32 copies (mixed precision)
     double *d_gtab = nullptr;                   // generated-V electron gas table
-    int64_t n_vp = 0, n_tp = 0, n_dp = 0, n_hhhh = 0, n_b = 0;
+    int64_t n_vp = 0, n_tp = 0, n_dp = 0, n_hhhh = 0, n_xp = 0, n_b = 0;
     double flops = 0, flops_pppp = 0;
     // a batch of independent solves merged into this context (nprob = 1 otherwise)
     int nprob = 1;
@@ -453,10 +456,13 @@ int launch_ladder_x(ccd_ctx *c, hipStream_t st) {
     if (c->iter_prec == CCD_PREC_FP32)
         hipLaunchKernelGGL(k_ladder_x<float>, dim3((unsigned)c->xtiles.size()), dim3(kThreads), 0, st, c->d_xtiles,
                            c->d_chans, c->d_vtp, reinterpret_cast<const float *>(c->d_tas), c->d_vh, c->d_zero,
-                           c->d_x, c->d_ctl, skip_ctl(c));
+                           c->d_x, c->d_xp, c->d_ctl, skip_ctl(c));
     else
         hipLaunchKernelGGL(k_ladder_x<double>, dim3((unsigned)c->xtiles.size()), dim3(kThreads), 0, st, c->d_xtiles,
-                           c->d_chans, c->d_vtp, c->d_tas, c->d_vh, c->d_zero, c->d_x, c->d_ctl, skip_ctl(c));
+                           c->d_chans, c->d_vtp, c->d_tas, c->d_vh, c->d_zero, c->d_x, c->d_xp, c->d_ctl, skip_ctl(c));
+    if (!c->xftiles.empty())
+        hipLaunchKernelGGL(k_fold_x, dim3((unsigned)c->xftiles.size()), dim3(kThreads), 0, st, c->d_xftiles, c->d_chans,
+                           c->d_vh, c->d_xp, c->d_x, c->d_ctl, skip_ctl(c));
     return check_launch("k_ladder_x");
 }
 
@@ -1247,7 +1253,7 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
     // ---- direct channels: swaps, packed rows, row pairs
     std::vector<int32_t> pp_swap(npp, -1), hh_old(nhh);
     std::vector<int32_t> prow, prow_sw;
-    int64_t off_t = 0, off_vp = 0, off_tp = 0, off_dp = 0, off_hhhh = 0, ipp = 0, ihh = 0;
+    int64_t off_t = 0, off_vp = 0, off_tp = 0, off_dp = 0, off_hhhh = 0, off_xp = 0, ipp = 0, ihh = 0;
     c->flops = 0; c->flops_pppp = 0;
     c->t2_lo.assign(nprob + 1, 0);
     std::unordered_map<int64_t, int32_t> pos;
@@ -1352,8 +1358,10 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
                                  (long long)npk * H);
             const int ldv = (npk + 15) / 16 * 16;
             const int nseg = ladder_segments(npk, (H + 15) / 16);
+            const int nxs = std::max(1, (npk + KXSEG - 1) / KXSEG);
             DChan ch{off_t, off_vp, off_tp, off_hhhh, P, H, (int32_t)ipp, (int32_t)ihh, npk, ldv, pr_off, ndiag,
-                     off_dp, nseg, prob, 0, hh2};
+                     off_dp, nseg, prob, 0, hh2, off_xp, nxs, 0};
+            if (nxs > 1) off_xp += (int64_t)nxs * H * H;
             c->chans.push_back(ch);
             off_vp += (int64_t)npk * ldv;
             off_tp += (int64_t)npk * H;
@@ -1368,7 +1376,7 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
     c->nt2 = off_t;
     while (prob + 1 < nprob) c->t2_lo[++prob] = off_t;
     c->t2_lo[nprob] = off_t;
-    c->n_vp = off_vp; c->n_tp = off_tp; c->n_dp = off_dp; c->n_hhhh = off_hhhh;
+    c->n_vp = off_vp; c->n_tp = off_tp; c->n_dp = off_dp; c->n_hhhh = off_hhhh; c->n_xp = off_xp;
     if (c->nt2 >= (int64_t)INT32_MAX)
         return ccd::fail(CCD_E_INVALID, "T2 has %lld elements (limit 2^31)", (long long)c->nt2);
 
@@ -1568,7 +1576,10 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
     for (int k = 0; k < (int)c->chans.size(); k++) {
         const DChan &ch = c->chans[k];
         for (int r = 0; r < ch.hh; r += 16)
-            for (int q = 0; q < ch.hh; q += 16) c->xtiles.push_back(DTile{k, r, 16, q});
+            for (int q = 0; q < ch.hh; q += 16) {
+                for (int sg = 0; sg < ch.nxs; sg++) c->xtiles.push_back(DTile{k, r, sg, q});
+                if (ch.nxs > 1) c->xftiles.push_back(DTile{k, r, 16, q});
+            }
         const int npairs = ch.npk + ch.ndiag, per = std::max(1, kThreads / ch.hh);
         for (int r = 0; r < npairs; r += per) {
             c->utiles.push_back(DTile{k, r, std::min(per, npairs - r), (int32_t)c->n_cp});
@@ -1607,7 +1618,8 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
     if ((rc = upload(c, &c->d_chans, c->chans)) || (rc = upload(c, &c->d_mods, c->mods)) ||
         (rc = upload(c, &c->d_rtiles, c->rtiles)) || (rc = upload(c, &c->d_ltiles, c->ltiles)) ||
         (rc = upload(c, &c->d_utiles, c->utiles)) ||
-        (rc = upload(c, &c->d_xtiles, c->xtiles)) || (rc = upload(c, &c->d_ttiles, c->ttiles)) || (rc = upload(c, &c->d_ritems, c->ritems)) ||
+        (rc = upload(c, &c->d_xtiles, c->xtiles)) || (rc = upload(c, &c->d_xftiles, c->xftiles)) ||
+        (rc = upload(c, &c->d_ttiles, c->ttiles)) || (rc = upload(c, &c->d_ritems, c->ritems)) ||
         (rc = upload(c, &c->d_rgroups, c->rgroups)) || (rc = upload(c, &c->d_lbptr, c->lbptr)) ||
         (rc = upload(c, &c->d_t2_in, t2_in)) || (rc = upload(c, &c->d_t2_out, t2_out)) ||
         (rc = upload(c, &c->d_pp_old, pp_old)) ||
@@ -1627,6 +1639,8 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
         (rc = dalloc(c, &c->d_g, nt)) || (rc = dalloc(c, &c->d_vt, nt)) || (rc = dalloc(c, &c->d_va, nt)) ||
         (rc = dalloc(c, &c->d_dp, (size_t)c->n_dp)) || (rc = dalloc(c, &c->d_tas, (size_t)c->n_tp)) ||
         (rc = dalloc(c, &c->d_vtp, (size_t)c->n_tp)) || (rc = dalloc(c, &c->d_x, (size_t)c->n_hhhh)) ||
+        (rc = dalloc(c, &c->d_xp, (size_t)std::max<int64_t>(c->n_xp, 1))) ||
+
         (rc = dalloc(c, &c->d_vh, (size_t)c->n_hhhh)) ||
         (rc = dalloc(c, &c->d_vp, c->matrix_free ? (size_t)1 : (size_t)c->n_vp)) ||
         (rc = dalloc(c, &c->d_vb, (size_t)c->n_b)) || (rc = dalloc(c, &c->d_red, (size_t)std::max<int64_t>(c->n_red, 1))) ||

@@ -61,6 +61,9 @@ struct DChan {          // active direct channel (pp > 0 and hh > 0)
                         // (two terms, no spin mask), see the pppp ladder
     int32_t hh2;        // columns of the hole pairs i != j: (i,j) at 2v, (j,i) at 2v + 1
                         // (the pairs (i,i) follow), so column IJ < hh2 has its partner at IJ ^ 1
+    int64_t off_xp;     // the nxs k-segment partial blocks (hh x hh) of X = V^hhhh + V^hhpp T
+    int32_t nxs;
+    int32_t pad;
 };
 
 struct DMod {           // active mod channel (ph > 0 and hp > 0)

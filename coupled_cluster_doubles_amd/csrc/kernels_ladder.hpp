@@ -13,15 +13,23 @@
 #include "device_common.hpp"
 
 constexpr int KC = 32;                 // k per chunk of k_ladder_x (k-slot q of step s: k = 8q + s)
+constexpr int KXSEG = 256;             // packed pairs per k segment of X (a partial X block each)
 constexpr int NTMAX = 8;               // 16-column tiles: hh <= 128
 constexpr int LROWS = 64;              // packed rows per ladder block (16 per wave)
 constexpr int WROWS = 16;              // packed rows per wave tile (T X term)
 constexpr int kGasTableMax = 4096;     // entries of the generated-V gas table (LDS, 32 KB)
 
-// X = V^hhhh + sum_{c<d} V^hhpp[kl][cd] tas[cd][ij], one 16x16 tile of X per
-// block; the four waves take every fourth k chunk and their partial tiles are
-// added in wave order through LDS.  FP64 arithmetic; tas in TT (FP32 in the
-// FP32 iterations, whose ladder reads it so).
+// X = V^hhhh + sum_{c<d} V^hhpp[kl][cd] tas[cd][ij], split over k segments of
+// KXSEG packed pairs: a block computes one 16x16 tile of one segment
+// (tile.row0 = first row kl, tile.aux = first column ij, tile.nrows =
+// segment), so that every wave has at most two k chunks in its dependent
+// chain; the four waves take every fourth chunk and their partial tiles are
+// added in wave order through LDS.  A channel of one segment writes X
+// (V^hhhh added) directly; the others write their segments' partial blocks
+// (xp), which k_fold_x (kernels_update.hpp) adds in segment order.  (A last-block fold inside
+// this kernel needs device-scope fences, which write back the L2 of the
+// XCD: X took 167 us instead of 14.)  FP64 arithmetic; tas in TT (FP32 in
+// the FP32 iterations, whose ladder reads it so).
 template <typename TT>
 __global__ __launch_bounds__(kThreads) void k_ladder_x(const DTile *__restrict__ tiles,
                                                        const DChan *__restrict__ chans,
@@ -30,6 +38,7 @@ __global__ __launch_bounds__(kThreads) void k_ladder_x(const DTile *__restrict__
                                                        const double *__restrict__ vh,
                                                        const double *__restrict__ zero,
                                                        double *__restrict__ x,
+                                                       double *__restrict__ xp,
                                                        const LoopCtl *__restrict__ ctl,
                                                        const LoopCtl *__restrict__ pctl) {
     if (ctl->done) return;
@@ -38,18 +47,19 @@ __global__ __launch_bounds__(kThreads) void k_ladder_x(const DTile *__restrict__
     const DTile tl = tiles[blockIdx.x];
     const DChan ch = chans[tl.chan];
     if (block_solve_stopped(pctl, ch.prob)) return;   // a stopped solve of a batch
-    const int H = ch.hh, npk = ch.npk, kl0 = tl.row0, ij0 = tl.aux;
+    const int H = ch.hh, kl0 = tl.row0, ij0 = tl.aux, seg = tl.nrows;
+    const int kend = min(ch.npk, (seg + 1) * KXSEG);
     const bool aok = kl0 + m < H, bok = ij0 + m < H;
     const TT *zt = reinterpret_cast<const TT *>(zero);
     dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = wave * KC; k0 < npk; k0 += KC * (kThreads / 64)) {
+    for (int k0 = seg * KXSEG + wave * KC; k0 < kend; k0 += KC * (kThreads / 64)) {
         // every load unconditional (elements outside the block read the zero
         // buffer), so that the 16 loads of a chunk are in flight together
         double a[8], b[8];
 #pragma unroll
         for (int s = 0; s < 8; s++) {
             const int w = k0 + kq * 8 + s;
-            const bool ok = w < npk;
+            const bool ok = w < kend;
             a[s] = *((ok && aok) ? vtp + ch.off_tp + (int64_t)w * H + kl0 + m : zero);
             b[s] = (double)*((ok && bok) ? tas + ch.off_tp + (int64_t)w * H + ij0 + m : zt);
         }
@@ -65,8 +75,9 @@ __global__ __launch_bounds__(kThreads) void k_ladder_x(const DTile *__restrict__
             const double s = ((part[0][lane][q] + part[1][lane][q]) + part[2][lane][q]) + part[3][lane][q];
             const int row = kl0 + q * 4 + kq, col = ij0 + m;
             if (row < H && col < H) {
-                const int64_t o = ch.off_hhhh + (int64_t)row * H + col;
-                x[o] = vh[o] + s;
+                const int64_t o = (int64_t)row * H + col;
+                if (ch.nxs == 1) x[ch.off_hhhh + o] = vh[ch.off_hhhh + o] + s;
+                else xp[ch.off_xp + (int64_t)seg * H * H + o] = s;
             }
         }
     }

@@ -416,3 +416,32 @@ __global__ __launch_bounds__(kThreads) void k_expand_d(UpdArgs a, double *__rest
         d[ch.off_t + (int64_t)r2 * H + IJ] = -v;
     }
 }
+
+// X of the channels of more than one k segment: V^hhhh plus the segments'
+// partial blocks in segment order, one 16x16 tile per block (tile.row0,
+// tile.aux: its first row and column).
+__global__ __launch_bounds__(kThreads) void k_fold_x(const DTile *__restrict__ tiles,
+                                                     const DChan *__restrict__ chans,
+                                                     const double *__restrict__ vh,
+                                                     const double *__restrict__ xp, double *__restrict__ x,
+                                                     const LoopCtl *__restrict__ ctl,
+                                                     const LoopCtl *__restrict__ pctl) {
+    if (ctl->done) return;
+    const DTile tl = tiles[blockIdx.x];
+    const DChan ch = chans[tl.chan];
+    if (wave_solve_stopped(pctl, ch.prob)) return;   // no barrier below: a per-wave answer suffices
+    const int H = ch.hh, row = tl.row0 + (threadIdx.x >> 4), col = tl.aux + (threadIdx.x & 15);
+    if (row >= H || col >= H) return;
+    const int64_t o = (int64_t)row * H + col, hh2 = (int64_t)H * H;
+    // the segments' loads issued four at a time (a clamped load is not added)
+    double v = vh[ch.off_hhhh + o];
+    for (int q0 = 0; q0 < ch.nxs; q0 += 4) {
+        double t[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) t[u] = xp[ch.off_xp + min(q0 + u, ch.nxs - 1) * hh2 + o];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (q0 + u < ch.nxs) v += t[u];
+    }
+    x[ch.off_hhhh + o] = v;
+}
