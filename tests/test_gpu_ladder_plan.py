@@ -113,6 +113,49 @@ def test_kernel_kinds_close_to_oracle(require_gpu, kind, args):
         assert np.array_equal(ctx.ladder(ta), Context(m, FP64).ladder(ta))
 
 
+def _antisym_fast(m, t):
+    """(1 - P_ab)(1 - P_ij) t over the pphh labels (vectorised _antisym)."""
+    lab = m.t2_labels().astype(np.int64)
+    n = int(lab.max()) + 1
+    key = lambda a, b, i, j: ((a * n + b) * n + i) * n + j
+    k0 = key(lab[:, 0], lab[:, 1], lab[:, 2], lab[:, 3])
+    order = np.argsort(k0)
+    def where(k):
+        pos = order[np.searchsorted(k0, k, sorter=order)]
+        assert np.array_equal(k0[pos], k)
+        return pos
+    sw_ab = where(key(lab[:, 1], lab[:, 0], lab[:, 2], lab[:, 3]))
+    sw_ij = where(key(lab[:, 0], lab[:, 1], lab[:, 3], lab[:, 2]))
+    ta = t - t[sw_ab]
+    return ta - ta[sw_ij]
+
+
+def test_ladder_kinds_at_n54_against_oracle(require_gpu):
+    """The production workload (electron gas N=54, matrix-free V): the pppp
+    ladder of every FP32 kernel kind against the oracle's, within the kind's
+    tolerance relative to the largest element, and the FP64 ladder within
+    1e-12.  Classes 1-4 of the plan (one to four 16-column tiles, hh up to
+    54) occur here."""
+    args = (1.0, 32, 4)
+    m, o = Model.electron_gas(*args), Oracle.electron_gas(*args, matrix_free=True)
+    ta = _antisym_fast(m, np.random.default_rng(54).uniform(-0.05, 0.05, m.n_t2))
+    d_o = o.ladder(ta)
+    o.close()
+    scale = np.max(np.abs(d_o))
+    ctx = Context(m, FP64, matrix_free=True)
+    err = np.max(np.abs(ctx.ladder(ta) - d_o)) / scale
+    print(f"N=54 FP64: max |D - D_oracle| / max|D_oracle| = {err:.2e}", flush=True)
+    assert err <= 1e-12
+    ctx.close()
+    for kinds, tol in ((["default"] * 8, F32_REL), (["mfma16"] * 8, F32_REL), (["bf16x3"] * 8, BF16X3_REL)):
+        ctx = Context(m, FP32, matrix_free=True)
+        _plan(ctx, kind_fp32=kinds)
+        err = np.max(np.abs(ctx.ladder(ta) - d_o)) / scale
+        print(f"N=54 kinds={kinds[0]}: max |D - D_oracle| / max|D_oracle| = {err:.2e}", flush=True)
+        assert err <= tol, (kinds, err)
+        ctx.close()
+
+
 @pytest.mark.parametrize("kinds", [["bf16x3"] * 8, ["mfma16"] * 8])
 def test_stored_and_generated_v_agree_bitwise_under_every_kind(require_gpu, kinds):
     m = Model.electron_gas(1.0, 19, 2)
