@@ -163,7 +163,7 @@ struct ccd_ctx {
     DTile *d_xftiles = nullptr;
     int2 *d_ppl = nullptr, *d_hhl = nullptr, *d_phl = nullptr, *d_hpl = nullptr;
     int32_t *d_hh_old = nullptr, *d_prow = nullptr, *d_prow_sw = nullptr;   // d_hh_old: caller column of each internal hh column
-    int32_t *d_red_ptr = nullptr, *d_cpos = nullptr, *d_rpos = nullptr;
+    int32_t *d_red_ptr = nullptr, *d_red_idx = nullptr;   // reduction lists: ranges, places in d_red
     int64_t n_red = 0;
     int32_t *d_tr_base = nullptr, *d_addr_pp = nullptr, *d_addr_ph = nullptr;
     // internal row order of the pphh layout (see build): element maps to and
@@ -176,7 +176,7 @@ struct ccd_ctx {
     int cur = 0;
     double *d_dp = nullptr, *d_tph = nullptr, *d_r = nullptr, *d_g = nullptr, *d_x = nullptr;
     double *d_xp = nullptr;      // k-segment partial blocks of X (channels of more than one segment)
-    double *d_tas = nullptr, *d_red = nullptr;  // d_red: W partial sums in reduction-list order
+    double *d_tas = nullptr, *d_red = nullptr;  // d_red: W partial sums (column partials, then row sums)
     double *d_vt = nullptr, *d_vtp = nullptr, *d_vh = nullptr, *d_vp = nullptr, *d_va = nullptr,
            *d_vb = nullptr;
     double *d_partials = nullptr, *d_scalar = nullptr;
@@ -289,7 +289,7 @@ UpdArgs upd_args(const ccd_ctx *c, const double *t_old, double *t_new, double mi
     a.prow = c->d_prow; a.prow_sw = c->d_prow_sw;
     a.eps = c->d_eps; a.chi = c->d_onebody; a.psi = c->d_onebody + c->nh;
     a.vt = c->d_vt; a.dp = c->d_dp; a.g = c->d_g; a.t_old = t_old; a.t_new = t_new;
-    a.tas = c->d_tas; a.red = c->d_red; a.cpos = c->d_cpos; a.rpos = c->d_rpos; a.mixing = mixing; a.nh = c->nh; a.ctl = c->d_ctl;
+    a.tas = c->d_tas; a.red = c->d_red; a.n_cp = c->n_cp; a.mixing = mixing; a.nh = c->nh; a.ctl = c->d_ctl;
     a.pctl = c->d_pctl;
     return a;
 }
@@ -576,7 +576,8 @@ int launch_update(ccd_ctx *c, const double *t_old, double *t_new, double mixing,
 // bookkeep = false: chi and psi only, whatever the loop state (no energy, no
 // loop step)
 int launch_reduce(ccd_ctx *c, hipStream_t st, bool bookkeep) {
-    hipLaunchKernelGGL(k_reduce, dim3((unsigned)c->nsp), dim3(kThreads), 0, st, c->d_red_ptr, c->nh, c->d_red,
+    hipLaunchKernelGGL(k_reduce, dim3((unsigned)c->nsp), dim3(kThreads), 0, st, c->d_red_ptr, c->d_red_idx, c->nh,
+                       c->d_red,
                        c->d_onebody, c->d_ticket, c->d_scalar, c->d_ctl, c->d_elog, c->nprob, c->d_hlo,
                        c->d_pctl, bookkeep ? 1 : 0);
     return check_launch("k_reduce");
@@ -1604,13 +1605,13 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
         red_idx.insert(red_idx.end(), v.begin(), v.end());
         red_ptr.push_back((int32_t)red_idx.size());
     }
-    // the update writes each partial straight to its place in the list order,
-    // so every chi_i / psi_a is a sum over a contiguous range
-    std::vector<int32_t> cpos((size_t)std::max<int64_t>(c->n_cp, 1), -1), rpos((size_t)std::max<int64_t>(npp, 1), -1);
-    for (int32_t q = 0; q < (int32_t)red_idx.size(); q++) {
-        if (q < red_ptr[nh]) cpos[red_idx[q]] = q;
-        else rpos[red_idx[q]] = q;
-    }
+    // the update stores the column partials at their slots and the row sums
+    // after them (n_cp + row; rows of channels without hole pairs have no
+    // sum and are in no list), so the lists hold places in that array
+    if (c->n_cp + npp >= (int64_t)INT32_MAX) return ccd::fail(CCD_E_INVALID, "too many partial sums");
+    if ((int64_t)red_idx.size() > c->n_cp + npp || red_ptr[nh] != c->n_cp)
+        return ccd::fail(CCD_E_INVALID, "reduction lists do not match the partial sums");
+    for (size_t q = (size_t)red_ptr[nh]; q < red_idx.size(); q++) red_idx[q] += (int32_t)c->n_cp;
     c->n_red = (int64_t)red_idx.size();
 
     // ---- upload tables
@@ -1627,7 +1628,7 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
         (rc = upload(c, &c->d_hhl, hhl)) || (rc = upload(c, &c->d_phl, phl)) || (rc = upload(c, &c->d_hpl, hpl)) ||
         (rc = upload(c, &c->d_hh_old, hh_old)) || (rc = upload(c, &c->d_prow, prow)) ||
         (rc = upload(c, &c->d_prow_sw, prow_sw)) || (rc = upload(c, &c->d_red_ptr, red_ptr)) ||
-        (rc = upload(c, &c->d_cpos, cpos)) || (rc = upload(c, &c->d_rpos, rpos)) || (rc = upload(c, &c->d_tr_base, tr_base)) ||
+        (rc = upload(c, &c->d_red_idx, red_idx)) || (rc = upload(c, &c->d_tr_base, tr_base)) ||
         (rc = upload(c, &c->d_addr_pp, addr_pp)) || (rc = upload(c, &c->d_addr_ph, addr_ph)) ||
         (rc = upload(c, &c->d_perm_ph, perm_ph)) || (rc = upload(c, &c->d_perm_pp, perm_pp)))
         return rc;
@@ -1643,7 +1644,7 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
 
         (rc = dalloc(c, &c->d_vh, (size_t)c->n_hhhh)) ||
         (rc = dalloc(c, &c->d_vp, c->matrix_free ? (size_t)1 : (size_t)c->n_vp)) ||
-        (rc = dalloc(c, &c->d_vb, (size_t)c->n_b)) || (rc = dalloc(c, &c->d_red, (size_t)std::max<int64_t>(c->n_red, 1))) ||
+        (rc = dalloc(c, &c->d_vb, (size_t)c->n_b)) || (rc = dalloc(c, &c->d_red, (size_t)std::max<int64_t>(c->n_cp + npp, 1))) ||
         (rc = dalloc(c, &c->d_partials, std::max(c->utiles.size(), (size_t)1))) ||
         (rc = dalloc(c, &c->d_scalar, (size_t)std::max(4, nprob))) ||
         (rc = dalloc(c, &c->d_zero, (size_t)kZeroDoubles)) || (rc = dalloc(c, &c->d_ctl, 1)) || (rc = dalloc(c, &c->d_ticket, 1)) ||
@@ -2408,7 +2409,7 @@ int ccd_kernel_work(const ccd_ctx *ctx, double *flops, double *bytes, double *el
     }
     // per element: <ij||ab>, T_old, the ring term (contraction type) in, T_new out
     b[CCD_KT_UPDATE] += (8.0 + 8.0 + cs + 8.0) * nt;
-    b[CCD_KT_REDUCE] = 8.0 * (double)ctx->n_red + 8.0 * ctx->nsp;
+    b[CCD_KT_REDUCE] = 12.0 * (double)ctx->n_red + 8.0 * ctx->nsp;
     for (int k = 0; k < n && k < CCD_KT_COUNT; k++) {
         if (flops) flops[k] = f[k];
         if (bytes) bytes[k] = b[k];

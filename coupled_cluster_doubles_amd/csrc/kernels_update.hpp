@@ -27,9 +27,9 @@ struct UpdArgs {
     const double *t_old;
     double *t_new;                   // new amplitudes
     void *tas;                       // their (a,b)-antisymmetric packed rows, in OT
-    double *red;                     // W = V o T partial sums in reduction-list order:
-    const int32_t *cpos;             //   column partial (tile aux + IJ) -> place (chi_i lists)
-    const int32_t *rpos;             //   row sum (pp_off + row) -> place (psi_a lists)
+    double *red;                     // W = V o T partial sums, each tile's stored contiguously:
+                                     //   the column partials at [tile aux + IJ], then,
+    int64_t n_cp;                    //   from n_cp on, the row sums at [pp_off + row]
     double mixing;
     int nh;
     const LoopCtl *ctl;              // loop state of the context
@@ -57,8 +57,8 @@ __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
     if (frozen && a.pctl[ch.prob].idle >= 2) return;
     const int H = ch.hh, n = tl.nrows * H;
     const int l = threadIdx.x & (kThreads - 1);
-    // places of this slot's sums in the reduction order, loaded with the
-    // element's indices so that the final stores need no further round trip
+    // places of this slot's sums (a tile's column partials and row sums are
+    // contiguous, so its stores coalesce; the reductions gather them)
     int rp1 = 0, rp2 = 0, cp = 0;
     bool pdiag = true;
     if (l < n) {
@@ -71,9 +71,9 @@ __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
         // 2u+1 (b,a), the pairs (a,a) after them (the prow / prow_sw lists)
         const bool diag = pr >= ch.npk;
         const int r1 = diag ? ch.npk + pr : 2 * pr, r2 = diag ? r1 : r1 + 1;
-        cp = a.cpos[tl.aux + IJ];
-        rp1 = a.rpos[ch.pp_off + r1];
-        rp2 = a.rpos[ch.pp_off + r2];
+        cp = tl.aux + IJ;
+        rp1 = ch.pp_off + r1;
+        rp2 = ch.pp_off + r2;
         pdiag = diag;
         const int64_t b1 = ch.off_t + (int64_t)r1 * H, b2 = ch.off_t + (int64_t)r2 * H;
         const double v1 = a.vt[b1 + IJ], v2 = a.vt[b2 + IJ];
@@ -166,8 +166,8 @@ __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
         __syncthreads();
     }
     if (l < n && cq == 0) {
-        a.red[rp1] = w1s[l];
-        if (!pdiag) a.red[rp2] = w2s[l];
+        a.red[a.n_cp + rp1] = w1s[l];
+        if (!pdiag) a.red[a.n_cp + rp2] = w2s[l];
     }
     if (l < n && pq == 0) a.red[cp] = cs[l];
 }
@@ -241,8 +241,9 @@ __global__ __launch_bounds__(kThreads) void k_transpose(const int32_t *__restric
 }
 
 // chi_i (blocks 0..nh-1) and psi_a (blocks nh..nsp-1) as fixed-order sums of
-// their column partials / row sums, which the update stored contiguously
-// (parts[ptr[k] .. ptr[k+1])).  The chi block that finishes last
+// their column partials / row sums: parts[idx[q]] for q in ptr[k] ..
+// ptr[k+1] (the update stores each tile's sums contiguously, the lists
+// gather them).  The chi block that finishes last
 // (ticket counter) then forms, for every solve p still running, E_p = 1/4
 // sum of chi_i over its holes [hlo[p], hlo[p+1]) (= 1/4 sum V o T: every
 // element has one first hole index) in a fixed order, and runs the loop
@@ -254,7 +255,8 @@ __global__ __launch_bounds__(kThreads) void k_transpose(const int32_t *__restric
 // (an empty problem) block 0 does that part.  bookkeep = 0: chi and psi only
 // (rebuilt from the current amplitudes), whatever the loop state, no energy
 // and no loop step.
-__global__ __launch_bounds__(kThreads) void k_reduce(const int32_t *__restrict__ ptr, int nh,
+__global__ __launch_bounds__(kThreads) void k_reduce(const int32_t *__restrict__ ptr,
+                                                     const int32_t *__restrict__ idx, int nh,
                                                      const double *__restrict__ parts,
                                                      double *__restrict__ out, int *__restrict__ ticket,
                                                      double *__restrict__ energy, LoopCtl *ctl,
@@ -265,20 +267,23 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const int32_t *__restrict__
     __shared__ double red[kThreads / 64];
     __shared__ int last, pdone;
     const int k = blockIdx.x;
-    // thread t adds parts[lo + t], parts[lo + t + 256], ... in order; the
-    // loads of 8 strides are issued before their adds (a slot past the list
-    // reads element lo and is not added), so a thread waits for one memory
-    // round trip per 8 elements
+    // thread t adds list entries lo + t, lo + t + 256, ... in order; the
+    // index and then the value loads of 8 strides are issued before their
+    // adds (a slot past the list reads entry lo and is not added), so a
+    // thread waits for two memory round trips per 8 elements
     constexpr int U = 8;
     const int lo = ptr[k], hi = ptr[k + 1];
     double s = 0.0;
     for (int q0 = lo + threadIdx.x; q0 < hi; q0 += kThreads * U) {
+        int ix[U];
         double v[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int q = q0 + u * kThreads;
-            v[u] = parts[q < hi ? q : lo];
+            ix[u] = idx[q < hi ? q : lo];
         }
+#pragma unroll
+        for (int u = 0; u < U; u++) v[u] = parts[ix[u]];
 #pragma unroll
         for (int u = 0; u < U; u++)
             if (q0 + u * kThreads < hi) s += v[u];
