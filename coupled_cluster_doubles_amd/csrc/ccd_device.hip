@@ -111,6 +111,7 @@ static double *scratch_ptr(int device) {
 // ------------------------------------------------------------------ context
 
 constexpr int kStale = -1;   // tph_prec / tas_prec: not derived from the current amplitudes, rebuild before iterating
+constexpr int kAnyPrec = -2;  // ob_prec: chi = psi = 0 (T = 0), right for either precision
 
 struct ccd_ctx {
     int device = 0;
@@ -156,6 +157,9 @@ struct ccd_ctx {
     // chi / psi may not be those of the current amplitudes: the psi blocks of
     // the reduction that stopped a device loop may have returned at entry
     bool onebody_stale = false;
+    // the V precision of the W sums behind chi and psi (FP32 iterations read
+    // V's FP32 rounding; kAnyPrec: chi = psi = 0, from T = 0)
+    int ob_prec = CCD_PREC_FP64;
     // device tables
     DChan *d_chans = nullptr;
     DMod *d_mods = nullptr;
@@ -202,7 +206,7 @@ struct ccd_ctx {
     int2 *d_rec1 = nullptr, *d_rec2 = nullptr, *d_rec1f = nullptr, *d_rec2f = nullptr, *d_rec1w = nullptr;  // generated-V pair records
     RingItem *d_ritems = nullptr;
     RingGroup *d_rgroups = nullptr;
-    float *d_vp32 = nullptr, *d_va32 = nullptr, *d_vb32 = nullptr;  // FP32 copies (mixed precision)
+    float *d_vp32 = nullptr, *d_va32 = nullptr, *d_vb32 = nullptr, *d_vt32 = nullptr;  // FP32 copies (mixed precision)
     double *d_gtab = nullptr;                   // generated-V electron gas table
     int64_t n_vp = 0, n_tp = 0, n_dp = 0, n_hhhh = 0, n_xp = 0, n_b = 0;
     double flops = 0, flops_pppp = 0;
@@ -288,7 +292,7 @@ UpdArgs upd_args(const ccd_ctx *c, const double *t_old, double *t_new, double mi
     a.tiles = c->d_utiles; a.chans = c->d_chans; a.ppl = c->d_ppl; a.hhl = c->d_hhl;
     a.prow = c->d_prow; a.prow_sw = c->d_prow_sw;
     a.eps = c->d_eps; a.chi = c->d_onebody; a.psi = c->d_onebody + c->nh;
-    a.vt = c->d_vt; a.dp = c->d_dp; a.g = c->d_g; a.t_old = t_old; a.t_new = t_new;
+    a.vt = c->d_vt; a.vt32 = c->d_vt32; a.dp = c->d_dp; a.g = c->d_g; a.t_old = t_old; a.t_new = t_new;
     a.tas = c->d_tas; a.red = c->d_red; a.n_cp = c->n_cp; a.mixing = mixing; a.nh = c->nh; a.ctl = c->d_ctl;
     a.pctl = c->d_pctl;
     return a;
@@ -415,7 +419,7 @@ int sync_state(ccd_ctx *c) {
     int rc;
     double *t = c->d_t[c->cur];
     if (c->tas_prec != c->iter_prec && (rc = launch_antisym(c, t))) return rc;
-    if (c->onebody_stale) {
+    if (c->onebody_stale || (c->ob_prec != kAnyPrec && c->ob_prec != c->iter_prec)) {
         if ((rc = launch_update<false>(c, t, nullptr, 0.0, c->stream)) || (rc = launch_reduce(c, c->stream, false)))
             return rc;
         c->onebody_stale = false;
@@ -566,7 +570,8 @@ int launch_update(ccd_ctx *c, const double *t_old, double *t_new, double mixing,
     if (UPDATE) c->tas_prec = c->iter_prec;   // the update writes them in OT
     if (c->utiles.empty()) return CCD_OK;
     UpdArgs a = upd_args(c, t_old, t_new, mixing);
-    if (UPDATE && c->iter_prec == CCD_PREC_FP32)
+    c->ob_prec = c->iter_prec;   // the W sums it writes
+    if (c->iter_prec == CCD_PREC_FP32)
         hipLaunchKernelGGL((k_update<UPDATE, float>), dim3((unsigned)c->utiles.size()), dim3(kThreads), 0, st, a);
     else
         hipLaunchKernelGGL((k_update<UPDATE, double>), dim3((unsigned)c->utiles.size()), dim3(kThreads), 0, st, a);
@@ -1778,6 +1783,7 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
         };
         // V^pppp: generated contexts built theirs above (or generate it in the ladder)
         if ((rc = to_f32(c->d_va, &c->d_va32, c->nt2)) || (rc = to_f32(c->d_vb, &c->d_vb32, c->n_b)) ||
+            (rc = to_f32(c->d_vt, &c->d_vt32, c->nt2)) ||
             (!c->d_vp32 && (rc = to_f32(c->d_vp, &c->d_vp32, c->matrix_free ? 0 : c->n_vp))))
             return rc;
     }
@@ -1914,6 +1920,7 @@ int zero_state(ccd_ctx *c) {
     c->tas_prec = c->iter_prec;
     HIP_TRY(hipMemsetAsync(c->d_onebody, 0, sizeof(double) * c->nsp, c->stream));
     c->onebody_stale = false;
+    c->ob_prec = kAnyPrec;
     c->cur = 0;
     return arm_free_run(c);
 }

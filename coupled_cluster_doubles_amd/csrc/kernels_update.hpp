@@ -21,6 +21,7 @@ struct UpdArgs {
     const int32_t *prow, *prow_sw;   // row of (a,b) and of (b,a) per pair
     const double *eps, *chi, *psi;   // chi [nh], psi [np]
     const double *vt;                // <ij||ab> in the pphh layout
+    const float *vt32;               // its FP32 rounding (what the update reads in FP32 iterations)
     const void *dp;                  // ladder D on packed rows: one slab per k segment, then 1/2 T X
     const void *g;                   // ring R gathered to the pphh layout: g[(a,b),(i,j)] = R[(a,i),(j,b)]
                                      // (both in the contractions' arithmetic type)
@@ -37,7 +38,8 @@ struct UpdArgs {
 };
 
 // UPDATE = true: one Jacobi step for both rows of each pair; UPDATE = false:
-// only the W sums of the current amplitudes (to derive chi, psi, E from them).
+// only the W sums of the current amplitudes (to derive chi, psi, E from them),
+// with V in the same arithmetic as the iterations that follow.
 // A solve of a batch that has stopped keeps its amplitudes: its new T is the
 // old one (written while the other buffer may still hold older amplitudes).  OT: type of the ladder and ring outputs (float after FP32
 // contractions, which they hold exactly); the update itself runs in FP64.
@@ -76,7 +78,16 @@ __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
         rp2 = ch.pp_off + r2;
         pdiag = diag;
         const int64_t b1 = ch.off_t + (int64_t)r1 * H, b2 = ch.off_t + (int64_t)r2 * H;
-        const double v1 = a.vt[b1 + IJ], v2 = a.vt[b2 + IJ];
+        // V in the arithmetic of the iteration: FP32 iterations add its FP32
+        // rounding, as their contractions are FP32 (the sums stay FP64)
+        double v1, v2;
+        if constexpr (sizeof(OT) == 4) {
+            v1 = a.vt32[b1 + IJ];
+            v2 = a.vt32[b2 + IJ];
+        } else {
+            v1 = a.vt[b1 + IJ];
+            v2 = a.vt[b2 + IJ];
+        }
         const double t1 = a.t_old[b1 + IJ], t2 = a.t_old[b2 + IJ];
         double w1, w2;
         if (UPDATE) {
