@@ -2414,8 +2414,9 @@ int ccd_kernel_work(const ccd_ctx *ctx, double *flops, double *bytes, double *el
         // the update: D read once per packed row (one array in the algorithm)
         b[CCD_KT_UPDATE] += cs * npk * hh + cs * npk * hh;               // D in, T rows out
     }
-    // per element: <ij||ab>, T_old, the ring term (contraction type) in, T_new out
-    b[CCD_KT_UPDATE] += (8.0 + 8.0 + cs + 8.0) * nt;
+    // per element: <ij||ab> and the ring term (both in the contraction type),
+    // T_old in, T_new out
+    b[CCD_KT_UPDATE] += (cs + 8.0 + cs + 8.0) * nt;
     b[CCD_KT_REDUCE] = 12.0 * (double)ctx->n_red + 8.0 * ctx->nsp;
     for (int k = 0; k < n && k < CCD_KT_COUNT; k++) {
         if (flops) flops[k] = f[k];
