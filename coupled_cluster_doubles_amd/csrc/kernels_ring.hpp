@@ -173,13 +173,11 @@ __device__ __forceinline__ void ring_staged_block(const RingGroup &grp, const DM
                                                   const CT *__restrict__ vb, const double *__restrict__ zero,
                                                   CT *__restrict__ r, unsigned char *lds) {
     typedef RingStage<CT, NT> S;
-    typedef typename Mma<CT>::acc Acc;
     constexpr int D = S::D, STR = S::STR, G = S::G;
-    constexpr bool F64 = sizeof(CT) == 8;
     // the wave index is uniform across the wave: kept in scalar registers, so
     // are the channel descriptor and the column-tile loop below
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
+    const int lane = threadIdx.x & 63, m = lane & 15;
     const CT *zc = reinterpret_cast<const CT *>(zero);
     DMod md[G];
 #pragma unroll
@@ -194,7 +192,7 @@ __device__ __forceinline__ void ring_staged_block(const RingGroup &grp, const DM
 #pragma unroll
     for (int g = 1; g < G; g++)
         if (g0 == g) my = md[g];
-    const int PH = my.ph, HP = my.hp;
+    const int HP = my.hp;
 
     // V^B(:, jt0), the initial value of X(:, jt0): issued first, consumed after the first product
     CT bv[NT][4];
