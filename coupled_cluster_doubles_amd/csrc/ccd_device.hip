@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <limits>
 #include <climits>
 #include <mutex>
@@ -1264,6 +1265,7 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
     c->t2_lo.assign(nprob + 1, 0);
     std::unordered_map<int64_t, int32_t> pos;
     std::vector<int32_t> order, hord, t2_in, t2_out, pp_old(npp);
+    std::vector<std::array<int32_t, 4>> pkey;   // sort keys of a channel's pairs a<b
     std::vector<int2> seg;
     for (int64_t k = 0, o = 0; k < p->n_chan; k++)
         for (int q = 0; q < p->pp_dim[k]; q++) pp_old[o++] = q;
@@ -1278,18 +1280,36 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
             pos.clear();
             for (int q = 0; q < P; q++) pos[(int64_t)ppl[ipp + q].x * nsp + ppl[ipp + q].y] = q;
             if ((int)pos.size() != P) return ccd::fail(CCD_E_INVALID, "pp channel %d lists a pair twice", k);
-            // internal row order: each pair (a,b) with a<b, in the caller's
-            // order, directly followed by its partner (b,a); then the pairs
-            // (a,a).  The update reads both rows of a pair, so its loads cover
-            // one contiguous run of rows.
+            // internal row order: each pair (a,b) with a<b directly followed
+            // by its partner (b,a); then the pairs (a,a).  The update reads
+            // both rows of a pair, so its loads cover one contiguous run of
+            // rows.  With quantum numbers the pairs a<b go in lexicographic
+            // order of the momentum (first three numbers) of their up-spin
+            // state, or of a when both spins agree, else in the caller's
+            // order: the generated ladder's table reads of neighbouring rows
+            // then fall on neighbouring entries (the table runs over |dz|
+            // fastest) instead of random banks.
             order.clear();
+            pkey.clear();
             for (int q = 0; q < P; q++)
                 if (ppl[ipp + q].x < ppl[ipp + q].y) {
-                    auto it = pos.find((int64_t)ppl[ipp + q].y * nsp + ppl[ipp + q].x);
-                    if (it == pos.end()) return ccd::fail(CCD_E_INVALID, "pp channel %d lacks a swapped pair", k);
-                    order.push_back(q);
-                    order.push_back(it->second);
+                    const int a = ppl[ipp + q].x, b = ppl[ipp + q].y;
+                    std::array<int32_t, 4> key{0, 0, 0, q};
+                    if (p->qn) {
+                        const int32_t *qa = p->qn + 5 * (size_t)a, *qb = p->qn + 5 * (size_t)b;
+                        const int32_t *qs = (qa[3] != qb[3] && qb[3] > qa[3]) ? qb : qa;
+                        key = {qs[0], qs[1], qs[2], q};
+                    }
+                    pkey.push_back(key);
                 }
+            std::sort(pkey.begin(), pkey.end());
+            for (const auto &key : pkey) {
+                const int q = key[3];
+                auto it = pos.find((int64_t)ppl[ipp + q].y * nsp + ppl[ipp + q].x);
+                if (it == pos.end()) return ccd::fail(CCD_E_INVALID, "pp channel %d lacks a swapped pair", k);
+                order.push_back(q);
+                order.push_back(it->second);
+            }
             for (int q = 0; q < P; q++)
                 if (ppl[ipp + q].x == ppl[ipp + q].y) order.push_back(q);
             if ((int)order.size() != P) return ccd::fail(CCD_E_INVALID, "pp channel %d lacks a swapped pair", k);
