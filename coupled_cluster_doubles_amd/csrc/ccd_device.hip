@@ -180,6 +180,7 @@ struct ccd_ctx {
     double *d_t[2] = {nullptr, nullptr};
     int cur = 0;
     double *d_dp = nullptr, *d_tph = nullptr, *d_r = nullptr, *d_g = nullptr, *d_x = nullptr;
+    float *d_dsym = nullptr;     // class 1's symmetric partial sums (LK_BF16X3SYM; FP32 contexts)
     double *d_xp = nullptr;      // k-segment partial blocks of X (channels of more than one segment)
     double *d_tas = nullptr, *d_red = nullptr;  // d_red: W partial sums (column partials, then row sums)
     double *d_vt = nullptr, *d_vtp = nullptr, *d_vh = nullptr, *d_vp = nullptr, *d_va = nullptr,
@@ -209,7 +210,7 @@ struct ccd_ctx {
     RingGroup *d_rgroups = nullptr;
     float *d_vp32 = nullptr, *d_va32 = nullptr, *d_vb32 = nullptr, *d_vt32 = nullptr;  // FP32 copies (mixed precision)
     double *d_gtab = nullptr;                   // generated-V electron gas table
-    int64_t n_vp = 0, n_tp = 0, n_dp = 0, n_hhhh = 0, n_xp = 0, n_b = 0;
+    int64_t n_vp = 0, n_tp = 0, n_dp = 0, n_hhhh = 0, n_xp = 0, n_b = 0, n_sym = 0;
     double flops = 0, flops_pppp = 0;
     // a batch of independent solves merged into this context (nprob = 1 otherwise)
     int nprob = 1;
@@ -296,6 +297,7 @@ UpdArgs upd_args(const ccd_ctx *c, const double *t_old, double *t_new, double mi
     a.vt = c->d_vt; a.vt32 = c->d_vt32; a.dp = c->d_dp; a.g = c->d_g; a.t_old = t_old; a.t_new = t_new;
     a.tas = c->d_tas; a.red = c->d_red; a.n_cp = c->n_cp; a.mixing = mixing; a.nh = c->nh; a.ctl = c->d_ctl;
     a.pctl = c->d_pctl;
+    a.dsym = c->iter_prec == CCD_PREC_FP32 && c->plan.kind[CCD_PREC_FP32][0] == CCD_LK_BF16X3SYM ? c->d_dsym : nullptr;
     return a;
 }
 
@@ -496,6 +498,7 @@ void launch_grp(ccd_ctx *c, const CT *vp, int g, int lo, int hi, hipStream_t st)
     static const int glo[5] = {1, 2, 3, 5, 1}, ghi[5] = {1, 2, 4, 8, 8};
     const int LO = glo[g], HI = ghi[g];
     LadderLaunch L{};
+    L.dsym = c->d_dsym;
     const int prec = sizeof(CT) == 4 ? CCD_PREC_FP32 : CCD_PREC_FP64;
     int nb = 0;
     size_t need = 0;
@@ -1145,10 +1148,14 @@ void plan_class_tiles(ccd_ctx *c, int nt) {
     if (G <= 0) return;
     const int kch = nt <= 5 ? LadderShape<1>::KCH : LadderShape<8>::KCH;
     const int kseg = nt <= 2 ? LadderShape<1>::KSEG : LadderShape<3>::KSEG;
+    // the symmetric class-1 kernel: a segment-0 tile covers k from its first
+    // row to the end of the channel, a later segment's tile only clears its slab
+    const bool sym = nt == 1 && c->plan.kind[CCD_PREC_FP32][0] == CCD_LK_BF16X3SYM;
     std::vector<int64_t> cost(n);
     for (int i = 0; i < n; i++) {   // k chunks of the tile, plus one for its prologue
         const DTile &t = c->ltiles0[lo + i];
-        const int klen = std::min(c->chans[t.chan].npk, (t.aux + 1) * kseg) - t.aux * kseg;
+        const int npk = c->chans[t.chan].npk;
+        const int klen = sym ? (t.aux == 0 ? npk - t.row0 : 0) : std::min(npk, (t.aux + 1) * kseg) - t.aux * kseg;
         cost[i] = (klen + kch - 1) / kch + 1;
     }
     auto cu_max = [&](const std::vector<std::vector<int>> &own) {
@@ -1261,6 +1268,7 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
     std::vector<int32_t> pp_swap(npp, -1), hh_old(nhh);
     std::vector<int32_t> prow, prow_sw;
     int64_t off_t = 0, off_vp = 0, off_tp = 0, off_dp = 0, off_hhhh = 0, off_xp = 0, ipp = 0, ihh = 0;
+    int64_t off_sym = 0;   // floats, in whole blocks of 1024
     c->flops = 0; c->flops_pppp = 0;
     c->t2_lo.assign(nprob + 1, 0);
     std::unordered_map<int64_t, int32_t> pos;
@@ -1385,8 +1393,18 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
             const int ldv = (npk + 15) / 16 * 16;
             const int nseg = ladder_segments(npk, (H + 15) / 16);
             const int nxs = std::max(1, (npk + KXSEG - 1) / KXSEG);
+            // class 1 on the symmetric kernel: the rows of row tile J hold J
+            // slabs of LROWS x hh partial sums
+            const int nrt = (npk + LROWS - 1) / LROWS;
+            int32_t sym = -1;
+            if (H <= 16 && nrt >= 2) {
+                if (off_sym / 1024 >= (int64_t)INT32_MAX)
+                    return ccd::fail(CCD_E_INVALID, "symmetric ladder sums too large (channel %d)", k);
+                sym = (int32_t)(off_sym / 1024);
+                off_sym += ((int64_t)nrt * (nrt - 1) / 2 * LROWS * H + 1023) / 1024 * 1024;
+            }
             DChan ch{off_t, off_vp, off_tp, off_hhhh, P, H, (int32_t)ipp, (int32_t)ihh, npk, ldv, pr_off, ndiag,
-                     off_dp, nseg, prob, 0, hh2, off_xp, nxs, 0};
+                     off_dp, nseg, prob, 0, hh2, off_xp, nxs, sym};
             if (nxs > 1) off_xp += (int64_t)nxs * H * H;
             c->chans.push_back(ch);
             off_vp += (int64_t)npk * ldv;
@@ -1403,6 +1421,7 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
     while (prob + 1 < nprob) c->t2_lo[++prob] = off_t;
     c->t2_lo[nprob] = off_t;
     c->n_vp = off_vp; c->n_tp = off_tp; c->n_dp = off_dp; c->n_hhhh = off_hhhh; c->n_xp = off_xp;
+    c->n_sym = off_sym;
     if (c->nt2 >= (int64_t)INT32_MAX)
         return ccd::fail(CCD_E_INVALID, "T2 has %lld elements (limit 2^31)", (long long)c->nt2);
 
@@ -1664,6 +1683,7 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
         (rc = dalloc(c, &c->d_t[1], nt)) || (rc = dalloc(c, &c->d_tph, nt)) || (rc = dalloc(c, &c->d_r, nt)) ||
         (rc = dalloc(c, &c->d_g, nt)) || (rc = dalloc(c, &c->d_vt, nt)) || (rc = dalloc(c, &c->d_va, nt)) ||
         (rc = dalloc(c, &c->d_dp, (size_t)c->n_dp)) || (rc = dalloc(c, &c->d_tas, (size_t)c->n_tp)) ||
+        (rc = dalloc(c, &c->d_dsym, c->precision == CCD_PREC_FP32 ? (size_t)c->n_sym : (size_t)1)) ||
         (rc = dalloc(c, &c->d_vtp, (size_t)c->n_tp)) || (rc = dalloc(c, &c->d_x, (size_t)c->n_hhhh)) ||
         (rc = dalloc(c, &c->d_xp, (size_t)std::max<int64_t>(c->n_xp, 1))) ||
 
@@ -1676,6 +1696,9 @@ int build(ccd_ctx *c, const ccd_problem *p, const BatchInfo &bi) {
         (rc = dalloc(c, &c->d_elog, (size_t)kElogCap * nprob)) || (rc = upload(c, &c->d_hlo, bi.hlo)) ||
         (rc = upload(c, &c->d_pprob, std::vector<int32_t>(sprob.begin() + nh, sprob.end()))))
         return rc;
+    // every sum the update reads is written by the ladder of the same
+    // iteration; zero for the solves of a batch that stop before their first
+    HIP_TRY(hipMemset(c->d_dsym, 0, sizeof(float) * (c->precision == CCD_PREC_FP32 ? (size_t)std::max<int64_t>(c->n_sym, 1) : 1)));
     HIP_TRY(hipHostMalloc((void **)&c->h_ctl, sizeof(LoopCtl), hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void **)&c->h_ctl2, sizeof(LoopCtl), hipHostMallocDefault));
     *c->h_ctl = LoopCtl{};
@@ -2122,10 +2145,12 @@ int time_iterations(ccd_ctx *c, double *us) {
 int plan_valid(const ccd_ladder_plan &p) {
     for (int n = 0; n < NTMAX; n++) {
         for (int q = 0; q < 2; q++)
-            if (p.kind[q][n] < CCD_LK_DEFAULT || p.kind[q][n] > CCD_LK_BF16X3)
+            if (p.kind[q][n] < CCD_LK_DEFAULT || p.kind[q][n] > CCD_LK_BF16X3SYM)
                 return ccd::fail(CCD_E_INVALID, "ladder plan: unknown kernel kind %d (class %d)", p.kind[q][n], n + 1);
-        if (p.kind[CCD_PREC_FP64][n] == CCD_LK_BF16X3)
+        if (p.kind[CCD_PREC_FP64][n] == CCD_LK_BF16X3 || p.kind[CCD_PREC_FP64][n] == CCD_LK_BF16X3SYM)
             return ccd::fail(CCD_E_INVALID, "ladder plan: BF16x3 products are for FP32 iterations (class %d)", n + 1);
+        if (n > 0 && p.kind[CCD_PREC_FP32][n] == CCD_LK_BF16X3SYM)
+            return ccd::fail(CCD_E_INVALID, "ladder plan: the symmetric kernel is for class 1 (class %d)", n + 1);
         if (p.blocks_per_cu[n] < 0 || p.blocks_per_cu[n] > 16)
             return ccd::fail(CCD_E_INVALID, "ladder plan: %d blocks per compute unit (class %d)", p.blocks_per_cu[n],
                              n + 1);
@@ -2182,8 +2207,10 @@ int tune_ladder(ccd_ctx *c, int flags) {
             return upload_ladder_tiles(c);
         };
         std::vector<int> ck{CCD_LK_DEFAULT};   // class 1's 16x16x4 kernel differs from its default
-        for (int k : kinds)
+        for (int k : kinds) {
             if (k == CCD_LK_BF16X3 || (k == CCD_LK_MFMA16 && nt == 1)) ck.push_back(k);
+            if (k == CCD_LK_BF16X3 && nt == 1) ck.push_back(CCD_LK_BF16X3SYM);
+        }
         std::vector<int> bpcs{1, 2, 3, 4, 6};
         if (!c->matrix_free) bpcs.insert(bpcs.begin(), 0);
         if ((rc = consider(p.kind[prec][nt - 1], ck)) || (rc = consider(p.blocks_per_cu[nt - 1], bpcs)) ||
@@ -2198,6 +2225,10 @@ int tune_ladder(ccd_ctx *c, int flags) {
         ccd_ladder_plan q = start;
         for (int n = 0; n < NTMAX; n++) q.kind[prec][n] = k;
         cands.push_back(q);
+        if (k == CCD_LK_BF16X3) {   // and class 1 on the symmetric kernel
+            q.kind[prec][0] = CCD_LK_BF16X3SYM;
+            cands.push_back(q);
+        }
     }
     double best = 0.0;
     ccd_ladder_plan pick = start;

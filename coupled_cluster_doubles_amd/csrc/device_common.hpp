@@ -63,7 +63,9 @@ struct DChan {          // active direct channel (pp > 0 and hh > 0)
                         // (the pairs (i,i) follow), so column IJ < hh2 has its partner at IJ ^ 1
     int64_t off_xp;     // the nxs k-segment partial blocks (hh x hh) of X = V^hhhh + V^hhpp T
     int32_t nxs;
-    int32_t pad;
+    int32_t sym;        // class-1 channels of two or more 64-row tiles: offset of their symmetric
+                        // partial sums of the pppp ladder (LK_BF16X3SYM), in units of 1024 floats;
+                        // -1 for the others
 };
 
 struct DMod {           // active mod channel (ph > 0 and hp > 0)

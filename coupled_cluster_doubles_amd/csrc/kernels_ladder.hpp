@@ -249,6 +249,7 @@ enum LadderKind {
                       // 16x16x4 one for the others; FP64: the 16x16x4 one
     LK_MFMA16 = 1,    // v_mfma_{f32,f64}_16x16x4 (ladder_run) in every class
     LK_BF16X3 = 2,    // FP32 iterations: BF16x3 split products, v_mfma_f32_16x16x32_bf16
+    LK_BF16X3SYM = 3, // FP32 iterations, class 1: BF16x3 on the symmetry of V^pppp (sym_tile)
 };
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -791,6 +792,233 @@ __device__ __forceinline__ void wave_tile(const DTile &tl, const DChan &ch, cons
     __syncthreads();   // the regions are reused by the next tile
 }
 
+// ---- FP32 pppp ladder of class 1 on the symmetry of V^pppp (LK_BF16X3SYM)
+//
+// V^pppp of a channel is symmetric, <ab||cd> = <cd||ab> (real states; the
+// generators and the stored blocks give V[P][Q] and V[Q][P] from the same
+// operations), so a tile -- 64 packed rows P from row0 = 64 J -- builds
+// only the elements V[P][Q] with Q >= row0 and uses each twice:
+//  * D[P] += V[P][Q] T[Q], the tile's own rows, for every Q >= row0;
+//  * D[Q] += V[P][Q] T[P] over the tile's rows, for Q past the tile
+//    (Q >= row0 + 64): the part of row Q that the rows of tile J carry.
+//    It goes to the channel's symmetric partial sums: the rows of row tile
+//    J' = Q / 64 hold J' slabs of 64 x hh, slab J at
+//    sym + ((J' (J' - 1) / 2 + J) 64 + Q - 64 J') hh; the update adds them
+//    after the channel's D_p slabs.
+// The first sum over the tile's rows and the second over the rows of the
+// earlier tiles add up to row Q's sum over every k.  Elements built: the
+// sum over the tiles of 64 (npk - row0), 0.52-0.57 of npk^2 for the
+// electron gases at N=54 and N=66.
+//
+// Waves take the tile's 16-wide k chunks in turn (wave w: chunks w, w + 4,
+// ...), so each wave's first chunk lies in the tile's diagonal block and
+// every later one past it.  Lane (q, m) builds the elements of rows
+// row0 + 16 rb + m (rb = 0..3) at k-slots k0 + 4 q + e (e = 0..3), from
+// the records of the k-slots broadcast from lanes e of its 16-lane row (one
+// broadcast serves four rows).  Both products are BF16x3 split products
+// (x = x_h + x_l with x_h = bf16(x), x_l = bf16(x - x_h); a b = a_l b_h +
+// a_h b_l + a_h b_h, FP32 accumulation):
+//  * own rows, per row block: three v_mfma_f32_16x16x16_bf16 with
+//    A[m][4q+e] = V[16 rb + m][k0 + 4 q + e], B[4q+e][n] = T[k0 + 4 q + e][n];
+//  * rows of the chunk: the two BF16 parts of each element go, as one
+//    32-bit word, into the wave's LDS block [k-slot][row] and come back
+//    transposed (lane (q, m) reads rows 32 g + 8 q .. + 7 of k-slot m: the A
+//    operand of v_mfma_f32_16x16x32_bf16, B = the tile's own T rows, split
+//    once per tile); 6 MFMAs per chunk, complete for the chunk's 16 rows.
+// Within a wave LDS accesses complete in order, so the transposition needs
+// no barrier.  The own-row partial tiles of the four waves are added in
+// wave order at the end of the tile.  The stored and generated modes use
+// the same elements in the same order, so they agree bit for bit.  A tile
+// of a later k segment (aux > 0: channels of more than one segment) only
+// clears its slab of D_p: the segment-0 tile covers the whole k range.
+constexpr int SKC = 16;     // k-slots per chunk
+constexpr int SPSTR = 68;   // words per k-slot row of a wave's transposition block: 64 rows + 4
+                            // (the 16 lanes of a row group write consecutive banks, the four
+                            // groups 16 banks apart; the 16-byte reads of a row group cover all banks)
+
+struct SymLadder {
+    static constexpr int WBLK = SKC * SPSTR;       // words of a wave's transposition block
+    static constexpr int RED = 4 * 4 * 4 * 64;     // floats of the four waves' own-row partial tiles
+    static constexpr size_t BYTES = (size_t)(4 * WBLK > RED ? 4 * WBLK : RED) * 4;
+};
+
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef short sh4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ flt4 mfma_bf16_k16(bf16x4 a, bf16x4 b, flt4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(sh4, a), __builtin_bit_cast(sh4, b), c, 0,
+                                                     0, 0);
+}
+
+// high part in the low half-word, low part in the high one
+__device__ __forceinline__ uint32_t pack_split(__bf16 h, __bf16 l) {
+    return (uint32_t)__builtin_bit_cast(uint16_t, h) | ((uint32_t)__builtin_bit_cast(uint16_t, l) << 16);
+}
+
+template <int SRC, int MODE>
+__device__ __forceinline__ void sym_tile(const DTile &tl, const DChan &ch, const float *__restrict__ vp,
+                                         const LadderGen &lg, const float *__restrict__ tas,
+                                         const double *__restrict__ zero, float *__restrict__ dp,
+                                         float *__restrict__ dsym, unsigned char *lds, uint32_t tbase, int wave,
+                                         int lane) {
+    constexpr bool ONE = MODE == 1, SAME = MODE == 2, GEN = SRC != LV_STORED;
+    constexpr bool GAS = gas_source(SRC) && MODE != 0;   // else the generic generator
+    const int H = ch.hh, npk = ch.npk, m = lane & 15, q = lane >> 4;
+    if (tl.aux > 0) {   // block-uniform
+        float *o = dp + ch.off_dp + (int64_t)tl.aux * npk * H + (int64_t)tl.row0 * H;
+        for (int l = wave * 64 + lane; l < tl.nrows * H; l += kThreads) o[l] = 0.f;
+        return;
+    }
+    const int row0 = tl.row0, J = row0 >> 6, kend = npk;
+    const int nch = (kend - row0 + SKC - 1) / SKC;
+    // the records of the split path: total-spin-0 channels with the sign field
+    const int2 *grec1 = GAS ? (ONE ? lg.rec1w : lg.rec1f) : lg.rec1;
+    // rows of this lane; rows past the tile repeat its first row (finite
+    // elements: their T rows read 0 in the second product, their own sums are
+    // never stored)
+    int2 ra[4], rb2[4];
+    const float *av[4];
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++) {
+        const int r = 16 * rb + m;
+        const int64_t u = row0 + (r < tl.nrows ? r : 0);
+        ra[rb] = GEN ? grec1[ch.pr_off + u] : int2{0, 0};
+        rb2[rb] = GAS && SAME ? lg.rec2f[ch.pr_off + u] : int2{0, 0};
+        av[rb] = vp + ch.off_vp + u * ch.ldv;
+    }
+    // buffer loads: records past kend and T rows past npk (or columns past hh) read 0
+    const BufRsrc rsrc = GEN ? make_rsrc(grec1 + ch.pr_off, (uint32_t)kend * 8u) : make_rsrc(zero, 0);
+    const BufRsrc bsrc = make_rsrc(tas + ch.off_tp, (uint32_t)kend * (uint32_t)H * 4u);
+    const uint32_t roff = m < 4 ? (uint32_t)(row0 + 4 * q + m) * 8u : kOOB;   // record of k-slot 4 q + m
+    const uint32_t boff = m < H ? (uint32_t)((row0 + 4 * q) * H + m) * 4u : kOOB;   // T[row0 + 4 q][m]
+    const uint32_t cstep = (uint32_t)SKC * (uint32_t)H * 4u;
+    // B of the second product: the tile's T rows 32 g + 8 q + e in column m
+    // (boff holds row row0 + 4 q)
+    bf16x8 tbh[2], tbl[2];
+#pragma unroll
+    for (int g = 0; g < 2; g++)
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const float x = buf_load<float>(bsrc, boff + (uint32_t)((32 * g + 4 * q + e) * H) * 4u);
+            tbh[g][e] = (__bf16)x;
+            tbl[g][e] = (__bf16)(x - (float)tbh[g][e]);
+        }
+    flt4 acc[4];
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++) acc[rb] = flt4{0.f, 0.f, 0.f, 0.f};
+    uint32_t *blk = reinterpret_cast<uint32_t *>(lds) + wave * SymLadder::WBLK;
+    float *sym = dsym + (int64_t)ch.sym * 1024;
+
+    if (wave < nch) {   // uniform
+        int2 rc = GEN ? buf_i2(rsrc, roff + (uint32_t)(wave * SKC) * 8u) : int2{0, 0};
+        float bc[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) bc[e] = buf_load<float>(bsrc, boff + (uint32_t)wave * cstep + (uint32_t)(e * H) * 4u);
+        for (int c = wave; c < nch; c += 4) {
+            // the next chunk's records and T rows (past the last chunk: 0)
+            const int cn = c + 4;
+            const int2 rn = GEN ? buf_i2(rsrc, roff + (uint32_t)(cn * SKC) * 8u) : int2{0, 0};
+            float bn[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) bn[e] = buf_load<float>(bsrc, boff + (uint32_t)cn * cstep + (uint32_t)(e * H) * 4u);
+            const int k0 = row0 + c * SKC;
+            float a[4][4];
+            if constexpr (!GEN) {
+                const bool kok = k0 + 4 * q < ch.ldv;
+#pragma unroll
+                for (int rb = 0; rb < 4; rb++) {
+                    const flt4 w = *reinterpret_cast<const flt4 *>(kok ? av[rb] + k0 + 4 * q
+                                                                       : reinterpret_cast<const float *>(zero));
+#pragma unroll
+                    for (int e = 0; e < 4; e++) a[rb][e] = w[e];
+                }
+            } else {
+                static_for<0, 4>([&](auto E) {
+                    constexpr int e = decltype(E)::value;
+                    const int2 r = {row_bcast<e>(rc.x), row_bcast<e>(rc.y)};
+#pragma unroll
+                    for (int rb = 0; rb < 4; rb++) {
+                        if (GAS && ONE) a[rb][e] = lds_loadf(gas_addr1(ra[rb], r, tbase));
+                        else if (GAS) a[rb][e] = lds_loadf(gas_addr1(ra[rb], r, tbase)) - lds_loadf(gas_addr1(rb2[rb], r, tbase));
+                        else a[rb][e] = gen_tbme_f32(lg.G, ra[rb].x, ra[rb].y, r.x, r.y);
+                    }
+                });
+            }
+            bf16x4 ah[4], al[4], bh, bl;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                bh[e] = (__bf16)bc[e];
+                bl[e] = (__bf16)(bc[e] - (float)bh[e]);
+#pragma unroll
+                for (int rb = 0; rb < 4; rb++) {
+                    ah[rb][e] = (__bf16)a[rb][e];
+                    al[rb][e] = (__bf16)(a[rb][e] - (float)ah[rb][e]);
+                }
+            }
+#pragma unroll
+            for (int rb = 0; rb < 4; rb++) {
+                acc[rb] = mfma_bf16_k16(al[rb], bh, acc[rb]);
+                acc[rb] = mfma_bf16_k16(ah[rb], bl, acc[rb]);
+                acc[rb] = mfma_bf16_k16(ah[rb], bh, acc[rb]);
+            }
+            if (c >= 4) {   // uniform: the chunk lies past the tile
+#pragma unroll
+                for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+                    for (int e = 0; e < 4; e++) blk[(4 * q + e) * SPSTR + 16 * rb + m] = pack_split(ah[rb][e], al[rb][e]);
+                flt4 tacc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int g = 0; g < 2; g++) {
+                    const u32x4 w0 = *reinterpret_cast<const u32x4 *>(blk + m * SPSTR + 32 * g + 8 * q);
+                    const u32x4 w1 = *reinterpret_cast<const u32x4 *>(blk + m * SPSTR + 32 * g + 8 * q + 4);
+                    u32x4 hv, lv;
+                    hv[0] = __builtin_amdgcn_perm(w0[1], w0[0], 0x05040100u);
+                    hv[1] = __builtin_amdgcn_perm(w0[3], w0[2], 0x05040100u);
+                    hv[2] = __builtin_amdgcn_perm(w1[1], w1[0], 0x05040100u);
+                    hv[3] = __builtin_amdgcn_perm(w1[3], w1[2], 0x05040100u);
+                    lv[0] = __builtin_amdgcn_perm(w0[1], w0[0], 0x07060302u);
+                    lv[1] = __builtin_amdgcn_perm(w0[3], w0[2], 0x07060302u);
+                    lv[2] = __builtin_amdgcn_perm(w1[1], w1[0], 0x07060302u);
+                    lv[3] = __builtin_amdgcn_perm(w1[3], w1[2], 0x07060302u);
+                    const bf16x8 xh = __builtin_bit_cast(bf16x8, hv), xl = __builtin_bit_cast(bf16x8, lv);
+                    tacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl, tbh[g], tacc, 0, 0, 0);
+                    tacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, tbl[g], tacc, 0, 0, 0);
+                    tacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, tbh[g], tacc, 0, 0, 0);
+                }
+                // rows k0 + 4 q + r of row tile Jt, slab J
+                const int Jt = J + (c >> 2);
+                float *o = sym + ((int64_t)(Jt * (Jt - 1) / 2 + J) * 64 + 16 * (c & 3)) * H;
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    if (m < H && k0 + 4 * q + r < kend) o[(4 * q + r) * H + m] = tacc[r];
+            }
+            rc = rn;
+#pragma unroll
+            for (int e = 0; e < 4; e++) bc[e] = bn[e];
+        }
+    }
+    // the own-row partial tiles, added in wave order: every wave parks its
+    // four (row block rb, register r at [wave][rb][r][lane]), wave w adds row
+    // block w and stores it
+    __syncthreads();   // every wave is out of its k loop: the LDS is free
+    float *red = reinterpret_cast<float *>(lds);
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) red[((wave * 4 + rb) * 4 + r) * 64 + lane] = acc[rb][r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        float v = red[((0 * 4 + wave) * 4 + r) * 64 + lane];
+#pragma unroll
+        for (int w = 1; w < 4; w++) v += red[((w * 4 + wave) * 4 + r) * 64 + lane];
+        const int row = 16 * wave + 4 * q + r;
+        if (row < tl.nrows && m < H) dp[ch.off_dp + (int64_t)(row0 + row) * H + m] = v;
+    }
+    __syncthreads();   // the LDS is reused by the next tile
+}
+
 // ---- the grouped pppp-ladder launch
 //
 // One persistent launch covers the classes LO..HI (channels with hh in
@@ -806,6 +1034,7 @@ struct LadderLaunch {
     int32_t ptr0[NTMAX + 1];
     int32_t blk0[NTMAX + 2];
     int32_t kind[NTMAX + 1];
+    float *dsym;   // symmetric partial sums of class 1 (LK_BF16X3SYM)
 };
 
 // Bytes of the B-tile LDS of class NT under kernel kind `kind` (CT: the
@@ -813,6 +1042,7 @@ struct LadderLaunch {
 template <typename CT, int NT>
 __host__ __device__ constexpr size_t ladder_class_bytes(int kind) {
     if (sizeof(CT) == 4 && kind == LK_BF16X3) return LadderTile<float, NT, true>::BYTES;
+    if (sizeof(CT) == 4 && NT == 1 && kind == LK_BF16X3SYM) return SymLadder::BYTES;
     if (sizeof(CT) == 4 && NT == 1 && kind == LK_DEFAULT) return (size_t)WaveLadder<1>::REGION * sizeof(float);
     return LadderTile<CT, NT, false>::BYTES;
 }
@@ -822,13 +1052,32 @@ __device__ __forceinline__ void ladder_class(int kind, const DTile *__restrict__
                                              const DChan *__restrict__ chans, const CT *__restrict__ vp,
                                              const LadderGen &lg, const CT *__restrict__ tas,
                                              const double *__restrict__ zero, CT *__restrict__ dp,
-                                             unsigned char *lds_b, uint32_t tbase, void *tab_lds) {
+                                             float *__restrict__ dsym, unsigned char *lds_b, uint32_t tbase,
+                                             void *tab_lds) {
     if constexpr (std::is_same<CT, float>::value) {
         if (kind == LK_BF16X3) {
             ladder_run<float, NT, SRC, true>(tiles, b0, b1, chans, vp, lg, tas, zero, dp, lds_b, tbase, tab_lds);
             return;
         }
         if constexpr (NT == 1) {
+            if (kind == LK_BF16X3SYM) {
+                const int tid = threadIdx.x & (kThreads - 1);
+                const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+                int prob = -1;
+                bool stopped = false;
+                for (int ti = b0; ti < b1; ti++) {
+                    const DTile tl = tiles[ti];
+                    const DChan ch = chans[tl.chan];
+                    if (!ladder_tile_begin<float, SRC>(lg, ch, prob, stopped, tab_lds)) continue;
+                    if (gas_source(SRC) && ch.spin0 == 1)
+                        sym_tile<SRC, 1>(tl, ch, vp, lg, tas, zero, dp, dsym, lds_b, tbase, wave, lane);
+                    else if (SRC == LV_GAS || (SRC == LV_GAS_BATCH && ch.spin0 == 2))
+                        sym_tile<SRC, 2>(tl, ch, vp, lg, tas, zero, dp, dsym, lds_b, tbase, wave, lane);
+                    else
+                        sym_tile<SRC, 0>(tl, ch, vp, lg, tas, zero, dp, dsym, lds_b, tbase, wave, lane);
+                }
+                return;
+            }
             if (kind == LK_DEFAULT) {
                 const int tid = threadIdx.x & (kThreads - 1);
                 const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
@@ -879,7 +1128,8 @@ __global__ __launch_bounds__(kThreads) void k_ladder_grp(LadderLaunch L, const D
     const int kind = L.kind[nt];
     static_for<LO, HI + 1>([&](auto N) {
         constexpr int n = decltype(N)::value;
-        if (nt == n) ladder_class<CT, n, SRC>(kind, tl, b0, b1, chans, vp, lg, tas, zero, dp, lds_b, tbase, lg_lds);
+        if (nt == n)
+            ladder_class<CT, n, SRC>(kind, tl, b0, b1, chans, vp, lg, tas, zero, dp, L.dsym, lds_b, tbase, lg_lds);
     });
 }
 

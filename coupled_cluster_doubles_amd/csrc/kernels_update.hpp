@@ -23,6 +23,8 @@ struct UpdArgs {
     const double *vt;                // <ij||ab> in the pphh layout
     const float *vt32;               // its FP32 rounding (what the update reads in FP32 iterations)
     const void *dp;                  // ladder D on packed rows: one slab per k segment, then 1/2 T X
+    const float *dsym;               // FP32 iterations with class 1 on LK_BF16X3SYM: its symmetric
+                                     // partial sums (rows of row tile J: J slabs), else nullptr
     const void *g;                   // ring R gathered to the pphh layout: g[(a,b),(i,j)] = R[(a,i),(j,b)]
                                      // (both in the contractions' arithmetic type)
     const double *t_old;
@@ -36,6 +38,26 @@ struct UpdArgs {
     const LoopCtl *ctl;              // loop state of the context
     const LoopCtl *pctl;             // per solve (a batch); pctl == ctl for a single solve
 };
+
+// The symmetric partial sums of the pppp ladder for packed row pr, column
+// IJ of a class-1 channel (the kernel kind LK_BF16X3SYM): the rows of row
+// tile J (64 rows) hold J slabs, those of the tiles before it, added in
+// order (loads issued four at a time; the clamped loads are not added).
+__device__ __forceinline__ double sym_rows(const float *__restrict__ dsym, const DChan &ch, int pr, int IJ) {
+    const int H = ch.hh, J = pr >> 6;
+    const float *s = dsym + (int64_t)ch.sym * 1024 + ((int64_t)(J * (J - 1) / 2) * 64 + (pr - 64 * J)) * H + IJ;
+    const int64_t slab = (int64_t)64 * H;
+    double d = 0.0;
+    for (int j0 = 0; j0 < J; j0 += 4) {
+        float v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) v[u] = s[min(j0 + u, J - 1) * slab];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (j0 + u < J) d += (double)v[u];
+    }
+    return d;
+}
 
 // UPDATE = true: one Jacobi step for both rows of each pair; UPDATE = false:
 // only the W sums of the current amplitudes (to derive chi, psi, E from them),
@@ -109,6 +131,7 @@ __global__ __launch_bounds__(kThreads) void k_update(UpdArgs a) {
                     for (int u = 0; u < 4; u++)
                         if (q0 + u <= ch.nseg) d += (double)dv[u];
                 }
+                if (a.dsym && ch.sym >= 0) d += sym_rows(a.dsym, ch, pr, IJ);
             }
             const double ci = 0.5 * a.chi[ii], cj = 0.5 * a.chi[jj];
             const double pa = -0.5 * a.psi[ia - a.nh], pb = -0.5 * a.psi[ib - a.nh];
@@ -428,6 +451,7 @@ __global__ __launch_bounds__(kThreads) void k_expand_d(UpdArgs a, double *__rest
         const OT *dq = static_cast<const OT *>(a.dp) + ch.off_dp + (int64_t)pr * H + IJ;
         double v = 0.0;
         for (int q = 0; q <= ch.nseg; q++) v += (double)dq[q * slab];
+        if (a.dsym && ch.sym >= 0) v += sym_rows(a.dsym, ch, pr, IJ);
         d[ch.off_t + (int64_t)r1 * H + IJ] = v;
         d[ch.off_t + (int64_t)r2 * H + IJ] = -v;
     }

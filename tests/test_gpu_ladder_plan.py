@@ -7,8 +7,10 @@ the summation (16x16x4 MFMA instead of 4x4x1 for class 1) or the arithmetic
 (BF16x3 split products, about 2^-16 relative per product): they are held to
 the oracle -- FP32 kinds within 2e-5 of the largest element of the ladder,
 BF16x3 within 2e-4 -- and a mixed solve with them to 1e-7 Ha per particle
-of the oracle's converged energy.  Stored and generated V stay bit-identical
-under every kind.
+of the oracle's converged energy.  The symmetric class-1 kind (bf16x3sym)
+builds each element of V^pppp once for both of its rows: held to the oracle
+like BF16x3, also through whole iterations.  Stored and generated V stay
+bit-identical under every kind.
 """
 import numpy as np
 import pytest
@@ -56,7 +58,9 @@ def test_default_plan_and_validation(require_gpu):
     ctx = Context(m, FP64)
     assert ctx.ladder_plan()["blocks_per_cu"] == [0] * 8
     p = ctx.ladder_plan()
-    for bad in ({"kind_fp64": ["bf16x3"] * 8}, {"blocks_per_cu": [17] * 8}, {"blocks_per_cu": [-1] * 8}):
+    for bad in ({"kind_fp64": ["bf16x3"] * 8}, {"blocks_per_cu": [17] * 8}, {"blocks_per_cu": [-1] * 8},
+                {"kind_fp64": ["bf16x3sym"] + ["default"] * 7},
+                {"kind_fp32": ["bf16x3", "bf16x3sym"] + ["bf16x3"] * 6}):
         q = dict(p, **bad)
         with pytest.raises(CCDError):
             ctx.set_ladder_plan(q)
@@ -69,9 +73,14 @@ def test_default_plan_and_validation(require_gpu):
     ("heg", (1.0, 10, 3), FP32, False),
     ("pairing", (16, 16, 0.5, 1.0), FP64, False),   # hh up to 128: classes 5-8
     ("pairing", (16, 16, 0.5, 1.0), FP32, False),
+    ("heg", (1.0, 19, 2), "sym", True),
+    ("heg", (1.0, 10, 3), "sym", False),
 ])
 def test_scheduling_fields_change_no_result(require_gpu, kind, args, prec, free):
     m = _model(kind, args)
+    sym = prec == "sym"   # FP32 iterations with class 1 on the symmetric kernel
+    if sym:
+        prec = FP32
     plans = [
         {},
         {"blocks_per_cu": [1] * 8, "assign": ["channel"] * 8},
@@ -82,7 +91,7 @@ def test_scheduling_fields_change_no_result(require_gpu, kind, args, prec, free)
     for q, kw in enumerate(plans):
         print(f"{kind}{args} prec={prec} free={free} plan {q}: {kw}", flush=True)
         ctx = Context(m, prec, matrix_free=free)
-        _plan(ctx, **kw)
+        _plan(ctx, **kw, **({"kind_fp32": ["bf16x3sym"] + ["bf16x3"] * 7} if sym else {}))
         got = (ctx.iterate_n(4), ctx.get_t2())
         ctx.close()
         if ref is None:
@@ -102,7 +111,8 @@ def test_kernel_kinds_close_to_oracle(require_gpu, kind, args):
     d_o = o.ladder(ta)
     scale = np.max(np.abs(d_o))
     for kinds, tol in ((["mfma16"] * 8, F32_REL), (["bf16x3"] * 8, BF16X3_REL),
-                       (["default", "bf16x3", "mfma16", "bf16x3", "default", "bf16x3", "mfma16", "bf16x3"], BF16X3_REL)):
+                       (["default", "bf16x3", "mfma16", "bf16x3", "default", "bf16x3", "mfma16", "bf16x3"], BF16X3_REL),
+                       (["bf16x3sym"] + ["default"] * 7, BF16X3_REL)):
         ctx = Context(m, FP32)
         _plan(ctx, kind_fp32=kinds)
         err = np.max(np.abs(ctx.ladder(ta) - d_o)) / scale
@@ -147,7 +157,8 @@ def test_ladder_kinds_at_n54_against_oracle(require_gpu):
     print(f"N=54 FP64: max |D - D_oracle| / max|D_oracle| = {err:.2e}", flush=True)
     assert err <= 1e-12
     ctx.close()
-    for kinds, tol in ((["default"] * 8, F32_REL), (["mfma16"] * 8, F32_REL), (["bf16x3"] * 8, BF16X3_REL)):
+    for kinds, tol in ((["default"] * 8, F32_REL), (["mfma16"] * 8, F32_REL), (["bf16x3"] * 8, BF16X3_REL),
+                       (["bf16x3sym"] + ["bf16x3"] * 7, BF16X3_REL)):
         ctx = Context(m, FP32, matrix_free=True)
         _plan(ctx, kind_fp32=kinds)
         err = np.max(np.abs(ctx.ladder(ta) - d_o)) / scale
@@ -156,7 +167,7 @@ def test_ladder_kinds_at_n54_against_oracle(require_gpu):
         ctx.close()
 
 
-@pytest.mark.parametrize("kinds", [["bf16x3"] * 8, ["mfma16"] * 8])
+@pytest.mark.parametrize("kinds", [["bf16x3"] * 8, ["mfma16"] * 8, ["bf16x3sym"] + ["bf16x3"] * 7])
 def test_stored_and_generated_v_agree_bitwise_under_every_kind(require_gpu, kinds):
     m = Model.electron_gas(1.0, 19, 2)
     a, b = Context(m, FP32), Context(m, FP32, matrix_free=True)
@@ -166,18 +177,41 @@ def test_stored_and_generated_v_agree_bitwise_under_every_kind(require_gpu, kind
     assert np.array_equal(a.get_t2(), b.get_t2())
 
 
-def test_mixed_solve_with_bf16x3_products_within_1e7_per_particle(require_gpu):
+@pytest.mark.parametrize("kinds", [["bf16x3"] * 8, ["bf16x3sym"] + ["bf16x3"] * 7])
+def test_mixed_solve_with_bf16x3_products_within_1e7_per_particle(require_gpu, kinds):
     """FP32 iterations on BF16x3 products, then FP64 sweeps: the converged
     energy within 1e-7 Ha per particle of the oracle's (N=14 electrons)."""
     args = (1.0, 19, 2)
     m, o = Model.electron_gas(*args), Oracle.electron_gas(*args)
     e_o, stop_o, _ = o.solve(1e-10)
     ctx = Context(m, FP32, matrix_free=True)
-    _plan(ctx, kind_fp32=["bf16x3"] * 8)
+    _plan(ctx, kind_fp32=kinds)
     e, stop, n, n32 = ctx.solve_mixed(1e-10, 1e-6)
-    print(f"bf16x3 mixed: {n32} + {n - n32} iterations, |dE|/N = {abs(e[-1] - e_o[-1]) / 14:.2e}", flush=True)
+    print(f"{kinds[0]} mixed: {n32} + {n - n32} iterations, |dE|/N = {abs(e[-1] - e_o[-1]) / 14:.2e}", flush=True)
     assert stop == stop_o == "converged" and n32 > 0
     assert abs(e[-1] - e_o[-1]) / 14 <= 1e-7
+
+
+def test_symmetric_kernel_at_n66_matches_bf16x3(require_gpu):
+    """Electron gas N=66 (matrix-free): class-1 channels of up to 1,576
+    packed pairs, so of two k segments (the symmetric kernel's tiles of the
+    second segment only clear their slab).  Its ladder against the BF16x3
+    kernel's, both within BF16X3_REL of the exact one (held to the oracle at
+    N=54), so within twice that of each other; and three whole FP32
+    iterations on each within 1e-6 of each other (relative energy)."""
+    m = Model.electron_gas(0.5, 52, 4)
+    ta = _antisym_fast(m, np.random.default_rng(66).uniform(-0.05, 0.05, m.n_t2))
+    got = {}
+    for kinds in (["bf16x3"] * 8, ["bf16x3sym"] + ["bf16x3"] * 7):
+        ctx = Context(m, FP32, matrix_free=True)
+        _plan(ctx, kind_fp32=kinds)
+        got[kinds[0]] = (ctx.ladder(ta), ctx.iterate_n(3))
+        ctx.close()
+    (d0, e0), (d1, e1) = got["bf16x3"], got["bf16x3sym"]
+    err = np.max(np.abs(d1 - d0)) / np.max(np.abs(d0))
+    print(f"N=66 ladder bf16x3sym vs bf16x3: {err:.2e}; energies {e0} {e1}", flush=True)
+    assert err <= 2 * BF16X3_REL
+    assert np.max(np.abs(e1 - e0) / np.abs(e0)) <= 1e-6
 
 
 def test_tuner_keeps_results_or_tolerance(require_gpu):
