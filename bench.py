@@ -85,7 +85,9 @@ KERNEL_OF_GROUP = {
 }
 # the size classes of the pppp ladder each launch group times
 CLASSES_OF_GROUP = {"ladder_v1": (1,), "ladder_v2": (2,), "ladder_v34": (3, 4), "ladder_v58": (5, 6, 7, 8)}
-KIND_LABEL = {"default": "{dflt}", "mfma16": "16x16x4 MFMA", "bf16x3": "BF16x3 split products, 16x16x32 BF16 MFMA"}
+KIND_LABEL = {"default": "{dflt}", "mfma16": "16x16x4 MFMA", "bf16x3": "BF16x3 split products, 16x16x32 BF16 MFMA",
+              "bf16x3sym": "BF16x3 split products on the symmetry of V^pppp, 16x16x16 and 16x16x32 BF16 MFMA"}
+BF16X3_KINDS = ("bf16x3", "bf16x3sym")
 BF16_PEAK_TFLOPS = 2500.0   # dense BF16 MFMA; a BF16x3 product costs three
 
 
@@ -317,8 +319,9 @@ def main():
     kind_ab = None
     if prec == FP32 and not args.no_kind_ab:
         kind_ab = {}
-        for kind in ("default", "mfma16", "bf16x3"):
-            q = dict(plan, kind_fp32=[kind] * 8)
+        for kind in ("default", "mfma16", "bf16x3", "bf16x3sym"):
+            # the symmetric kernel is for class 1: the others on BF16x3
+            q = dict(plan, kind_fp32=[kind] * 8 if kind != "bf16x3sym" else [kind] + ["bf16x3"] * 7)
             ctx.set_ladder_plan(q)
             tk = ctx.time_kernels(n_timed, REPS)
             kind_ab[kind] = {g: tk[g] * 1e3 for g in CLASSES_OF_GROUP}
@@ -335,17 +338,29 @@ def main():
 
     # roofline of the dominant launch group, measured on the stream it runs on
     dom = max(kern, key=lambda g: kern[g]["ms"])
-    prof = load_json(os.path.join(ROOT, "profiles", f"{dom}_{args.config}_{args.precision}_{args.mode}.json"))
+    # the committed counter summary of this workload: per class-1 kernel kind
+    # (profiles/ladder_v1_<config>_<precision>_<mode>_<kind>.json, with the
+    # calibrated traffic of every launch group); a ladder group's own figures
+    # only where its kernel kind is the profiled one
+    k1 = plan["kind_fp32" if ct == "float" else "kind_fp64"][0]
+    prof_name = f"ladder_v1_{args.config}_{args.precision}_{args.mode}_{k1}.json"
+    prof_all = load_json(os.path.join(ROOT, "profiles", prof_name))
+    prof = None
+    if prof_all:
+        if dom == "ladder_v1":
+            prof = prof_all
+        elif not dom.startswith("ladder_v") and prof_all.get("traffic_per_group", {}).get(dom):
+            prof = {"bytes_per_launch": prof_all["traffic_per_group"][dom]}
 
     def roof_of(g, rep, peak_tf):
         r = rep[g]
         if g.startswith("ladder_v") and free:
-            if r.get("kind") == "bf16x3":   # three BF16 products per FP32 product
+            if r.get("kind") in BF16X3_KINDS:   # three BF16 products per FP32 product
                 peak_tf = BF16_PEAK_TFLOPS / 3
             d = {"bound": "mfma", "kernel": r["kernel"], "achieved": r["tflops"], "peak": peak_tf,
                  "unit": "TFLOP/s", "frac": r["tflops"] / peak_tf, "algorithmic_flops": r["flops"],
                  "algorithmic_bytes": r["bytes"], "launch_ms": r["ms"]}
-            if r.get("kind") == "bf16x3":
+            if r.get("kind") in BF16X3_KINDS:
                 d["peak_note"] = "dense BF16 MFMA peak / 3 (each product is three BF16 products)"
         elif g in ("ring", "ladder_x", "ladder_tx") or g.startswith("ladder_v"):
             d = {"bound": "mfma" if g == "ring" else "hbm", "kernel": r["kernel"], "launch_ms": r["ms"],
@@ -363,7 +378,7 @@ def main():
     peak = FP64_PEAK_TFLOPS if prec == FP64 else FP32_PEAK_TFLOPS
     roof = roof_of(dom, kern, peak)
     roof["traffic"] = prof.get("bytes_per_launch") if prof else None
-    roof["traffic_source"] = (f"profiles/{dom}_{args.config}_{args.precision}_{args.mode}.json "
+    roof["traffic_source"] = (f"profiles/{prof_name} "
                               "(rocprofv3 FETCH_SIZE / WRITE_SIZE, calibrated)") if prof else None
     if prof and prof.get("valu_insts_per_element") and kern[dom].get("generated_elements"):
         # VALU-issue roof of the generated ladder: every generated element costs
@@ -503,7 +518,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == FP64 else ("f32+bf16x3+f64" if "bf16x3" in plan["kind_fp32"] else "f32+f64"),
+        "dtype": "f64" if prec == FP64 else ("f32+bf16x3+f64" if set(BF16X3_KINDS) & set(plan["kind_fp32"]) else "f32+f64"),
         "mixed_vs_fp64": mixed,
         "data": "synthetic: the reference's closed-form basis and Coulomb interaction, T2 from zero",
         "config": {"workload": cfg["name"], "precision": args.precision, "mode": args.mode,

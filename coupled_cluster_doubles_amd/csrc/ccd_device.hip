@@ -2460,8 +2460,23 @@ int ccd_kernel_work(const ccd_ctx *ctx, double *flops, double *bytes, double *el
         const int g = nt16 == 1 ? CCD_KT_LADDER1 : nt16 == 2 ? CCD_KT_LADDER2 : nt16 <= 4 ? CCD_KT_LADDER34
                                                                                           : CCD_KT_LADDER58;
         f[g] += 2.0 * ppk * hh;
-        b[g] += (ctx->matrix_free ? 0.0 : cs * ppk) + 2.0 * cs * npk * hh;   // V^pppp (stored), T rows in, D out
-        if (ctx->matrix_free) e[g] += ppk;
+        if (g == CCD_KT_LADDER1 && ch.sym >= 0 && ctx->iter_prec == CCD_PREC_FP32 &&
+            ctx->plan.kind[CCD_PREC_FP32][0] == CCD_LK_BF16X3SYM) {
+            // the symmetric kernel: tile J builds (or reads) the elements of its
+            // rows from k = 64 J on, and writes the sums of the rows past it,
+            // which the update reads
+            double el = 0.0, tr = 0.0;
+            for (int r0 = 0; r0 < ch.npk; r0 += LROWS) {
+                el += (double)std::min(LROWS, ch.npk - r0) * (ch.npk - r0);
+                tr += (double)std::max(0, ch.npk - r0 - LROWS) * hh;
+            }
+            b[g] += (ctx->matrix_free ? 0.0 : cs * el) + 2.0 * cs * npk * hh + 4.0 * tr;
+            b[CCD_KT_UPDATE] += 4.0 * tr;
+            if (ctx->matrix_free) e[g] += el;
+        } else {
+            b[g] += (ctx->matrix_free ? 0.0 : cs * ppk) + 2.0 * cs * npk * hh;   // V^pppp (stored), T rows in, D out
+            if (ctx->matrix_free) e[g] += ppk;
+        }
         // the update: D read once per packed row (one array in the algorithm)
         b[CCD_KT_UPDATE] += cs * npk * hh + cs * npk * hh;               // D in, T rows out
     }

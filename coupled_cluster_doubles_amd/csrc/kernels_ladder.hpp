@@ -253,6 +253,21 @@ enum LadderKind {
 };
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float flt2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+// The BF16x3 parts of two FP32 values, each pair in one 32-bit word (the
+// first value in the low half): [0] = bf16(x) (round to nearest even), [1] =
+// bf16(x - [0]).  A pair per conversion (v_cvt_pk_bf16_f32 takes two
+// values); the same numbers as converting one value at a time.
+__device__ __forceinline__ u32x2 split_pair(float x0, float x1) {
+    const flt2 x = {x0, x1};
+    const uint32_t hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(x, bf16x2));
+    const flt2 h = {__uint_as_float(hi << 16), __uint_as_float(hi & 0xffff0000u)};
+    return u32x2{hi, __builtin_bit_cast(uint32_t, __builtin_convertvector(x - h, bf16x2))};
+}
 
 // B-tile LDS of ladder_run: two buffers of the chunk's KCH x 16 NT
 // amplitudes.  Split products hold them as BF16 high and low parts, column
@@ -446,14 +461,12 @@ __device__ __forceinline__ void ladder_run(const DTile *__restrict__ tiles, int 
                         // k-slot of row 2p: lane group ((2p) % 16) / 4, element
                         // 4 ((2p) / 16) + (2p) % 4 (row 2p + 1 is the next element)
                         const int k = 2 * p, pos = ((k & 15) >> 2) * 8 + 4 * (k >> 4) + (k & 3);
-                        const float x0 = b[2 * q], x1 = b[2 * q + 1];
-                        const __bf16 h0 = (__bf16)x0, h1 = (__bf16)x1;
-                        const __bf16 l0 = (__bf16)(x0 - (float)h0), l1 = (__bf16)(x1 - (float)h1);
+                        const u32x2 hl = split_pair(b[2 * q], b[2 * q + 1]);
+                        const uint32_t hw = hl[0], lw = hl[1];
                         __bf16 *ph = Bsp + ((2 * buf) * W + col) * CSTR + pos;
                         __bf16 *pl = Bsp + ((2 * buf + 1) * W + col) * CSTR + pos;
-                        typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
-                        *reinterpret_cast<bf2 *>(ph) = bf2{h0, h1};
-                        *reinterpret_cast<bf2 *>(pl) = bf2{l0, l1};
+                        *reinterpret_cast<uint32_t *>(ph) = hw;
+                        *reinterpret_cast<uint32_t *>(pl) = lw;
                     }
                 } else {
 #pragma unroll
@@ -469,13 +482,14 @@ __device__ __forceinline__ void ladder_run(const DTile *__restrict__ tiles, int 
             for (int n = 0; n < NT; n++) acc[n] = Acc{};
             auto mfma_chunk = [&](int buf, const CT (&a)[G]) {
                 if constexpr (SPLIT) {
-                    bf16x8 ah, al;
+                    u32x4 hv, lv;
 #pragma unroll
-                    for (int s = 0; s < G; s++) {
-                        const __bf16 h = (__bf16)a[s];
-                        ah[s] = h;
-                        al[s] = (__bf16)(a[s] - (float)h);
+                    for (int j = 0; j < G / 2; j++) {
+                        const u32x2 hl = split_pair(a[2 * j], a[2 * j + 1]);
+                        hv[j] = hl[0];
+                        lv[j] = hl[1];
                     }
+                    const bf16x8 ah = __builtin_bit_cast(bf16x8, hv), al = __builtin_bit_cast(bf16x8, lv);
 #pragma unroll
                     for (int n = 0; n < NT; n++) {
                         const __bf16 *col = Bsp + ((2 * buf) * W + n * 16 + m) * CSTR + 8 * kq;
@@ -820,11 +834,12 @@ __device__ __forceinline__ void wave_tile(const DTile &tl, const DChan &ch, cons
 // a_h b_l + a_h b_h, FP32 accumulation):
 //  * own rows, per row block: three v_mfma_f32_16x16x16_bf16 with
 //    A[m][4q+e] = V[16 rb + m][k0 + 4 q + e], B[4q+e][n] = T[k0 + 4 q + e][n];
-//  * rows of the chunk: the two BF16 parts of each element go, as one
-//    32-bit word, into the wave's LDS block [k-slot][row] and come back
-//    transposed (lane (q, m) reads rows 32 g + 8 q .. + 7 of k-slot m: the A
-//    operand of v_mfma_f32_16x16x32_bf16, B = the tile's own T rows, split
-//    once per tile); 6 MFMAs per chunk, complete for the chunk's 16 rows.
+//  * rows of the chunk: the two BF16 parts of each element go, as 16-bit
+//    words, into the wave's LDS blocks [k-slot][row] (one per part) and come
+//    back transposed (lane (q, m) reads rows 32 g + 8 q .. + 7 of k-slot m,
+//    16 bytes: the A operand of v_mfma_f32_16x16x32_bf16, B = the tile's own
+//    T rows, split once per tile); 6 MFMAs per chunk, complete for the
+//    chunk's 16 rows.
 // Within a wave LDS accesses complete in order, so the transposition needs
 // no barrier.  The own-row partial tiles of the four waves are added in
 // wave order at the end of the tile.  The stored and generated modes use
@@ -832,28 +847,22 @@ __device__ __forceinline__ void wave_tile(const DTile &tl, const DChan &ch, cons
 // of a later k segment (aux > 0: channels of more than one segment) only
 // clears its slab of D_p: the segment-0 tile covers the whole k range.
 constexpr int SKC = 16;     // k-slots per chunk
-constexpr int SPSTR = 68;   // words per k-slot row of a wave's transposition block: 64 rows + 4
-                            // (the 16 lanes of a row group write consecutive banks, the four
-                            // groups 16 banks apart; the 16-byte reads of a row group cover all banks)
+constexpr int SPH = 72;     // 16-bit words per k-slot row of a wave's transposition blocks: 64 rows
+                            // + 8 (the four row groups' writes 16 banks apart; the 16-byte reads
+                            // of a row group cover all 64 banks once)
 
 struct SymLadder {
-    static constexpr int WBLK = SKC * SPSTR;       // words of a wave's transposition block
+    static constexpr int WBLK = 2 * SKC * SPH;     // 16-bit words of a wave's two blocks (high, low parts)
     static constexpr int RED = 4 * 4 * 4 * 64;     // floats of the four waves' own-row partial tiles
-    static constexpr size_t BYTES = (size_t)(4 * WBLK > RED ? 4 * WBLK : RED) * 4;
+    static constexpr size_t BYTES = (size_t)4 * WBLK * 2 > (size_t)RED * 4 ? (size_t)4 * WBLK * 2 : (size_t)RED * 4;
 };
 
-typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef short sh4 __attribute__((ext_vector_type(4)));
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ flt4 mfma_bf16_k16(bf16x4 a, bf16x4 b, flt4 c) {
+// v_mfma_f32_16x16x16_bf16 on operands given as two words of BF16 pairs
+__device__ __forceinline__ flt4 mfma_bf16_k16(u32x2 a, u32x2 b, flt4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(sh4, a), __builtin_bit_cast(sh4, b), c, 0,
                                                      0, 0);
-}
-
-// high part in the low half-word, low part in the high one
-__device__ __forceinline__ uint32_t pack_split(__bf16 h, __bf16 l) {
-    return (uint32_t)__builtin_bit_cast(uint16_t, h) | ((uint32_t)__builtin_bit_cast(uint16_t, l) << 16);
 }
 
 template <int SRC, int MODE>
@@ -897,17 +906,24 @@ __device__ __forceinline__ void sym_tile(const DTile &tl, const DChan &ch, const
     // (boff holds row row0 + 4 q)
     bf16x8 tbh[2], tbl[2];
 #pragma unroll
-    for (int g = 0; g < 2; g++)
+    for (int g = 0; g < 2; g++) {
+        float x[8];
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const float x = buf_load<float>(bsrc, boff + (uint32_t)((32 * g + 4 * q + e) * H) * 4u);
-            tbh[g][e] = (__bf16)x;
-            tbl[g][e] = (__bf16)(x - (float)tbh[g][e]);
+        for (int e = 0; e < 8; e++) x[e] = buf_load<float>(bsrc, boff + (uint32_t)((32 * g + 4 * q + e) * H) * 4u);
+        u32x4 hv, lv;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const u32x2 hl = split_pair(x[2 * j], x[2 * j + 1]);
+            hv[j] = hl[0];
+            lv[j] = hl[1];
         }
+        tbh[g] = __builtin_bit_cast(bf16x8, hv);
+        tbl[g] = __builtin_bit_cast(bf16x8, lv);
+    }
     flt4 acc[4];
 #pragma unroll
     for (int rb = 0; rb < 4; rb++) acc[rb] = flt4{0.f, 0.f, 0.f, 0.f};
-    uint32_t *blk = reinterpret_cast<uint32_t *>(lds) + wave * SymLadder::WBLK;
+    uint16_t *blh = reinterpret_cast<uint16_t *>(lds) + wave * SymLadder::WBLK, *bll = blh + SKC * SPH;
     float *sym = dsym + (int64_t)ch.sym * 1024;
 
     if (wave < nch) {   // uniform
@@ -945,16 +961,15 @@ __device__ __forceinline__ void sym_tile(const DTile &tl, const DChan &ch, const
                     }
                 });
             }
-            bf16x4 ah[4], al[4], bh, bl;
+            // BF16 pairs: element 2 j + i of a row in half i of word j
+            u32x2 ah[4], al[4];
+            const u32x2 b01 = split_pair(bc[0], bc[1]), b23 = split_pair(bc[2], bc[3]);
+            const u32x2 bh = {b01[0], b23[0]}, bl = {b01[1], b23[1]};
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                bh[e] = (__bf16)bc[e];
-                bl[e] = (__bf16)(bc[e] - (float)bh[e]);
-#pragma unroll
-                for (int rb = 0; rb < 4; rb++) {
-                    ah[rb][e] = (__bf16)a[rb][e];
-                    al[rb][e] = (__bf16)(a[rb][e] - (float)ah[rb][e]);
-                }
+            for (int rb = 0; rb < 4; rb++) {
+                const u32x2 p01 = split_pair(a[rb][0], a[rb][1]), p23 = split_pair(a[rb][2], a[rb][3]);
+                ah[rb] = u32x2{p01[0], p23[0]};
+                al[rb] = u32x2{p01[1], p23[1]};
             }
 #pragma unroll
             for (int rb = 0; rb < 4; rb++) {
@@ -966,22 +981,17 @@ __device__ __forceinline__ void sym_tile(const DTile &tl, const DChan &ch, const
 #pragma unroll
                 for (int rb = 0; rb < 4; rb++)
 #pragma unroll
-                    for (int e = 0; e < 4; e++) blk[(4 * q + e) * SPSTR + 16 * rb + m] = pack_split(ah[rb][e], al[rb][e]);
+                    for (int e = 0; e < 4; e++) {
+                        const int o = (4 * q + e) * SPH + 16 * rb + m;
+                        const int sh = 16 * (e & 1);
+                        blh[o] = (uint16_t)(ah[rb][e >> 1] >> sh);
+                        bll[o] = (uint16_t)(al[rb][e >> 1] >> sh);
+                    }
                 flt4 tacc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int g = 0; g < 2; g++) {
-                    const u32x4 w0 = *reinterpret_cast<const u32x4 *>(blk + m * SPSTR + 32 * g + 8 * q);
-                    const u32x4 w1 = *reinterpret_cast<const u32x4 *>(blk + m * SPSTR + 32 * g + 8 * q + 4);
-                    u32x4 hv, lv;
-                    hv[0] = __builtin_amdgcn_perm(w0[1], w0[0], 0x05040100u);
-                    hv[1] = __builtin_amdgcn_perm(w0[3], w0[2], 0x05040100u);
-                    hv[2] = __builtin_amdgcn_perm(w1[1], w1[0], 0x05040100u);
-                    hv[3] = __builtin_amdgcn_perm(w1[3], w1[2], 0x05040100u);
-                    lv[0] = __builtin_amdgcn_perm(w0[1], w0[0], 0x07060302u);
-                    lv[1] = __builtin_amdgcn_perm(w0[3], w0[2], 0x07060302u);
-                    lv[2] = __builtin_amdgcn_perm(w1[1], w1[0], 0x07060302u);
-                    lv[3] = __builtin_amdgcn_perm(w1[3], w1[2], 0x07060302u);
-                    const bf16x8 xh = __builtin_bit_cast(bf16x8, hv), xl = __builtin_bit_cast(bf16x8, lv);
+                    const bf16x8 xh = *reinterpret_cast<const bf16x8 *>(blh + m * SPH + 32 * g + 8 * q);
+                    const bf16x8 xl = *reinterpret_cast<const bf16x8 *>(bll + m * SPH + 32 * g + 8 * q);
                     tacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl, tbh[g], tacc, 0, 0, 0);
                     tacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, tbl[g], tacc, 0, 0, 0);
                     tacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, tbh[g], tacc, 0, 0, 0);
