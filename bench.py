@@ -86,8 +86,9 @@ KERNEL_OF_GROUP = {
 # the size classes of the pppp ladder each launch group times
 CLASSES_OF_GROUP = {"ladder_v1": (1,), "ladder_v2": (2,), "ladder_v34": (3, 4), "ladder_v58": (5, 6, 7, 8)}
 KIND_LABEL = {"default": "{dflt}", "mfma16": "16x16x4 MFMA", "bf16x3": "BF16x3 split products, 16x16x32 BF16 MFMA",
-              "bf16x3sym": "BF16x3 split products on the symmetry of V^pppp, 16x16x16 and 16x16x32 BF16 MFMA"}
-BF16X3_KINDS = ("bf16x3", "bf16x3sym")
+              "bf16x3sym": "BF16x3 split products on the symmetry of V^pppp, 16x16x16 and 16x16x32 BF16 MFMA",
+              "bf16x3w": "BF16x3 split products, waves split k, four rows per lane, 16x16x16 BF16 MFMA"}
+BF16X3_KINDS = ("bf16x3", "bf16x3sym", "bf16x3w")
 BF16_PEAK_TFLOPS = 2500.0   # dense BF16 MFMA; a BF16x3 product costs three
 
 
@@ -319,9 +320,9 @@ def main():
     kind_ab = None
     if prec == FP32 and not args.no_kind_ab:
         kind_ab = {}
-        for kind in ("default", "mfma16", "bf16x3", "bf16x3sym"):
-            # the symmetric kernel is for class 1: the others on BF16x3
-            q = dict(plan, kind_fp32=[kind] * 8 if kind != "bf16x3sym" else [kind] + ["bf16x3"] * 7)
+        for kind in ("default", "mfma16", "bf16x3", "bf16x3sym", "bf16x3w"):
+            # the wave-k kernels are for class 1: the others on BF16x3
+            q = dict(plan, kind_fp32=[kind] * 8 if kind not in ("bf16x3sym", "bf16x3w") else [kind] + ["bf16x3"] * 7)
             ctx.set_ladder_plan(q)
             tk = ctx.time_kernels(n_timed, REPS)
             kind_ab[kind] = {g: tk[g] * 1e3 for g in CLASSES_OF_GROUP}

@@ -22,7 +22,7 @@ STOP_REASONS = {0: "converged", 1: "max_iterations", 2: "nan"}
 # iteration schedules (CCD_SCHED_*): how the same kernels are launched
 SCHEDULES = {"auto": 0, "streams": 1, "serial": 2, "serial_graph": 3}
 # pppp-ladder plan vocabulary (CCD_LK_*, CCD_LA_*, CCD_LP_*, CCD_TUNE_*)
-LADDER_KINDS = {"default": 0, "mfma16": 1, "bf16x3": 2, "bf16x3sym": 3}
+LADDER_KINDS = {"default": 0, "mfma16": 1, "bf16x3": 2, "bf16x3sym": 3, "bf16x3w": 4}
 LADDER_ASSIGN = {"model": 0, "channel": 1, "longest": 2, "runs": 3}
 LADDER_PARTITIONS = {"split": 0, "one": 1}
 TUNE_KINDS = 1

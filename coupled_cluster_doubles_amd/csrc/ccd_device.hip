@@ -2145,12 +2145,13 @@ int time_iterations(ccd_ctx *c, double *us) {
 int plan_valid(const ccd_ladder_plan &p) {
     for (int n = 0; n < NTMAX; n++) {
         for (int q = 0; q < 2; q++)
-            if (p.kind[q][n] < CCD_LK_DEFAULT || p.kind[q][n] > CCD_LK_BF16X3SYM)
+            if (p.kind[q][n] < CCD_LK_DEFAULT || p.kind[q][n] > CCD_LK_BF16X3W)
                 return ccd::fail(CCD_E_INVALID, "ladder plan: unknown kernel kind %d (class %d)", p.kind[q][n], n + 1);
-        if (p.kind[CCD_PREC_FP64][n] == CCD_LK_BF16X3 || p.kind[CCD_PREC_FP64][n] == CCD_LK_BF16X3SYM)
+        if (p.kind[CCD_PREC_FP64][n] >= CCD_LK_BF16X3)
             return ccd::fail(CCD_E_INVALID, "ladder plan: BF16x3 products are for FP32 iterations (class %d)", n + 1);
-        if (n > 0 && p.kind[CCD_PREC_FP32][n] == CCD_LK_BF16X3SYM)
-            return ccd::fail(CCD_E_INVALID, "ladder plan: the symmetric kernel is for class 1 (class %d)", n + 1);
+        if (n > 0 && p.kind[CCD_PREC_FP32][n] >= CCD_LK_BF16X3SYM)
+            return ccd::fail(CCD_E_INVALID, "ladder plan: kernel kind %d is for class 1 (class %d)",
+                             p.kind[CCD_PREC_FP32][n], n + 1);
         if (p.blocks_per_cu[n] < 0 || p.blocks_per_cu[n] > 16)
             return ccd::fail(CCD_E_INVALID, "ladder plan: %d blocks per compute unit (class %d)", p.blocks_per_cu[n],
                              n + 1);
@@ -2209,7 +2210,10 @@ int tune_ladder(ccd_ctx *c, int flags) {
         std::vector<int> ck{CCD_LK_DEFAULT};   // class 1's 16x16x4 kernel differs from its default
         for (int k : kinds) {
             if (k == CCD_LK_BF16X3 || (k == CCD_LK_MFMA16 && nt == 1)) ck.push_back(k);
-            if (k == CCD_LK_BF16X3 && nt == 1) ck.push_back(CCD_LK_BF16X3SYM);
+            if (k == CCD_LK_BF16X3 && nt == 1) {
+                ck.push_back(CCD_LK_BF16X3SYM);
+                ck.push_back(CCD_LK_BF16X3W);
+            }
         }
         std::vector<int> bpcs{1, 2, 3, 4, 6};
         if (!c->matrix_free) bpcs.insert(bpcs.begin(), 0);
@@ -2225,8 +2229,10 @@ int tune_ladder(ccd_ctx *c, int flags) {
         ccd_ladder_plan q = start;
         for (int n = 0; n < NTMAX; n++) q.kind[prec][n] = k;
         cands.push_back(q);
-        if (k == CCD_LK_BF16X3) {   // and class 1 on the symmetric kernel
+        if (k == CCD_LK_BF16X3) {   // and class 1 on the wave-k kernels
             q.kind[prec][0] = CCD_LK_BF16X3SYM;
+            cands.push_back(q);
+            q.kind[prec][0] = CCD_LK_BF16X3W;
             cands.push_back(q);
         }
     }

@@ -249,7 +249,8 @@ enum LadderKind {
                       // 16x16x4 one for the others; FP64: the 16x16x4 one
     LK_MFMA16 = 1,    // v_mfma_{f32,f64}_16x16x4 (ladder_run) in every class
     LK_BF16X3 = 2,    // FP32 iterations: BF16x3 split products, v_mfma_f32_16x16x32_bf16
-    LK_BF16X3SYM = 3, // FP32 iterations, class 1: BF16x3 on the symmetry of V^pppp (sym_tile)
+    LK_BF16X3SYM = 3, // FP32 iterations, class 1: BF16x3 on the symmetry of V^pppp (wave_k_tile)
+    LK_BF16X3W = 4,   // FP32 iterations, class 1: BF16x3, waves split k, four rows per lane (wave_k_tile)
 };
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -846,6 +847,12 @@ __device__ __forceinline__ void wave_tile(const DTile &tl, const DChan &ch, cons
 // the same elements in the same order, so they agree bit for bit.  A tile
 // of a later k segment (aux > 0: channels of more than one segment) only
 // clears its slab of D_p: the segment-0 tile covers the whole k range.
+//
+// SYM = false (LK_BF16X3W): the same kernel without the symmetry -- every
+// tile takes the k range of its segment (all of it from k = 0, like the
+// other kinds) and only the own-row product, written to its segment's slab
+// -- for the layout alone: one record broadcast per four rows (the
+// 16x16x32 kernel broadcasts per row) and no partial sums for the update.
 constexpr int SKC = 16;     // k-slots per chunk
 constexpr int SPH = 72;     // 16-bit words per k-slot row of a wave's transposition blocks: 64 rows
                             // + 8 (the four row groups' writes 16 banks apart; the 16-byte reads
@@ -865,8 +872,8 @@ __device__ __forceinline__ flt4 mfma_bf16_k16(u32x2 a, u32x2 b, flt4 c) {
                                                      0, 0);
 }
 
-template <int SRC, int MODE>
-__device__ __forceinline__ void sym_tile(const DTile &tl, const DChan &ch, const float *__restrict__ vp,
+template <int SRC, int MODE, bool SYM>
+__device__ __forceinline__ void wave_k_tile(const DTile &tl, const DChan &ch, const float *__restrict__ vp,
                                          const LadderGen &lg, const float *__restrict__ tas,
                                          const double *__restrict__ zero, float *__restrict__ dp,
                                          float *__restrict__ dsym, unsigned char *lds, uint32_t tbase, int wave,
@@ -874,13 +881,15 @@ __device__ __forceinline__ void sym_tile(const DTile &tl, const DChan &ch, const
     constexpr bool ONE = MODE == 1, SAME = MODE == 2, GEN = SRC != LV_STORED;
     constexpr bool GAS = gas_source(SRC) && MODE != 0;   // else the generic generator
     const int H = ch.hh, npk = ch.npk, m = lane & 15, q = lane >> 4;
-    if (tl.aux > 0) {   // block-uniform
+    if (SYM && tl.aux > 0) {   // block-uniform
         float *o = dp + ch.off_dp + (int64_t)tl.aux * npk * H + (int64_t)tl.row0 * H;
         for (int l = wave * 64 + lane; l < tl.nrows * H; l += kThreads) o[l] = 0.f;
         return;
     }
-    const int row0 = tl.row0, J = row0 >> 6, kend = npk;
-    const int nch = (kend - row0 + SKC - 1) / SKC;
+    constexpr int KSEG = LadderShape<1>::KSEG;
+    const int row0 = tl.row0, J = row0 >> 6, seg = SYM ? 0 : tl.aux;
+    const int kbeg = SYM ? row0 : seg * KSEG, kend = SYM ? npk : min(npk, kbeg + KSEG);
+    const int nch = (kend - kbeg + SKC - 1) / SKC;
     // the records of the split path: total-spin-0 channels with the sign field
     const int2 *grec1 = GAS ? (ONE ? lg.rec1w : lg.rec1f) : lg.rec1;
     // rows of this lane; rows past the tile repeat its first row (finite
@@ -896,20 +905,20 @@ __device__ __forceinline__ void sym_tile(const DTile &tl, const DChan &ch, const
         rb2[rb] = GAS && SAME ? lg.rec2f[ch.pr_off + u] : int2{0, 0};
         av[rb] = vp + ch.off_vp + u * ch.ldv;
     }
-    // buffer loads: records past kend and T rows past npk (or columns past hh) read 0
+    // buffer loads: records and T rows past kend (or columns past hh) read 0
     const BufRsrc rsrc = GEN ? make_rsrc(grec1 + ch.pr_off, (uint32_t)kend * 8u) : make_rsrc(zero, 0);
     const BufRsrc bsrc = make_rsrc(tas + ch.off_tp, (uint32_t)kend * (uint32_t)H * 4u);
-    const uint32_t roff = m < 4 ? (uint32_t)(row0 + 4 * q + m) * 8u : kOOB;   // record of k-slot 4 q + m
-    const uint32_t boff = m < H ? (uint32_t)((row0 + 4 * q) * H + m) * 4u : kOOB;   // T[row0 + 4 q][m]
+    const uint32_t roff = m < 4 ? (uint32_t)(kbeg + 4 * q + m) * 8u : kOOB;   // record of k-slot 4 q + m
+    const uint32_t boff = m < H ? (uint32_t)((kbeg + 4 * q) * H + m) * 4u : kOOB;   // T[kbeg + 4 q][m]
     const uint32_t cstep = (uint32_t)SKC * (uint32_t)H * 4u;
     // B of the second product: the tile's T rows 32 g + 8 q + e in column m
-    // (boff holds row row0 + 4 q)
     bf16x8 tbh[2], tbl[2];
+    const uint32_t toff = m < H ? (uint32_t)((row0 + 8 * q) * H + m) * 4u : kOOB;
 #pragma unroll
     for (int g = 0; g < 2; g++) {
         float x[8];
 #pragma unroll
-        for (int e = 0; e < 8; e++) x[e] = buf_load<float>(bsrc, boff + (uint32_t)((32 * g + 4 * q + e) * H) * 4u);
+        for (int e = 0; e < 8; e++) x[e] = SYM ? buf_load<float>(bsrc, toff + (uint32_t)((32 * g + e) * H) * 4u) : 0.f;
         u32x4 hv, lv;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
@@ -938,7 +947,7 @@ __device__ __forceinline__ void sym_tile(const DTile &tl, const DChan &ch, const
             float bn[4];
 #pragma unroll
             for (int e = 0; e < 4; e++) bn[e] = buf_load<float>(bsrc, boff + (uint32_t)cn * cstep + (uint32_t)(e * H) * 4u);
-            const int k0 = row0 + c * SKC;
+            const int k0 = kbeg + c * SKC;
             float a[4][4];
             if constexpr (!GEN) {
                 const bool kok = k0 + 4 * q < ch.ldv;
@@ -977,7 +986,7 @@ __device__ __forceinline__ void sym_tile(const DTile &tl, const DChan &ch, const
                 acc[rb] = mfma_bf16_k16(ah[rb], bl, acc[rb]);
                 acc[rb] = mfma_bf16_k16(ah[rb], bh, acc[rb]);
             }
-            if (c >= 4) {   // uniform: the chunk lies past the tile
+            if (SYM && c >= 4) {   // uniform: the chunk lies past the tile
 #pragma unroll
                 for (int rb = 0; rb < 4; rb++)
 #pragma unroll
@@ -1024,7 +1033,7 @@ __device__ __forceinline__ void sym_tile(const DTile &tl, const DChan &ch, const
 #pragma unroll
         for (int w = 1; w < 4; w++) v += red[((w * 4 + wave) * 4 + r) * 64 + lane];
         const int row = 16 * wave + 4 * q + r;
-        if (row < tl.nrows && m < H) dp[ch.off_dp + (int64_t)(row0 + row) * H + m] = v;
+        if (row < tl.nrows && m < H) dp[ch.off_dp + (int64_t)seg * npk * H + (int64_t)(row0 + row) * H + m] = v;
     }
     __syncthreads();   // the LDS is reused by the next tile
 }
@@ -1052,7 +1061,7 @@ struct LadderLaunch {
 template <typename CT, int NT>
 __host__ __device__ constexpr size_t ladder_class_bytes(int kind) {
     if (sizeof(CT) == 4 && kind == LK_BF16X3) return LadderTile<float, NT, true>::BYTES;
-    if (sizeof(CT) == 4 && NT == 1 && kind == LK_BF16X3SYM) return SymLadder::BYTES;
+    if (sizeof(CT) == 4 && NT == 1 && (kind == LK_BF16X3SYM || kind == LK_BF16X3W)) return SymLadder::BYTES;
     if (sizeof(CT) == 4 && NT == 1 && kind == LK_DEFAULT) return (size_t)WaveLadder<1>::REGION * sizeof(float);
     return LadderTile<CT, NT, false>::BYTES;
 }
@@ -1070,7 +1079,7 @@ __device__ __forceinline__ void ladder_class(int kind, const DTile *__restrict__
             return;
         }
         if constexpr (NT == 1) {
-            if (kind == LK_BF16X3SYM) {
+            if (kind == LK_BF16X3SYM || kind == LK_BF16X3W) {
                 const int tid = threadIdx.x & (kThreads - 1);
                 const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
                 int prob = -1;
@@ -1079,12 +1088,17 @@ __device__ __forceinline__ void ladder_class(int kind, const DTile *__restrict__
                     const DTile tl = tiles[ti];
                     const DChan ch = chans[tl.chan];
                     if (!ladder_tile_begin<float, SRC>(lg, ch, prob, stopped, tab_lds)) continue;
-                    if (gas_source(SRC) && ch.spin0 == 1)
-                        sym_tile<SRC, 1>(tl, ch, vp, lg, tas, zero, dp, dsym, lds_b, tbase, wave, lane);
-                    else if (SRC == LV_GAS || (SRC == LV_GAS_BATCH && ch.spin0 == 2))
-                        sym_tile<SRC, 2>(tl, ch, vp, lg, tas, zero, dp, dsym, lds_b, tbase, wave, lane);
-                    else
-                        sym_tile<SRC, 0>(tl, ch, vp, lg, tas, zero, dp, dsym, lds_b, tbase, wave, lane);
+                    auto run = [&](auto sym_) {
+                        constexpr bool S = decltype(sym_)::value;
+                        if (gas_source(SRC) && ch.spin0 == 1)
+                            wave_k_tile<SRC, 1, S>(tl, ch, vp, lg, tas, zero, dp, dsym, lds_b, tbase, wave, lane);
+                        else if (SRC == LV_GAS || (SRC == LV_GAS_BATCH && ch.spin0 == 2))
+                            wave_k_tile<SRC, 2, S>(tl, ch, vp, lg, tas, zero, dp, dsym, lds_b, tbase, wave, lane);
+                        else
+                            wave_k_tile<SRC, 0, S>(tl, ch, vp, lg, tas, zero, dp, dsym, lds_b, tbase, wave, lane);
+                    };
+                    if (kind == LK_BF16X3SYM) run(std::true_type{});
+                    else run(std::false_type{});
                 }
                 return;
             }

@@ -27,7 +27,7 @@ void launch_src(int g, unsigned grid, size_t lds, hipStream_t st, const LadderLa
 template <typename CT, int NT>
 size_t max_class_bytes() {
     size_t b = 0;
-    for (int k = LK_DEFAULT; k <= LK_BF16X3SYM; k++) b = b > ladder_class_bytes<CT, NT>(k) ? b : ladder_class_bytes<CT, NT>(k);
+    for (int k = LK_DEFAULT; k <= LK_BF16X3W; k++) b = b > ladder_class_bytes<CT, NT>(k) ? b : ladder_class_bytes<CT, NT>(k);
     return b;
 }
 

@@ -175,6 +175,9 @@ int ccd_get_schedule(const ccd_ctx *ctx, int *sched, double *us_per_iter);
                                 symmetry of V^pppp (<ab||cd> = <cd||ab>): each element with
                                 c,d at or past the row tile of a,b is built once and used
                                 for both rows, about 0.55 of the elements of the others */
+#define CCD_LK_BF16X3W 4     /* FP32 iterations, class 1 only: the kernel of CCD_LK_BF16X3SYM
+                                without the symmetry (every element built; waves split k,
+                                four rows per lane) */
 #define CCD_LA_MODEL 0       /* tile assignment: the one a model of the CU loads prefers
                                 (a matrix-free batch: CCD_LA_RUNS) */
 #define CCD_LA_CHANNEL 1     /*   round robin in channel order */
@@ -200,8 +203,9 @@ typedef struct ccd_ladder_plan {
 int ccd_get_ladder_plan(const ccd_ctx *ctx, ccd_ladder_plan *plan);
 /* Use `plan` from the next iteration on (its us fields are ignored).  Fails
  * with CCD_E_INVALID on an unknown kind, assignment or partition, a
- * CCD_LK_BF16X3 or CCD_LK_BF16X3SYM entry for FP64 iterations, a
- * CCD_LK_BF16X3SYM entry outside class 1, or a block count outside 0..16. */
+ * CCD_LK_BF16X3, CCD_LK_BF16X3SYM or CCD_LK_BF16X3W entry for FP64
+ * iterations, a CCD_LK_BF16X3SYM or CCD_LK_BF16X3W entry outside class 1,
+ * or a block count outside 0..16. */
 int ccd_set_ladder_plan(ccd_ctx *ctx, const ccd_ladder_plan *plan);
 /* Tune the plan on this problem, for the arithmetic of the following
  * iterations.  Per class: the kernel kinds that `flags` allow, then the
@@ -212,8 +216,8 @@ int ccd_set_ladder_plan(ccd_ctx *ctx, const ccd_ladder_plan *plan);
  * with each allowed kind in every class, each under both partitions
  * (us_partition: the chosen plan's).  Resets T2 to zero. */
 #define CCD_TUNE_KINDS 1     /* also try CCD_LK_MFMA16 where it differs from the default */
-#define CCD_TUNE_BF16X3 2    /* also try CCD_LK_BF16X3 and, for class 1, CCD_LK_BF16X3SYM (FP32
-                                iterations) */
+#define CCD_TUNE_BF16X3 2    /* also try CCD_LK_BF16X3 and, for class 1, CCD_LK_BF16X3SYM and
+                                CCD_LK_BF16X3W (FP32 iterations) */
 int ccd_tune_ladder(ccd_ctx *ctx, int flags);
 
 /* MBPT2 correlation energy, 1/4 sum v^2/(e_i+e_j-e_a-e_b)

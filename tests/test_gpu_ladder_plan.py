@@ -8,9 +8,10 @@ the summation (16x16x4 MFMA instead of 4x4x1 for class 1) or the arithmetic
 the oracle -- FP32 kinds within 2e-5 of the largest element of the ladder,
 BF16x3 within 2e-4 -- and a mixed solve with them to 1e-7 Ha per particle
 of the oracle's converged energy.  The symmetric class-1 kind (bf16x3sym)
-builds each element of V^pppp once for both of its rows: held to the oracle
-like BF16x3, also through whole iterations.  Stored and generated V stay
-bit-identical under every kind.
+builds each element of V^pppp once for both of its rows, and bf16x3w is its
+kernel without the symmetry: both held to the oracle like BF16x3, also
+through whole iterations.  Stored and generated V stay bit-identical under
+every kind.
 """
 import numpy as np
 import pytest
@@ -60,7 +61,9 @@ def test_default_plan_and_validation(require_gpu):
     p = ctx.ladder_plan()
     for bad in ({"kind_fp64": ["bf16x3"] * 8}, {"blocks_per_cu": [17] * 8}, {"blocks_per_cu": [-1] * 8},
                 {"kind_fp64": ["bf16x3sym"] + ["default"] * 7},
-                {"kind_fp32": ["bf16x3", "bf16x3sym"] + ["bf16x3"] * 6}):
+                {"kind_fp32": ["bf16x3", "bf16x3sym"] + ["bf16x3"] * 6},
+                {"kind_fp64": ["bf16x3w"] + ["default"] * 7},
+                {"kind_fp32": ["bf16x3", "bf16x3w"] + ["bf16x3"] * 6}):
         q = dict(p, **bad)
         with pytest.raises(CCDError):
             ctx.set_ladder_plan(q)
@@ -75,11 +78,12 @@ def test_default_plan_and_validation(require_gpu):
     ("pairing", (16, 16, 0.5, 1.0), FP32, False),
     ("heg", (1.0, 19, 2), "sym", True),
     ("heg", (1.0, 10, 3), "sym", False),
+    ("heg", (1.0, 19, 2), "w", True),
 ])
 def test_scheduling_fields_change_no_result(require_gpu, kind, args, prec, free):
     m = _model(kind, args)
-    sym = prec == "sym"   # FP32 iterations with class 1 on the symmetric kernel
-    if sym:
+    k1 = {"sym": "bf16x3sym", "w": "bf16x3w"}.get(prec)   # FP32 iterations, class 1 on a wave-k kernel
+    if k1:
         prec = FP32
     plans = [
         {},
@@ -91,7 +95,7 @@ def test_scheduling_fields_change_no_result(require_gpu, kind, args, prec, free)
     for q, kw in enumerate(plans):
         print(f"{kind}{args} prec={prec} free={free} plan {q}: {kw}", flush=True)
         ctx = Context(m, prec, matrix_free=free)
-        _plan(ctx, **kw, **({"kind_fp32": ["bf16x3sym"] + ["bf16x3"] * 7} if sym else {}))
+        _plan(ctx, **kw, **({"kind_fp32": [k1] + ["bf16x3"] * 7} if k1 else {}))
         got = (ctx.iterate_n(4), ctx.get_t2())
         ctx.close()
         if ref is None:
@@ -112,7 +116,7 @@ def test_kernel_kinds_close_to_oracle(require_gpu, kind, args):
     scale = np.max(np.abs(d_o))
     for kinds, tol in ((["mfma16"] * 8, F32_REL), (["bf16x3"] * 8, BF16X3_REL),
                        (["default", "bf16x3", "mfma16", "bf16x3", "default", "bf16x3", "mfma16", "bf16x3"], BF16X3_REL),
-                       (["bf16x3sym"] + ["default"] * 7, BF16X3_REL)):
+                       (["bf16x3sym"] + ["default"] * 7, BF16X3_REL), (["bf16x3w"] + ["default"] * 7, BF16X3_REL)):
         ctx = Context(m, FP32)
         _plan(ctx, kind_fp32=kinds)
         err = np.max(np.abs(ctx.ladder(ta) - d_o)) / scale
@@ -158,7 +162,7 @@ def test_ladder_kinds_at_n54_against_oracle(require_gpu):
     assert err <= 1e-12
     ctx.close()
     for kinds, tol in ((["default"] * 8, F32_REL), (["mfma16"] * 8, F32_REL), (["bf16x3"] * 8, BF16X3_REL),
-                       (["bf16x3sym"] + ["bf16x3"] * 7, BF16X3_REL)):
+                       (["bf16x3sym"] + ["bf16x3"] * 7, BF16X3_REL), (["bf16x3w"] + ["bf16x3"] * 7, BF16X3_REL)):
         ctx = Context(m, FP32, matrix_free=True)
         _plan(ctx, kind_fp32=kinds)
         err = np.max(np.abs(ctx.ladder(ta) - d_o)) / scale
@@ -167,7 +171,8 @@ def test_ladder_kinds_at_n54_against_oracle(require_gpu):
         ctx.close()
 
 
-@pytest.mark.parametrize("kinds", [["bf16x3"] * 8, ["mfma16"] * 8, ["bf16x3sym"] + ["bf16x3"] * 7])
+@pytest.mark.parametrize("kinds", [["bf16x3"] * 8, ["mfma16"] * 8, ["bf16x3sym"] + ["bf16x3"] * 7,
+                                   ["bf16x3w"] + ["bf16x3"] * 7])
 def test_stored_and_generated_v_agree_bitwise_under_every_kind(require_gpu, kinds):
     m = Model.electron_gas(1.0, 19, 2)
     a, b = Context(m, FP32), Context(m, FP32, matrix_free=True)
@@ -177,7 +182,7 @@ def test_stored_and_generated_v_agree_bitwise_under_every_kind(require_gpu, kind
     assert np.array_equal(a.get_t2(), b.get_t2())
 
 
-@pytest.mark.parametrize("kinds", [["bf16x3"] * 8, ["bf16x3sym"] + ["bf16x3"] * 7])
+@pytest.mark.parametrize("kinds", [["bf16x3"] * 8, ["bf16x3sym"] + ["bf16x3"] * 7, ["bf16x3w"] + ["bf16x3"] * 7])
 def test_mixed_solve_with_bf16x3_products_within_1e7_per_particle(require_gpu, kinds):
     """FP32 iterations on BF16x3 products, then FP64 sweeps: the converged
     energy within 1e-7 Ha per particle of the oracle's (N=14 electrons)."""
