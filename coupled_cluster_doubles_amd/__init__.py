@@ -23,6 +23,7 @@ STOP_REASONS = {0: "converged", 1: "max_iterations", 2: "nan"}
 SCHEDULES = {"auto": 0, "streams": 1, "serial": 2, "serial_graph": 3}
 # pppp-ladder plan vocabulary (CCD_LK_*, CCD_LA_*, CCD_LP_*, CCD_TUNE_*)
 LADDER_KINDS = {"default": 0, "mfma16": 1, "bf16x3": 2, "bf16x3sym": 3, "bf16x3w": 4}
+RING_KINDS = {"fp32": 0, "bf16x3": 1}
 LADDER_ASSIGN = {"model": 0, "channel": 1, "longest": 2, "runs": 3}
 LADDER_PARTITIONS = {"split": 0, "one": 1}
 TUNE_KINDS = 1
@@ -218,8 +219,9 @@ class Context:
     def ladder_plan(self) -> dict:
         """The pppp-ladder plan: per size class 1..8 (16(n-1) < hh <= 16n) the
         kernel kind for FP64 / FP32 iterations, blocks per compute unit (0: one
-        per tile) and tile assignment; the launch partition; the tuner's
-        microseconds per class launch and per iteration of each partition."""
+        per tile) and tile assignment; the launch partition; the ring's kernel
+        in FP32 iterations; the tuner's microseconds per class launch and per
+        iteration of each partition."""
         p = LadderPlan()
         check(lib().ccd_get_ladder_plan(self.h, C.byref(p)), "ccd_get_ladder_plan")
         kinds = {v: k for k, v in LADDER_KINDS.items()}
@@ -234,6 +236,7 @@ class Context:
             "us_fp64": [float(p.us[0][n]) for n in range(8)],
             "us_fp32": [float(p.us[1][n]) for n in range(8)],
             "us_partition": {"split": float(p.us_partition[0]), "one": float(p.us_partition[1])},
+            "ring_fp32": {v: k for k, v in RING_KINDS.items()}[p.ring_fp32],
         }
 
     def set_ladder_plan(self, plan: dict):
@@ -245,6 +248,7 @@ class Context:
             p.blocks_per_cu[n] = plan["blocks_per_cu"][n]
             p.assign[n] = LADDER_ASSIGN[plan["assign"][n]]
         p.partition = LADDER_PARTITIONS[plan["partition"]]
+        p.ring_fp32 = RING_KINDS[plan.get("ring_fp32", "fp32")]
         check(lib().ccd_set_ladder_plan(self.h, C.byref(p)), "ccd_set_ladder_plan")
 
     def tune_ladder(self, flags: int = 0) -> dict:

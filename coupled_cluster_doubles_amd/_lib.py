@@ -38,6 +38,7 @@ class LadderPlan(C.Structure):
     _fields_ = [
         ("kind", (C.c_int32 * 8) * 2), ("blocks_per_cu", C.c_int32 * 8), ("assign", C.c_int32 * 8),
         ("partition", C.c_int32), ("us", (C.c_double * 8) * 2), ("us_partition", C.c_double * 2),
+        ("ring_fp32", C.c_int32),
     ]
 
 

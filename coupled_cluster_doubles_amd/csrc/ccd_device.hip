@@ -328,27 +328,27 @@ size_t ring_stage_bytes(int nt) {
 // LDS-staged ring: one launch over the groups of 1-4 tile channels, one over
 // the 5-tile channels (whose register and LDS footprint would lower the
 // occupancy of the others)
-template <typename CT>
+template <typename CT, bool SPLIT = false>
 void launch_ring_staged(ccd_ctx *c, const CT *va, const CT *vb, hipStream_t st) {
     const int n5 = c->n_rg_five, n = (int)c->rgroups.size();
     if (n5 > 0) {
         const size_t lds = RingStage<CT, 5>::BYTES;
-        hipLaunchKernelGGL((k_ring_staged<CT, 5, 5>), dim3((unsigned)n5), dim3(kThreads), lds, st, c->d_rgroups,
-                           c->d_mods, reinterpret_cast<const CT *>(c->d_tph), va, vb, c->d_zero,
+        hipLaunchKernelGGL((k_ring_staged<CT, 5, 5, SPLIT>), dim3((unsigned)n5), dim3(kThreads), lds, st,
+                           c->d_rgroups, c->d_mods, reinterpret_cast<const CT *>(c->d_tph), va, vb, c->d_zero,
                            reinterpret_cast<CT *>(c->d_r), c->d_ctl, skip_ctl(c));
     }
     if (n > n5) {
         const size_t lds = c->rg_lds[sizeof(CT) == 8 ? 0 : 1];
-        hipLaunchKernelGGL((k_ring_staged<CT, 1, 4>), dim3((unsigned)(n - n5)), dim3(kThreads), lds, st,
+        hipLaunchKernelGGL((k_ring_staged<CT, 1, 4, SPLIT>), dim3((unsigned)(n - n5)), dim3(kThreads), lds, st,
                            c->d_rgroups + n5, c->d_mods, reinterpret_cast<const CT *>(c->d_tph), va, vb, c->d_zero,
                            reinterpret_cast<CT *>(c->d_r), c->d_ctl, skip_ctl(c));
     }
 }
 
 // The staged instances that may use more than 64 KB of LDS need the limit raised.
-template <typename CT, int LO, int HI>
+template <typename CT, int LO, int HI, bool SPLIT = false>
 int allow_ring_lds(size_t bytes) {
-    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_ring_staged<CT, LO, HI>),
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_ring_staged<CT, LO, HI, SPLIT>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     return CCD_OK;
 }
@@ -357,7 +357,8 @@ int allow_ring_lds_all() {
     int rc;
     if ((rc = allow_ring_lds<double, 1, 4>(RingStage<double, 4>::BYTES)) ||
         (rc = allow_ring_lds<double, 5, 5>(RingStage<double, 5>::BYTES)) ||
-        (rc = allow_ring_lds<float, 5, 5>(RingStage<float, 5>::BYTES)))
+        (rc = allow_ring_lds<float, 5, 5>(RingStage<float, 5>::BYTES)) ||
+        (rc = allow_ring_lds<float, 5, 5, true>(RingStage<float, 5>::BYTES)))
         return rc;
     return CCD_OK;
 }
@@ -366,7 +367,8 @@ int launch_ring(ccd_ctx *c, hipStream_t st) {
     const int nw = c->n_ring_wide;
     if (c->iter_prec == CCD_PREC_FP32) {
         launch_ring_ct<float, true>(c, c->d_va32, c->d_vb32, 0, nw, st);
-        launch_ring_staged<float>(c, c->d_va32, c->d_vb32, st);
+        if (c->plan.ring_fp32 == CCD_RK_BF16X3) launch_ring_staged<float, true>(c, c->d_va32, c->d_vb32, st);
+        else launch_ring_staged<float>(c, c->d_va32, c->d_vb32, st);
     } else {
         launch_ring_ct<double, true>(c, c->d_va, c->d_vb, 0, nw, st);
         launch_ring_staged<double>(c, c->d_va, c->d_vb, st);
@@ -1126,6 +1128,7 @@ void default_plan(ccd_ctx *c) {
         p.assign[n] = CCD_LA_MODEL;
     }
     p.partition = CCD_LP_SPLIT;
+    p.ring_fp32 = CCD_RK_FP32;
 }
 
 // Block runs of class nt's ladder tiles under the plan (host tables ltiles,
@@ -2160,6 +2163,8 @@ int plan_valid(const ccd_ladder_plan &p) {
     }
     if (p.partition != CCD_LP_SPLIT && p.partition != CCD_LP_ONE)
         return ccd::fail(CCD_E_INVALID, "ladder plan: unknown partition %d", p.partition);
+    if (p.ring_fp32 != CCD_RK_FP32 && p.ring_fp32 != CCD_RK_BF16X3)
+        return ccd::fail(CCD_E_INVALID, "ladder plan: unknown ring kind %d", p.ring_fp32);
     return CCD_OK;
 }
 
@@ -2236,6 +2241,14 @@ int tune_ladder(ccd_ctx *c, int flags) {
             cands.push_back(q);
         }
     }
+    if (prec == CCD_PREC_FP32 && (flags & CCD_TUNE_BF16X3)) {   // each of them with the BF16x3 ring
+        const size_t nc = cands.size();
+        for (size_t q = 0; q < nc; q++) {
+            ccd_ladder_plan r = cands[q];
+            r.ring_fp32 = CCD_RK_BF16X3;
+            cands.push_back(r);
+        }
+    }
     double best = 0.0;
     ccd_ladder_plan pick = start;
     for (size_t q = 0; q < cands.size() && !rc; q++)
@@ -2258,7 +2271,8 @@ int tune_ladder(ccd_ctx *c, int flags) {
         for (size_t q = 0; q < cands.size(); q++)   // the partitions' timings of the plan chosen
             if (std::memcmp(cands[q].kind, pick.kind, sizeof pick.kind) == 0 &&
                 std::memcmp(cands[q].blocks_per_cu, pick.blocks_per_cu, sizeof pick.blocks_per_cu) == 0 &&
-                std::memcmp(cands[q].assign, pick.assign, sizeof pick.assign) == 0) {
+                std::memcmp(cands[q].assign, pick.assign, sizeof pick.assign) == 0 &&
+                cands[q].ring_fp32 == pick.ring_fp32) {
                 p.us_partition[0] = cands[q].us_partition[0];
                 p.us_partition[1] = cands[q].us_partition[1];
             }

@@ -250,6 +250,51 @@ __device__ __forceinline__ flt4 mfma16(float a, float b, flt4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+// BF16x3 split products (x = x_h + x_l in bfloat16, a b ~ a_l b_h + a_h b_l +
+// a_h b_h with FP32 accumulation).
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef short sh4 __attribute__((ext_vector_type(4)));
+
+// The BF16x3 parts of two FP32 values, each pair in one 32-bit word (the
+// first value in the low half): [0] = bf16(x) (round to nearest even), [1] =
+// bf16(x - [0]).  A pair per conversion (v_cvt_pk_bf16_f32 takes two
+// values); the same numbers as converting one value at a time.
+__device__ __forceinline__ u32x2 split_pair(float x0, float x1) {
+    const flt2 x = {x0, x1};
+    const uint32_t hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(x, bf16x2));
+    const flt2 h = {__uint_as_float(hi << 16), __uint_as_float(hi & 0xffff0000u)};
+    return u32x2{hi, __builtin_bit_cast(uint32_t, __builtin_convertvector(x - h, bf16x2))};
+}
+
+// The parts of four values: [0] = the high parts, [1] = the low parts, as
+// the operand of v_mfma_f32_16x16x16_bf16 (value e of the four in half e % 2
+// of word e / 2).
+struct Split4 {
+    u32x2 h, l;
+};
+__device__ __forceinline__ Split4 split4(float x0, float x1, float x2, float x3) {
+    const u32x2 p01 = split_pair(x0, x1), p23 = split_pair(x2, x3);
+    return Split4{u32x2{p01[0], p23[0]}, u32x2{p01[1], p23[1]}};
+}
+
+// v_mfma_f32_16x16x16_bf16: A is 16x16 (lane l: row l%16, k-slots 4 (l/16) ..
+// + 3), B is 16x16 (lane l: k-slots 4 (l/16) .. + 3, column l%16), C as
+// v_mfma_f32_16x16x4f32's; operands given as two words of BF16 pairs.
+__device__ __forceinline__ flt4 mfma_bf16_k16(u32x2 a, u32x2 b, flt4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(sh4, a), __builtin_bit_cast(sh4, b), c, 0,
+                                                     0, 0);
+}
+
+// a b in BF16x3: the three products in the order of the ladder's split kernels
+__device__ __forceinline__ flt4 mfma_bf16x3_k16(const Split4 &a, const Split4 &b, flt4 c) {
+    c = mfma_bf16_k16(a.l, b.h, c);
+    c = mfma_bf16_k16(a.h, b.l, c);
+    return mfma_bf16_k16(a.h, b.h, c);
+}
+
 // Per-type view of the 16x16 MFMA tile: accumulator type and the output row
 // that register q of lane l holds.
 template <typename CT> struct Mma;

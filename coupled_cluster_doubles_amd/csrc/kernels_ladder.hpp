@@ -253,22 +253,6 @@ enum LadderKind {
     LK_BF16X3W = 4,   // FP32 iterations, class 1: BF16x3, waves split k, four rows per lane (wave_k_tile)
 };
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-typedef float flt2 __attribute__((ext_vector_type(2)));
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
-
-// The BF16x3 parts of two FP32 values, each pair in one 32-bit word (the
-// first value in the low half): [0] = bf16(x) (round to nearest even), [1] =
-// bf16(x - [0]).  A pair per conversion (v_cvt_pk_bf16_f32 takes two
-// values); the same numbers as converting one value at a time.
-__device__ __forceinline__ u32x2 split_pair(float x0, float x1) {
-    const flt2 x = {x0, x1};
-    const uint32_t hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(x, bf16x2));
-    const flt2 h = {__uint_as_float(hi << 16), __uint_as_float(hi & 0xffff0000u)};
-    return u32x2{hi, __builtin_bit_cast(uint32_t, __builtin_convertvector(x - h, bf16x2))};
-}
 
 // B-tile LDS of ladder_run: two buffers of the chunk's KCH x 16 NT
 // amplitudes.  Split products hold them as BF16 high and low parts, column
@@ -864,13 +848,6 @@ struct SymLadder {
     static constexpr size_t BYTES = (size_t)4 * WBLK * 2 > (size_t)RED * 4 ? (size_t)4 * WBLK * 2 : (size_t)RED * 4;
 };
 
-typedef short sh4 __attribute__((ext_vector_type(4)));
-
-// v_mfma_f32_16x16x16_bf16 on operands given as two words of BF16 pairs
-__device__ __forceinline__ flt4 mfma_bf16_k16(u32x2 a, u32x2 b, flt4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(sh4, a), __builtin_bit_cast(sh4, b), c, 0,
-                                                     0, 0);
-}
 
 template <int SRC, int MODE, bool SYM>
 __device__ __forceinline__ void wave_k_tile(const DTile &tl, const DChan &ch, const float *__restrict__ vp,

@@ -118,7 +118,16 @@ struct RingGroup {      // the mod channels of one block of the staged ring (-1:
 // Column tile jt of a staged channel (As = V^A, Ts = T^ph in LDS; bv = V^B(:, jt)):
 // X(:, jt) = V^B(:, jt) + 1/2 V^A T(:, jt), then R(:, jt) = T X(:, jt), with the
 // k-slot of MFMA step j for lane group kq as in ring_wave.
-template <typename CT, int NT, int STR>
+//
+// SPLIT (CT = float): both products as BF16x3 split products on
+// v_mfma_f32_16x16x16_bf16, three per 16 k instead of four
+// v_mfma_f32_16x16x4f32 (about 16 against 34 SIMD cycles each).  Its
+// operand lanes hold the values the FP32 steps use (lane group kq: k-slots
+// 4 kq .. + 3 of the 16-wide k tile), and its accumulator holds X(4 kq + r,
+// col) in register r like the FP32 one, so X is again the B operand of the
+// second product as it stands.  Each B operand is split once and used for
+// every row tile.
+template <typename CT, int NT, int STR, bool SPLIT>
 __device__ __forceinline__ void ring_staged_tile(CT (*As)[STR], CT (*Ts)[STR], const DMod &my, int jt,
                                                  const CT (&bv)[NT][4], CT *__restrict__ r) {
     typedef typename Mma<CT>::acc Acc;
@@ -126,6 +135,43 @@ __device__ __forceinline__ void ring_staged_tile(CT (*As)[STR], CT (*Ts)[STR], c
     const int lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
     const int PH = my.ph, HP = my.hp, col = jt * 16 + m;
     const bool cok = col < HP;
+    if constexpr (SPLIT) {
+        static_assert(!F64, "split products from FP32 operands");
+        Split4 tb[NT];   // B of the first product: Ts(pt*16 + 4 kq + e, col)
+#pragma unroll
+        for (int pt = 0; pt < NT; pt++)
+            tb[pt] = split4(Ts[pt * 16 + 4 * kq][col], Ts[pt * 16 + 4 * kq + 1][col], Ts[pt * 16 + 4 * kq + 2][col],
+                            Ts[pt * 16 + 4 * kq + 3][col]);
+        Split4 xs[NT];   // X(:, jt), split: B of the second product
+#pragma unroll
+        for (int kt = 0; kt < NT; kt++) {
+            flt4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int pt = 0; pt < NT; pt++) {
+                const flt4 a4 = *reinterpret_cast<const flt4 *>(&As[kt * 16 + m][pt * 16 + 4 * kq]);
+                acc = mfma_bf16x3_k16(split4(a4[0], a4[1], a4[2], a4[3]), tb[pt], acc);
+            }
+            float xv[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) xv[q] = 0.5f * acc[q] + bv[kt][q];
+            xs[kt] = split4(xv[0], xv[1], xv[2], xv[3]);
+        }
+#pragma unroll
+        for (int pt = 0; pt < NT; pt++) {
+            flt4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kt = 0; kt < NT; kt++) {
+                const flt4 a4 = *reinterpret_cast<const flt4 *>(&Ts[pt * 16 + m][kt * 16 + 4 * kq]);
+                acc = mfma_bf16x3_k16(split4(a4[0], a4[1], a4[2], a4[3]), xs[kt], acc);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int i = pt * 16 + Mma<CT>::row(q, lane);
+                if (i < PH && cok) r[my.off_ph + i * HP + col] = acc[q];
+            }
+        }
+        return;
+    }
     Acc x[NT];
 #pragma unroll
     for (int kt = 0; kt < NT; kt++) {
@@ -167,7 +213,7 @@ __device__ __forceinline__ void ring_staged_tile(CT (*As)[STR], CT (*Ts)[STR], c
     }
 }
 
-template <typename CT, int NT>
+template <typename CT, int NT, bool SPLIT>
 __device__ __forceinline__ void ring_staged_block(const RingGroup &grp, const DMod *__restrict__ mods,
                                                   const CT *__restrict__ tph, const CT *__restrict__ va,
                                                   const CT *__restrict__ vb, const double *__restrict__ zero,
@@ -246,7 +292,7 @@ __device__ __forceinline__ void ring_staged_block(const RingGroup &grp, const DM
 
     CT(*As)[STR] = reinterpret_cast<CT(*)[STR]>(lds + g0 * S::CH_BYTES);
     CT(*Ts)[STR] = reinterpret_cast<CT(*)[STR]>(lds + g0 * S::CH_BYTES + sizeof(CT) * D * STR);
-    if (jt0 * 16 < HP) ring_staged_tile<CT, NT, STR>(As, Ts, my, jt0, bv, r);
+    if (jt0 * 16 < HP) ring_staged_tile<CT, NT, STR, SPLIT>(As, Ts, my, jt0, bv, r);
     if (NT == 5 && wave == 0 && 4 * 16 < HP) {   // the fifth column tile
         const CT *B = vb + my.off_b;
         const int col = 4 * 16 + m;
@@ -257,12 +303,13 @@ __device__ __forceinline__ void ring_staged_block(const RingGroup &grp, const DM
                 const int i = kt * 16 + Mma<CT>::row(q, lane);
                 bv[kt][q] = *((i < HP && col < HP) ? B + i * HP + col : zc);
             }
-        ring_staged_tile<CT, NT, STR>(As, Ts, my, 4, bv, r);
+        ring_staged_tile<CT, NT, STR, SPLIT>(As, Ts, my, 4, bv, r);
     }
 }
 
 // Groups of channels with LO..HI tiles (largest first), one group per block.
-template <typename CT, int LO, int HI>
+// SPLIT: BF16x3 split products (FP32 iterations).
+template <typename CT, int LO, int HI, bool SPLIT = false>
 __global__ __launch_bounds__(kThreads) void k_ring_staged(const RingGroup *__restrict__ groups,
                                                           const DMod *__restrict__ mods,
                                                           const CT *__restrict__ tph,
@@ -285,7 +332,7 @@ __global__ __launch_bounds__(kThreads) void k_ring_staged(const RingGroup *__res
     const int nt = (max(m0.ph, m0.hp) + 15) / 16;
     static_for<LO, HI + 1>([&](auto N) {
         constexpr int NT = decltype(N)::value;
-        if (nt == NT) ring_staged_block<CT, NT>(grp, mods, tph, va, vb, zero, r, ring_lds);
+        if (nt == NT) ring_staged_block<CT, NT, SPLIT>(grp, mods, tph, va, vb, zero, r, ring_lds);
     });
 }
 

@@ -178,6 +178,8 @@ int ccd_get_schedule(const ccd_ctx *ctx, int *sched, double *us_per_iter);
 #define CCD_LK_BF16X3W 4     /* FP32 iterations, class 1 only: the kernel of CCD_LK_BF16X3SYM
                                 without the symmetry (every element built; waves split k,
                                 four rows per lane) */
+#define CCD_RK_FP32 0        /* ring: v_mfma_f32_16x16x4f32 (the start) */
+#define CCD_RK_BF16X3 1      /* ring: BF16x3 split products on v_mfma_f32_16x16x16_bf16 */
 #define CCD_LA_MODEL 0       /* tile assignment: the one a model of the CU loads prefers
                                 (a matrix-free batch: CCD_LA_RUNS) */
 #define CCD_LA_CHANNEL 1     /*   round robin in channel order */
@@ -196,6 +198,9 @@ typedef struct ccd_ladder_plan {
                                               each class as chosen (0: not measured) */
     double us_partition[2];                /* ccd_tune_ladder: microseconds per iteration of
                                               CCD_LP_SPLIT and CCD_LP_ONE (0: not measured) */
+    int32_t ring_fp32;                     /* the ring of FP32 iterations (calcChannelT2Diff_phhp,
+                                              src/calc.cpp:264-313) in its mod channels of 1-5
+                                              column tiles: CCD_RK_* */
 } ccd_ladder_plan;
 /* The plan in use.  A context starts with CCD_LK_DEFAULT everywhere, 3
  * blocks per compute unit (generated V) or one block per tile (stored V),
@@ -205,7 +210,7 @@ int ccd_get_ladder_plan(const ccd_ctx *ctx, ccd_ladder_plan *plan);
  * with CCD_E_INVALID on an unknown kind, assignment or partition, a
  * CCD_LK_BF16X3, CCD_LK_BF16X3SYM or CCD_LK_BF16X3W entry for FP64
  * iterations, a CCD_LK_BF16X3SYM or CCD_LK_BF16X3W entry outside class 1,
- * or a block count outside 0..16. */
+ * a block count outside 0..16, or an unknown ring kind. */
 int ccd_set_ladder_plan(ccd_ctx *ctx, const ccd_ladder_plan *plan);
 /* Tune the plan on this problem, for the arithmetic of the following
  * iterations.  Per class: the kernel kinds that `flags` allow, then the
@@ -213,11 +218,14 @@ int ccd_set_ladder_plan(ccd_ctx *ctx, const ccd_ladder_plan *plan);
  * then the assignment, each timed with HIP events over repeated launches of
  * that class alone (us).  Then whole iterations in the context's schedule
  * decide among the plan in use, that per-class result and the plan in use
- * with each allowed kind in every class, each under both partitions
- * (us_partition: the chosen plan's).  Resets T2 to zero. */
+ * with each allowed kind in every class (BF16x3 also with class 1 on
+ * CCD_LK_BF16X3SYM or CCD_LK_BF16X3W), with CCD_TUNE_BF16X3 each also with
+ * the BF16x3 ring, each under both partitions (us_partition: the chosen
+ * plan's).  Resets T2 to zero. */
 #define CCD_TUNE_KINDS 1     /* also try CCD_LK_MFMA16 where it differs from the default */
 #define CCD_TUNE_BF16X3 2    /* also try CCD_LK_BF16X3 and, for class 1, CCD_LK_BF16X3SYM and
-                                CCD_LK_BF16X3W (FP32 iterations) */
+                                CCD_LK_BF16X3W, and the ring on CCD_RK_BF16X3 (FP32
+                                iterations) */
 int ccd_tune_ladder(ccd_ctx *ctx, int flags);
 
 /* MBPT2 correlation energy, 1/4 sum v^2/(e_i+e_j-e_a-e_b)

@@ -182,17 +182,21 @@ def test_stored_and_generated_v_agree_bitwise_under_every_kind(require_gpu, kind
     assert np.array_equal(a.get_t2(), b.get_t2())
 
 
-@pytest.mark.parametrize("kinds", [["bf16x3"] * 8, ["bf16x3sym"] + ["bf16x3"] * 7, ["bf16x3w"] + ["bf16x3"] * 7])
-def test_mixed_solve_with_bf16x3_products_within_1e7_per_particle(require_gpu, kinds):
-    """FP32 iterations on BF16x3 products, then FP64 sweeps: the converged
-    energy within 1e-7 Ha per particle of the oracle's (N=14 electrons)."""
+@pytest.mark.parametrize("kinds,ring", [(["bf16x3"] * 8, "fp32"), (["bf16x3sym"] + ["bf16x3"] * 7, "fp32"),
+                                        (["bf16x3w"] + ["bf16x3"] * 7, "fp32"), (["bf16x3"] * 8, "bf16x3"),
+                                        (["bf16x3w"] + ["bf16x3"] * 7, "bf16x3")])
+def test_mixed_solve_with_bf16x3_products_within_1e7_per_particle(require_gpu, kinds, ring):
+    """FP32 iterations on BF16x3 products (the ladder, with or without the
+    ring), then FP64 sweeps: the converged energy within 1e-7 Ha per particle
+    of the oracle's (N=14 electrons)."""
     args = (1.0, 19, 2)
     m, o = Model.electron_gas(*args), Oracle.electron_gas(*args)
     e_o, stop_o, _ = o.solve(1e-10)
     ctx = Context(m, FP32, matrix_free=True)
-    _plan(ctx, kind_fp32=kinds)
+    _plan(ctx, kind_fp32=kinds, ring_fp32=ring)
     e, stop, n, n32 = ctx.solve_mixed(1e-10, 1e-6)
-    print(f"{kinds[0]} mixed: {n32} + {n - n32} iterations, |dE|/N = {abs(e[-1] - e_o[-1]) / 14:.2e}", flush=True)
+    print(f"{kinds[0]} ring {ring} mixed: {n32} + {n - n32} iterations, |dE|/N = {abs(e[-1] - e_o[-1]) / 14:.2e}",
+          flush=True)
     assert stop == stop_o == "converged" and n32 > 0
     assert abs(e[-1] - e_o[-1]) / 14 <= 1e-7
 

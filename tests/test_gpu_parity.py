@@ -281,7 +281,9 @@ def test_fp32_diagrams_close_to_oracle(require_gpu):
 def test_ring_every_tile_class(require_gpu, kind, args):
     """calcT2Diff_phhp on random T^ph for every mod-channel width the
     LDS-staged (1-5 tiles) and per-wave (6-8 tiles) ring kernels take, in
-    FP64 (1e-12) and FP32 (2e-5, relative to the largest element)."""
+    FP64 (1e-12) and FP32 (2e-5, relative to the largest element), and with
+    the staged ring on BF16x3 split products (2e-4; channels of 6-8 tiles
+    stay FP32)."""
     from coupled_cluster_doubles_amd import FP32
     m, o = _pair(kind, args)
     tph = np.random.default_rng(5).uniform(-0.05, 0.05, m.n_t2)
@@ -290,6 +292,17 @@ def test_ring_every_tile_class(require_gpu, kind, args):
     assert np.max(np.abs(r64 - r_o)) <= A_REL * np.max(np.abs(r_o))
     r32 = Context(m, FP32).ring(tph)
     assert np.max(np.abs(r32 - r_o)) <= F32_REL * np.max(np.abs(r_o))
+    ctx = Context(m, FP32)
+    p = ctx.ladder_plan()
+    p["ring_fp32"] = "bf16x3"
+    ctx.set_ladder_plan(p)
+    rb = ctx.ring(tph)
+    err = np.max(np.abs(rb - r_o)) / np.max(np.abs(r_o))
+    print(f"{kind}{args} BF16x3 ring: max |R - R_oracle| / max|R_oracle| = {err:.2e}", flush=True)
+    assert err <= 2e-4
+    # the FP64 iterations of the same context keep the FP64 ring
+    ctx.set_precision(FP64)
+    assert np.array_equal(ctx.ring(tph), r64)
 
 
 @pytest.mark.parametrize("free", [False, True])
