@@ -194,6 +194,9 @@ def kernel_report(ctx, iters, ct, src):
         e = {"kernel": lab, "ms": ms, "flops": w[g]["flops"], "bytes": w[g]["bytes"]}
         if g in CLASSES_OF_GROUP:
             e["kind"] = group_kind(g, ct, plan)
+        if g == "ring" and ct == "float" and plan.get("ring_fp32") == "bf16x3":
+            e["kernel"] += " (staged ring: BF16x3 split products, 16x16x16 BF16 MFMA)"
+            e["kind"] = "bf16x3"
         if sec > 0:
             e["tflops"] = w[g]["flops"] / sec / 1e12
             e["gbs"] = w[g]["bytes"] / sec / 1e9
@@ -326,6 +329,10 @@ def main():
             ctx.set_ladder_plan(q)
             tk = ctx.time_kernels(n_timed, REPS)
             kind_ab[kind] = {g: tk[g] * 1e3 for g in CLASSES_OF_GROUP}
+        # and the staged ring's two kernels (microseconds per launch of the ring group)
+        for ring in ("fp32", "bf16x3"):
+            ctx.set_ladder_plan(dict(plan, ring_fp32=ring))
+            kind_ab[f"ring_{ring}"] = {"ring": ctx.time_kernels(n_timed, REPS)["ring"] * 1e3}
         ctx.set_ladder_plan(plan)
 
     # executed flops of an iteration: the products on packed pairs
@@ -519,7 +526,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == FP64 else ("f32+bf16x3+f64" if set(BF16X3_KINDS) & set(plan["kind_fp32"]) else "f32+f64"),
+        "dtype": "f64" if prec == FP64 else ("f32+bf16x3+f64" if set(BF16X3_KINDS) & set(plan["kind_fp32"]) or plan.get("ring_fp32") == "bf16x3"
+                                             else "f32+f64"),
         "mixed_vs_fp64": mixed,
         "data": "synthetic: the reference's closed-form basis and Coulomb interaction, T2 from zero",
         "config": {"workload": cfg["name"], "precision": args.precision, "mode": args.mode,

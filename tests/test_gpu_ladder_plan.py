@@ -202,13 +202,14 @@ def test_mixed_solve_with_bf16x3_products_within_1e7_per_particle(require_gpu, k
 
 
 def test_symmetric_kernel_at_n66_matches_bf16x3(require_gpu):
-    """Electron gas N=66 (matrix-free): class-1 channels of up to 1,576
-    packed pairs, so of two k segments (the symmetric kernel's tiles of the
-    second segment only clear their slab).  Its ladder against the BF16x3
-    kernel's, both within BF16X3_REL of the exact one (held to the oracle at
-    N=54), so within twice that of each other; and three whole FP32
-    iterations on each within 1e-6 of each other (relative energy)."""
-    m = Model.electron_gas(0.5, 52, 4)
+    """Electron gas N=66 (matrix-free, r_s=0.5, 3,150 states): class-1
+    channels of up to 1,154 packed pairs, so of two k segments (the symmetric
+    kernel's tiles of the second segment only clear their slab).  Its ladder
+    against the BF16x3 kernel's, both within BF16X3_REL of the exact one
+    (held to the oracle at N=54), so within twice that of each other; and
+    three whole FP32 iterations on each within 1e-6 of each other (relative
+    energy)."""
+    m = Model.electron_gas(0.5, 46, 5)
     ta = _antisym_fast(m, np.random.default_rng(66).uniform(-0.05, 0.05, m.n_t2))
     got = {}
     for kinds in (["bf16x3"] * 8, ["bf16x3sym"] + ["bf16x3"] * 7):
