@@ -28,12 +28,13 @@ LADDER_ASSIGN = {"model": 0, "channel": 1, "longest": 2, "runs": 3}
 LADDER_PARTITIONS = {"split": 0, "one": 1}
 TUNE_KINDS = 1
 TUNE_BF16X3 = 2
+TUNE_RING_BF16X3 = 4
 # launch groups of an iteration, in CCD_KT_* order (include/ccd_amd.h)
 KERNEL_GROUPS = ("ring", "remap_r", "ladder_x", "ladder_tx", "ladder_v1", "ladder_v2", "ladder_v34",
                  "ladder_v58", "update", "remap_t", "reduce")
 
 __all__ = ["Model", "Context", "Batch", "CCDError", "FP64", "FP32", "MATRIX_FREE", "STOP_REASONS",
-           "TUNE_KINDS", "TUNE_BF16X3"]
+           "TUNE_KINDS", "TUNE_BF16X3", "TUNE_RING_BF16X3"]
 
 
 def _ptr(a: np.ndarray):
@@ -253,7 +254,8 @@ class Context:
 
     def tune_ladder(self, flags: int = 0) -> dict:
         """Tune the ladder plan for the current arithmetic (TUNE_KINDS,
-        TUNE_BF16X3 widen the kernels tried); resets T2 to zero."""
+        TUNE_BF16X3, TUNE_RING_BF16X3 widen the kernels tried); resets T2 to
+        zero."""
         check(lib().ccd_tune_ladder(self.h, flags), "ccd_tune_ladder")
         return self.ladder_plan()
 

@@ -2241,7 +2241,7 @@ int tune_ladder(ccd_ctx *c, int flags) {
             cands.push_back(q);
         }
     }
-    if (prec == CCD_PREC_FP32 && (flags & CCD_TUNE_BF16X3)) {   // each of them with the BF16x3 ring
+    if (prec == CCD_PREC_FP32 && (flags & CCD_TUNE_RING_BF16X3)) {   // each of them with the BF16x3 ring
         const size_t nc = cands.size();
         for (size_t q = 0; q < nc; q++) {
             ccd_ladder_plan r = cands[q];

@@ -219,13 +219,16 @@ int ccd_set_ladder_plan(ccd_ctx *ctx, const ccd_ladder_plan *plan);
  * that class alone (us).  Then whole iterations in the context's schedule
  * decide among the plan in use, that per-class result and the plan in use
  * with each allowed kind in every class (BF16x3 also with class 1 on
- * CCD_LK_BF16X3SYM or CCD_LK_BF16X3W), with CCD_TUNE_BF16X3 each also with
- * the BF16x3 ring, each under both partitions (us_partition: the chosen
- * plan's).  Resets T2 to zero. */
+ * CCD_LK_BF16X3SYM or CCD_LK_BF16X3W), with CCD_TUNE_RING_BF16X3 each also
+ * with the BF16x3 ring, each under both partitions (us_partition: the
+ * chosen plan's).  Resets T2 to zero. */
 #define CCD_TUNE_KINDS 1     /* also try CCD_LK_MFMA16 where it differs from the default */
 #define CCD_TUNE_BF16X3 2    /* also try CCD_LK_BF16X3 and, for class 1, CCD_LK_BF16X3SYM and
-                                CCD_LK_BF16X3W, and the ring on CCD_RK_BF16X3 (FP32
-                                iterations) */
+                                CCD_LK_BF16X3W (FP32 iterations) */
+#define CCD_TUNE_RING_BF16X3 4  /* also try the ring on CCD_RK_BF16X3 (FP32 iterations).  Its
+                                rounding moves the fixed point of the FP32 phase of a mixed
+                                solve further: at N=54 the phase then stops after 24
+                                iterations and needs 10 FP64 ones instead of 26 + 5. */
 int ccd_tune_ladder(ccd_ctx *ctx, int flags);
 
 /* MBPT2 correlation energy, 1/4 sum v^2/(e_i+e_j-e_a-e_b)

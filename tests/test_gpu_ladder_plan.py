@@ -16,7 +16,8 @@ every kind.
 import numpy as np
 import pytest
 
-from coupled_cluster_doubles_amd import (FP32, FP64, TUNE_BF16X3, TUNE_KINDS, CCDError, Context, Model)
+from coupled_cluster_doubles_amd import (FP32, FP64, TUNE_BF16X3, TUNE_KINDS, TUNE_RING_BF16X3, CCDError, Context,
+                                         Model)
 from oracle.pyoracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -238,7 +239,7 @@ def test_tuner_keeps_results_or_tolerance(require_gpu):
     assert p["us_fp32"][0] > 0 and all(v > 0 for v in p["us_partition"].values())
     got = ctx.solve_mixed(1e-10, 1e-6)
     assert np.array_equal(got[0], ref[0]) and got[1:] == ref[1:]
-    p = ctx.tune_ladder(TUNE_KINDS | TUNE_BF16X3)
+    p = ctx.tune_ladder(TUNE_KINDS | TUNE_BF16X3 | TUNE_RING_BF16X3)
     print(f"tuned plan: {p}", flush=True)
     assert p["kind_fp64"] == ["default"] * 8
     e, stop, n, n32 = ctx.solve_mixed(1e-10, 1e-6)
