@@ -2643,7 +2643,8 @@ int ccd_set_ladder_plan(ccd_ctx *c, const ccd_ladder_plan *plan) {
 int ccd_tune_ladder(ccd_ctx *c, int flags) {
     ccd::clear_error();
     if (!c) return ccd::fail(CCD_E_INVALID, "null context");
-    if (flags & ~(CCD_TUNE_KINDS | CCD_TUNE_BF16X3)) return ccd::fail(CCD_E_INVALID, "unknown tuner flags %d", flags);
+    if (flags & ~(CCD_TUNE_KINDS | CCD_TUNE_BF16X3 | CCD_TUNE_RING_BF16X3))
+        return ccd::fail(CCD_E_INVALID, "unknown tuner flags %d", flags);
     const int saved = c->iter_prec;
     int rc = tune_ladder(c, flags);
     c->iter_prec = saved;
