@@ -168,3 +168,33 @@ def test_batch_iterate_n_keeps_going_past_a_diverging_solve(require_gpu):
     assert np.array_equal(en[0, :n], e_div, equal_nan=True)
     assert np.all(np.isnan(en[0, n:]))
     assert np.array_equal(en[1], Context(ok, FP64).iterate_n(25))
+
+
+@pytest.mark.parametrize("k1", ["bf16x3w", "bf16x3sym"])
+def test_batch_wave_k_ladder_kinds_match_single_contexts_bitwise(require_gpu, k1):
+    """The class-1 wave-k ladder kernels (bf16x3w, and the symmetric kind with
+    its partial sums read by the update) in a matrix-free mixed batch: each
+    electron-gas solve generates V from its own table, the others through
+    the generic generator, stopped solves are skipped, and every solve equals
+    a context of its own on the same kernel kinds bit for bit."""
+    _, models = _small()
+    kinds = [k1] + ["bf16x3"] * 7
+
+    def pinned(ctx):
+        p = ctx.ladder_plan()
+        p["kind_fp32"] = kinds
+        ctx.set_ladder_plan(p)
+        return ctx
+
+    b = pinned(Batch(models, FP32, matrix_free=True))
+    res = b.solve_mixed(1e-10, 1e-6)
+    assert len({r[2] for r in res}) > 1          # the solves stop at different iterations
+    for q, m in enumerate(models):
+        c = pinned(Context(m, FP32, matrix_free=True))
+        e, stop, n, n32 = c.solve_mixed(1e-10, 1e-6)
+        eb, stop_b, n_b, n32_b = res[q]
+        print(f"{k1} solve {q}: {stop_b} after {n_b} ({n32_b} FP32) iterations", flush=True)
+        assert (stop_b, n_b, n32_b) == (stop, n, n32), q
+        assert np.array_equal(eb, e), q
+        assert np.array_equal(b.t2(q), c.get_t2()), q
+        c.close()
