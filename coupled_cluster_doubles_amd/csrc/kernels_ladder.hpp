@@ -910,7 +910,11 @@ __device__ __forceinline__ void wave_k_tile(const DTile &tl, const DChan &ch, co
 #pragma unroll
     for (int rb = 0; rb < 4; rb++) acc[rb] = flt4{0.f, 0.f, 0.f, 0.f};
     uint16_t *blh = reinterpret_cast<uint16_t *>(lds) + wave * SymLadder::WBLK, *bll = blh + SKC * SPH;
-    float *sym = dsym + (int64_t)ch.sym * 1024;
+    // the channel's symmetric partial sums (the host gives every class-1
+    // channel of two or more row tiles a place; a chunk past the tile exists
+    // only in those)
+    const bool has_sym = SYM && ch.sym >= 0;
+    float *sym = has_sym ? dsym + (int64_t)ch.sym * 1024 : dsym;
 
     if (wave < nch) {   // uniform
         int2 rc = GEN ? buf_i2(rsrc, roff + (uint32_t)(wave * SKC) * 8u) : int2{0, 0};
@@ -963,7 +967,7 @@ __device__ __forceinline__ void wave_k_tile(const DTile &tl, const DChan &ch, co
                 acc[rb] = mfma_bf16_k16(ah[rb], bl, acc[rb]);
                 acc[rb] = mfma_bf16_k16(ah[rb], bh, acc[rb]);
             }
-            if (SYM && c >= 4) {   // uniform: the chunk lies past the tile
+            if (has_sym && c >= 4) {   // uniform: the chunk lies past the tile
 #pragma unroll
                 for (int rb = 0; rb < 4; rb++)
 #pragma unroll
