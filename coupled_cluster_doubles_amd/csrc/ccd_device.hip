@@ -327,13 +327,20 @@ size_t ring_stage_bytes(int nt) {
 
 // LDS-staged ring: one launch over the groups of 1-4 tile channels, one over
 // the 5-tile channels (whose register and LDS footprint would lower the
-// occupancy of the others)
+// occupancy of the others).  The 5-tile blocks have a wave per column tile
+// for FP64 and BF16x3 products (2.5-3 us less of a 40-100 us ring at 66
+// electrons) and four waves for plain FP32 products, which measured 3 us
+// slower with five.
+template <typename CT, bool SPLIT>
+constexpr int ring_five_waves() { return sizeof(CT) == 8 || SPLIT ? 5 : 4; }
+
 template <typename CT, bool SPLIT = false>
 void launch_ring_staged(ccd_ctx *c, const CT *va, const CT *vb, hipStream_t st) {
     const int n5 = c->n_rg_five, n = (int)c->rgroups.size();
     if (n5 > 0) {
+        constexpr int NW = ring_five_waves<CT, SPLIT>();
         const size_t lds = RingStage<CT, 5>::BYTES;
-        hipLaunchKernelGGL((k_ring_staged<CT, 5, 5, SPLIT>), dim3((unsigned)n5), dim3(kThreads), lds, st,
+        hipLaunchKernelGGL((k_ring_staged<CT, 5, 5, SPLIT, NW>), dim3((unsigned)n5), dim3(NW * 64), lds, st,
                            c->d_rgroups, c->d_mods, reinterpret_cast<const CT *>(c->d_tph), va, vb, c->d_zero,
                            reinterpret_cast<CT *>(c->d_r), c->d_ctl, skip_ctl(c));
     }
@@ -346,9 +353,9 @@ void launch_ring_staged(ccd_ctx *c, const CT *va, const CT *vb, hipStream_t st) 
 }
 
 // The staged instances that may use more than 64 KB of LDS need the limit raised.
-template <typename CT, int LO, int HI, bool SPLIT = false>
+template <typename CT, int LO, int HI, bool SPLIT = false, int NW = 4>
 int allow_ring_lds(size_t bytes) {
-    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_ring_staged<CT, LO, HI, SPLIT>),
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_ring_staged<CT, LO, HI, SPLIT, NW>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     return CCD_OK;
 }
@@ -356,9 +363,9 @@ int allow_ring_lds(size_t bytes) {
 int allow_ring_lds_all() {
     int rc;
     if ((rc = allow_ring_lds<double, 1, 4>(RingStage<double, 4>::BYTES)) ||
-        (rc = allow_ring_lds<double, 5, 5>(RingStage<double, 5>::BYTES)) ||
-        (rc = allow_ring_lds<float, 5, 5>(RingStage<float, 5>::BYTES)) ||
-        (rc = allow_ring_lds<float, 5, 5, true>(RingStage<float, 5>::BYTES)))
+        (rc = allow_ring_lds<double, 5, 5, false, ring_five_waves<double, false>()>(RingStage<double, 5>::BYTES)) ||
+        (rc = allow_ring_lds<float, 5, 5, false, ring_five_waves<float, false>()>(RingStage<float, 5>::BYTES)) ||
+        (rc = allow_ring_lds<float, 5, 5, true, ring_five_waves<float, true>()>(RingStage<float, 5>::BYTES)))
         return rc;
     return CCD_OK;
 }

@@ -213,7 +213,7 @@ __device__ __forceinline__ void ring_staged_tile(CT (*As)[STR], CT (*Ts)[STR], c
     }
 }
 
-template <typename CT, int NT, bool SPLIT>
+template <typename CT, int NT, bool SPLIT, int NW>
 __device__ __forceinline__ void ring_staged_block(const RingGroup &grp, const DMod *__restrict__ mods,
                                                   const CT *__restrict__ tph, const CT *__restrict__ va,
                                                   const CT *__restrict__ vb, const double *__restrict__ zero,
@@ -255,11 +255,12 @@ __device__ __forceinline__ void ring_staged_block(const RingGroup &grp, const DM
     }
 
     // staging in rounds of up to 8 elements per thread (every load of a round
-    // is issued before its LDS stores): element e = thread + 256 q of the group
-    // lies in channel e / D^2 = q / NT^2 (D^2 = 256 NT^2), a constant of the
+    // is issued before its LDS stores): element e = thread + TB q of the group
+    // (TB threads) lies in channel e / D^2 = q / (D^2 / TB), a constant of the
     // unrolled loops
-    constexpr int PER = G * D * D / kThreads;
-    static_assert(G * D * D % kThreads == 0, "whole staging rounds");
+    constexpr int TB = 64 * NW, CR = D * D / TB;   // threads; rounds per channel
+    constexpr int PER = G * CR;
+    static_assert(D * D % TB == 0, "whole staging rounds");
     constexpr int RND = PER < 8 ? PER : 8;
 #pragma unroll
     for (int q0 = 0; q0 < PER; q0 += RND) {
@@ -268,7 +269,7 @@ __device__ __forceinline__ void ring_staged_block(const RingGroup &grp, const DM
         for (int u = 0; u < RND; u++) {
             const int q = q0 + u;
             if (q < PER) {
-                const int g = q / (NT * NT), rem = threadIdx.x + (q - g * NT * NT) * kThreads;
+                const int g = q / CR, rem = threadIdx.x + (q - g * CR) * TB;
                 const int row = rem / D, c = rem - row * D;
                 const DMod &h = md[g];
                 sa[u] = *((row < h.hp && c < h.ph) ? va + h.off_ph + row * h.ph + c : zc);
@@ -279,7 +280,7 @@ __device__ __forceinline__ void ring_staged_block(const RingGroup &grp, const DM
         for (int u = 0; u < RND; u++) {
             const int q = q0 + u;
             if (q < PER) {
-                const int g = q / (NT * NT), rem = threadIdx.x + (q - g * NT * NT) * kThreads;
+                const int g = q / CR, rem = threadIdx.x + (q - g * CR) * TB;
                 const int row = rem / D, c = rem - row * D;
                 CT(*As)[STR] = reinterpret_cast<CT(*)[STR]>(lds + g * S::CH_BYTES);
                 CT(*Ts)[STR] = reinterpret_cast<CT(*)[STR]>(lds + g * S::CH_BYTES + sizeof(CT) * D * STR);
@@ -293,7 +294,7 @@ __device__ __forceinline__ void ring_staged_block(const RingGroup &grp, const DM
     CT(*As)[STR] = reinterpret_cast<CT(*)[STR]>(lds + g0 * S::CH_BYTES);
     CT(*Ts)[STR] = reinterpret_cast<CT(*)[STR]>(lds + g0 * S::CH_BYTES + sizeof(CT) * D * STR);
     if (jt0 * 16 < HP) ring_staged_tile<CT, NT, STR, SPLIT>(As, Ts, my, jt0, bv, r);
-    if (NT == 5 && wave == 0 && 4 * 16 < HP) {   // the fifth column tile
+    if (NT == 5 && NW == 4 && wave == 0 && 4 * 16 < HP) {   // the fifth column tile (four waves)
         const CT *B = vb + my.off_b;
         const int col = 4 * 16 + m;
 #pragma unroll
@@ -307,10 +308,12 @@ __device__ __forceinline__ void ring_staged_block(const RingGroup &grp, const DM
     }
 }
 
-// Groups of channels with LO..HI tiles (largest first), one group per block.
-// SPLIT: BF16x3 split products (FP32 iterations).
-template <typename CT, int LO, int HI, bool SPLIT = false>
-__global__ __launch_bounds__(kThreads) void k_ring_staged(const RingGroup *__restrict__ groups,
+// Groups of channels with LO..HI tiles (largest first), one group per block
+// of NW waves.  SPLIT: BF16x3 split products (FP32 iterations).  Channels of
+// 5 tiles run with NW = 5, a wave per column tile (with four waves, wave 0
+// would take the fifth tile as well, and the block would wait for it).
+template <typename CT, int LO, int HI, bool SPLIT = false, int NW = 4>
+__global__ __launch_bounds__(64 * NW) void k_ring_staged(const RingGroup *__restrict__ groups,
                                                           const DMod *__restrict__ mods,
                                                           const CT *__restrict__ tph,
                                                           const CT *__restrict__ va, const CT *__restrict__ vb,
@@ -332,7 +335,7 @@ __global__ __launch_bounds__(kThreads) void k_ring_staged(const RingGroup *__res
     const int nt = (max(m0.ph, m0.hp) + 15) / 16;
     static_for<LO, HI + 1>([&](auto N) {
         constexpr int NT = decltype(N)::value;
-        if (nt == NT) ring_staged_block<CT, NT, SPLIT>(grp, mods, tph, va, vb, zero, r, ring_lds);
+        if (nt == NT) ring_staged_block<CT, NT, SPLIT, NW>(grp, mods, tph, va, vb, zero, r, ring_lds);
     });
 }
 
