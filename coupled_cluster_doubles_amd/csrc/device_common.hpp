@@ -13,6 +13,17 @@ constexpr int kZeroDoubles = 32;  // size of the zero buffer masked-off loads re
 constexpr int kElogCap = 1024;    // energies logged by a device-run iteration loop
 constexpr int kSolveBatch = 4;    // iterations queued between two reads of the loop state
 
+// The workgroups of a launch are dispatched to the 8 XCDs in turn (block b to
+// XCD (b + c) mod 8 for some c), and each XCD has an L2 of its own.  The work
+// item of block b of an n-block grid such that each XCD takes a contiguous
+// run of the n items: XCD x gets items x (n / 8) + min(x, n mod 8) onward, in
+// block order, so that neighbouring items that read the same rows share an
+// L2.  A bijection of [0, n).
+__host__ __device__ __forceinline__ int xcd_slot(int b, int n) {
+    const int x = b & 7, q = b >> 3, per = n >> 3, rem = n & 7;
+    return x * per + (x < rem ? x : rem) + q;
+}
+
 // Raw (unswizzled) buffer loads.  A lane whose byte offset is at or beyond the
 // buffer's size reads 0 without touching memory (the hardware range check), so
 // an element outside a tile needs no select: its offset is kOOB (the sum of

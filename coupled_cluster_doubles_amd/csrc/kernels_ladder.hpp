@@ -44,7 +44,8 @@ __global__ __launch_bounds__(kThreads) void k_ladder_x(const DTile *__restrict__
     if (ctl->done) return;
     __shared__ double part[kThreads / 64][64][4];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
-    const DTile tl = tiles[blockIdx.x];
+    // the tiles of a channel (which read the same rows) on one XCD
+    const DTile tl = tiles[xcd_slot(blockIdx.x, gridDim.x)];
     const DChan ch = chans[tl.chan];
     if (block_solve_stopped(pctl, ch.prob)) return;   // a stopped solve of a batch
     const int H = ch.hh, kl0 = tl.row0, ij0 = tl.aux, seg = tl.nrows;
