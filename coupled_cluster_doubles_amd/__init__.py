@@ -24,7 +24,7 @@ SCHEDULES = {"auto": 0, "streams": 1, "serial": 2, "serial_graph": 3}
 # pppp-ladder plan vocabulary (CCD_LK_*, CCD_LA_*, CCD_LP_*, CCD_TUNE_*)
 LADDER_KINDS = {"default": 0, "mfma16": 1, "bf16x3": 2, "bf16x3sym": 3, "bf16x3w": 4}
 RING_KINDS = {"fp32": 0, "bf16x3": 1}
-LADDER_ASSIGN = {"model": 0, "channel": 1, "longest": 2, "runs": 3}
+LADDER_ASSIGN = {"model": 0, "channel": 1, "longest": 2, "runs": 3, "xcd": 4}
 LADDER_PARTITIONS = {"split": 0, "one": 1}
 TUNE_KINDS = 1
 TUNE_BF16X3 = 2

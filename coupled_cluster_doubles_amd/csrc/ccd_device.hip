@@ -1196,22 +1196,56 @@ void plan_class_tiles(ccd_ctx *c, int nt) {
         return;
     }
     if (mode != CCD_LA_CHANNEL) {
+        // XCD groups of the blocks: the dispatcher sends block b to XCD
+        // (b + c) mod 8, so the blocks b = x mod 8 share an L2 (CCD_LA_XCD);
+        // one group of all blocks otherwise
+        const int nx = mode == CCD_LA_XCD ? std::min(G, 8) : 1;
+        std::vector<int> grp(n, 0);
+        if (nx > 1) {
+            // units: the tiles of a channel in order, cut where a unit would
+            // pass 1/32 of the class; largest unit first to the least loaded XCD
+            int64_t total = 0;
+            for (int i = 0; i < n; i++) total += cost[i];
+            const int64_t cap = std::max<int64_t>(1, total / 32);
+            std::vector<std::pair<int64_t, int>> units;   // (cost, first tile)
+            std::vector<int> unit_of(n);
+            for (int i = 0; i < n; i++) {
+                if (i == 0 || c->ltiles0[lo + i].chan != c->ltiles0[lo + i - 1].chan ||
+                    units.back().first + cost[i] > cap)
+                    units.push_back({0, i});
+                units.back().first += cost[i];
+                unit_of[i] = (int)units.size() - 1;
+            }
+            std::vector<int> uidx(units.size()), ux(units.size());
+            for (size_t u = 0; u < units.size(); u++) uidx[u] = (int)u;
+            std::stable_sort(uidx.begin(), uidx.end(), [&](int a, int b) { return units[a].first > units[b].first; });
+            std::vector<int64_t> xload(nx, 0);
+            for (int u : uidx) {
+                const int x = (int)(std::min_element(xload.begin(), xload.end()) - xload.begin());
+                ux[u] = x;
+                xload[x] += units[u].first;
+            }
+            for (int i = 0; i < n; i++) grp[i] = ux[unit_of[i]];
+        }
         std::vector<int> idx(n);
         for (int i = 0; i < n; i++) idx[i] = i;
         std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-        // least loaded block, lowest index on ties
-        std::priority_queue<std::pair<int64_t, int>, std::vector<std::pair<int64_t, int>>,
-                            std::greater<std::pair<int64_t, int>>>
-            heap;
-        for (int b = 0; b < G; b++) heap.push({0, b});
+        // least loaded block of the tile's group, lowest index on ties
+        typedef std::priority_queue<std::pair<int64_t, int>, std::vector<std::pair<int64_t, int>>,
+                                    std::greater<std::pair<int64_t, int>>>
+            Heap;
+        std::vector<Heap> heap(nx);
+        for (int b = 0; b < G; b++) heap[b % nx].push({0, b});
         for (int i : idx) {
-            auto [load, b] = heap.top();
-            heap.pop();
+            Heap &h = heap[grp[i]];
+            auto [load, b] = h.top();
+            h.pop();
             lpt[b].push_back(i);
-            heap.push({load + cost[i], b});
+            h.push({load + cost[i], b});
         }
     }
-    const bool use_lpt = mode == CCD_LA_LONGEST || (mode == CCD_LA_MODEL && 10 * cu_max(lpt) <= 9 * cu_max(rr));
+    const bool use_lpt = mode == CCD_LA_LONGEST || mode == CCD_LA_XCD ||
+                         (mode == CCD_LA_MODEL && 10 * cu_max(lpt) <= 9 * cu_max(rr));
     const auto &own = use_lpt ? lpt : rr;
     int k = 0;
     for (int b = 0; b < G; b++) {
@@ -2165,7 +2199,7 @@ int plan_valid(const ccd_ladder_plan &p) {
         if (p.blocks_per_cu[n] < 0 || p.blocks_per_cu[n] > 16)
             return ccd::fail(CCD_E_INVALID, "ladder plan: %d blocks per compute unit (class %d)", p.blocks_per_cu[n],
                              n + 1);
-        if (p.assign[n] < CCD_LA_MODEL || p.assign[n] > CCD_LA_RUNS)
+        if (p.assign[n] < CCD_LA_MODEL || p.assign[n] > CCD_LA_XCD)
             return ccd::fail(CCD_E_INVALID, "ladder plan: unknown assignment %d (class %d)", p.assign[n], n + 1);
     }
     if (p.partition != CCD_LP_SPLIT && p.partition != CCD_LP_ONE)
@@ -2181,8 +2215,9 @@ int plan_valid(const ccd_ladder_plan &p) {
 // timed alone do not show how they share the GPU with each other and with
 // the ring and the remaps in the four-stream schedule, so the final choice
 // is made on whole iterations in the context's schedule, among: the plan
-// the tuner started from, the per-class result, the starting plan with each
-// other allowed kind in every class, each under both partitions.  The
+// the tuner started from, the per-class result, the starting plan and the
+// per-class one with each other allowed kind in every class, each under
+// both partitions.  The
 // amplitudes of two free iterations give the kernels realistic operands;
 // T2 goes back to zero at the end.
 int tune_ladder(ccd_ctx *c, int flags) {
@@ -2230,24 +2265,27 @@ int tune_ladder(ccd_ctx *c, int flags) {
         std::vector<int> bpcs{1, 2, 3, 4, 6};
         if (!c->matrix_free) bpcs.insert(bpcs.begin(), 0);
         if ((rc = consider(p.kind[prec][nt - 1], ck)) || (rc = consider(p.blocks_per_cu[nt - 1], bpcs)) ||
-            (rc = consider(p.assign[nt - 1], {CCD_LA_CHANNEL, CCD_LA_LONGEST, CCD_LA_RUNS})))
+            (rc = consider(p.assign[nt - 1], {CCD_LA_CHANNEL, CCD_LA_LONGEST, CCD_LA_RUNS, CCD_LA_XCD})))
             break;
         us_class[nt - 1] = best;
     }
-    // whole plans, whole iterations
+    // whole plans, whole iterations: each kind in every class on the
+    // starting plan's blocks and assignments and on the per-class ones
+    const ccd_ladder_plan per_class = p;
     std::vector<ccd_ladder_plan> cands{start, p};
-    for (int k : kinds) {
-        if (k == CCD_LK_DEFAULT) continue;
-        ccd_ladder_plan q = start;
-        for (int n = 0; n < NTMAX; n++) q.kind[prec][n] = k;
-        cands.push_back(q);
-        if (k == CCD_LK_BF16X3) {   // and class 1 on the wave-k kernels
-            q.kind[prec][0] = CCD_LK_BF16X3SYM;
+    for (const ccd_ladder_plan *base : {&start, &per_class})
+        for (int k : kinds) {
+            if (k == CCD_LK_DEFAULT) continue;
+            ccd_ladder_plan q = *base;
+            for (int n = 0; n < NTMAX; n++) q.kind[prec][n] = k;
             cands.push_back(q);
-            q.kind[prec][0] = CCD_LK_BF16X3W;
-            cands.push_back(q);
+            if (k == CCD_LK_BF16X3) {   // and class 1 on the wave-k kernels
+                q.kind[prec][0] = CCD_LK_BF16X3SYM;
+                cands.push_back(q);
+                q.kind[prec][0] = CCD_LK_BF16X3W;
+                cands.push_back(q);
+            }
         }
-    }
     if (prec == CCD_PREC_FP32 && (flags & CCD_TUNE_RING_BF16X3)) {   // each of them with the BF16x3 ring
         const size_t nc = cands.size();
         for (size_t q = 0; q < nc; q++) {
@@ -2266,12 +2304,20 @@ int tune_ladder(ccd_ctx *c, int flags) {
             double us = 0.0;
             if ((rc = upload_ladder_tiles(c)) || (rc = drop_graphs(c)) || (rc = time_iterations(c, &us))) break;
             cands[q].us_partition[part] = us;
-            if (best == 0.0 || us < best) {
-                best = us;
-                pick = cands[q];
-                pick.partition = part;
-            }
+            if (best == 0.0 || us < best) best = us;
         }
+    // whole iterations of plans this close differ by the noise of the
+    // timing: the first candidate within 1% of the fastest (the per-class
+    // result comes right after the starting plan)
+    for (size_t q = 0; q < cands.size() && !rc; q++) {
+        int part = CCD_LP_SPLIT;
+        while (part <= CCD_LP_ONE && !(cands[q].us_partition[part] <= 1.01 * best)) part++;
+        if (part <= CCD_LP_ONE) {
+            pick = cands[q];
+            pick.partition = part;
+            break;
+        }
+    }
     if (!rc) {
         p = pick;
         for (int n = 0; n < NTMAX; n++) p.us[prec][n] = us_class[n];

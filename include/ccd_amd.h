@@ -185,6 +185,10 @@ int ccd_get_schedule(const ccd_ctx *ctx, int *sched, double *us_per_iter);
 #define CCD_LA_CHANNEL 1     /*   round robin in channel order */
 #define CCD_LA_LONGEST 2     /*   longest tile first to the least loaded block */
 #define CCD_LA_RUNS 3        /*   contiguous runs of about equal cost, in channel order */
+#define CCD_LA_XCD 4         /*   channels (the largest cut into pieces of at most 1/32 of the
+                                class) to the 8 XCDs, least loaded first, then longest tile
+                                first over each XCD's blocks: the tiles of a channel, which
+                                read the same packed T rows, share one L2 */
 #define CCD_LP_SPLIT 0       /* launches: classes 1, 2, 3-4 and 5-8 (in the four-stream
                                 schedule classes 1-2 and 3-8 run on two streams) */
 #define CCD_LP_ONE 1         /* one grouped launch of every class */

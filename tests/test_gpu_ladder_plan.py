@@ -91,6 +91,7 @@ def test_scheduling_fields_change_no_result(require_gpu, kind, args, prec, free)
         {"blocks_per_cu": [1] * 8, "assign": ["channel"] * 8},
         {"blocks_per_cu": [6] * 8, "assign": ["longest"] * 8, "partition": "one"},
         {"blocks_per_cu": [0, 2, 4, 0, 1, 3, 0, 2], "assign": ["longest", "channel"] * 4, "partition": "one"},
+        {"blocks_per_cu": [2, 3, 1, 6, 4, 2, 3, 1], "assign": ["xcd", "runs", "xcd", "model"] * 2},
     ]
     ref = None
     for q, kw in enumerate(plans):
